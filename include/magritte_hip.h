@@ -1,0 +1,230 @@
+/*
+ * magritte_hip.h -- C-ABI of the MI355X (gfx950) radiation-field and
+ * level-population library (libmagritte_hip.so).
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, no C++ or torch
+ * types.  Every entry point returns an int status (MGR_OK = 0, negative on
+ * failure) and never throws; the message of the last failure on a context is
+ * returned by mgr_last_error().
+ *
+ * Each entry point names the reference (UCL/Magritte) interface it replaces,
+ * as path:line inside the reference tree.  The model is described by the
+ * io-layout arrays the reference reads in Model::read (src/Model/model.cpp:54-86):
+ * units are SI, velocities in units of c, vturb2 in (v/c)^2.
+ *
+ * One line-producing species per context (the reference loops over
+ * lines.lineProducingSpecies; the configurations served here all have one).
+ */
+#ifndef MAGRITTE_HIP_H
+#define MAGRITTE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------ */
+#define MGR_OK                 0
+#define MGR_ERR_ARGUMENT      -1   /* bad argument / precondition (reference returns -1) */
+#define MGR_ERR_DEVICE        -2   /* HIP runtime error                                 */
+#define MGR_ERR_NOMEM         -3   /* device allocation failed                          */
+#define MGR_ERR_TRACE         -4   /* a ray found no forward neighbour inside the mesh  */
+#define MGR_ERR_NUMERIC       -5   /* singular rate matrix                              */
+#define MGR_ERR_STATE         -6   /* called out of order (e.g. no spectral discretisation) */
+#define MGR_ERR_UNSUPPORTED   -7   /* feature outside this library's scope              */
+
+/* Precision of the radiation-field solve (opacity, emissivity, source function,
+ * Feautrier elimination).  J is accumulated over rays in FP64 in both modes. */
+#define MGR_FP32 0
+#define MGR_FP64 1
+
+/* Convergence criteria of the ALI driver. */
+#define MGR_CRIT_FRACTION 0  /* fraction of levels with rel. change > pop_prec must be <= 0.005
+                                (src/Simulation/sim_lines.tpp:140)                          */
+#define MGR_CRIT_MAX      1  /* max relative change over levels with n > 1e-10 n_tot < pop_prec
+                                (magritte/run/levelpops.py:11-13,44)                        */
+
+/* ------------------------------------------------------------------------ */
+/* Model view: what Model::read loads (src/Model/model.cpp:54-86)            */
+/* ------------------------------------------------------------------------ */
+typedef struct mgr_model_view {
+    /* Parameters (src/Model/parameters.hpp:18-96) */
+    int64_t ncells;
+    int64_t nrays;
+    int64_t nboundary;
+    int64_t nspecs;
+    int64_t nquads;
+    double  pop_prec;            /* required precision for populations           */
+    double  max_width_fraction;  /* Parameters::max_width_fraction, default 0.5  */
+    int64_t n_off_diag;          /* ALO bandwidth; only 0 is supported here      */
+
+    /* Geometry/Cells (src/Model/Geometry/Cells/cells.cpp:20-93) */
+    const double  *position;     /* [ncells][3] m                                 */
+    const double  *velocity;     /* [ncells][3] units of c                        */
+    const int64_t *n_neighbors;  /* [ncells]                                      */
+    const int64_t *neighbors;    /* [sum(n_neighbors)] flattened, in list order   */
+
+    /* Geometry/Rays (src/Model/Geometry/Rays/rays.cpp:22-52) */
+    const double  *rays;         /* [nrays][3] unit vectors                       */
+    const double  *weights;      /* [nrays]                                       */
+
+    /* Geometry/Boundary (src/Model/Geometry/Boundary/boundary.cpp:19-67) */
+    const int64_t *boundary2cell_nr;      /* [nboundary]                          */
+
+    /* Thermodynamics (src/Model/Thermodynamics) */
+    const double  *temperature;  /* [ncells] gas temperature, K                   */
+    const double  *vturb2;       /* [ncells] (v_turb/c)^2                         */
+
+    /* Chemistry/Species (src/Model/Chemistry/Species/species.cpp:19-56) */
+    const double  *abundance;    /* [ncells][nspecs] number densities, m^-3       */
+
+    /* Lines/LineProducingSpecies_0/Linedata (Linedata/linedata.cpp:25-88) */
+    int64_t        num;          /* species index of the line producer            */
+    double         inverse_mass; /* 1 / (mass in amu)                             */
+    int64_t        nlev;
+    int64_t        nrad;
+    const double  *energy;       /* [nlev] J                                      */
+    const double  *weight;       /* [nlev] statistical weights                    */
+    const int64_t *irad;         /* [nrad] upper level                            */
+    const int64_t *jrad;         /* [nrad] lower level                            */
+    const double  *A;            /* [nrad] Einstein A, s^-1                       */
+    const double  *Bs;           /* [nrad] stimulated emission coefficient        */
+    const double  *Ba;           /* [nrad] absorption coefficient                 */
+    const double  *frequency;    /* [nrad] Hz                                     */
+
+    /* .../Linedata/CollisionPartner_<c> (CollisionPartner/collisionPartner.cpp:20-60),
+     * concatenated over partners c = 0..ncolpar-1                                */
+    int64_t        ncolpar;
+    const int64_t *colpar_num;   /* [ncolpar] species index of the partner        */
+    const char    *colpar_orth_or_para; /* [ncolpar] 'n', 'o' or 'p'              */
+    const int64_t *colpar_ncol;  /* [ncolpar]                                     */
+    const int64_t *colpar_ntmp;  /* [ncolpar]                                     */
+    const int64_t *icol;         /* [sum ncol] upper level                        */
+    const int64_t *jcol;         /* [sum ncol] lower level                        */
+    const double  *tmp;          /* [sum ntmp] table temperatures, K              */
+    const double  *Cd;           /* [sum ntmp*ncol], partner-major, then [t][k], m^3 s^-1 */
+    const double  *Ce;           /* same layout                                   */
+
+    /* Lines/LineProducingSpecies_0/Quadrature (Quadrature/quadrature.cpp:19-40) */
+    const double  *roots;        /* [nquads]                                      */
+    const double  *qweights;     /* [nquads]                                      */
+} mgr_model_view;
+
+/* Convergence statistics of one level-population update
+ * (LineProducingSpecies::check_for_convergence,
+ *  src/Model/Lines/LineProducingSpecies/lineProducingSpecies.tpp:120-171).      */
+typedef struct mgr_conv {
+    double fraction_not_converged;
+    double relative_change_mean;
+    double relative_change_max;   /* computed here; the reference leaves it at 0 */
+} mgr_conv;
+
+typedef struct mgr_ctx mgr_ctx;   /* opaque: device-resident model + state       */
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Uploads the model to device `device` (SoA layout) and builds the derived
+ * static tables.  Replaces Model::read (src/Model/model.cpp:54-86) plus the
+ * antipodal-ray set-up Rays::setup_antipodal_rays (Rays/rays.cpp:90-103).     */
+int  mgr_create(const mgr_model_view *model, int device, mgr_ctx **out);
+void mgr_destroy(mgr_ctx *ctx);
+const char *mgr_last_error(const mgr_ctx *ctx);   /* ctx may be NULL (create errors) */
+int  mgr_device_count(int *count);
+
+/* ------------------------------------------------------------------------ */
+/* Model set-up (once per model)                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Simulation::compute_spectral_discretisation (src/Simulation/sim_spectrum.tpp:13-196) */
+int mgr_compute_spectral_discretisation(mgr_ctx *ctx);
+
+/* Simulation::compute_boundary_intensities (src/Simulation/sim_radiation.tpp:17-48).
+ * temperatures: [nboundary] K, or NULL for the CMB (T_CMB = 2.72548 K).      */
+int mgr_compute_boundary_intensities(mgr_ctx *ctx, const double *temperatures);
+
+/* Simulation::compute_LTE_level_populations (src/Simulation/sim_lines.tpp:10-16) */
+int mgr_compute_LTE_level_populations(mgr_ctx *ctx);
+
+/* Geometry::trace_ray<CoMoving> for every (origin, ray), cached on the device
+ * (src/Model/Geometry/geometry.tpp:21-207,328-465).  The reference re-traces
+ * inside every compute_radiation_field; traces here are computed once.       */
+int mgr_trace(mgr_ctx *ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Hot path                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Simulation::compute_radiation_field (src/Simulation/sim_radiation.tpp:56-338):
+ * J(p,f) = sum_r 2 w_r u_r(p,f) and the Lambda* diagonal, n_off_diag = 0.     */
+int mgr_compute_radiation_field(mgr_ctx *ctx, int precision);
+
+/* Simulation::compute_Jeff (src/Simulation/sim_lines.tpp:176-220)             */
+int mgr_compute_Jeff(mgr_ctx *ctx);
+
+/* Simulation::compute_level_populations_from_stateq (src/Simulation/sim_lines.tpp:24-30):
+ * per-cell statistical equilibrium, convergence check, emissivity/opacity.    */
+int mgr_compute_level_populations_from_stateq(mgr_ctx *ctx, mgr_conv *conv);
+
+/* Lines::iteration_using_Ng_acceleration (src/Model/Lines/lines.cpp:131-149)  */
+int mgr_iteration_using_Ng_acceleration(mgr_ctx *ctx, mgr_conv *conv);
+
+/* Simulation::compute_level_populations(io, use_Ng, max_niterations)
+ * (src/Simulation/sim_lines.tpp:61-164): the ALI loop, one hipGraph replay per
+ * iteration.  iterations_out receives the iteration count (the reference's
+ * return value).                                                              */
+int mgr_compute_level_populations(mgr_ctx *ctx, int use_Ng_acceleration,
+                                  int64_t max_niterations, int criterion,
+                                  int precision, int64_t *iterations_out);
+
+/* ------------------------------------------------------------------------ */
+/* Data movement (the reference keeps these as Model members)                */
+/* ------------------------------------------------------------------------ */
+
+/* LineProducingSpecies::population [ncells*nlev], index i + p*nlev.          */
+int mgr_set_populations(mgr_ctx *ctx, const double *population);
+int mgr_get_populations(mgr_ctx *ctx, double *population);
+/* population_prev1..3 (Ng history), each [ncells*nlev]; any pointer may be NULL. */
+int mgr_get_population_history(mgr_ctx *ctx, double *prev1, double *prev2, double *prev3);
+int mgr_set_population_history(mgr_ctx *ctx, const double *prev1, const double *prev2,
+                               const double *prev3);
+/* Radiation::J [ncells*nfreqs], index f + p*nfreqs (radiation.tpp:11-15).    */
+int mgr_get_J(mgr_ctx *ctx, double *J);
+/* Lambda::Ls for n_off_diag = 0: one element per (p,k) at nr = p, [ncells*nrad]. */
+int mgr_get_lambda(mgr_ctx *ctx, double *Ls);
+/* Jlin, Jeff, Jdif [ncells*nrad]; any pointer may be NULL.                    */
+int mgr_get_Jeff(mgr_ctx *ctx, double *Jlin, double *Jeff, double *Jdif);
+/* Frequencies::nu [ncells*nfreqs] and LineProducingSpecies::nr_line
+ * [ncells*nrad*nquads].                                                       */
+int mgr_get_frequencies(mgr_ctx *ctx, double *nu, int64_t *nr_line);
+/* Radiation::I_bdy for one reduced ray: [nboundary*nfreqs] (all rays share it). */
+int mgr_get_boundary_intensities(mgr_ctx *ctx, double *I_bdy);
+/* Lines::emissivity / opacity [ncells*nrad] (lines.tpp:44-60).               */
+int mgr_get_emissivity_opacity(mgr_ctx *ctx, double *emissivity, double *opacity);
+
+/* Traced projected cells of one ray, expanded with the interpolation points
+ * of Geometry::set_data (geometry.tpp:100-207) exactly as the solver consumes
+ * them: cellNr, shift, dZ per point.  n_out receives the number of points;
+ * at most `capacity` are written.                                             */
+int mgr_get_ray(mgr_ctx *ctx, int64_t origin, int64_t ray, int64_t capacity,
+                int64_t *cellNr, double *shift, double *dZ, int64_t *n_out);
+/* Simulation::get_npoints_on_rays<CoMoving> (simulation.tpp:540-552):
+ * npoints[(r)*ncells + o] = n_ar + n_r + 1 for r < nrays/2.                   */
+int mgr_get_npoints_on_rays(mgr_ctx *ctx, int64_t *npoints);
+
+/* Kernel time of the last mgr_compute_radiation_field, measured with HIP
+ * events on the context's stream (ms).                                        */
+int mgr_last_radiation_field_ms(mgr_ctx *ctx, double *ms);
+/* Sum over the cached traces of the expanded points of every ray pair
+ * (sum of ndep) and the number of ray pairs with ndep > 1.                    */
+int mgr_trace_stats(mgr_ctx *ctx, int64_t *sum_ndep, int64_t *n_pairs_solved);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAGRITTE_HIP_H */

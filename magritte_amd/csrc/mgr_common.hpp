@@ -1,0 +1,200 @@
+// mgr_common.hpp -- shared definitions of the gfx950 radiation-field /
+// level-population library (device layout, constants, context).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/magritte_hip.h"
+
+namespace mgr {
+
+// Physical constants, src/Tools/constants.hpp:26-63 (the same literal values
+// and derivations, so that every derived quantity rounds as the reference's).
+constexpr double PI              = 3.14159265358979323846;
+constexpr double FOUR_PI         = 4.0 * PI;
+constexpr double INVERSE_SQRT_PI = 0.5 * 1.12837916709551257390;   // 0.5*M_2_SQRTPI
+constexpr double CC              = 2.99792458E+8;
+constexpr double HH              = 6.62607004E-34;
+constexpr double KB              = 1.38064852E-23;
+constexpr double AMU             = 1.66053904E-27;
+constexpr double T_CMB           = 2.7254800;
+constexpr double C_SQUARED       = 8.98755179E+16;
+constexpr double TWO_KB_OVER_AMU_C_SQUARED = 2.0 * KB / (AMU * C_SQUARED);
+constexpr double TWO_HH_OVER_CC_SQUARED    = 2.0 * HH / C_SQUARED;
+constexpr double HH_OVER_KB                = HH / KB;
+constexpr double HH_OVER_FOUR_PI           = HH / FOUR_PI;
+
+// Line window of get_eta_and_chi (src/Simulation/simulation.tpp:403-404) and
+// opacity floor (:397).
+constexpr double LINE_WINDOW = 11.0;
+constexpr double CHI_FLOOR   = 1.0E-26;
+
+// Exact power-of-two scalings of the radiation-field solve.  Opacities are
+// carried as chi * CHI_SCALE and path lengths as dZ / CHI_SCALE (optical depth
+// unchanged); intensities and source functions as I * I_SCALE.  In FP32 this
+// keeps chi, S, dtau^2 and S*dtau^2 inside the normal range for SI inputs;
+// powers of two make the scaling exact in either precision.
+constexpr double CHI_SCALE = 73786976294838206464.0;     // 2^66 ~ 7.4e19
+constexpr double I_SCALE   = 1125899906842624.0;         // 2^50 ~ 1.1e15
+
+constexpr int WAVE = 64;                 // CDNA wavefront width
+constexpr int MAX_INTERPOLATION = 10000; // geometry.tpp:138
+
+// Small per-model line tables, read by every kernel.
+struct LineTables {
+    int    L;              // nrad (one species)
+    int    V;              // nlev
+    int    Q;              // nquads
+    int    F;              // nfreqs = L*Q
+    double inverse_mass;
+    double line[64];       // sorted line frequencies (Lines::line)
+    int    line_index[64]; // transition of sorted line l (Lines::line_index)
+};
+constexpr int MAX_LINES = 64;
+
+#define MGR_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            ctx->fail(MGR_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return MGR_ERR_DEVICE;                                                 \
+        }                                                                          \
+    } while (0)
+
+#define MGR_LAUNCH_CHECK()                                                         \
+    do {                                                                           \
+        hipError_t e_ = hipGetLastError();                                         \
+        if (e_ != hipSuccess) {                                                    \
+            ctx->fail(MGR_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+            return MGR_ERR_DEVICE;                                                 \
+        }                                                                          \
+    } while (0)
+
+}  // namespace mgr
+
+// Device-resident model and state (SoA).  Indices follow the reference:
+// J(p,f) at f + p*F (radiation.tpp:11-15), populations at i + p*V
+// (lineProducingSpecies.tpp:24-30), emissivity/opacity at k + p*L (lines.tpp:19-34).
+struct mgr_ctx {
+    int         device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int         status_sticky = MGR_OK;
+
+    // sizes
+    int64_t N = 0, R = 0, NB = 0, F = 0, L = 0, V = 0, Q = 0, nspecs = 0;
+    int64_t ncolpar = 0, ncol_tot = 0, ntmp_tot = 0, ctab_tot = 0;
+    double  pop_prec = 1e-6, max_width_fraction = 0.5;
+    int64_t num = 0;
+
+    // host copies
+    std::vector<int64_t> antipod;          // [R]
+    std::vector<int64_t> boundary2cell;    // [NB]
+    std::vector<int64_t> lam_f_off, lam_f; // CSR: sorted f of each transition k
+
+    // device: static model
+    double  *pos = nullptr, *vel = nullptr;        // [N*3]
+    int64_t *nb_off = nullptr;                     // [N+1]
+    int32_t *nb = nullptr;                         // [sum n_neighbors]
+    uint8_t *is_boundary = nullptr;                // [N]
+    int32_t *cell2bdy = nullptr;                   // [N], -1 for interior cells
+    int32_t *bdy2cell = nullptr;                   // [NB]
+    double  *rays = nullptr, *ray_w = nullptr;     // [R*3], [R]
+    int32_t *d_antipod = nullptr;                  // [R]
+    double  *temperature = nullptr, *vturb2 = nullptr;   // [N]
+    double  *abundance = nullptr;                  // [N*nspecs]
+    mgr::LineTables *tables = nullptr;             // device copy
+    mgr::LineTables  h_tables;                     // host copy
+    double  *A = nullptr, *Bs = nullptr, *Ba = nullptr, *frequency = nullptr; // [L]
+    int32_t *irad = nullptr, *jrad = nullptr;      // [L]
+    double  *energy = nullptr, *lweight = nullptr; // [V]
+    double  *roots = nullptr, *qweights = nullptr; // [Q]
+    int32_t *cp_num = nullptr, *cp_ncol = nullptr, *cp_ntmp = nullptr;  // [ncolpar]
+    int32_t *cp_koff = nullptr, *cp_toff = nullptr, *cp_coff = nullptr; // [ncolpar]
+    int32_t *cp_op = nullptr;                      // [ncolpar] 0 n, 1 o, 2 p
+    int32_t *icol = nullptr, *jcol = nullptr;      // [ncol_tot]
+    double  *tmp = nullptr, *Cd = nullptr, *Ce = nullptr;
+
+    // device: derived per-cell tables
+    double  *pw = nullptr;          // [N] profile width / line frequency
+    double  *dshift_max = nullptr;  // [N]
+    double  *W11 = nullptr;         // [N*L] 11 * width of sorted line l
+    double  *IW64 = nullptr;        // [N*L] 1 / width
+    float   *IW32 = nullptr;        // [N*L]
+    double  *E64 = nullptr, *K64 = nullptr;   // [N*L] emissivity/opacity prefactors (scaled)
+    float   *E32 = nullptr, *K32 = nullptr;   // [N*L]
+    double   pw_max = 0.0;
+
+    // device: spectral discretisation
+    double  *nu = nullptr;          // [N*F]
+    int32_t *nr_line = nullptr;     // [N*L*Q]
+    int32_t *f_appears = nullptr, *f_k = nullptr, *f_z = nullptr;   // [F]
+    int32_t *d_lam_f_off = nullptr, *d_lam_f = nullptr;             // CSR
+    bool     spectral_set = false;
+    double  *I_bdy = nullptr;       // [NB*F]
+    bool     bdy_set = false;
+
+    // device: populations and line quantities
+    double  *pop = nullptr, *prev1 = nullptr, *prev2 = nullptr, *prev3 = nullptr; // [N*V]
+    double  *pop_tot = nullptr;     // [N]
+    double  *emis = nullptr, *opac = nullptr;   // [N*L]
+    bool     pops_set = false;
+
+    // device: radiation field
+    double  *J = nullptr;           // [N*F]
+    double  *Lam = nullptr;         // [N*F]  sum_r 2 w_r Lambda_ar / chi
+    double  *Ls = nullptr;          // [N*L]
+    double  *Jlin = nullptr, *Jeff = nullptr, *Jdif = nullptr;  // [N*L]
+    int32_t *err_flag = nullptr;    // device error word
+    bool     J_set = false;
+    float    last_rf_ms = 0.0f;
+
+    // device: trace cache
+    int64_t *seg_off = nullptr;     // [N*R + 1]
+    int32_t *nseg = nullptr;        // [N*R]
+    int32_t *npts = nullptr;        // [N*R]
+    int32_t *seg_cell = nullptr;    // [total segments]
+    double  *seg_shift = nullptr, *seg_dZ = nullptr;
+    int64_t  total_segments = 0;
+    uint64_t *shift_min_bits = nullptr;  // min shift over all points (FP64 bits)
+    double   shift_min = 1.0;
+    bool     traced = false;
+    int64_t  sum_ndep = 0, n_pairs_solved = 0;
+
+    // reductions
+    double  *partials = nullptr;    // [MAX_PARTIALS * 8]
+    double  *reduced = nullptr;     // [8]
+    double  *h_reduced = nullptr;   // pinned [8]
+
+    // graphs of one ALI iteration
+    hipGraphExec_t graph_normal = nullptr, graph_ng = nullptr;
+    int graph_precision = -1;
+
+    int fail(int code, const std::string &msg) {
+        err = msg;
+        return code;
+    }
+};
+
+namespace mgr {
+constexpr int RED_BLOCK = 256;
+constexpr int MAX_PARTIALS = 4096;
+
+// entry points shared between translation units
+int upload_model(mgr_ctx *ctx, const mgr_model_view *m);
+int free_model(mgr_ctx *ctx);
+int launch_cell_tables(mgr_ctx *ctx);
+int launch_emissivity_opacity(mgr_ctx *ctx);
+int launch_trace(mgr_ctx *ctx);
+int launch_radiation_field(mgr_ctx *ctx, int precision);
+int launch_jeff(mgr_ctx *ctx);
+int launch_stateq(mgr_ctx *ctx);
+int launch_convergence(mgr_ctx *ctx);
+int launch_ng(mgr_ctx *ctx);
+int read_convergence(mgr_ctx *ctx, mgr_conv *conv);
+}  // namespace mgr

@@ -1,0 +1,673 @@
+// mgr_context.hip -- context life cycle, model upload, once-per-model set-up
+// kernels (cell tables, spectral discretisation, boundary intensities, LTE)
+// and the emissivity/opacity update, plus the data-movement entry points.
+//
+// Compiled with -ffp-contract=off: these kernels restate FP64 expressions of
+// the reference and must round exactly as its scalar code does.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "mgr_common.hpp"
+
+using namespace mgr;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+template <typename T>
+int dalloc(mgr_ctx *ctx, T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return ctx->fail(MGR_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) +
+                                            " bytes failed: " + hipGetErrorString(e));
+    }
+    return MGR_OK;
+}
+
+template <typename T>
+int upload(mgr_ctx *ctx, T **p, const T *src, size_t n)
+{
+    int rc = dalloc(ctx, p, n);
+    if (rc) return rc;
+    if (n) MGR_HIP(hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return MGR_OK;
+}
+
+template <typename T>
+int upload_cast(mgr_ctx *ctx, T **p, const int64_t *src, size_t n)
+{
+    std::vector<T> tmpv(n ? n : 1);
+    for (size_t i = 0; i < n; i++) tmpv[i] = (T)src[i];
+    return upload(ctx, p, tmpv.data(), n);
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+// ---- kernels ---------------------------------------------------------------
+
+// Thermodynamics::profile_width(inverse_mass, p) (thermodynamics.tpp:62-69),
+// Simulation::get_dshift_max (simulation.tpp:15-37), and the per-(cell, sorted
+// line) window 11*width, 1/width used by get_eta_and_chi (simulation.tpp:403-446).
+__global__ void k_cell_tables(int64_t N, const LineTables *__restrict__ tb,
+                              const double *__restrict__ T, const double *__restrict__ vturb2,
+                              double max_width_fraction, double *__restrict__ pw,
+                              double *__restrict__ dshift_max, double *__restrict__ W11,
+                              double *__restrict__ IW64, float *__restrict__ IW32)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const int L = tb->L;
+    const double w = sqrt(TWO_KB_OVER_AMU_C_SQUARED * tb->inverse_mass * T[p] + vturb2[p]);
+    pw[p] = w;
+    dshift_max[p] = max_width_fraction * w;
+    for (int l = 0; l < L; l++) {
+        const double width = tb->line[l] * w;
+        W11[p * L + l] = LINE_WINDOW * width;
+        const double iw = 1.0 / width;
+        IW64[p * L + l] = iw;
+        IW32[p * L + l] = (float)iw;
+    }
+}
+
+// Simulation::compute_spectral_discretisation (sim_spectrum.tpp:13-196):
+// nu(p, i) = line[k] + line[k]*pw(p)*root_z for i = k*Q + z, sorted per cell.
+// One thread per (p, i) ranks its value among the cell's F values (ties by
+// original index), which gives heapsort's order for distinct values.
+__global__ void k_spectral(int64_t N, const LineTables *__restrict__ tb,
+                           const double *__restrict__ pw, const double *__restrict__ roots,
+                           double *__restrict__ nu, int32_t *__restrict__ nr_line)
+{
+    const int F = tb->F, Q = tb->Q, L = tb->L;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * F) return;
+    const int64_t p = t / F;
+    const int i = (int)(t % F);
+    const double w = pw[p];
+    const int ki = i / Q, zi = i % Q;
+    const double li = tb->line[ki];
+    const double vi = li + (li * w) * roots[zi];
+    int rank = 0;
+    for (int j = 0; j < F; j++) {
+        const int kj = j / Q, zj = j % Q;
+        const double lj = tb->line[kj];
+        const double vj = lj + (lj * w) * roots[zj];
+        rank += (vj < vi) || (vj == vi && j < i);
+    }
+    nu[p * F + rank] = vi;
+    nr_line[(p * L + ki) * Q + zi] = rank;
+    (void)L;
+}
+
+// Simulation::compute_boundary_intensities (sim_radiation.tpp:17-48) with
+// planck (src/Functions/planck.tpp:15-24).
+__global__ void k_boundary_intensities(int64_t NB, int F, const int32_t *__restrict__ bdy2cell,
+                                       const double *__restrict__ temps, double T_default,
+                                       const double *__restrict__ nu, double *__restrict__ I_bdy)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= NB * F) return;
+    const int64_t b = t / F;
+    const int f = (int)(t % F);
+    const int64_t p = bdy2cell[b];
+    const double T = temps ? temps[b] : T_default;
+    const double freq = nu[p * F + f];
+    const double h_over_kbT = HH_OVER_KB / T;
+    I_bdy[t] = TWO_HH_OVER_CC_SQUARED * (freq * freq * freq) / expm1(h_over_kbT * freq);
+}
+
+// LineProducingSpecies::update_using_LTE (lineProducingSpecies.tpp:82-115).
+__global__ void k_lte(int64_t N, int V, int nspecs, int num, const double *__restrict__ abundance,
+                      const double *__restrict__ T, const double *__restrict__ weight,
+                      const double *__restrict__ energy, double *__restrict__ pop,
+                      double *__restrict__ pop_tot)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const double ptot = abundance[p * nspecs + num];
+    pop_tot[p] = ptot;
+    double pf = 0.0;
+    for (int i = 0; i < V; i++) {
+        const double v = weight[i] * exp(-energy[i] / (KB * T[p]));
+        pop[p * V + i] = v;
+        pf += v;
+    }
+    for (int i = 0; i < V; i++) pop[p * V + i] *= ptot / pf;
+}
+
+__global__ void k_pop_tot(int64_t N, int nspecs, int num, const double *__restrict__ abundance,
+                          double *__restrict__ pop_tot)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    pop_tot[p] = abundance[p * nspecs + num];
+}
+
+}  // namespace
+
+// LineProducingSpecies::get_emissivity / get_opacity (lineProducingSpecies.tpp:41-70),
+// Lines::set_emissivity_and_opacity (lines.tpp:44-60), and the scaled line
+// prefactors of the radiation-field kernel: eta = sum emissivity nu phi with
+// phi = (1/width)(1/sqrt(pi)) exp(-x^2), so E = emissivity/(width sqrt(pi))
+// scaled by CHI_SCALE*I_SCALE, and K = opacity/(width sqrt(pi)) * CHI_SCALE,
+// both stored at the sorted line position l.
+__global__ void k_emissivity_opacity(int64_t N, const LineTables *__restrict__ tb,
+                                     const double *__restrict__ A, const double *__restrict__ Bs,
+                                     const double *__restrict__ Ba, const int32_t *__restrict__ irad,
+                                     const int32_t *__restrict__ jrad, const double *__restrict__ pop,
+                                     const double *__restrict__ IW64, double *__restrict__ emis,
+                                     double *__restrict__ opac, double *__restrict__ E64,
+                                     double *__restrict__ K64, float *__restrict__ E32,
+                                     float *__restrict__ K32)
+{
+    const int L = tb->L, V = tb->V;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * L) return;
+    const int64_t p = t / L;
+    const int l = (int)(t % L);
+    const int k = tb->line_index[l];
+    const int i = irad[k], j = jrad[k];
+    const double e = HH_OVER_FOUR_PI * A[k] * pop[p * V + i];
+    const double o = HH_OVER_FOUR_PI * (pop[p * V + j] * Ba[k] - pop[p * V + i] * Bs[k]);
+    emis[p * L + k] = e;
+    opac[p * L + k] = o;
+    const double pref = INVERSE_SQRT_PI * IW64[p * L + l];
+    const double es = e * (CHI_SCALE * I_SCALE) * pref;
+    const double ks = o * CHI_SCALE * pref;
+    E64[p * L + l] = es;
+    K64[p * L + l] = ks;
+    E32[p * L + l] = (float)es;
+    K32[p * L + l] = (float)ks;
+}
+
+namespace mgr {
+
+static unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+int launch_cell_tables(mgr_ctx *ctx)
+{
+    hipLaunchKernelGGL(k_cell_tables, dim3(grid_for(ctx->N, 256)), dim3(256), 0, ctx->stream,
+                       ctx->N, ctx->tables, ctx->temperature, ctx->vturb2, ctx->max_width_fraction,
+                       ctx->pw, ctx->dshift_max, ctx->W11, ctx->IW64, ctx->IW32);
+    MGR_LAUNCH_CHECK();
+    return MGR_OK;
+}
+
+int launch_emissivity_opacity(mgr_ctx *ctx)
+{
+    hipLaunchKernelGGL(k_emissivity_opacity, dim3(grid_for(ctx->N * ctx->L, 256)), dim3(256), 0,
+                       ctx->stream, ctx->N, ctx->tables, ctx->A, ctx->Bs, ctx->Ba, ctx->irad,
+                       ctx->jrad, ctx->pop, ctx->IW64, ctx->emis, ctx->opac, ctx->E64, ctx->K64,
+                       ctx->E32, ctx->K32);
+    MGR_LAUNCH_CHECK();
+    return MGR_OK;
+}
+
+int upload_model(mgr_ctx *ctx, const mgr_model_view *m)
+{
+    int rc;
+    ctx->N = m->ncells; ctx->R = m->nrays; ctx->NB = m->nboundary; ctx->nspecs = m->nspecs;
+    ctx->L = m->nrad; ctx->V = m->nlev; ctx->Q = m->nquads; ctx->F = m->nrad * m->nquads;
+    ctx->pop_prec = m->pop_prec; ctx->max_width_fraction = m->max_width_fraction;
+    ctx->num = m->num; ctx->ncolpar = m->ncolpar;
+    const int64_t N = ctx->N, R = ctx->R, NB = ctx->NB, L = ctx->L, V = ctx->V, Q = ctx->Q;
+
+    if (N <= 0 || R <= 0 || (R % 2) != 0 || L <= 0 || V < 2 || Q <= 0 || NB < 0)
+        return ctx->fail(MGR_ERR_ARGUMENT, "model sizes out of range (need ncells, nrad, nquads > 0, even nrays, nlev >= 2)");
+    if (L > MAX_LINES) return ctx->fail(MGR_ERR_UNSUPPORTED, "more than 64 line transitions");
+    if (m->n_off_diag != 0)
+        return ctx->fail(MGR_ERR_UNSUPPORTED, "n_off_diag > 0 (banded ALO) is not supported");
+    if (N > INT32_MAX / 2) return ctx->fail(MGR_ERR_UNSUPPORTED, "ncells exceeds the 32-bit cell index");
+    if (m->num < 0 || m->num >= m->nspecs) return ctx->fail(MGR_ERR_ARGUMENT, "line species index out of range");
+
+    // Lines::setup (src/Model/Lines/lines.cpp:43-87): sorted line list.
+    LineTables &tb = ctx->h_tables;
+    std::memset(&tb, 0, sizeof(tb));
+    tb.L = (int)L; tb.V = (int)V; tb.Q = (int)Q; tb.F = (int)ctx->F; tb.inverse_mass = m->inverse_mass;
+    {
+        std::vector<int> order(L);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return m->frequency[a] < m->frequency[b]; });
+        for (int l = 0; l < L; l++) { tb.line[l] = m->frequency[order[l]]; tb.line_index[l] = order[l]; }
+    }
+    for (int64_t k = 0; k < L; k++) {
+        if (m->irad[k] < 0 || m->irad[k] >= V || m->jrad[k] < 0 || m->jrad[k] >= V)
+            return ctx->fail(MGR_ERR_ARGUMENT, "radiative transition level out of range");
+    }
+
+    // neighbours (CSR) with bounds checks: the tracer trusts these indices
+    std::vector<int64_t> off(N + 1, 0);
+    for (int64_t p = 0; p < N; p++) {
+        if (m->n_neighbors[p] < 0) return ctx->fail(MGR_ERR_ARGUMENT, "negative n_neighbors");
+        off[p + 1] = off[p] + m->n_neighbors[p];
+    }
+    const int64_t M = off[N];
+    for (int64_t q = 0; q < M; q++)
+        if (m->neighbors[q] < 0 || m->neighbors[q] >= N)
+            return ctx->fail(MGR_ERR_ARGUMENT, "neighbour index out of range");
+
+    // Boundary::read (boundary.cpp:19-51)
+    std::vector<int32_t> c2b(N, -1), isb_i(N, 0);
+    std::vector<uint8_t> isb(N, 0);
+    ctx->boundary2cell.assign(m->boundary2cell_nr, m->boundary2cell_nr + NB);
+    for (int64_t b = 0; b < NB; b++) {
+        const int64_t c = m->boundary2cell_nr[b];
+        if (c < 0 || c >= N) return ctx->fail(MGR_ERR_ARGUMENT, "boundary2cell_nr out of range");
+        c2b[c] = (int32_t)b;
+        isb[c] = 1;
+    }
+
+    // Rays::setup_antipodal_rays (rays.cpp:90-103): the last match wins
+    ctx->antipod.assign(R, -1);
+    for (int64_t r1 = 0; r1 < R; r1++)
+        for (int64_t r2 = 0; r2 < R; r2++) {
+            const double x = m->rays[3 * r1 + 0] + m->rays[3 * r2 + 0];
+            const double y = m->rays[3 * r1 + 1] + m->rays[3 * r2 + 1];
+            const double z = m->rays[3 * r1 + 2] + m->rays[3 * r2 + 2];
+            if (((x * x + y * y) + z * z) < 1.0E-9) ctx->antipod[r1] = r2;
+        }
+    for (int64_t r = 0; r < R / 2; r++)
+        if (ctx->antipod[r] < 0) return ctx->fail(MGR_ERR_ARGUMENT, "a ray has no antipode");
+
+    // collision partners: offsets into the concatenated tables
+    std::vector<int32_t> cpn(ctx->ncolpar + 1), cpc(ctx->ncolpar + 1), cpt(ctx->ncolpar + 1),
+        cpk(ctx->ncolpar + 1), cptoff(ctx->ncolpar + 1), cpco(ctx->ncolpar + 1), cpop(ctx->ncolpar + 1);
+    int64_t koff = 0, toff = 0, coff = 0;
+    for (int64_t c = 0; c < ctx->ncolpar; c++) {
+        if (m->colpar_num[c] < 0 || m->colpar_num[c] >= m->nspecs)
+            return ctx->fail(MGR_ERR_ARGUMENT, "collision partner species out of range");
+        if (m->colpar_ntmp[c] < 1) return ctx->fail(MGR_ERR_ARGUMENT, "collision partner without temperatures");
+        cpn[c] = (int32_t)m->colpar_num[c]; cpc[c] = (int32_t)m->colpar_ncol[c];
+        cpt[c] = (int32_t)m->colpar_ntmp[c];
+        cpk[c] = (int32_t)koff; cptoff[c] = (int32_t)toff; cpco[c] = (int32_t)coff;
+        const char op = m->colpar_orth_or_para[c];
+        cpop[c] = (op == 'o') ? 1 : (op == 'p') ? 2 : 0;
+        for (int64_t k = 0; k < m->colpar_ncol[c]; k++)
+            if (m->icol[koff + k] < 0 || m->icol[koff + k] >= V || m->jcol[koff + k] < 0 || m->jcol[koff + k] >= V)
+                return ctx->fail(MGR_ERR_ARGUMENT, "collisional transition level out of range");
+        koff += m->colpar_ncol[c]; toff += m->colpar_ntmp[c];
+        coff += m->colpar_ncol[c] * m->colpar_ntmp[c];
+    }
+    ctx->ncol_tot = koff; ctx->ntmp_tot = toff; ctx->ctab_tot = coff;
+
+    if ((rc = upload(ctx, &ctx->pos, m->position, N * 3))) return rc;
+    if ((rc = upload(ctx, &ctx->vel, m->velocity, N * 3))) return rc;
+    if ((rc = upload(ctx, &ctx->nb_off, off.data(), N + 1))) return rc;
+    if ((rc = upload_cast<int32_t>(ctx, &ctx->nb, m->neighbors, M))) return rc;
+    if ((rc = upload(ctx, &ctx->is_boundary, isb.data(), N))) return rc;
+    if ((rc = upload(ctx, &ctx->cell2bdy, c2b.data(), N))) return rc;
+    if ((rc = upload_cast<int32_t>(ctx, &ctx->bdy2cell, m->boundary2cell_nr, NB))) return rc;
+    if ((rc = upload(ctx, &ctx->rays, m->rays, R * 3))) return rc;
+    if ((rc = upload(ctx, &ctx->ray_w, m->weights, R))) return rc;
+    if ((rc = upload_cast<int32_t>(ctx, &ctx->d_antipod, ctx->antipod.data(), R))) return rc;
+    if ((rc = upload(ctx, &ctx->temperature, m->temperature, N))) return rc;
+    if ((rc = upload(ctx, &ctx->vturb2, m->vturb2, N))) return rc;
+    if ((rc = upload(ctx, &ctx->abundance, m->abundance, N * m->nspecs))) return rc;
+    if ((rc = upload(ctx, &ctx->tables, &tb, 1))) return rc;
+    if ((rc = upload(ctx, &ctx->A, m->A, L))) return rc;
+    if ((rc = upload(ctx, &ctx->Bs, m->Bs, L))) return rc;
+    if ((rc = upload(ctx, &ctx->Ba, m->Ba, L))) return rc;
+    if ((rc = upload(ctx, &ctx->frequency, m->frequency, L))) return rc;
+    if ((rc = upload_cast<int32_t>(ctx, &ctx->irad, m->irad, L))) return rc;
+    if ((rc = upload_cast<int32_t>(ctx, &ctx->jrad, m->jrad, L))) return rc;
+    if ((rc = upload(ctx, &ctx->energy, m->energy, V))) return rc;
+    if ((rc = upload(ctx, &ctx->lweight, m->weight, V))) return rc;
+    if ((rc = upload(ctx, &ctx->roots, m->roots, Q))) return rc;
+    if ((rc = upload(ctx, &ctx->qweights, m->qweights, Q))) return rc;
+    if ((rc = upload(ctx, &ctx->cp_num, cpn.data(), ctx->ncolpar))) return rc;
+    if ((rc = upload(ctx, &ctx->cp_ncol, cpc.data(), ctx->ncolpar))) return rc;
+    if ((rc = upload(ctx, &ctx->cp_ntmp, cpt.data(), ctx->ncolpar))) return rc;
+    if ((rc = upload(ctx, &ctx->cp_koff, cpk.data(), ctx->ncolpar))) return rc;
+    if ((rc = upload(ctx, &ctx->cp_toff, cptoff.data(), ctx->ncolpar))) return rc;
+    if ((rc = upload(ctx, &ctx->cp_coff, cpco.data(), ctx->ncolpar))) return rc;
+    if ((rc = upload(ctx, &ctx->cp_op, cpop.data(), ctx->ncolpar))) return rc;
+    if ((rc = upload_cast<int32_t>(ctx, &ctx->icol, m->icol, koff))) return rc;
+    if ((rc = upload_cast<int32_t>(ctx, &ctx->jcol, m->jcol, koff))) return rc;
+    if ((rc = upload(ctx, &ctx->tmp, m->tmp, toff))) return rc;
+    if ((rc = upload(ctx, &ctx->Cd, m->Cd, coff))) return rc;
+    if ((rc = upload(ctx, &ctx->Ce, m->Ce, coff))) return rc;
+
+    // derived and state arrays
+    const int64_t F = ctx->F;
+    if ((rc = dalloc(ctx, &ctx->pw, N))) return rc;
+    if ((rc = dalloc(ctx, &ctx->dshift_max, N))) return rc;
+    if ((rc = dalloc(ctx, &ctx->W11, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->IW64, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->IW32, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->E64, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->K64, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->E32, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->K32, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->nu, N * F))) return rc;
+    if ((rc = dalloc(ctx, &ctx->nr_line, N * L * Q))) return rc;
+    if ((rc = dalloc(ctx, &ctx->I_bdy, NB * F))) return rc;
+    if ((rc = dalloc(ctx, &ctx->pop, N * V))) return rc;
+    if ((rc = dalloc(ctx, &ctx->prev1, N * V))) return rc;
+    if ((rc = dalloc(ctx, &ctx->prev2, N * V))) return rc;
+    if ((rc = dalloc(ctx, &ctx->prev3, N * V))) return rc;
+    if ((rc = dalloc(ctx, &ctx->pop_tot, N))) return rc;
+    if ((rc = dalloc(ctx, &ctx->emis, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->opac, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->J, N * F))) return rc;
+    if ((rc = dalloc(ctx, &ctx->Lam, N * F))) return rc;
+    if ((rc = dalloc(ctx, &ctx->Ls, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->Jlin, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->Jeff, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->Jdif, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->err_flag, 1))) return rc;
+    if ((rc = dalloc(ctx, &ctx->partials, (size_t)MAX_PARTIALS * 8))) return rc;
+    if ((rc = dalloc(ctx, &ctx->reduced, 8))) return rc;
+    if ((rc = dalloc(ctx, &ctx->shift_min_bits, 1))) return rc;
+    MGR_HIP(hipHostMalloc((void **)&ctx->h_reduced, 8 * sizeof(double), hipHostMallocDefault));
+    MGR_HIP(hipMemset(ctx->err_flag, 0, sizeof(int32_t)));
+    MGR_HIP(hipMemset(ctx->prev1, 0, N * V * sizeof(double)));
+    MGR_HIP(hipMemset(ctx->prev2, 0, N * V * sizeof(double)));
+    MGR_HIP(hipMemset(ctx->prev3, 0, N * V * sizeof(double)));
+    MGR_HIP(hipMemset(ctx->Ls, 0, N * L * sizeof(double)));
+    MGR_HIP(hipMemset(ctx->J, 0, N * F * sizeof(double)));
+
+    hipLaunchKernelGGL(k_pop_tot, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N,
+                       (int)ctx->nspecs, (int)ctx->num, ctx->abundance, ctx->pop_tot);
+    MGR_LAUNCH_CHECK();
+    if ((rc = launch_cell_tables(ctx))) return rc;
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    {
+        std::vector<double> pwh(N);
+        MGR_HIP(hipMemcpy(pwh.data(), ctx->pw, N * sizeof(double), hipMemcpyDeviceToHost));
+        ctx->pw_max = *std::max_element(pwh.begin(), pwh.end());
+    }
+    return MGR_OK;
+}
+
+int free_model(mgr_ctx *ctx)
+{
+    dfree(ctx->pos); dfree(ctx->vel); dfree(ctx->nb_off); dfree(ctx->nb); dfree(ctx->is_boundary);
+    dfree(ctx->cell2bdy); dfree(ctx->bdy2cell); dfree(ctx->rays); dfree(ctx->ray_w);
+    dfree(ctx->d_antipod); dfree(ctx->temperature); dfree(ctx->vturb2); dfree(ctx->abundance);
+    dfree(ctx->tables); dfree(ctx->A); dfree(ctx->Bs); dfree(ctx->Ba); dfree(ctx->frequency);
+    dfree(ctx->irad); dfree(ctx->jrad); dfree(ctx->energy); dfree(ctx->lweight); dfree(ctx->roots);
+    dfree(ctx->qweights); dfree(ctx->cp_num); dfree(ctx->cp_ncol); dfree(ctx->cp_ntmp);
+    dfree(ctx->cp_koff); dfree(ctx->cp_toff); dfree(ctx->cp_coff); dfree(ctx->cp_op);
+    dfree(ctx->icol); dfree(ctx->jcol); dfree(ctx->tmp); dfree(ctx->Cd); dfree(ctx->Ce);
+    dfree(ctx->pw); dfree(ctx->dshift_max); dfree(ctx->W11); dfree(ctx->IW64); dfree(ctx->IW32);
+    dfree(ctx->E64); dfree(ctx->K64); dfree(ctx->E32); dfree(ctx->K32); dfree(ctx->nu);
+    dfree(ctx->nr_line); dfree(ctx->f_appears); dfree(ctx->f_k); dfree(ctx->f_z);
+    dfree(ctx->d_lam_f_off); dfree(ctx->d_lam_f); dfree(ctx->I_bdy); dfree(ctx->pop);
+    dfree(ctx->prev1); dfree(ctx->prev2); dfree(ctx->prev3); dfree(ctx->pop_tot); dfree(ctx->emis);
+    dfree(ctx->opac); dfree(ctx->J); dfree(ctx->Lam); dfree(ctx->Ls); dfree(ctx->Jlin);
+    dfree(ctx->Jeff); dfree(ctx->Jdif); dfree(ctx->err_flag); dfree(ctx->seg_off); dfree(ctx->nseg);
+    dfree(ctx->npts); dfree(ctx->seg_cell); dfree(ctx->seg_shift); dfree(ctx->seg_dZ);
+    dfree(ctx->shift_min_bits); dfree(ctx->partials); dfree(ctx->reduced);
+    if (ctx->h_reduced) (void)hipHostFree(ctx->h_reduced);
+    ctx->h_reduced = nullptr;
+    if (ctx->graph_normal) (void)hipGraphExecDestroy(ctx->graph_normal);
+    if (ctx->graph_ng) (void)hipGraphExecDestroy(ctx->graph_ng);
+    ctx->graph_normal = ctx->graph_ng = nullptr;
+    return MGR_OK;
+}
+
+}  // namespace mgr
+
+// ---- C-ABI -------------------------------------------------------------------
+
+extern "C" {
+
+int mgr_device_count(int *count)
+{
+    if (!count) return MGR_ERR_ARGUMENT;
+    *count = 0;
+    return hipGetDeviceCount(count) == hipSuccess ? MGR_OK : MGR_ERR_DEVICE;
+}
+
+int mgr_create(const mgr_model_view *model, int device, mgr_ctx **out)
+{
+    if (!out) return MGR_ERR_ARGUMENT;
+    *out = nullptr;
+    if (!model) { g_create_error = "null model"; return MGR_ERR_ARGUMENT; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        g_create_error = "no HIP device available";
+        return MGR_ERR_DEVICE;
+    }
+    if (device < 0 || device >= ndev) { g_create_error = "device index out of range"; return MGR_ERR_ARGUMENT; }
+    mgr_ctx *ctx = new mgr_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "could not set device / create stream";
+        delete ctx;
+        return MGR_ERR_DEVICE;
+    }
+    const int rc = upload_model(ctx, model);
+    if (rc != MGR_OK) {
+        g_create_error = ctx->err;
+        free_model(ctx);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return MGR_OK;
+}
+
+void mgr_destroy(mgr_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_model(ctx);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *mgr_last_error(const mgr_ctx *ctx)
+{
+    if (!ctx) return g_create_error.c_str();
+    return ctx->err.c_str();
+}
+
+int mgr_compute_spectral_discretisation(mgr_ctx *ctx)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    MGR_HIP(hipSetDevice(ctx->device));
+    const int64_t N = ctx->N, F = ctx->F, L = ctx->L, Q = ctx->Q;
+    hipLaunchKernelGGL(k_spectral, dim3(grid_for(N * F, 256)), dim3(256), 0, ctx->stream, N,
+                       ctx->tables, ctx->pw, ctx->roots, ctx->nu, ctx->nr_line);
+    MGR_LAUNCH_CHECK();
+    // appears_in_line_integral / corresponding_{k,z}, indexed by the unsorted
+    // position (sim_spectrum.tpp:153-184), and per transition the sorted
+    // frequency indices f whose back-map names it (used for Lambda*).
+    std::vector<int32_t> appears(F, 0), fk(F, (int32_t)F), fz(F, (int32_t)F);
+    for (int64_t k = 0; k < L; k++)
+        for (int64_t z = 0; z < Q; z++) { appears[k * Q + z] = 1; fk[k * Q + z] = (int32_t)k; fz[k * Q + z] = (int32_t)z; }
+    ctx->lam_f_off.assign(L + 1, 0);
+    ctx->lam_f.clear();
+    for (int64_t k = 0; k < L; k++) {
+        for (int64_t f = 0; f < F; f++)
+            if (appears[f] && fk[f] == k) ctx->lam_f.push_back(f);
+        ctx->lam_f_off[k + 1] = (int64_t)ctx->lam_f.size();
+    }
+    int rc;
+    std::vector<int32_t> off32(ctx->lam_f_off.begin(), ctx->lam_f_off.end()), f32(ctx->lam_f.begin(), ctx->lam_f.end());
+    mgr_ctx *c = ctx;
+    if (!c->f_appears) {
+        if ((rc = upload(c, &c->f_appears, appears.data(), F))) return rc;
+        if ((rc = upload(c, &c->f_k, fk.data(), F))) return rc;
+        if ((rc = upload(c, &c->f_z, fz.data(), F))) return rc;
+        if ((rc = upload(c, &c->d_lam_f_off, off32.data(), L + 1))) return rc;
+        if ((rc = upload(c, &c->d_lam_f, f32.data(), f32.size()))) return rc;
+    }
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->spectral_set = true;
+    return MGR_OK;
+}
+
+int mgr_compute_boundary_intensities(mgr_ctx *ctx, const double *temperatures)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (!ctx->spectral_set) return ctx->fail(MGR_ERR_STATE, "compute_spectral_discretisation first");
+    MGR_HIP(hipSetDevice(ctx->device));
+    double *dT = nullptr;
+    if (temperatures) {
+        int rc = upload(ctx, &dT, temperatures, ctx->NB);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_boundary_intensities, dim3(grid_for(ctx->NB * ctx->F, 256)), dim3(256), 0,
+                       ctx->stream, ctx->NB, (int)ctx->F, ctx->bdy2cell, dT, T_CMB, ctx->nu, ctx->I_bdy);
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(ctx->stream);
+    if (dT) (void)hipFree(dT);
+    if (le != hipSuccess || se != hipSuccess)
+        return ctx->fail(MGR_ERR_DEVICE, "boundary intensities kernel failed");
+    ctx->bdy_set = true;
+    return MGR_OK;
+}
+
+int mgr_compute_LTE_level_populations(mgr_ctx *ctx)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    MGR_HIP(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(k_lte, dim3(grid_for(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N,
+                       (int)ctx->V, (int)ctx->nspecs, (int)ctx->num, ctx->abundance, ctx->temperature,
+                       ctx->lweight, ctx->energy, ctx->pop, ctx->pop_tot);
+    MGR_LAUNCH_CHECK();
+    int rc = launch_emissivity_opacity(ctx);
+    if (rc) return rc;
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->pops_set = true;
+    return MGR_OK;
+}
+
+static int copy_out(mgr_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!dst) return MGR_OK;
+    MGR_HIP(hipSetDevice(ctx->device));
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    MGR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return MGR_OK;
+}
+
+static int copy_in(mgr_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!src) return MGR_OK;
+    MGR_HIP(hipSetDevice(ctx->device));
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    MGR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return MGR_OK;
+}
+
+int mgr_set_populations(mgr_ctx *ctx, const double *population)
+{
+    if (!ctx || !population) return MGR_ERR_ARGUMENT;
+    int rc = copy_in(ctx, ctx->pop, population, ctx->N * ctx->V * sizeof(double));
+    if (rc) return rc;
+    if ((rc = launch_emissivity_opacity(ctx))) return rc;
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->pops_set = true;
+    return MGR_OK;
+}
+
+int mgr_get_populations(mgr_ctx *ctx, double *population)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    return copy_out(ctx, population, ctx->pop, ctx->N * ctx->V * sizeof(double));
+}
+
+int mgr_get_population_history(mgr_ctx *ctx, double *p1, double *p2, double *p3)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    const size_t n = ctx->N * ctx->V * sizeof(double);
+    int rc;
+    if ((rc = copy_out(ctx, p1, ctx->prev1, n))) return rc;
+    if ((rc = copy_out(ctx, p2, ctx->prev2, n))) return rc;
+    return copy_out(ctx, p3, ctx->prev3, n);
+}
+
+int mgr_set_population_history(mgr_ctx *ctx, const double *p1, const double *p2, const double *p3)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    const size_t n = ctx->N * ctx->V * sizeof(double);
+    int rc;
+    if ((rc = copy_in(ctx, ctx->prev1, p1, n))) return rc;
+    if ((rc = copy_in(ctx, ctx->prev2, p2, n))) return rc;
+    return copy_in(ctx, ctx->prev3, p3, n);
+}
+
+int mgr_get_J(mgr_ctx *ctx, double *J)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    return copy_out(ctx, J, ctx->J, ctx->N * ctx->F * sizeof(double));
+}
+
+int mgr_get_lambda(mgr_ctx *ctx, double *Ls)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    return copy_out(ctx, Ls, ctx->Ls, ctx->N * ctx->L * sizeof(double));
+}
+
+int mgr_get_Jeff(mgr_ctx *ctx, double *Jlin, double *Jeff, double *Jdif)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    const size_t n = ctx->N * ctx->L * sizeof(double);
+    int rc;
+    if ((rc = copy_out(ctx, Jlin, ctx->Jlin, n))) return rc;
+    if ((rc = copy_out(ctx, Jeff, ctx->Jeff, n))) return rc;
+    return copy_out(ctx, Jdif, ctx->Jdif, n);
+}
+
+int mgr_get_frequencies(mgr_ctx *ctx, double *nu, int64_t *nr_line)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (!ctx->spectral_set) return ctx->fail(MGR_ERR_STATE, "compute_spectral_discretisation first");
+    int rc = copy_out(ctx, nu, ctx->nu, ctx->N * ctx->F * sizeof(double));
+    if (rc || !nr_line) return rc;
+    const int64_t n = ctx->N * ctx->L * ctx->Q;
+    std::vector<int32_t> tmpv(n);
+    if ((rc = copy_out(ctx, tmpv.data(), ctx->nr_line, n * sizeof(int32_t)))) return rc;
+    for (int64_t i = 0; i < n; i++) nr_line[i] = tmpv[i];
+    return MGR_OK;
+}
+
+int mgr_get_boundary_intensities(mgr_ctx *ctx, double *I_bdy)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    return copy_out(ctx, I_bdy, ctx->I_bdy, ctx->NB * ctx->F * sizeof(double));
+}
+
+int mgr_get_emissivity_opacity(mgr_ctx *ctx, double *emissivity, double *opacity)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    const size_t n = ctx->N * ctx->L * sizeof(double);
+    int rc = copy_out(ctx, emissivity, ctx->emis, n);
+    if (rc) return rc;
+    return copy_out(ctx, opacity, ctx->opac, n);
+}
+
+int mgr_last_radiation_field_ms(mgr_ctx *ctx, double *ms)
+{
+    if (!ctx || !ms) return MGR_ERR_ARGUMENT;
+    *ms = ctx->last_rf_ms;
+    return MGR_OK;
+}
+
+int mgr_trace_stats(mgr_ctx *ctx, int64_t *sum_ndep, int64_t *n_pairs_solved)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (!ctx->traced) return ctx->fail(MGR_ERR_STATE, "mgr_trace first");
+    if (sum_ndep) *sum_ndep = ctx->sum_ndep;
+    if (n_pairs_solved) *n_pairs_solved = ctx->n_pairs_solved;
+    return MGR_OK;
+}
+
+}  // extern "C"

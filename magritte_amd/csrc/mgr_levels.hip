@@ -1,0 +1,568 @@
+// mgr_levels.hip -- level populations: J_eff, the per-cell statistical
+// equilibrium (FP64 LU), convergence statistics, Ng acceleration and the ALI
+// driver that replays one hipGraph per iteration.
+//
+// Replaces Simulation::compute_Jeff (src/Simulation/sim_lines.tpp:176-220),
+// LineProducingSpecies::update_using_statistical_equilibrium
+// (lineProducingSpecies.tpp:249-495), ::check_for_convergence (:120-171),
+// ::update_using_Ng_acceleration (:181-238) and
+// Simulation::compute_level_populations (sim_lines.tpp:61-164).
+//
+// With n_off_diag = 0 every rate-equation triplet couples levels of a single
+// cell, so the reference's global SparseLU<COLAMD> of size (N V)^2 is block
+// diagonal: here one thread solves one cell's V x V system with partial
+// pivoting in FP64 registers (V known at compile time for V <= 8).
+#include <cmath>
+
+#include "mgr_common.hpp"
+
+using namespace mgr;
+
+namespace {
+
+constexpr int32_t ERR_SINGULAR = 16;
+
+// Simulation::compute_Jeff with the Lambda* diagonal of update_Lambda
+// (raypair_lambda.tpp:94-149): Ls(p,k) = sum over bins f whose back-map names
+// transition k of A_k w_z nu phi_k(nu) * [sum_r 2 w_r L_diag / chi](p,f).
+__global__ void k_jeff(int64_t N, int L, int Q, int F, int V, const double *__restrict__ J,
+                       const double *__restrict__ Lam, const int32_t *__restrict__ nr_line,
+                       const double *__restrict__ qw, const double *__restrict__ nu,
+                       const double *__restrict__ pw, const double *__restrict__ A,
+                       const double *__restrict__ frequency, const int32_t *__restrict__ irad,
+                       const int32_t *__restrict__ lam_f_off, const int32_t *__restrict__ lam_f,
+                       const int32_t *__restrict__ f_z, const double *__restrict__ pop,
+                       double *__restrict__ Ls, double *__restrict__ Jlin,
+                       double *__restrict__ Jeff, double *__restrict__ Jdif)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * L) return;
+    const int64_t p = t / L;
+    const int k = (int)(t % L);
+    double jl = 0.0;
+    for (int z = 0; z < Q; z++) jl += qw[z] * J[p * F + nr_line[(p * L + k) * Q + z]];
+    const double freq_line = frequency[k];
+    const double width = freq_line * pw[p];
+    const double iw = 1.0 / width;
+    double ls = 0.0;
+    for (int q = lam_f_off[k]; q < lam_f_off[k + 1]; q++) {
+        const int f = lam_f[q];
+        const double nuf = nu[p * F + f];
+        const double x = iw * (nuf - freq_line);
+        const double prof = iw * INVERSE_SQRT_PI * (1.0 / exp(x * x));
+        ls += A[k] * qw[f_z[f]] * (nuf * prof * Lam[p * F + f]);
+    }
+    Ls[p * L + k] = ls;
+    const double diff = ls * pop[p * V + irad[k]];
+    Jlin[p * L + k] = jl;
+    Jeff[p * L + k] = jl - HH_OVER_FOUR_PI * diff;
+    Jdif[p * L + k] = HH_OVER_FOUR_PI * diff;
+}
+
+// search (src/Functions/interpolation.tpp:14-54)
+__device__ __forceinline__ int search_table(const double *x, int n, double value)
+{
+    int start = 0, stop = n - 1;
+    if (value >= x[stop]) return stop;
+    else if (value <= x[start]) return start;
+    while (stop > start + 1) {
+        const int middle = (stop + start) / 2;
+        if (value > x[middle]) start = middle;
+        else if (value < x[middle]) stop = middle;
+        else return middle;
+    }
+    return stop;
+}
+
+struct StateqArgs {
+    int64_t N;
+    int V, L, nspecs, ncolpar;
+    const double *Jeff, *Ls, *A, *Bs, *Ba, *abundance, *T, *pop_tot;
+    const int32_t *irad, *jrad;
+    const int32_t *cp_num, *cp_ncol, *cp_ntmp, *cp_koff, *cp_toff, *cp_coff, *cp_op, *icol, *jcol;
+    const double *tmp, *Cd, *Ce;
+    double *pop, *prev1, *prev2, *prev3;
+    int32_t *err_flag;
+};
+
+template <int VV>
+__device__ __forceinline__ void stateq_cell(const StateqArgs &a, int64_t p, int V, double *M, double *y)
+{
+    const int Vn = (VV > 0) ? VV : V;
+    // history: prev3 <- prev2 <- prev1 <- population (lineProducingSpecies.tpp:260-262)
+    for (int i = 0; i < Vn; i++) {
+        const int64_t ind = p * Vn + i;
+        const double x = a.pop[ind];
+        a.prev3[ind] = a.prev2[ind];
+        a.prev2[ind] = a.prev1[ind];
+        a.prev1[ind] = x;
+        y[i] = 0.0;
+        for (int j = 0; j < Vn; j++) M[i * Vn + j] = 0.0;
+    }
+    const double *pop = a.prev1 + p * Vn;
+    // radiative transitions (transpose convention of the reference)
+    for (int k = 0; k < a.L; k++) {
+        const double Je = a.Jeff[p * a.L + k];
+        const double v_IJ = a.A[k] + a.Bs[k] * Je;
+        const double v_JI = a.Ba[k] * Je;
+        const int I = a.irad[k], Jl = a.jrad[k];
+        if (Jl != Vn - 1) { M[Jl * Vn + I] += v_IJ; M[Jl * Vn + Jl] += -v_JI; }
+        if (I != Vn - 1) { M[I * Vn + Jl] += v_JI; M[I * Vn + I] += -v_IJ; }
+    }
+    // approximated Lambda operator with the pre-solve opacity
+    for (int k = 0; k < a.L; k++) {
+        const int I = a.irad[k], Jl = a.jrad[k];
+        const double opacity = HH_OVER_FOUR_PI * (pop[Jl] * a.Ba[k] - pop[I] * a.Bs[k]);
+        const double v_IJ = -a.Ls[p * a.L + k] * opacity;
+        if (Jl != Vn - 1) M[Jl * Vn + I] += v_IJ;
+        if (I != Vn - 1) M[I * Vn + I] += -v_IJ;
+    }
+    // collisional transitions (collisionPartner.tpp:15-72)
+    const double Tp = a.T[p];
+    for (int c = 0; c < a.ncolpar; c++) {
+        double abn = a.abundance[p * a.nspecs + a.cp_num[c]];
+        if (a.cp_op[c] != 0) {
+            const double frac_H2_para = 1.0 / (1.0 + 9.0 * exp(-170.5 / Tp));
+            if (a.cp_op[c] == 1) abn *= (1.0 - frac_H2_para);
+            if (a.cp_op[c] == 2) abn *= frac_H2_para;
+        }
+        const int ncol = a.cp_ncol[c], ntmp = a.cp_ntmp[c];
+        const double *tmp = a.tmp + a.cp_toff[c];
+        const double *Cd = a.Cd + a.cp_coff[c];
+        const double *Ce = a.Ce + a.cp_coff[c];
+        const int t = search_table(tmp, ntmp, Tp);
+        const double step = (t == 0) ? 0.0 : (Tp - tmp[t - 1]) / (tmp[t] - tmp[t - 1]);
+        for (int k = 0; k < ncol; k++) {
+            double ce, cd;
+            if (t == 0) {
+                ce = Ce[k];
+                cd = Cd[k];
+            } else {
+                ce = Ce[(t - 1) * ncol + k] + (Ce[t * ncol + k] - Ce[(t - 1) * ncol + k]) * step;
+                cd = Cd[(t - 1) * ncol + k] + (Cd[t * ncol + k] - Cd[(t - 1) * ncol + k]) * step;
+            }
+            const double v_IJ = cd * abn, v_JI = ce * abn;
+            const int I = a.icol[a.cp_koff[c] + k], Jl = a.jcol[a.cp_koff[c] + k];
+            if (Jl != Vn - 1) { M[Jl * Vn + I] += v_IJ; M[Jl * Vn + Jl] += -v_JI; }
+            if (I != Vn - 1) { M[I * Vn + Jl] += v_JI; M[I * Vn + I] += -v_IJ; }
+        }
+    }
+    // conservation replaces the last level's equation
+    for (int i = 0; i < Vn; i++) M[(Vn - 1) * Vn + i] += 1.0;
+    y[Vn - 1] = a.pop_tot[p];
+
+    // LU with partial pivoting
+    for (int c = 0; c < Vn; c++) {
+        int piv = c;
+        double best = fabs(M[c * Vn + c]);
+        for (int r = c + 1; r < Vn; r++) {
+            const double v = fabs(M[r * Vn + c]);
+            if (v > best) { best = v; piv = r; }
+        }
+        if (best == 0.0) {
+            atomicOr(a.err_flag, ERR_SINGULAR);
+            return;
+        }
+        if (piv != c) {
+            for (int j = 0; j < Vn; j++) {
+                const double tsw = M[c * Vn + j];
+                M[c * Vn + j] = M[piv * Vn + j];
+                M[piv * Vn + j] = tsw;
+            }
+            const double tsw = y[c];
+            y[c] = y[piv];
+            y[piv] = tsw;
+        }
+        const double inv = 1.0 / M[c * Vn + c];
+        for (int r = c + 1; r < Vn; r++) {
+            const double l = M[r * Vn + c] * inv;
+            for (int j = c + 1; j < Vn; j++) M[r * Vn + j] -= l * M[c * Vn + j];
+            y[r] -= l * y[c];
+        }
+    }
+    for (int r = Vn - 1; r >= 0; r--) {
+        double acc = y[r];
+        for (int j = r + 1; j < Vn; j++) acc -= M[r * Vn + j] * y[j];
+        y[r] = acc / M[r * Vn + r];
+    }
+    for (int i = 0; i < Vn; i++) a.pop[p * Vn + i] = y[i];
+}
+
+template <int VV>
+__global__ void k_stateq(StateqArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.N) return;
+    double M[VV * VV], y[VV];
+    stateq_cell<VV>(a, p, VV, M, y);
+}
+
+constexpr int VMAX_GENERIC = 32;
+__global__ void k_stateq_generic(StateqArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.N) return;
+    double M[VMAX_GENERIC * VMAX_GENERIC], y[VMAX_GENERIC];
+    stateq_cell<0>(a, p, a.V, M, y);
+}
+
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double block_max(double v, double *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// check_for_convergence (lineProducingSpecies.tpp:120-171): per block partial
+// sums of the not-converged weight and the mean relative change, and the
+// maximum relative change; levels with n <= 1e-10 n_tot are ignored.
+__global__ void k_conv_partial(int64_t N, int V, double pop_prec, const double *__restrict__ pop,
+                               const double *__restrict__ prev1, const double *__restrict__ pop_tot,
+                               double *__restrict__ partials)
+{
+    __shared__ double sh[RED_BLOCK];
+    const double weight = 1.0 / (double)(N * V);
+    double fnc = 0.0, rcm = 0.0, rmax = 0.0;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const double min_pop = 1.0E-10 * pop_tot[p];
+        for (int i = 0; i < V; i++) {
+            const int64_t ind = p * V + i;
+            if (pop[ind] > min_pop) {
+                double rc = 2.0;
+                rc *= fabs(pop[ind] - prev1[ind]);
+                rc /= (pop[ind] + prev1[ind]);
+                if (rc > pop_prec) fnc += weight;
+                rcm += weight * rc;
+                rmax = fmax(rmax, rc);
+            }
+        }
+    }
+    fnc = block_sum(fnc, sh);
+    rcm = block_sum(rcm, sh);
+    rmax = block_max(rmax, sh);
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x + 0] = fnc;
+        partials[3 * blockIdx.x + 1] = rcm;
+        partials[3 * blockIdx.x + 2] = rmax;
+    }
+}
+
+__global__ void k_conv_final(int nparts, const double *__restrict__ partials, double *__restrict__ out,
+                             const int32_t *__restrict__ err_flag)
+{
+    __shared__ double sh[RED_BLOCK];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        a += partials[3 * i + 0];
+        b += partials[3 * i + 1];
+        c = fmax(c, partials[3 * i + 2]);
+    }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    c = block_max(c, sh);
+    if (threadIdx.x == 0) {
+        out[0] = a;
+        out[1] = b;
+        out[2] = c;
+        out[3] = (double)(*err_flag);
+    }
+}
+
+// Ng acceleration (lineProducingSpecies.tpp:181-238): five global dot
+// products, then x <- (1-a-b) x + a x1 + b x2 and the history shift when the
+// determinant is non-zero.
+__global__ void k_ng_partial(int64_t n, const double *__restrict__ x, const double *__restrict__ x1,
+                             const double *__restrict__ x2, const double *__restrict__ x3,
+                             double *__restrict__ partials)
+{
+    __shared__ double sh[RED_BLOCK];
+    double A1 = 0, A2 = 0, B2 = 0, C1 = 0, C2 = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double Q1 = x[i] - 2.0 * x1[i] + x2[i];
+        const double Q2 = x[i] - x1[i] - x2[i] + x3[i];
+        const double Q3 = x[i] - x1[i];
+        A1 += Q1 * Q1; A2 += Q1 * Q2; B2 += Q2 * Q2; C1 += Q1 * Q3; C2 += Q2 * Q3;
+    }
+    A1 = block_sum(A1, sh); A2 = block_sum(A2, sh); B2 = block_sum(B2, sh);
+    C1 = block_sum(C1, sh); C2 = block_sum(C2, sh);
+    if (threadIdx.x == 0) {
+        double *p = partials + 5 * blockIdx.x;
+        p[0] = A1; p[1] = A2; p[2] = B2; p[3] = C1; p[4] = C2;
+    }
+}
+
+__global__ void k_ng_final(int nparts, const double *__restrict__ partials, double *__restrict__ coef)
+{
+    __shared__ double sh[RED_BLOCK];
+    double s[5] = {0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+        for (int q = 0; q < 5; q++) s[q] += partials[5 * i + q];
+    for (int q = 0; q < 5; q++) s[q] = block_sum(s[q], sh);
+    if (threadIdx.x == 0) {
+        const double A1 = s[0], A2 = s[1], B2 = s[2], C1 = s[3], C2 = s[4];
+        const double B1 = A2;
+        const double denominator = A1 * B2 - A2 * B1;
+        coef[4] = denominator;
+        coef[5] = (denominator != 0.0) ? (C1 * B2 - C2 * B1) / denominator : 0.0;
+        coef[6] = (denominator != 0.0) ? (C2 * A1 - C1 * A2) / denominator : 0.0;
+    }
+}
+
+__global__ void k_ng_apply(int64_t n, const double *__restrict__ coef, double *__restrict__ x,
+                           double *__restrict__ x1, double *__restrict__ x2, double *__restrict__ x3)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (coef[4] == 0.0) return;
+    const double a = coef[5], b = coef[6];
+    const double xo = x[i];
+    x[i] = (1.0 - a - b) * xo + a * x1[i] + b * x2[i];
+    x3[i] = x2[i];
+    x2[i] = x1[i];
+    x1[i] = xo;
+}
+
+__global__ void k_zero_flag(int32_t *flag) { *flag = 0; }
+
+unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+int conv_blocks(mgr_ctx *ctx)
+{
+    return (int)std::min<int64_t>(MAX_PARTIALS, std::max<int64_t>(1, (ctx->N + RED_BLOCK - 1) / RED_BLOCK));
+}
+
+}  // namespace
+
+namespace mgr {
+
+int launch_jeff(mgr_ctx *ctx)
+{
+    hipLaunchKernelGGL(k_jeff, dim3(grid_for(ctx->N * ctx->L, 256)), dim3(256), 0, ctx->stream,
+                       ctx->N, (int)ctx->L, (int)ctx->Q, (int)ctx->F, (int)ctx->V, ctx->J, ctx->Lam,
+                       ctx->nr_line, ctx->qweights, ctx->nu, ctx->pw, ctx->A, ctx->frequency, ctx->irad,
+                       ctx->d_lam_f_off, ctx->d_lam_f, ctx->f_z, ctx->pop, ctx->Ls, ctx->Jlin,
+                       ctx->Jeff, ctx->Jdif);
+    MGR_LAUNCH_CHECK();
+    return MGR_OK;
+}
+
+int launch_stateq(mgr_ctx *ctx)
+{
+    StateqArgs a;
+    a.N = ctx->N; a.V = (int)ctx->V; a.L = (int)ctx->L; a.nspecs = (int)ctx->nspecs;
+    a.ncolpar = (int)ctx->ncolpar;
+    a.Jeff = ctx->Jeff; a.Ls = ctx->Ls; a.A = ctx->A; a.Bs = ctx->Bs; a.Ba = ctx->Ba;
+    a.abundance = ctx->abundance; a.T = ctx->temperature; a.pop_tot = ctx->pop_tot;
+    a.irad = ctx->irad; a.jrad = ctx->jrad;
+    a.cp_num = ctx->cp_num; a.cp_ncol = ctx->cp_ncol; a.cp_ntmp = ctx->cp_ntmp;
+    a.cp_koff = ctx->cp_koff; a.cp_toff = ctx->cp_toff; a.cp_coff = ctx->cp_coff; a.cp_op = ctx->cp_op;
+    a.icol = ctx->icol; a.jcol = ctx->jcol; a.tmp = ctx->tmp; a.Cd = ctx->Cd; a.Ce = ctx->Ce;
+    a.pop = ctx->pop; a.prev1 = ctx->prev1; a.prev2 = ctx->prev2; a.prev3 = ctx->prev3;
+    a.err_flag = ctx->err_flag;
+    const dim3 g(grid_for(ctx->N, 128)), b(128);
+    switch (ctx->V) {
+        case 2: hipLaunchKernelGGL(k_stateq<2>, g, b, 0, ctx->stream, a); break;
+        case 3: hipLaunchKernelGGL(k_stateq<3>, g, b, 0, ctx->stream, a); break;
+        case 4: hipLaunchKernelGGL(k_stateq<4>, g, b, 0, ctx->stream, a); break;
+        case 5: hipLaunchKernelGGL(k_stateq<5>, g, b, 0, ctx->stream, a); break;
+        case 6: hipLaunchKernelGGL(k_stateq<6>, g, b, 0, ctx->stream, a); break;
+        case 7: hipLaunchKernelGGL(k_stateq<7>, g, b, 0, ctx->stream, a); break;
+        case 8: hipLaunchKernelGGL(k_stateq<8>, g, b, 0, ctx->stream, a); break;
+        default:
+            if (ctx->V > VMAX_GENERIC)
+                return ctx->fail(MGR_ERR_UNSUPPORTED, "more than 32 levels");
+            hipLaunchKernelGGL(k_stateq_generic, g, b, 0, ctx->stream, a);
+    }
+    MGR_LAUNCH_CHECK();
+    return MGR_OK;
+}
+
+int launch_convergence(mgr_ctx *ctx)
+{
+    const int nb = conv_blocks(ctx);
+    hipLaunchKernelGGL(k_conv_partial, dim3(nb), dim3(RED_BLOCK), 0, ctx->stream, ctx->N, (int)ctx->V,
+                       ctx->pop_prec, ctx->pop, ctx->prev1, ctx->pop_tot, ctx->partials);
+    MGR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_conv_final, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, nb, ctx->partials,
+                       ctx->reduced, ctx->err_flag);
+    MGR_LAUNCH_CHECK();
+    return MGR_OK;
+}
+
+int launch_ng(mgr_ctx *ctx)
+{
+    const int64_t n = ctx->N * ctx->V;
+    const int nb = (int)std::min<int64_t>(MAX_PARTIALS, std::max<int64_t>(1, (n + RED_BLOCK - 1) / RED_BLOCK));
+    hipLaunchKernelGGL(k_ng_partial, dim3(nb), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->pop, ctx->prev1,
+                       ctx->prev2, ctx->prev3, ctx->partials);
+    MGR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_ng_final, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, nb, ctx->partials, ctx->reduced);
+    MGR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_ng_apply, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, ctx->reduced,
+                       ctx->pop, ctx->prev1, ctx->prev2, ctx->prev3);
+    MGR_LAUNCH_CHECK();
+    return MGR_OK;
+}
+
+// Enqueue the copy of the convergence words to pinned host memory.
+static int enqueue_conv_copy(mgr_ctx *ctx)
+{
+    MGR_HIP(hipMemcpyAsync(ctx->h_reduced, ctx->reduced, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return MGR_OK;
+}
+
+int read_convergence(mgr_ctx *ctx, mgr_conv *conv)
+{
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    const int32_t errv = (int32_t)ctx->h_reduced[3];
+    if (conv) {
+        conv->fraction_not_converged = ctx->h_reduced[0];
+        conv->relative_change_mean = ctx->h_reduced[1];
+        conv->relative_change_max = ctx->h_reduced[2];
+    }
+    if (errv & ERR_SINGULAR) return ctx->fail(MGR_ERR_NUMERIC, "singular rate matrix in a cell");
+    if (errv) return ctx->fail(MGR_ERR_TRACE, "a ray pair ended off the boundary or an interior origin has an empty ray");
+    return MGR_OK;
+}
+
+// One ALI iteration, enqueued on the context stream (graph-capturable: no
+// memset nodes, the error word is cleared by a kernel).
+static int enqueue_iteration(mgr_ctx *ctx, bool ng, int precision)
+{
+    int rc;
+    hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, ctx->stream, ctx->err_flag);
+    MGR_LAUNCH_CHECK();
+    if (ng) {
+        if ((rc = launch_ng(ctx))) return rc;
+    } else {
+        if ((rc = launch_radiation_field(ctx, precision))) return rc;
+        if ((rc = launch_jeff(ctx))) return rc;
+        if ((rc = launch_stateq(ctx))) return rc;
+    }
+    if ((rc = launch_convergence(ctx))) return rc;
+    if ((rc = launch_emissivity_opacity(ctx))) return rc;
+    return enqueue_conv_copy(ctx);
+}
+
+static int build_graph(mgr_ctx *ctx, bool ng, int precision, hipGraphExec_t *out)
+{
+    hipGraph_t g = nullptr;
+    MGR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_iteration(ctx, ng, precision);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return ctx->fail(MGR_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return ctx->fail(MGR_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+    return MGR_OK;
+}
+
+}  // namespace mgr
+
+extern "C" {
+
+int mgr_compute_Jeff(mgr_ctx *ctx)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (!ctx->J_set) return ctx->fail(MGR_ERR_STATE, "compute_radiation_field first");
+    MGR_HIP(hipSetDevice(ctx->device));
+    int rc = launch_jeff(ctx);
+    if (rc) return rc;
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    return MGR_OK;
+}
+
+int mgr_compute_level_populations_from_stateq(mgr_ctx *ctx, mgr_conv *conv)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (!ctx->J_set) return ctx->fail(MGR_ERR_STATE, "compute_radiation_field and compute_Jeff first");
+    MGR_HIP(hipSetDevice(ctx->device));
+    int rc;
+    hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, ctx->stream, ctx->err_flag);
+    MGR_LAUNCH_CHECK();
+    if ((rc = launch_stateq(ctx))) return rc;
+    if ((rc = launch_convergence(ctx))) return rc;
+    if ((rc = launch_emissivity_opacity(ctx))) return rc;
+    if ((rc = enqueue_conv_copy(ctx))) return rc;
+    return read_convergence(ctx, conv);
+}
+
+int mgr_iteration_using_Ng_acceleration(mgr_ctx *ctx, mgr_conv *conv)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (!ctx->pops_set) return ctx->fail(MGR_ERR_STATE, "level populations not set");
+    MGR_HIP(hipSetDevice(ctx->device));
+    int rc;
+    hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, ctx->stream, ctx->err_flag);
+    MGR_LAUNCH_CHECK();
+    if ((rc = launch_ng(ctx))) return rc;
+    if ((rc = launch_convergence(ctx))) return rc;
+    if ((rc = launch_emissivity_opacity(ctx))) return rc;
+    if ((rc = enqueue_conv_copy(ctx))) return rc;
+    return read_convergence(ctx, conv);
+}
+
+int mgr_compute_level_populations(mgr_ctx *ctx, int use_Ng_acceleration, int64_t max_niterations,
+                                  int criterion, int precision, int64_t *iterations_out)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (iterations_out) *iterations_out = -1;
+    // sim_lines.tpp:67-71: requires the line spectral discretisation
+    if (!ctx->spectral_set) return ctx->fail(MGR_ERR_STATE, "Spectral discretisation was not set for Lines");
+    if (!ctx->bdy_set) return ctx->fail(MGR_ERR_STATE, "compute_boundary_intensities first");
+    if (!ctx->pops_set) return ctx->fail(MGR_ERR_STATE, "level populations not set");
+    if (precision != MGR_FP32 && precision != MGR_FP64) return ctx->fail(MGR_ERR_ARGUMENT, "bad precision");
+    if (criterion != MGR_CRIT_FRACTION && criterion != MGR_CRIT_MAX) return ctx->fail(MGR_ERR_ARGUMENT, "bad criterion");
+    MGR_HIP(hipSetDevice(ctx->device));
+    int rc;
+    if (!ctx->traced && (rc = launch_trace(ctx))) return rc;
+    if (ctx->graph_precision != precision) {
+        if (ctx->graph_normal) (void)hipGraphExecDestroy(ctx->graph_normal);
+        if (ctx->graph_ng) (void)hipGraphExecDestroy(ctx->graph_ng);
+        ctx->graph_normal = ctx->graph_ng = nullptr;
+        if ((rc = build_graph(ctx, false, precision, &ctx->graph_normal))) return rc;
+        if ((rc = build_graph(ctx, true, precision, &ctx->graph_ng))) return rc;
+        ctx->graph_precision = precision;
+    }
+    int64_t iteration = 0, iteration_normal = 0;
+    bool some_not_converged = true;
+    while (some_not_converged && iteration < max_niterations) {
+        iteration++;
+        const bool ng = use_Ng_acceleration && (iteration_normal == 4);
+        MGR_HIP(hipGraphLaunch(ng ? ctx->graph_ng : ctx->graph_normal, ctx->stream));
+        iteration_normal = ng ? 0 : iteration_normal + 1;
+        mgr_conv conv;
+        if ((rc = read_convergence(ctx, &conv))) return rc;
+        if (criterion == MGR_CRIT_MAX)
+            some_not_converged = conv.relative_change_max >= ctx->pop_prec;
+        else
+            some_not_converged = conv.fraction_not_converged > 0.005;
+    }
+    ctx->J_set = true;
+    if (iterations_out) *iterations_out = iteration;
+    return MGR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// mgr_ray.hpp -- FP64 geometry arithmetic shared by the tracer and the
+// radiation-field kernels.  Every expression keeps the reference's operation
+// order and is compiled without floating-point contraction, so traced cell
+// sequences, shifts, dZ and interpolation points are bit-identical to the
+// reference's scalar code.
+#pragma once
+
+#include <cfloat>
+
+#include "mgr_common.hpp"
+
+namespace mgr {
+
+// Eigen's Vector3d::dot on x86-64 (SSE2 packet of two, then the tail):
+// (a.x*b.x + a.y*b.y) + a.z*b.z.
+__device__ __forceinline__ double dot3(double ax, double ay, double az, double bx, double by, double bz)
+{
+#pragma clang fp contract(off)
+    return (ax * bx + ay * by) + az * bz;
+}
+
+// Geometry::get_next_general (src/Model/Geometry/geometry.tpp:328-421):
+// the neighbour with Z_new > Z closest to the ray (strict <, first wins).
+__device__ __forceinline__ int32_t get_next(const double *__restrict__ pos,
+                                            const int64_t *__restrict__ nb_off,
+                                            const int32_t *__restrict__ nb, int32_t origin,
+                                            double nx, double ny, double nz, int32_t current,
+                                            double &Z, double &dZ)
+{
+#pragma clang fp contract(off)
+    double dmin = DBL_MAX;
+    int32_t next = -1;
+    const double ox = pos[3 * (int64_t)origin + 0];
+    const double oy = pos[3 * (int64_t)origin + 1];
+    const double oz = pos[3 * (int64_t)origin + 2];
+    const int64_t q1 = nb_off[current + 1];
+    for (int64_t q = nb_off[current]; q < q1; q++) {
+        const int32_t c = nb[q];
+        const double Rx = pos[3 * (int64_t)c + 0] - ox;
+        const double Ry = pos[3 * (int64_t)c + 1] - oy;
+        const double Rz = pos[3 * (int64_t)c + 2] - oz;
+        const double Z_new = dot3(Rx, Ry, Rz, nx, ny, nz);
+        if (Z_new > Z) {
+            const double distance_from_ray2 = dot3(Rx, Ry, Rz, Rx, Ry, Rz) - Z_new * Z_new;
+            if (distance_from_ray2 < dmin) {
+                dmin = distance_from_ray2;
+                next = c;
+                dZ = Z_new - Z;
+            }
+        }
+    }
+    Z = Z + dZ;
+    return next;
+}
+
+// Geometry::get_doppler_shift<CoMoving> (geometry.tpp:433-446)
+__device__ __forceinline__ double doppler_shift(const double *__restrict__ vel, int32_t origin,
+                                                int32_t current, double nx, double ny, double nz)
+{
+#pragma clang fp contract(off)
+    const double dx = vel[3 * (int64_t)current + 0] - vel[3 * (int64_t)origin + 0];
+    const double dy = vel[3 * (int64_t)current + 1] - vel[3 * (int64_t)origin + 1];
+    const double dz = vel[3 * (int64_t)current + 2] - vel[3 * (int64_t)origin + 2];
+    return 1.0 - dot3(dx, dy, dz, nx, ny, nz);
+}
+
+// Interpolation points of one traced step (Geometry::set_data,
+// geometry.tpp:100-207).  Returns the number of projected points the step
+// emits: 1 for a resolved velocity step, n_interpl for an interpolated one,
+// 0 when more than 10000 points would be needed (set_data returns early).
+struct StepSplit {
+    int    n;       // points emitted
+    int    interp;  // 1 if interpolated
+    int    half;    // sub-points m < half belong to the current cell
+    double dZ;      // dZ of each sub-point
+    double dshift;  // shift increment of each sub-point
+};
+
+__device__ __forceinline__ StepSplit split_step(double shift_crt, double shift_nxt,
+                                                double dZ_loc, double dshift_max)
+{
+#pragma clang fp contract(off)
+    StepSplit s;
+    const double dshift = shift_nxt - shift_crt;
+    const double dshift_abs = fabs(dshift);
+    if (dshift_abs > dshift_max) {
+        const double nd = dshift_abs / dshift_max + 1;
+        if (!(nd < (double)(MAX_INTERPOLATION + 1))) {   // (long)nd > 10000, or NaN
+            s.n = 0; s.interp = 1; s.half = 0; s.dZ = 0.0; s.dshift = 0.0;
+            return s;
+        }
+        const long n_interpl = (long)nd;
+        s.n = (int)n_interpl;
+        s.interp = 1;
+        s.half = (int)(long)(0.5 * (double)n_interpl);
+        s.dZ = dZ_loc / (double)n_interpl;
+        s.dshift = dshift / (double)n_interpl;
+    } else {
+        s.n = 1; s.interp = 0; s.half = 0; s.dZ = dZ_loc; s.dshift = 0.0;
+    }
+    return s;
+}
+
+// shift of interpolation sub-point m: shift_crt + m * dshift_interpl
+__device__ __forceinline__ double sub_shift(double shift_crt, int m, double dshift_interpl)
+{
+#pragma clang fp contract(off)
+    return shift_crt + (double)m * dshift_interpl;
+}
+
+}  // namespace mgr

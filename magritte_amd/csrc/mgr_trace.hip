@@ -1,0 +1,256 @@
+// mgr_trace.hip -- ray tracer (Geometry::trace_ray<CoMoving>,
+// src/Model/Geometry/geometry.tpp:21-95) over the point-cloud neighbour lists.
+//
+// One thread per (origin, ray).  A counting pass sizes every ray, the host
+// scans the counts, and a filling pass writes the traced steps (next cell,
+// shift, dZ) into one device-resident SoA cache that every later radiation
+// field reuses: geometry and velocities are static, so the reference's
+// re-tracing on every ALI iteration is not repeated.  Interpolation points
+// (set_data) are not stored; the solver expands them from the stored steps
+// with the same FP64 arithmetic (mgr_ray.hpp).
+//
+// Compiled with -ffp-contract=off.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "mgr_common.hpp"
+#include "mgr_ray.hpp"
+
+using namespace mgr;
+
+namespace {
+
+constexpr int32_t ERR_NO_NEIGHBOUR = 1;
+constexpr int32_t ERR_TOO_LONG     = 8;
+
+__device__ __forceinline__ void atomic_min_pos_double(uint64_t *addr, double v)
+{
+    // positive doubles order like their bit patterns
+    if (v > 0.0) atomicMin((unsigned long long *)addr, (unsigned long long)__double_as_longlong(v));
+}
+
+template <bool FILL>
+__global__ void k_trace(int64_t N, int R, const double *__restrict__ pos,
+                        const double *__restrict__ vel, const int64_t *__restrict__ nb_off,
+                        const int32_t *__restrict__ nb, const uint8_t *__restrict__ is_boundary,
+                        const double *__restrict__ rays, const double *__restrict__ dshift_max,
+                        int32_t *__restrict__ nseg, int32_t *__restrict__ npts,
+                        const int64_t *__restrict__ seg_off, int32_t *__restrict__ seg_cell,
+                        double *__restrict__ seg_shift, double *__restrict__ seg_dZ,
+                        uint64_t *__restrict__ shift_min_bits, int32_t *__restrict__ err_flag)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N * R) return;
+    const int32_t o = (int32_t)(q / R);
+    const int r = (int)(q % R);
+    const double nx = rays[3 * r + 0], ny = rays[3 * r + 1], nz = rays[3 * r + 2];
+    const double dsm = dshift_max[o];
+    int64_t w = FILL ? seg_off[q] : 0;
+    int32_t steps = 0, points = 0;
+    double smin = 1.0;
+
+    double Z = 0.0, dZ = 0.0;
+    int32_t nxt = get_next(pos, nb_off, nb, o, nx, ny, nz, o, Z, dZ);
+    if (nxt != -1) {
+        double shift_crt = doppler_shift(vel, o, o, nx, ny, nz);
+        double shift_nxt = doppler_shift(vel, o, nxt, nx, ny, nz);
+        for (;;) {
+            const StepSplit sp = split_step(shift_crt, shift_nxt, dZ, dsm);
+            points += sp.n;
+            smin = fmin(smin, shift_nxt);
+            if (FILL) {
+                seg_cell[w] = nxt;
+                seg_shift[w] = shift_nxt;
+                seg_dZ[w] = dZ;
+                w++;
+            }
+            steps++;
+            if (is_boundary[nxt]) break;
+            if (steps >= N) { atomicOr(err_flag, ERR_TOO_LONG); break; }
+            shift_crt = shift_nxt;
+            nxt = get_next(pos, nb_off, nb, o, nx, ny, nz, nxt, Z, dZ);
+            if (nxt < 0) { atomicOr(err_flag, ERR_NO_NEIGHBOUR); break; }
+            shift_nxt = doppler_shift(vel, o, nxt, nx, ny, nz);
+        }
+    }
+    if (!FILL) {
+        nseg[q] = steps;
+        npts[q] = points;
+        atomic_min_pos_double(shift_min_bits, smin);
+    }
+}
+
+// Expands the cached steps of one ray into its projected points (what
+// Geometry::trace_ray returns), with the solver's expansion arithmetic.
+__global__ void k_expand_ray(int32_t o, int64_t q, const int64_t *__restrict__ seg_off,
+                             const int32_t *__restrict__ nseg, const int32_t *__restrict__ seg_cell,
+                             const double *__restrict__ seg_shift, const double *__restrict__ seg_dZ,
+                             const double *__restrict__ dshift_max, int64_t capacity,
+                             int64_t *__restrict__ out_cell, double *__restrict__ out_shift,
+                             double *__restrict__ out_dZ, int64_t *__restrict__ n_out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int64_t s0 = seg_off[q];
+    const int ns = nseg[q];
+    const double dsm = dshift_max[o];
+    int64_t n = 0;
+    int32_t crt = o;
+    double s_crt = 1.0;
+    for (int j = 0; j < ns; j++) {
+        const int32_t c = seg_cell[s0 + j];
+        const double s = seg_shift[s0 + j];
+        const double z = seg_dZ[s0 + j];
+        const StepSplit sp = split_step(s_crt, s, z, dsm);
+        if (!sp.interp) {
+            if (n < capacity) { out_cell[n] = c; out_shift[n] = s; out_dZ[n] = z; }
+            n++;
+        } else {
+            for (int m = 1; m <= sp.n; m++) {
+                if (n < capacity) {
+                    out_cell[n] = (m < sp.half) ? crt : c;
+                    out_shift[n] = sub_shift(s_crt, m, sp.dshift);
+                    out_dZ[n] = sp.dZ;
+                }
+                n++;
+            }
+        }
+        crt = c;
+        s_crt = s;
+    }
+    *n_out = n;
+}
+
+}  // namespace
+
+namespace mgr {
+
+int launch_trace(mgr_ctx *ctx)
+{
+    const int64_t N = ctx->N, R = ctx->R, NR = N * R;
+    const unsigned grid = (unsigned)((NR + 127) / 128);
+    if (!ctx->nseg) {
+        MGR_HIP(hipMalloc((void **)&ctx->nseg, NR * sizeof(int32_t)));
+        MGR_HIP(hipMalloc((void **)&ctx->npts, NR * sizeof(int32_t)));
+        MGR_HIP(hipMalloc((void **)&ctx->seg_off, (NR + 1) * sizeof(int64_t)));
+    }
+    const uint64_t one_bits = 0x3FF0000000000000ull;   // 1.0
+    MGR_HIP(hipMemcpyAsync(ctx->shift_min_bits, &one_bits, sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+    MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
+    hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->pos,
+                       ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->dshift_max,
+                       ctx->nseg, ctx->npts, nullptr, nullptr, nullptr, nullptr, ctx->shift_min_bits,
+                       ctx->err_flag);
+    MGR_LAUNCH_CHECK();
+    std::vector<int32_t> ns(NR), np(NR);
+    int32_t errv = 0;
+    uint64_t smin_bits = 0;
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    MGR_HIP(hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (errv) {
+        return ctx->fail(MGR_ERR_TRACE, errv & ERR_NO_NEIGHBOUR
+                                            ? "a ray found no forward neighbour inside the mesh"
+                                            : "a ray visited more steps than there are cells");
+    }
+    MGR_HIP(hipMemcpy(ns.data(), ctx->nseg, NR * sizeof(int32_t), hipMemcpyDeviceToHost));
+    MGR_HIP(hipMemcpy(np.data(), ctx->npts, NR * sizeof(int32_t), hipMemcpyDeviceToHost));
+    MGR_HIP(hipMemcpy(&smin_bits, ctx->shift_min_bits, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    std::vector<int64_t> off(NR + 1, 0);
+    for (int64_t q = 0; q < NR; q++) off[q + 1] = off[q] + ns[q];
+    ctx->total_segments = off[NR];
+    double smin;
+    std::memcpy(&smin, &smin_bits, sizeof(double));
+    ctx->shift_min = std::min(1.0, smin);
+
+    // ray-pair statistics: ndep = n_ar + n_r + 1 (sim_radiation.tpp:164-178)
+    int64_t sum_ndep = 0, solved = 0;
+    for (int64_t o = 0; o < N; o++)
+        for (int64_t r = 0; r < R / 2; r++) {
+            const int64_t nd = (int64_t)np[o * R + r] + np[o * R + ctx->antipod[r]] + 1;
+            sum_ndep += nd;
+            solved += (nd > 1);
+        }
+    ctx->sum_ndep = sum_ndep;
+    ctx->n_pairs_solved = solved;
+
+    if (ctx->seg_cell) {
+        (void)hipFree(ctx->seg_cell); (void)hipFree(ctx->seg_shift); (void)hipFree(ctx->seg_dZ);
+        ctx->seg_cell = nullptr; ctx->seg_shift = nullptr; ctx->seg_dZ = nullptr;
+    }
+    const int64_t T = std::max<int64_t>(ctx->total_segments, 1);
+    MGR_HIP(hipMalloc((void **)&ctx->seg_cell, T * sizeof(int32_t)));
+    MGR_HIP(hipMalloc((void **)&ctx->seg_shift, T * sizeof(double)));
+    MGR_HIP(hipMalloc((void **)&ctx->seg_dZ, T * sizeof(double)));
+    MGR_HIP(hipMemcpy(ctx->seg_off, off.data(), (NR + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->pos,
+                       ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->dshift_max,
+                       nullptr, nullptr, ctx->seg_off, ctx->seg_cell, ctx->seg_shift, ctx->seg_dZ,
+                       nullptr, ctx->err_flag);
+    MGR_LAUNCH_CHECK();
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->traced = true;
+    return MGR_OK;
+}
+
+}  // namespace mgr
+
+extern "C" {
+
+int mgr_trace(mgr_ctx *ctx)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    MGR_HIP(hipSetDevice(ctx->device));
+    return launch_trace(ctx);
+}
+
+int mgr_get_ray(mgr_ctx *ctx, int64_t origin, int64_t ray, int64_t capacity, int64_t *cellNr,
+                double *shift, double *dZ, int64_t *n_out)
+{
+    if (!ctx || !n_out || capacity < 0) return MGR_ERR_ARGUMENT;
+    if (!ctx->traced) return ctx->fail(MGR_ERR_STATE, "mgr_trace first");
+    if (origin < 0 || origin >= ctx->N || ray < 0 || ray >= ctx->R)
+        return ctx->fail(MGR_ERR_ARGUMENT, "origin or ray out of range");
+    if (capacity > 0 && (!cellNr || !shift || !dZ)) return MGR_ERR_ARGUMENT;
+    MGR_HIP(hipSetDevice(ctx->device));
+    const int64_t cap = std::max<int64_t>(capacity, 1);
+    int64_t *dc = nullptr, *dn = nullptr;
+    double *ds = nullptr, *dz = nullptr;
+    MGR_HIP(hipMalloc((void **)&dc, cap * sizeof(int64_t)));
+    MGR_HIP(hipMalloc((void **)&ds, cap * sizeof(double)));
+    MGR_HIP(hipMalloc((void **)&dz, cap * sizeof(double)));
+    MGR_HIP(hipMalloc((void **)&dn, sizeof(int64_t)));
+    hipLaunchKernelGGL(k_expand_ray, dim3(1), dim3(64), 0, ctx->stream, (int32_t)origin,
+                       origin * ctx->R + ray, ctx->seg_off, ctx->nseg, ctx->seg_cell, ctx->seg_shift,
+                       ctx->seg_dZ, ctx->dshift_max, capacity, dc, ds, dz, dn);
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(ctx->stream);
+    int rc = MGR_OK;
+    if (le != hipSuccess || se != hipSuccess) rc = ctx->fail(MGR_ERR_DEVICE, "ray expansion kernel failed");
+    if (rc == MGR_OK) {
+        (void)hipMemcpy(n_out, dn, sizeof(int64_t), hipMemcpyDeviceToHost);
+        const int64_t n = std::min<int64_t>(*n_out, capacity);
+        if (n > 0) {
+            (void)hipMemcpy(cellNr, dc, n * sizeof(int64_t), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(shift, ds, n * sizeof(double), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(dZ, dz, n * sizeof(double), hipMemcpyDeviceToHost);
+        }
+    }
+    (void)hipFree(dc); (void)hipFree(ds); (void)hipFree(dz); (void)hipFree(dn);
+    return rc;
+}
+
+int mgr_get_npoints_on_rays(mgr_ctx *ctx, int64_t *npoints)
+{
+    if (!ctx || !npoints) return MGR_ERR_ARGUMENT;
+    if (!ctx->traced) return ctx->fail(MGR_ERR_STATE, "mgr_trace first");
+    MGR_HIP(hipSetDevice(ctx->device));
+    const int64_t N = ctx->N, R = ctx->R;
+    std::vector<int32_t> np(N * R);
+    MGR_HIP(hipMemcpy(np.data(), ctx->npts, N * R * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int64_t r = 0; r < R / 2; r++)
+        for (int64_t o = 0; o < N; o++)
+            npoints[r * N + o] = (int64_t)np[o * R + r] + np[o * R + ctx->antipod[r]] + 1;
+    return MGR_OK;
+}
+
+}  // extern "C"

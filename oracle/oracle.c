@@ -1,0 +1,1081 @@
+/*
+ * oracle.c -- CPU FP64 restatement ("cpu_ref") of Magritte's RayPair
+ * radiation-field path and ALI driver.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the timed CPU
+ * baseline.  Nothing in the product links or loads it.
+ *
+ * Built with -O2 -ffp-contract=off (no fused multiply-add), scalar FP64,
+ * OpenMP over origin cells exactly as the reference: ray pairs outer, origins
+ * strided over threads (src/Simulation/sim_radiation.tpp:130-148).
+ *
+ * Deviations, all stated where they occur:
+ *  - the global Eigen SparseLU<COLAMD> of the rate equations is replaced by one
+ *    dense LU with partial pivoting per cell; with n_off_diag = 0 every triplet
+ *    couples levels of one cell, so the global matrix is block diagonal and the
+ *    solutions agree up to rounding;
+ *  - heapsort (src/Functions/heapsort.cpp:77-114) is replaced by a sort on
+ *    (value, original index); for distinct values the permutation is the same;
+ *  - relative_change_max is computed (the reference leaves it at 0 because its
+ *    update is commented out as not thread safe, lineProducingSpecies.tpp:157-161);
+ *  - a segment needing more than 10000 interpolation points contributes no
+ *    points, as set_data's early return does (geometry.tpp:138-147);
+ *  - a ray with no forward neighbour stops and is reported (-1) instead of
+ *    reading velocity[-1] (geometry.tpp:68-77).
+ *  - Lambda* is stored for n_off_diag = 0 only (one element per (p,k) at nr = p).
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---- constants: src/Tools/constants.hpp:26-63 ---------------------------- */
+#define O_PI              3.14159265358979323846
+#define O_FOUR_PI         (4.0 * O_PI)
+#define O_INVERSE_SQRT_PI (0.5 * 1.12837916709551257390) /* 0.5*M_2_SQRTPI */
+#define O_CC              2.99792458E+8
+#define O_HH              6.62607004E-34
+#define O_KB              1.38064852E-23
+#define O_AMU             1.66053904E-27
+#define O_T_CMB           2.7254800
+#define O_C_SQUARED       8.98755179E+16
+static const double TWO_KB_OVER_AMU_C_SQUARED = 2.0 * O_KB / (O_AMU * O_C_SQUARED);
+static const double TWO_HH_OVER_CC_SQUARED    = 2.0 * O_HH / O_C_SQUARED;
+static const double HH_OVER_KB                = O_HH / O_KB;
+static const double HH_OVER_FOUR_PI           = O_HH / O_FOUR_PI;
+
+struct orc_state {
+    mgr_model_view m;
+    int64_t N, R, NB, F, L, V, Q;
+    double  *line;        int64_t *line_index;            /* [L] sorted line list */
+    int64_t *cell2boundary_nr; unsigned char *boundary;   /* [N] */
+    int64_t *antipod;                                     /* [R] */
+    int64_t *nb_off;                                      /* [N+1] */
+    double  *nu;          int64_t *nr_line;               /* [N*F], [N*L*Q] */
+    unsigned char *appears; int64_t *corr_l, *corr_k, *corr_z; /* [F] */
+    int      spec_set;
+    double  *I_bdy;                                       /* [R/2][NB][F] */
+    double  *pop, *pop_tot, *prev1, *prev2, *prev3;       /* [N*V], [N] */
+    double  *emissivity, *opacity;                        /* [N*L] */
+    double  *J;                                           /* [N*F] */
+    double  *Ls;                                          /* [N*L] */
+    double  *Jlin, *Jeff, *Jdif;                          /* [N*L] */
+    double   fnc, rcm, rcmax;
+};
+
+int orc_nthreads_max(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
+
+/* Stable sort of (value, index) pairs; stands in for heapsort
+ * (src/Functions/heapsort.cpp:77-114), identical for distinct values. */
+typedef struct { double v; int64_t i; } vi_pair;
+static int vi_cmp(const void *a, const void *b)
+{
+    const vi_pair *x = (const vi_pair *)a, *y = (const vi_pair *)b;
+    if (x->v < y->v) return -1;
+    if (x->v > y->v) return 1;
+    return (x->i < y->i) ? -1 : (x->i > y->i);
+}
+
+orc_state *orc_create(const mgr_model_view *m)
+{
+    orc_state *s = (orc_state *)xcalloc(1, sizeof(orc_state));
+    s->m  = *m;
+    s->N  = m->ncells;
+    s->R  = m->nrays;
+    s->NB = m->nboundary;
+    s->L  = m->nrad;
+    s->V  = m->nlev;
+    s->Q  = m->nquads;
+    s->F  = m->nrad * m->nquads;   /* nfreqs = nlines * nquads (one species) */
+
+    /* Lines::setup (src/Model/Lines/lines.cpp:43-87): sorted line list */
+    s->line = (double *)xcalloc(s->L, sizeof(double));
+    s->line_index = (int64_t *)xcalloc(s->L, sizeof(int64_t));
+    {
+        vi_pair *tmp = (vi_pair *)xcalloc(s->L, sizeof(vi_pair));
+        for (int64_t k = 0; k < s->L; k++) { tmp[k].v = m->frequency[k]; tmp[k].i = k; }
+        qsort(tmp, (size_t)s->L, sizeof(vi_pair), vi_cmp);
+        for (int64_t k = 0; k < s->L; k++) { s->line[k] = tmp[k].v; s->line_index[k] = tmp[k].i; }
+        free(tmp);
+    }
+
+    /* Boundary::read (src/Model/Geometry/Boundary/boundary.cpp:19-51) */
+    s->cell2boundary_nr = (int64_t *)xcalloc(s->N, sizeof(int64_t));
+    s->boundary = (unsigned char *)xcalloc(s->N, 1);
+    for (int64_t p = 0; p < s->N; p++) { s->cell2boundary_nr[p] = s->N; s->boundary[p] = 0; }
+    for (int64_t b = 0; b < s->NB; b++) {
+        const int64_t c = m->boundary2cell_nr[b];
+        s->cell2boundary_nr[c] = b;
+        s->boundary[c] = 1;
+    }
+
+    /* Rays::setup_antipodal_rays (src/Model/Geometry/Rays/rays.cpp:90-103):
+     * the last r2 with |r1+r2|^2 < 1e-9 wins. */
+    s->antipod = (int64_t *)xcalloc(s->R, sizeof(int64_t));
+    for (int64_t r1 = 0; r1 < s->R; r1++) {
+        for (int64_t r2 = 0; r2 < s->R; r2++) {
+            const double x = m->rays[3*r1+0] + m->rays[3*r2+0];
+            const double y = m->rays[3*r1+1] + m->rays[3*r2+1];
+            const double z = m->rays[3*r1+2] + m->rays[3*r2+2];
+            if (((x*x + y*y) + z*z) < 1.0E-9) s->antipod[r1] = r2;
+        }
+    }
+
+    s->nb_off = (int64_t *)xcalloc(s->N + 1, sizeof(int64_t));
+    for (int64_t p = 0; p < s->N; p++) s->nb_off[p+1] = s->nb_off[p] + m->n_neighbors[p];
+
+    s->nu       = (double *)xcalloc(s->N * s->F, sizeof(double));
+    s->nr_line  = (int64_t *)xcalloc(s->N * s->L * s->Q, sizeof(int64_t));
+    s->appears  = (unsigned char *)xcalloc(s->F, 1);
+    s->corr_l   = (int64_t *)xcalloc(s->F, sizeof(int64_t));
+    s->corr_k   = (int64_t *)xcalloc(s->F, sizeof(int64_t));
+    s->corr_z   = (int64_t *)xcalloc(s->F, sizeof(int64_t));
+    s->I_bdy    = (double *)xcalloc((s->R / 2) * s->NB * s->F, sizeof(double));
+    s->pop      = (double *)xcalloc(s->N * s->V, sizeof(double));
+    s->prev1    = (double *)xcalloc(s->N * s->V, sizeof(double));
+    s->prev2    = (double *)xcalloc(s->N * s->V, sizeof(double));
+    s->prev3    = (double *)xcalloc(s->N * s->V, sizeof(double));
+    s->pop_tot  = (double *)xcalloc(s->N, sizeof(double));
+    s->emissivity = (double *)xcalloc(s->N * s->L, sizeof(double));
+    s->opacity  = (double *)xcalloc(s->N * s->L, sizeof(double));
+    s->J        = (double *)xcalloc(s->N * s->F, sizeof(double));
+    s->Ls       = (double *)xcalloc(s->N * s->L, sizeof(double));
+    s->Jlin     = (double *)xcalloc(s->N * s->L, sizeof(double));
+    s->Jeff     = (double *)xcalloc(s->N * s->L, sizeof(double));
+    s->Jdif     = (double *)xcalloc(s->N * s->L, sizeof(double));
+    return s;
+}
+
+void orc_destroy(orc_state *s)
+{
+    if (!s) return;
+    free(s->line); free(s->line_index); free(s->cell2boundary_nr); free(s->boundary);
+    free(s->antipod); free(s->nb_off); free(s->nu); free(s->nr_line); free(s->appears);
+    free(s->corr_l); free(s->corr_k); free(s->corr_z); free(s->I_bdy); free(s->pop);
+    free(s->prev1); free(s->prev2); free(s->prev3); free(s->pop_tot); free(s->emissivity);
+    free(s->opacity); free(s->J); free(s->Ls); free(s->Jlin); free(s->Jeff); free(s->Jdif);
+    free(s);
+}
+
+/* ---- Functions ---------------------------------------------------------- */
+
+/* planck (src/Functions/planck.tpp:15-24) */
+double orc_planck(double temperature, double freq)
+{
+    const double h_over_kbT = HH_OVER_KB / temperature;
+    return TWO_HH_OVER_CC_SQUARED * (freq*freq*freq) / expm1(h_over_kbT*freq);
+}
+
+/* Thermodynamics::profile(width, freq_diff) (thermodynamics.tpp:16-29) with the
+ * scalar vExpMinus = 1/exp (src/Tools/Parallel/wrap_Grid.hpp:145-163). */
+double orc_profile(double width, double freq_diff)
+{
+    const double inverse_width = 1.0 / width;
+    const double sqrtExponent  = inverse_width * freq_diff;
+    const double exponent      = sqrtExponent * sqrtExponent;
+    return inverse_width * O_INVERSE_SQRT_PI * (1.0 / exp(exponent));
+}
+
+/* Thermodynamics::profile_width(inverse_mass, p) (thermodynamics.tpp:62-69) */
+double orc_profile_width(const orc_state *s, int64_t p)
+{
+    return sqrt(TWO_KB_OVER_AMU_C_SQUARED * s->m.inverse_mass * s->m.temperature[p]
+                + s->m.vturb2[p]);
+}
+
+/* search (src/Functions/interpolation.tpp:14-54) */
+static long o_search(const double *x, long n, double value)
+{
+    long start = 0, stop = n - 1;
+    if (value >= x[stop]) return stop;
+    else if (value <= x[start]) return start;
+    while (stop > start + 1) {
+        const long middle = (stop + start) / 2;
+        if (value > x[middle]) start = middle;
+        else if (value < x[middle]) stop = middle;
+        else return middle;
+    }
+    return stop;
+}
+
+/* CollisionPartner::adjust_abundance_for_ortho_or_para and
+ * ::interpolate_collision_coefficients (collisionPartner.tpp:15-72).
+ * Partner c's tables start at the concatenated offsets. */
+void orc_collision_rates(const orc_state *s, int64_t c, double temperature,
+                         double *Ce_intpld, double *Cd_intpld, double *abundance_factor)
+{
+    const mgr_model_view *m = &s->m;
+    int64_t koff = 0, toff = 0, ctoff = 0;
+    for (int64_t q = 0; q < c; q++) {
+        koff  += m->colpar_ncol[q];
+        toff  += m->colpar_ntmp[q];
+        ctoff += m->colpar_ncol[q] * m->colpar_ntmp[q];
+    }
+    (void)koff;
+    const int64_t ncol = m->colpar_ncol[c], ntmp = m->colpar_ntmp[c];
+    const double *tmp = m->tmp + toff;
+    const double *Ce = m->Ce + ctoff, *Cd = m->Cd + ctoff;
+
+    double factor = 1.0;
+    if (m->colpar_orth_or_para[c] != 'n') {
+        const double frac_H2_para = 1.0 / (1.0 + 9.0*exp(-170.5/temperature));
+        if (m->colpar_orth_or_para[c] == 'o') factor *= (1.0 - frac_H2_para);
+        if (m->colpar_orth_or_para[c] == 'p') factor *= frac_H2_para;
+    }
+    *abundance_factor = factor;
+
+    const long t = o_search(tmp, (long)ntmp, temperature);
+    if (t == 0) {
+        for (int64_t k = 0; k < ncol; k++) { Ce_intpld[k] = Ce[k]; Cd_intpld[k] = Cd[k]; }
+    } else if (t == ntmp) {   /* unreachable: search returns at most ntmp-1 */
+        for (int64_t k = 0; k < ncol; k++) {
+            Ce_intpld[k] = Ce[(ntmp-1)*ncol + k]; Cd_intpld[k] = Cd[(ntmp-1)*ncol + k];
+        }
+    } else {
+        const double step = (temperature - tmp[t-1]) / (tmp[t] - tmp[t-1]);
+        for (int64_t k = 0; k < ncol; k++) {
+            Ce_intpld[k] = Ce[(t-1)*ncol + k] + (Ce[t*ncol + k] - Ce[(t-1)*ncol + k]) * step;
+            Cd_intpld[k] = Cd[(t-1)*ncol + k] + (Cd[t*ncol + k] - Cd[(t-1)*ncol + k]) * step;
+        }
+    }
+}
+
+/* ---- Model set-up ------------------------------------------------------- */
+
+/* Simulation::compute_spectral_discretisation (src/Simulation/sim_spectrum.tpp:13-196) */
+int orc_compute_spectral_discretisation(orc_state *s)
+{
+    const int64_t F = s->F, L = s->L, Q = s->Q;
+    #pragma omp parallel
+    {
+        vi_pair *fr = (vi_pair *)xcalloc(F, sizeof(vi_pair));
+        int64_t *inv = (int64_t *)xcalloc(F, sizeof(int64_t));
+        #pragma omp for schedule(static)
+        for (int64_t p = 0; p < s->N; p++) {
+            const double pw = orc_profile_width(s, p);
+            int64_t index1 = 0;
+            for (int64_t k = 0; k < L; k++) {
+                /* uses the sorted list at the running index (sim_spectrum.tpp:31) */
+                const double freqs_line = s->line[k];
+                const double width = freqs_line * pw;
+                for (int64_t z = 0; z < Q; z++) {
+                    fr[index1].v = freqs_line + width * s->m.roots[z];
+                    fr[index1].i = index1;
+                    index1++;
+                }
+            }
+            qsort(fr, (size_t)F, sizeof(vi_pair), vi_cmp);
+            for (int64_t fl = 0; fl < F; fl++) {
+                s->nu[p*F + fl] = fr[fl].v;
+                inv[fr[fl].i] = fl;
+            }
+            int64_t index2 = 0;
+            for (int64_t k = 0; k < L; k++)
+                for (int64_t z = 0; z < Q; z++) {
+                    s->nr_line[(p*L + k)*Q + z] = inv[index2];
+                    index2++;
+                }
+        }
+        free(fr); free(inv);
+    }
+    /* cell-independent back-maps, indexed by the unsorted position (P11) */
+    for (int64_t fl = 0; fl < F; fl++) {
+        s->appears[fl] = 0; s->corr_l[fl] = F; s->corr_k[fl] = F; s->corr_z[fl] = F;
+    }
+    int64_t index2 = 0;
+    for (int64_t k = 0; k < L; k++)
+        for (int64_t z = 0; z < Q; z++) {
+            s->appears[index2] = 1; s->corr_l[index2] = 0;
+            s->corr_k[index2] = k;  s->corr_z[index2] = z;
+            index2++;
+        }
+    s->spec_set = 1;
+    return 0;
+}
+
+/* Simulation::compute_boundary_intensities (src/Simulation/sim_radiation.tpp:17-48) */
+int orc_compute_boundary_intensities(orc_state *s, const double *temperatures)
+{
+    const int64_t F = s->F;
+    for (int64_t r = 0; r < s->R / 2; r++)
+        for (int64_t b = 0; b < s->NB; b++) {
+            const int64_t p = s->m.boundary2cell_nr[b];
+            const double T = temperatures ? temperatures[b] : O_T_CMB;
+            for (int64_t f = 0; f < F; f++)
+                s->I_bdy[(r*s->NB + b)*F + f] = orc_planck(T, s->nu[p*F + f]);
+        }
+    return 0;
+}
+
+/* LineProducingSpecies::get_emissivity / get_opacity (lineProducingSpecies.tpp:41-70)
+ * and Lines::set_emissivity_and_opacity (lines.tpp:44-60). */
+void orc_set_emissivity_and_opacity(orc_state *s)
+{
+    const int64_t V = s->V, L = s->L;
+    for (int64_t p = 0; p < s->N; p++)
+        for (int64_t k = 0; k < L; k++) {
+            const int64_t i = s->m.irad[k], j = s->m.jrad[k];
+            s->emissivity[p*L + k] = HH_OVER_FOUR_PI * s->m.A[k] * s->pop[p*V + i];
+            s->opacity[p*L + k] = HH_OVER_FOUR_PI * (s->pop[p*V + j] * s->m.Ba[k]
+                                                     - s->pop[p*V + i] * s->m.Bs[k]);
+        }
+}
+
+/* LineProducingSpecies::update_using_LTE (lineProducingSpecies.tpp:82-115) */
+int orc_compute_LTE_level_populations(orc_state *s)
+{
+    const int64_t V = s->V;
+    for (int64_t p = 0; p < s->N; p++) {
+        s->pop_tot[p] = s->m.abundance[p*s->m.nspecs + s->m.num];
+        double pf = 0.0;
+        for (int64_t i = 0; i < V; i++) {
+            s->pop[p*V + i] = s->m.weight[i] * exp(-s->m.energy[i] / (O_KB * s->m.temperature[p]));
+            pf += s->pop[p*V + i];
+        }
+        for (int64_t i = 0; i < V; i++) s->pop[p*V + i] *= s->pop_tot[p] / pf;
+    }
+    orc_set_emissivity_and_opacity(s);
+    return 0;
+}
+
+/* ---- Geometry ----------------------------------------------------------- */
+
+/* Simulation::get_dshift_max (src/Simulation/simulation.tpp:15-37), one species */
+double orc_get_dshift_max(const orc_state *s, int64_t o)
+{
+    double dshift_max = DBL_MAX;
+    const double nd = s->m.max_width_fraction * orc_profile_width(s, o);
+    if (dshift_max > nd) dshift_max = nd;
+    return dshift_max;
+}
+
+/* Geometry::get_doppler_shift (geometry.tpp:433-465); Eigen's Vector3d dot is
+ * evaluated as (x*x' + y*y') + z*z' (SSE2 packet of two, then the tail). */
+double orc_get_doppler_shift(const orc_state *s, int64_t o, int64_t r, int64_t c, int rest_frame)
+{
+    const double *v = s->m.velocity;
+    if (!rest_frame) {
+        const double *n = s->m.rays + 3*r;
+        const double dx = v[3*c+0] - v[3*o+0];
+        const double dy = v[3*c+1] - v[3*o+1];
+        const double dz = v[3*c+2] - v[3*o+2];
+        return 1.0 - ((dx*n[0] + dy*n[1]) + dz*n[2]);
+    } else {
+        int64_t rc = r;
+        if (r >= s->R / 2) rc = s->antipod[r];
+        const double *n = s->m.rays + 3*rc;
+        return 1.0 - ((v[3*c+0]*n[0] + v[3*c+1]*n[1]) + v[3*c+2]*n[2]);
+    }
+}
+
+/* Geometry::get_next_general (geometry.tpp:328-421) */
+static int64_t o_get_next(const orc_state *s, int64_t origin, int64_t ray, int64_t current,
+                          double *Z, double *dZ)
+{
+    const double *pos = s->m.position;
+    const double *n = s->m.rays + 3*ray;
+    double dmin = DBL_MAX;
+    int64_t next = -1;
+    for (int64_t q = s->nb_off[current]; q < s->nb_off[current+1]; q++) {
+        const int64_t nb = s->m.neighbors[q];
+        const double Rx = pos[3*nb+0] - pos[3*origin+0];
+        const double Ry = pos[3*nb+1] - pos[3*origin+1];
+        const double Rz = pos[3*nb+2] - pos[3*origin+2];
+        const double Z_new = (Rx*n[0] + Ry*n[1]) + Rz*n[2];
+        if (Z_new > *Z) {
+            const double distance_from_ray2 = ((Rx*Rx + Ry*Ry) + Rz*Rz) - Z_new*Z_new;
+            if (distance_from_ray2 < dmin) {
+                dmin = distance_from_ray2;
+                next = nb;
+                *dZ = Z_new - *Z;
+            }
+        }
+    }
+    *Z = *Z + *dZ;
+    return next;
+}
+
+/* Geometry::set_data (geometry.tpp:100-207).  Appends to the output arrays;
+ * returns the new count (unchanged when more than 10000 points are needed). */
+static int64_t o_set_data(int64_t crt, int64_t nxt, double shift_crt, double shift_nxt,
+                          double dZ_loc, double dshift_max, int64_t cnt, int64_t cap,
+                          int64_t *o_crt, int64_t *o_cell, double *o_shift, double *o_dZ)
+{
+    const double dshift     = shift_nxt - shift_crt;
+    const double dshift_abs = fabs(dshift);
+    if (dshift_abs > dshift_max) {
+        const long   n_interpl      = (long)(dshift_abs / dshift_max + 1);
+        const long   half_n_interpl = (long)(0.5 * n_interpl);
+        const double dZ_interpl     = dZ_loc / n_interpl;
+        const double dshift_interpl = dshift / n_interpl;
+        if ((n_interpl > 10000) || (n_interpl < 0)) return cnt;
+        for (long m = 1; m < half_n_interpl; m++) {
+            if (cnt < cap) {
+                if (o_crt) o_crt[cnt] = -1;
+                o_cell[cnt] = crt; o_shift[cnt] = shift_crt + m * dshift_interpl; o_dZ[cnt] = dZ_interpl;
+            }
+            cnt++;
+        }
+        for (long m = half_n_interpl; m <= n_interpl; m++) {
+            if (cnt < cap) {
+                if (o_crt) o_crt[cnt] = -1;
+                o_cell[cnt] = nxt; o_shift[cnt] = shift_crt + m * dshift_interpl; o_dZ[cnt] = dZ_interpl;
+            }
+            cnt++;
+        }
+    } else {
+        if (cnt < cap) {
+            if (o_crt) o_crt[cnt] = crt;
+            o_cell[cnt] = nxt; o_shift[cnt] = shift_nxt; o_dZ[cnt] = dZ_loc;
+        }
+        cnt++;
+    }
+    return cnt;
+}
+
+/* Geometry::trace_ray<CoMoving> (geometry.tpp:21-95) */
+int64_t orc_trace_ray(const orc_state *s, int64_t o, int64_t r, double dshift_max,
+                      int64_t cap, int64_t *crt_out, int64_t *cellNr, double *shift, double *dZs)
+{
+    double Z = 0.0, dZ = 0.0;
+    int64_t cnt = 0;
+    int64_t nxt = o_get_next(s, o, r, o, &Z, &dZ);
+    if (nxt != -1) {
+        int64_t crt = o;
+        double shift_crt = orc_get_doppler_shift(s, o, r, crt, 0);
+        double shift_nxt = orc_get_doppler_shift(s, o, r, nxt, 0);
+        cnt = o_set_data(crt, nxt, shift_crt, shift_nxt, dZ, dshift_max, cnt, cap,
+                         crt_out, cellNr, shift, dZs);
+        while (!s->boundary[nxt]) {
+            shift_crt = shift_nxt;
+            crt = nxt;
+            nxt = o_get_next(s, o, r, nxt, &Z, &dZ);
+            if (nxt < 0) return -1;
+            shift_nxt = orc_get_doppler_shift(s, o, r, nxt, 0);
+            cnt = o_set_data(crt, nxt, shift_crt, shift_nxt, dZ, dshift_max, cnt, cap,
+                             crt_out, cellNr, shift, dZs);
+        }
+    }
+    return cnt;
+}
+
+/* get_required_npoints / Geometry::get_npoints_on_ray (geometry.tpp:470-543) */
+int64_t orc_get_npoints_on_ray(const orc_state *s, int64_t o, int64_t r, double dshift_max)
+{
+    int64_t npoints = 0;
+    double Z = 0.0, dZ = 0.0;
+    int64_t nxt = o_get_next(s, o, r, o, &Z, &dZ);
+    if (nxt != -1) {
+        double shift_crt = orc_get_doppler_shift(s, o, r, o, 0);
+        double shift_nxt = orc_get_doppler_shift(s, o, r, nxt, 0);
+        for (;;) {
+            const double dshift_abs = fabs(shift_nxt - shift_crt);
+            npoints += (dshift_abs > dshift_max) ? (int64_t)(dshift_abs/dshift_max + 1) : 1;
+            if (s->boundary[nxt]) break;
+            shift_crt = shift_nxt;
+            nxt = o_get_next(s, o, r, nxt, &Z, &dZ);
+            if (nxt < 0) return -1;
+            shift_nxt = orc_get_doppler_shift(s, o, r, nxt, 0);
+        }
+    }
+    return npoints;
+}
+
+/* ---- Line emissivity and opacity ---------------------------------------- */
+
+/* Simulation::get_eta_and_chi (src/Simulation/simulation.tpp:385-481) with
+ * get_line_width (:363-373) for a single species.  lnotch is stateful. */
+void orc_get_eta_and_chi(const orc_state *s, double freq_scaled, int64_t p,
+                         int64_t *lnotch, double *eta, double *chi)
+{
+    const int64_t L = s->L;
+    const double pw = orc_profile_width(s, p);
+    *eta = 0.0;
+    *chi = 1.0E-26;
+    const double lower = -11.0, upper = +11.0;
+    double freq_diff = freq_scaled - s->line[*lnotch];
+    double width = s->line[*lnotch] * pw;
+    while (freq_diff > upper*width) {
+        if (*lnotch >= L-1) break;
+        (*lnotch)++;
+        freq_diff = freq_scaled - s->line[*lnotch];
+        width = s->line[*lnotch] * pw;
+    }
+    int64_t lindex = *lnotch;
+    while (freq_diff >= lower*width) {
+        const double line_profile = orc_profile(width, freq_diff);
+        const int64_t ind = s->line_index[lindex] + p*L;
+        *eta += s->emissivity[ind] * freq_scaled * line_profile;
+        *chi += s->opacity[ind] * freq_scaled * line_profile;
+        if (lindex >= L-1) break;
+        lindex++;
+        freq_diff = freq_scaled - s->line[lindex];
+        width = s->line[lindex] * pw;
+    }
+}
+
+/* Radiation::rescale_I_bdy, scalar branch (src/Model/Radiation/radiation.tpp:341-373),
+ * with search_with_notch from notch 0 (interpolation.tpp:96-110) and
+ * interpolate_linear (interpolation.tpp:126-134). */
+static double o_rescale_I_bdy(const orc_state *s, double freq_scaled, int64_t R, int64_t p,
+                              int64_t b)
+{
+    const int64_t F = s->F;
+    const double *nup = s->nu + p*F;
+    const double *Ib = s->I_bdy + (R*s->NB + b)*F;
+    int64_t notch = 0;
+    while (notch < F) { if (freq_scaled <= nup[notch]) break; notch++; }
+    if (notch >= F) notch = F - 1;
+    if ((notch == 0) || (notch == F-1)) return Ib[notch];
+    const double nu1 = nup[notch], nu2 = nup[notch-1];
+    const double I1 = Ib[notch], I2 = Ib[notch-1];
+    return (I2 - I1) / (nu2 - nu1) * (freq_scaled - nu1) + I1;
+}
+
+/* ---- Feautrier solve ---------------------------------------------------- */
+
+/* RayPair::solve (src/Simulation/Raypair/raypair_solver.tpp:200-420) */
+void orc_raypair_solve(int64_t ndep, int64_t n_ar, int64_t first, int64_t last,
+                       int64_t n_off_diag, const double *term1, const double *dtau,
+                       double I_bdy_0, double I_bdy_n,
+                       double *Su, double *L_diag, double *L_upper, double *L_lower,
+                       double *A_out, double *C_out)
+{
+    double *A  = (double *)xcalloc(ndep, sizeof(double));
+    double *C  = (double *)xcalloc(ndep, sizeof(double));
+    double *F  = (double *)xcalloc(ndep, sizeof(double));
+    double *G  = (double *)xcalloc(ndep, sizeof(double));
+    double *iA = (double *)xcalloc(ndep, sizeof(double));
+    double *iC = (double *)xcalloc(ndep, sizeof(double));
+    double *i1F = (double *)xcalloc(ndep, sizeof(double));
+    double *i1G = (double *)xcalloc(ndep, sizeof(double));
+    double *GoG = (double *)xcalloc(ndep, sizeof(double));
+
+    const double inverse_dtau0 = 1.0 / dtau[first];
+    const double inverse_dtaud = 1.0 / dtau[last-1];
+    C[first] = 2.0 * inverse_dtau0 * inverse_dtau0;
+    const double B0_min_C0 = 1.0 + 2.0 * inverse_dtau0;
+    const double B0 = B0_min_C0 + C[first];
+    const double inverse_B0 = 1.0 / B0;
+    Su[first] = term1[first] + 2.0 * I_bdy_0 * inverse_dtau0;
+    for (int64_t n = first+1; n < last; n++) {
+        iA[n] = 0.5 * (dtau[n-1] + dtau[n]) * dtau[n-1];
+        A[n]  = 1.0 / iA[n];
+        iC[n] = 0.5 * (dtau[n-1] + dtau[n]) * dtau[n];
+        C[n]  = 1.0 / iC[n];
+        Su[n] = term1[n];
+    }
+    A[last] = 2.0 * inverse_dtaud * inverse_dtaud;
+    const double Bd_min_Ad = 1.0 + 2.0 * inverse_dtaud;
+    const double Bd = Bd_min_Ad + A[last];
+    Su[last] = term1[last] + 2.0 * I_bdy_n * inverse_dtaud;
+
+    /* elimination */
+    Su[first] = Su[first] * inverse_B0;
+    F[first] = 0.5 * B0_min_C0 * dtau[first] * dtau[first];
+    i1F[first] = 1.0 / (1.0 + F[first]);
+    for (int64_t n = first+1; n < last; n++) {
+        F[n] = (1.0 + A[n]*F[n-1]*i1F[n-1]) * iC[n];
+        i1F[n] = 1.0 / (1.0 + F[n]);
+        Su[n] = (Su[n] + A[n]*Su[n-1]) * i1F[n] * iC[n];
+    }
+    const double denominator = 1.0 / (Bd_min_Ad + Bd*F[last-1]);
+    Su[last] = (Su[last] + A[last]*Su[last-1]) * (1.0 + F[last-1]) * denominator;
+
+    if (n_off_diag == 0) {
+        if (n_ar < last) {
+            G[last] = 0.5 * Bd_min_Ad * dtau[last-1] * dtau[last-1];
+            GoG[last] = G[last] / (1.0 + G[last]);
+            for (int64_t n = last-1; n > n_ar; n--) {
+                Su[n] = Su[n] + Su[n+1] * i1F[n];
+                G[n] = (1.0 + C[n]*GoG[n+1]) * iA[n];
+                GoG[n] = G[n] / (1.0 + G[n]);
+            }
+            Su[n_ar] = Su[n_ar] + Su[n_ar+1] * i1F[n_ar];
+            L_diag[n_ar] = iC[n_ar] / (F[n_ar] + GoG[n_ar+1]);
+        } else {
+            L_diag[last] = (1.0 + F[last-1]) / (Bd_min_Ad + Bd*F[last-1]);
+        }
+    } else {
+        G[last] = 0.5 * Bd_min_Ad * dtau[last-1] * dtau[last-1];
+        i1G[last] = 1.0 / (1.0 + G[last]);
+        for (int64_t n = last-1; n > first; n--) {
+            Su[n] = Su[n] + Su[n+1] * i1F[n];
+            G[n] = (1.0 + C[n]*G[n+1]*i1G[n+1]) * iA[n];
+            i1G[n] = 1.0 / (1.0 + G[n]);
+        }
+        Su[first] = Su[first] + Su[first+1] * i1F[first];
+        L_diag[last] = (1.0 + F[last-1]) / (Bd_min_Ad + Bd*F[last-1]);
+        for (int64_t n = last-1; n >= first+1; n--)
+            L_diag[n] = iC[n] / (F[n] + G[n+1] * i1G[n+1]);
+        L_diag[first] = (1.0 + G[first+1]) / (B0_min_C0 + B0*G[first+1]);
+        for (int64_t n = last-1; n >= first; n--) {
+            L_upper[0*ndep + n] = L_diag[n+1] * i1F[n];
+            L_lower[0*ndep + n] = L_diag[n] * i1G[n+1];
+        }
+        for (int64_t m = 1; m < n_off_diag; m++)
+            for (int64_t n = last-1-m; n >= first; n--) {
+                L_upper[m*ndep + n] = L_upper[(m-1)*ndep + n+1] * i1F[n];
+                L_lower[m*ndep + n] = L_lower[(m-1)*ndep + n] * i1G[n+m+1];
+            }
+    }
+    if (A_out) for (int64_t n = 0; n < ndep; n++) A_out[n] = A[n];
+    if (C_out) for (int64_t n = 0; n < ndep; n++) C_out[n] = C[n];
+    free(A); free(C); free(F); free(G); free(iA); free(iC); free(i1F); free(i1G); free(GoG);
+}
+
+/* ---- Radiation field ---------------------------------------------------- */
+
+typedef struct {
+    int64_t cap;
+    int64_t *cell_r, *cell_ar; double *shift_r, *shift_ar, *dZ_r, *dZ_ar;
+    int64_t *lnotch_r, *lnotch_ar;
+    double *term1, *chi, *frs, *dtau, *Su, *L_diag;
+    int64_t *nrs;
+} o_pair_buf;
+
+static void o_buf_alloc(o_pair_buf *b, int64_t cap)
+{
+    b->cap = cap;
+    b->cell_r = (int64_t *)xcalloc(cap, 8); b->cell_ar = (int64_t *)xcalloc(cap, 8);
+    b->shift_r = (double *)xcalloc(cap, 8); b->shift_ar = (double *)xcalloc(cap, 8);
+    b->dZ_r = (double *)xcalloc(cap, 8);    b->dZ_ar = (double *)xcalloc(cap, 8);
+    b->lnotch_r = (int64_t *)xcalloc(cap, 8); b->lnotch_ar = (int64_t *)xcalloc(cap, 8);
+    const int64_t d = 2*cap + 1;
+    b->term1 = (double *)xcalloc(d, 8); b->chi = (double *)xcalloc(d, 8);
+    b->frs = (double *)xcalloc(d, 8);   b->dtau = (double *)xcalloc(d, 8);
+    b->Su = (double *)xcalloc(d, 8);    b->L_diag = (double *)xcalloc(d, 8);
+    b->nrs = (int64_t *)xcalloc(d, 8);
+}
+static void o_buf_free(o_pair_buf *b)
+{
+    free(b->cell_r); free(b->cell_ar); free(b->shift_r); free(b->shift_ar); free(b->dZ_r);
+    free(b->dZ_ar); free(b->lnotch_r); free(b->lnotch_ar); free(b->term1); free(b->chi);
+    free(b->frs); free(b->dtau); free(b->Su); free(b->L_diag); free(b->nrs);
+}
+
+/* Simulation::setup (src/Simulation/simulation.tpp:185-353) followed by
+ * RayPair::solve (n_off_diag = 0).  Side "ar" is the longer ray.  Returns u. */
+static double o_setup_and_solve(const orc_state *s, int64_t R, int64_t origin, int64_t f,
+                                int64_t n_ar, const int64_t *c_ar, const double *s_ar,
+                                const double *z_ar, int64_t *ln_ar,
+                                int64_t n_r, const int64_t *c_r, const double *s_r,
+                                const double *z_r, int64_t *ln_r,
+                                int64_t *lnotch_at_origin, o_pair_buf *b, double *L_origin,
+                                double *chi_origin)
+{
+    const int64_t F = s->F;
+    const int64_t ndep = n_ar + n_r + 1;
+    int64_t first = 0, last = n_ar;
+    double eta, chi, I_bdy_0, I_bdy_n;
+    double freq_scaled = s->nu[origin*F + f];
+    int64_t index = n_ar;
+
+    orc_get_eta_and_chi(s, freq_scaled, origin, lnotch_at_origin, &eta, &chi);
+    b->term1[index] = eta / chi;
+    b->chi[index] = chi; b->nrs[index] = origin; b->frs[index] = freq_scaled;
+    double chi_prev = chi;
+    const double chi_o = chi;
+
+    if (n_ar > 0) {
+        index = n_ar - 1;
+        for (int64_t i = 0; i < n_ar; i++) {
+            freq_scaled = s_ar[i] * s->nu[origin*F + f];
+            orc_get_eta_and_chi(s, freq_scaled, c_ar[i], &ln_ar[i], &eta, &chi);
+            b->term1[index] = eta / chi;
+            b->dtau[index] = 0.5 * (chi + chi_prev) * z_ar[i];
+            b->chi[index] = chi; b->nrs[index] = c_ar[i]; b->frs[index] = freq_scaled;
+            chi_prev = chi;
+            index--;
+        }
+        const int64_t cellNr = c_ar[n_ar-1];
+        I_bdy_0 = o_rescale_I_bdy(s, freq_scaled, R, cellNr, s->cell2boundary_nr[cellNr]);
+        first = index + 1;
+        chi_prev = chi_o;
+    } else {
+        I_bdy_0 = s->I_bdy[(R*s->NB + s->cell2boundary_nr[origin])*F + f];
+    }
+
+    if (n_r > 0) {
+        index = n_ar + 1;
+        for (int64_t i = 0; i < n_r; i++) {
+            freq_scaled = s_r[i] * s->nu[origin*F + f];
+            orc_get_eta_and_chi(s, freq_scaled, c_r[i], &ln_r[i], &eta, &chi);
+            b->term1[index] = eta / chi;
+            b->dtau[index-1] = 0.5 * (chi + chi_prev) * z_r[i];
+            b->chi[index] = chi; b->nrs[index] = c_r[i]; b->frs[index] = freq_scaled;
+            chi_prev = chi;
+            index++;
+        }
+        const int64_t cellNr = c_r[n_r-1];
+        I_bdy_n = o_rescale_I_bdy(s, freq_scaled, R, cellNr, s->cell2boundary_nr[cellNr]);
+        last = index - 1;
+    } else {
+        I_bdy_n = s->I_bdy[(R*s->NB + s->cell2boundary_nr[origin])*F + f];
+    }
+
+    orc_raypair_solve(ndep, n_ar, first, last, 0, b->term1, b->dtau, I_bdy_0, I_bdy_n,
+                      b->Su, b->L_diag, NULL, NULL, NULL, NULL);
+    *L_origin = b->L_diag[n_ar];
+    *chi_origin = b->chi[n_ar];
+    return b->Su[n_ar];
+}
+
+/* Simulation::compute_radiation_field (src/Simulation/sim_radiation.tpp:56-338),
+ * RayPair branch, n_off_diag = 0, with RayPair::update_Lambda
+ * (raypair_lambda.tpp:94-149) and Lambda::add_element (lambda.tpp:130-143). */
+int orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, int64_t o_end)
+{
+    const int64_t F = s->F, L = s->L;
+    if (o_begin < 0) o_begin = 0;
+    if (o_end > s->N || o_end < 0) o_end = s->N;
+    for (int64_t p = o_begin; p < o_end; p++) {
+        for (int64_t f = 0; f < F; f++) s->J[p*F + f] = 0.0;    /* initialize_J */
+        for (int64_t k = 0; k < L; k++) s->Ls[p*L + k] = 0.0;   /* lambda.clear */
+    }
+    int err = 0;
+    if (nthreads <= 0) nthreads = orc_nthreads_max();
+    for (int64_t r = 0; r < s->R / 2; r++) {
+        const int64_t R = r;
+        #pragma omp parallel num_threads(nthreads) reduction(|:err)
+        {
+            o_pair_buf b;
+            int64_t cap = 256;
+            o_buf_alloc(&b, cap);
+            int tid = 0, nth = 1;
+#ifdef _OPENMP
+            tid = omp_get_thread_num(); nth = omp_get_num_threads();
+#endif
+            for (int64_t o = o_begin + tid; o < o_end; o += nth) {
+                const int64_t ar = s->antipod[r];
+                const double weight_ang = s->m.weights[r];
+                const double dshift_max = orc_get_dshift_max(s, o);
+                int64_t n_r, n_ar_t;
+                for (;;) {
+                    n_r = orc_trace_ray(s, o, r, dshift_max, b.cap, NULL, b.cell_r, b.shift_r, b.dZ_r);
+                    n_ar_t = orc_trace_ray(s, o, ar, dshift_max, b.cap, NULL, b.cell_ar, b.shift_ar, b.dZ_ar);
+                    if (n_r > b.cap || n_ar_t > b.cap) {
+                        cap = 2 * (n_r > n_ar_t ? n_r : n_ar_t);
+                        o_buf_free(&b); o_buf_alloc(&b, cap);
+                        continue;
+                    }
+                    break;
+                }
+                if (n_r < 0 || n_ar_t < 0) { err = 1; continue; }
+                for (int64_t i = 0; i < n_r; i++) b.lnotch_r[i] = 0;
+                for (int64_t i = 0; i < n_ar_t; i++) b.lnotch_ar[i] = 0;
+                /* the longer ray goes on the ar side (sim_radiation.tpp:164-171) */
+                const int r_longer = (n_r > n_ar_t);
+                const int64_t nA = r_longer ? n_r : n_ar_t, nB = r_longer ? n_ar_t : n_r;
+                int64_t *cA = r_longer ? b.cell_r : b.cell_ar, *cB = r_longer ? b.cell_ar : b.cell_r;
+                double *sA = r_longer ? b.shift_r : b.shift_ar, *sB = r_longer ? b.shift_ar : b.shift_r;
+                double *zA = r_longer ? b.dZ_r : b.dZ_ar, *zB = r_longer ? b.dZ_ar : b.dZ_r;
+                int64_t *lA = r_longer ? b.lnotch_r : b.lnotch_ar, *lB = r_longer ? b.lnotch_ar : b.lnotch_r;
+                const int64_t ndep = nA + nB + 1;
+                if (ndep > 1) {
+                    int64_t lnotch_at_origin = 0;
+                    for (int64_t f = 0; f < F; f++) {
+                        double Lo, chio;
+                        const double u = o_setup_and_solve(s, R, o, f, nA, cA, sA, zA, lA,
+                                                           nB, cB, sB, zB, lB,
+                                                           &lnotch_at_origin, &b, &Lo, &chio);
+                        if (s->appears[f]) {
+                            const int64_t k = s->corr_k[f], z = s->corr_z[f];
+                            const double freq_line = s->m.frequency[k];
+                            const double weight = s->m.qweights[z] * 2.0 * weight_ang;
+                            const double factor = s->m.A[k] * weight;
+                            const double frs = s->nu[o*F + f];
+                            const double prof = orc_profile(freq_line * orc_profile_width(s, o),
+                                                            frs - freq_line);
+                            const double Lel = factor * (frs * prof * Lo / chio);
+                            s->Ls[o*L + k] += Lel;
+                        }
+                        s->J[o*F + f] += 2.0 * weight_ang * u;
+                    }
+                } else {
+                    const int64_t bb = s->cell2boundary_nr[o];
+                    for (int64_t f = 0; f < F; f++) {
+                        const double I = s->I_bdy[(R*s->NB + bb)*F + f];
+                        const double u = 0.5 * (I + I);
+                        s->J[o*F + f] += 2.0 * weight_ang * u;
+                    }
+                }
+            }
+            o_buf_free(&b);
+        }
+    }
+    return err ? -1 : 0;
+}
+
+/* ---- Level populations -------------------------------------------------- */
+
+/* Simulation::compute_Jeff (src/Simulation/sim_lines.tpp:176-220), n_off_diag = 0 */
+void orc_compute_Jeff(orc_state *s)
+{
+    const int64_t F = s->F, L = s->L, Q = s->Q, V = s->V;
+    #pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < s->N; p++)
+        for (int64_t k = 0; k < L; k++) {
+            double Jlin = 0.0;
+            for (int64_t z = 0; z < Q; z++)
+                Jlin += s->m.qweights[z] * s->J[p*F + s->nr_line[(p*L + k)*Q + z]];
+            double diff = 0.0;
+            diff += s->Ls[p*L + k] * s->pop[p*V + s->m.irad[k]];
+            s->Jlin[p*L + k] = Jlin;
+            s->Jeff[p*L + k] = Jlin - HH_OVER_FOUR_PI * diff;
+            s->Jdif[p*L + k] = HH_OVER_FOUR_PI * diff;
+        }
+}
+
+/* Dense LU with partial pivoting, row-major M [n][n], solves M x = y in place. */
+static int o_lu_solve(int64_t n, double *M, double *y)
+{
+    for (int64_t c = 0; c < n; c++) {
+        int64_t piv = c; double best = fabs(M[c*n + c]);
+        for (int64_t r = c+1; r < n; r++) if (fabs(M[r*n + c]) > best) { best = fabs(M[r*n + c]); piv = r; }
+        if (best == 0.0) return -1;
+        if (piv != c) {
+            for (int64_t j = 0; j < n; j++) { const double t = M[c*n+j]; M[c*n+j] = M[piv*n+j]; M[piv*n+j] = t; }
+            const double t = y[c]; y[c] = y[piv]; y[piv] = t;
+        }
+        for (int64_t r = c+1; r < n; r++) {
+            const double l = M[r*n + c] / M[c*n + c];
+            M[r*n + c] = l;
+            for (int64_t j = c+1; j < n; j++) M[r*n + j] -= l * M[c*n + j];
+            y[r] -= l * y[c];
+        }
+    }
+    for (int64_t r = n-1; r >= 0; r--) {
+        double acc = y[r];
+        for (int64_t j = r+1; j < n; j++) acc -= M[r*n + j] * y[j];
+        y[r] = acc / M[r*n + r];
+    }
+    return 0;
+}
+
+/* LineProducingSpecies::update_using_statistical_equilibrium
+ * (lineProducingSpecies.tpp:249-495): history shift, per-cell assembly of the
+ * same triplets (transpose convention, last row = conservation), one dense LU
+ * per cell in place of the global SparseLU (block diagonal for n_off_diag = 0). */
+int orc_update_using_statistical_equilibrium(orc_state *s)
+{
+    const mgr_model_view *m = &s->m;
+    const int64_t V = s->V, L = s->L, NV = s->N * s->V;
+    memcpy(s->prev3, s->prev2, NV * sizeof(double));
+    memcpy(s->prev2, s->prev1, NV * sizeof(double));
+    memcpy(s->prev1, s->pop,   NV * sizeof(double));
+    int err = 0;
+    int64_t ncol_max = 1;
+    for (int64_t c = 0; c < m->ncolpar; c++) if (m->colpar_ncol[c] > ncol_max) ncol_max = m->colpar_ncol[c];
+    #pragma omp parallel reduction(|:err)
+    {
+        double *M = (double *)xcalloc(V*V, sizeof(double));
+        double *y = (double *)xcalloc(V, sizeof(double));
+        double *Ce = (double *)xcalloc(ncol_max, sizeof(double));
+        double *Cd = (double *)xcalloc(ncol_max, sizeof(double));
+        #pragma omp for schedule(static)
+        for (int64_t p = 0; p < s->N; p++) {
+            for (int64_t q = 0; q < V*V; q++) M[q] = 0.0;
+            for (int64_t q = 0; q < V; q++) y[q] = 0.0;
+            const double *pop = s->prev1 + p*V;   /* == population before the solve */
+            /* radiative transitions */
+            for (int64_t k = 0; k < L; k++) {
+                const double Jeff = s->Jeff[p*L + k];
+                const double v_IJ = m->A[k] + m->Bs[k] * Jeff;
+                const double v_JI = m->Ba[k] * Jeff;
+                const int64_t I = m->irad[k], J = m->jrad[k];
+                if (J != V-1) { M[J*V + I] += v_IJ; M[J*V + J] += -v_JI; }
+                if (I != V-1) { M[I*V + J] += v_JI; M[I*V + I] += -v_IJ; }
+            }
+            /* approximated Lambda operator, kappa from the current populations */
+            for (int64_t k = 0; k < L; k++) {
+                const int64_t I = m->irad[k], J = m->jrad[k];
+                const double opacity = HH_OVER_FOUR_PI * (pop[J] * m->Ba[k] - pop[I] * m->Bs[k]);
+                const double v_IJ = -s->Ls[p*L + k] * opacity;
+                if (J != V-1) M[J*V + I] += v_IJ;
+                if (I != V-1) M[I*V + I] += -v_IJ;
+            }
+            /* collisional transitions */
+            int64_t koff = 0;
+            for (int64_t c = 0; c < m->ncolpar; c++) {
+                double factor;
+                const double tmp = m->temperature[p];
+                orc_collision_rates(s, c, tmp, Ce, Cd, &factor);
+                const double abn = m->abundance[p*m->nspecs + m->colpar_num[c]] * factor;
+                for (int64_t k = 0; k < m->colpar_ncol[c]; k++) {
+                    const double v_IJ = Cd[k] * abn, v_JI = Ce[k] * abn;
+                    const int64_t I = m->icol[koff + k], J = m->jcol[koff + k];
+                    if (J != V-1) { M[J*V + I] += v_IJ; M[J*V + J] += -v_JI; }
+                    if (I != V-1) { M[I*V + J] += v_JI; M[I*V + I] += -v_IJ; }
+                }
+                koff += m->colpar_ncol[c];
+            }
+            for (int64_t i = 0; i < V; i++) M[(V-1)*V + i] += 1.0;
+            y[V-1] = s->pop_tot[p];
+            if (o_lu_solve(V, M, y) != 0) { err = 1; continue; }
+            for (int64_t i = 0; i < V; i++) s->pop[p*V + i] = y[i];
+        }
+        free(M); free(y); free(Ce); free(Cd);
+    }
+    return err ? -1 : 0;
+}
+
+/* LineProducingSpecies::check_for_convergence (lineProducingSpecies.tpp:120-171) */
+void orc_check_for_convergence(orc_state *s, double pop_prec)
+{
+    const int64_t V = s->V;
+    const double weight = 1.0 / (double)(s->N * V);
+    double fnc = 0.0, rcm = 0.0, rcmax = 0.0;
+    #pragma omp parallel for reduction(+:fnc, rcm) reduction(max:rcmax)
+    for (int64_t p = 0; p < s->N; p++) {
+        const double min_pop = 1.0E-10 * s->pop_tot[p];
+        for (int64_t i = 0; i < V; i++) {
+            const int64_t ind = p*V + i;
+            if (s->pop[ind] > min_pop) {
+                double relative_change = 2.0;
+                relative_change *= fabs(s->pop[ind] - s->prev1[ind]);
+                relative_change /= (s->pop[ind] + s->prev1[ind]);
+                if (relative_change > pop_prec) fnc += weight;
+                rcm += (weight * relative_change);
+                if (relative_change > rcmax) rcmax = relative_change;
+            }
+        }
+    }
+    s->fnc = fnc; s->rcm = rcm; s->rcmax = rcmax;
+}
+
+/* LineProducingSpecies::update_using_Ng_acceleration (lineProducingSpecies.tpp:181-238) */
+void orc_update_using_Ng_acceleration(orc_state *s)
+{
+    const int64_t NV = s->N * s->V;
+    double A1 = 0.0, A2 = 0.0, B2 = 0.0, C1 = 0.0, C2 = 0.0;
+    for (int64_t i = 0; i < NV; i++) {
+        const double x = s->pop[i], x1 = s->prev1[i], x2 = s->prev2[i], x3 = s->prev3[i];
+        const double Q1 = x - 2.0*x1 + x2;
+        const double Q2 = x - x1 - x2 + x3;
+        const double Q3 = x - x1;
+        A1 += Q1*Q1; A2 += Q1*Q2; B2 += Q2*Q2; C1 += Q1*Q3; C2 += Q2*Q3;
+    }
+    const double B1 = A2;
+    const double denominator = A1*B2 - A2*B1;
+    if (denominator != 0.0) {
+        const double a = (C1*B2 - C2*B1) / denominator;
+        const double b = (C2*A1 - C1*A2) / denominator;
+        for (int64_t i = 0; i < NV; i++) {
+            const double x = s->pop[i];
+            s->pop[i] = (1.0 - a - b)*x + a*s->prev1[i] + b*s->prev2[i];
+            s->prev3[i] = s->prev2[i];
+            s->prev2[i] = s->prev1[i];
+            s->prev1[i] = x;
+        }
+    }
+}
+
+/* Simulation::compute_level_populations (src/Simulation/sim_lines.tpp:61-164) */
+int64_t orc_compute_level_populations(orc_state *s, int use_Ng, int64_t max_niterations,
+                                      int criterion, int nthreads)
+{
+    if (!s->spec_set) return -1;
+    int64_t iteration = 0, iteration_normal = 0;
+    int some_not_converged = 1;
+    while (some_not_converged && (iteration < max_niterations)) {
+        iteration++;
+        some_not_converged = 0;
+        if (use_Ng && (iteration_normal == 4)) {
+            orc_update_using_Ng_acceleration(s);
+            orc_check_for_convergence(s, s->m.pop_prec);
+            orc_set_emissivity_and_opacity(s);
+            iteration_normal = 0;
+        } else {
+            if (orc_compute_radiation_field(s, nthreads, 0, s->N) != 0) return -1;
+            orc_compute_Jeff(s);
+            if (orc_update_using_statistical_equilibrium(s) != 0) return -1;
+            orc_check_for_convergence(s, s->m.pop_prec);
+            orc_set_emissivity_and_opacity(s);
+            iteration_normal++;
+        }
+        if (criterion == MGR_CRIT_MAX) {
+            if (s->rcmax >= s->m.pop_prec) some_not_converged = 1;
+        } else {
+            if (s->fnc > 0.005) some_not_converged = 1;
+        }
+    }
+    return iteration;
+}
+
+/* ---- accessors ---------------------------------------------------------- */
+void orc_get_frequencies(const orc_state *s, double *nu, int64_t *nr_line)
+{
+    if (nu) memcpy(nu, s->nu, s->N * s->F * sizeof(double));
+    if (nr_line) memcpy(nr_line, s->nr_line, s->N * s->L * s->Q * sizeof(int64_t));
+}
+void orc_get_boundary_intensities(const orc_state *s, double *I_bdy)
+{
+    memcpy(I_bdy, s->I_bdy, s->NB * s->F * sizeof(double));
+}
+void orc_get_J(const orc_state *s, double *J) { memcpy(J, s->J, s->N * s->F * sizeof(double)); }
+void orc_get_lambda(const orc_state *s, double *Ls) { memcpy(Ls, s->Ls, s->N * s->L * sizeof(double)); }
+void orc_get_Jeff(const orc_state *s, double *Jlin, double *Jeff, double *Jdif)
+{
+    const size_t n = s->N * s->L * sizeof(double);
+    if (Jlin) memcpy(Jlin, s->Jlin, n);
+    if (Jeff) memcpy(Jeff, s->Jeff, n);
+    if (Jdif) memcpy(Jdif, s->Jdif, n);
+}
+void orc_get_populations(const orc_state *s, double *pop) { memcpy(pop, s->pop, s->N * s->V * sizeof(double)); }
+void orc_set_populations(orc_state *s, const double *pop)
+{
+    memcpy(s->pop, pop, s->N * s->V * sizeof(double));
+    for (int64_t p = 0; p < s->N; p++) s->pop_tot[p] = s->m.abundance[p*s->m.nspecs + s->m.num];
+    orc_set_emissivity_and_opacity(s);
+}
+void orc_get_population_history(const orc_state *s, double *p1, double *p2, double *p3)
+{
+    const size_t n = s->N * s->V * sizeof(double);
+    if (p1) memcpy(p1, s->prev1, n);
+    if (p2) memcpy(p2, s->prev2, n);
+    if (p3) memcpy(p3, s->prev3, n);
+}
+void orc_set_population_history(orc_state *s, const double *p1, const double *p2, const double *p3)
+{
+    const size_t n = s->N * s->V * sizeof(double);
+    if (p1) memcpy(s->prev1, p1, n);
+    if (p2) memcpy(s->prev2, p2, n);
+    if (p3) memcpy(s->prev3, p3, n);
+}
+void orc_get_emissivity_opacity(const orc_state *s, double *emis, double *opac)
+{
+    const size_t n = s->N * s->L * sizeof(double);
+    if (emis) memcpy(emis, s->emissivity, n);
+    if (opac) memcpy(opac, s->opacity, n);
+}
+void orc_get_convergence(const orc_state *s, double *fnc, double *rcm, double *rcmax)
+{
+    *fnc = s->fnc; *rcm = s->rcm; *rcmax = s->rcmax;
+}
+void orc_get_antipod(const orc_state *s, int64_t *antipod) { memcpy(antipod, s->antipod, s->R * sizeof(int64_t)); }
+void orc_get_lines(const orc_state *s, double *line, int64_t *line_index)
+{
+    memcpy(line, s->line, s->L * sizeof(double));
+    memcpy(line_index, s->line_index, s->L * sizeof(int64_t));
+}
+void orc_get_line_maps(const orc_state *s, int64_t *appears, int64_t *l, int64_t *k, int64_t *z)
+{
+    for (int64_t f = 0; f < s->F; f++) {
+        appears[f] = s->appears[f]; l[f] = s->corr_l[f]; k[f] = s->corr_k[f]; z[f] = s->corr_z[f];
+    }
+}

@@ -1,0 +1,151 @@
+"""GPU parity: the HIP library (through its C-ABI) against the FP64 CPU oracle
+on the same seeded models.
+
+Bars (from the workload definition):
+  * traced cell sequences, shifts and dZ: bit-exact;
+  * spectral grid (nu, nr_line): bit-exact; boundary intensities and LTE
+    populations within 1e-14 relative (device exp/expm1 vs glibc);
+  * J: within 1e-5 relative for the FP32 solve, 1e-10 for the FP64 solve;
+  * Lambda* diagonal: 1e-5 (FP32) / 1e-10 (FP64) relative;
+  * converged populations: within 1e-5 relative at the 1e-6 criterion.
+"""
+import numpy as np
+import pytest
+
+from magritte_amd.setup import synthetic
+from oracle.py_oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+J_TOL = {"fp32": 1e-5, "fp64": 1e-10}
+
+
+def _ctx(model):
+    from magritte_amd.hip import Context
+    return Context(model).setup()
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+MODELS = {
+    "regular16": lambda: synthetic.regular_grid_model(n=16),
+    "envelope8": lambda: synthetic.envelope_model(n=8, seed=7),
+    "stress8": lambda: synthetic.stress_model(n=8, seed=11),
+}
+
+
+@pytest.fixture(scope="module", params=list(MODELS))
+def pair(request):
+    print(f"\n[parity] model {request.param}", flush=True)
+    model = MODELS[request.param]()
+    orc = Oracle(model).setup()
+    ctx = _ctx(model)
+    ctx.trace()
+    return request.param, model, orc, ctx
+
+
+def test_spectral_discretisation_bit_exact(pair):
+    _, model, orc, ctx = pair
+    nu_o, nr_o = orc.frequencies()
+    nu_g, nr_g = ctx.frequencies()
+    assert np.array_equal(nr_o, nr_g)
+    assert np.array_equal(nu_o, nu_g)
+
+
+def test_boundary_intensities(pair):
+    _, model, orc, ctx = pair
+    assert rel_err(ctx.boundary_intensities(), orc.boundary_intensities()) < 1e-14
+
+
+def test_lte_populations(pair):
+    _, model, orc, ctx = pair
+    assert rel_err(ctx.populations(), orc.populations()) < 1e-14
+    eo, oo = orc.emissivity_opacity()
+    eg, og = ctx.emissivity_opacity()
+    assert rel_err(eg, eo) < 1e-13
+    assert np.max(np.abs(og - oo) / np.maximum(np.abs(oo), 1e-300)) < 1e-12
+
+
+def test_npoints_on_rays_match(pair):
+    _, model, orc, ctx = pair
+    npg = ctx.npoints_on_rays()
+    ant = orc.antipod()
+    for r in range(model.nrays // 2):
+        for o in range(0, model.ncells, max(1, model.ncells // 97)):
+            n = orc.npoints_on_ray(o, r) + orc.npoints_on_ray(o, int(ant[r])) + 1
+            assert npg[r, o] == n, (r, o)
+
+
+def test_traced_sequences_bit_exact(pair):
+    name, model, orc, ctx = pair
+    rng = np.random.default_rng(3)
+    origins = rng.choice(model.ncells, size=min(model.ncells, 60), replace=False)
+    checked = 0
+    for o in origins:
+        for r in range(model.nrays):
+            _, c_o, s_o, z_o = orc.trace_ray(int(o), r)
+            c_g, s_g, z_g = ctx.ray(int(o), r)
+            assert np.array_equal(c_o, c_g), (o, r)
+            assert np.array_equal(s_o, s_g), (o, r)
+            assert np.array_equal(z_o, z_g), (o, r)
+            checked += len(c_o)
+    assert checked > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_radiation_field(pair, precision):
+    name, model, orc, ctx = pair
+    orc.compute_radiation_field()
+    ctx.compute_radiation_field(precision)
+    J_o, J_g = orc.J(), ctx.J()
+    err = rel_err(J_g, J_o)
+    print(f"[parity] {name} {precision} J max rel err {err:.3e}", flush=True)
+    assert err < J_TOL[precision]
+    ctx.compute_Jeff()
+    orc.compute_Jeff()
+    L_o, L_g = orc.lambda_diag(), ctx.lambda_diag()
+    mask = np.abs(L_o) > 0
+    errL = rel_err(L_g[mask], L_o[mask])
+    print(f"[parity] {name} {precision} Lambda* max rel err {errL:.3e}", flush=True)
+    assert errL < J_TOL[precision]
+    assert np.all(L_g[~mask] == 0.0)
+    Jl_o, Je_o, Jd_o = orc.Jeff()
+    Jl_g, Je_g, _ = ctx.Jeff()
+    assert rel_err(Jl_g, Jl_o) < J_TOL[precision]
+    # J_eff = J_lin - J_dif cancels in optically thick cells: its error is
+    # bounded relative to the terms of the difference, not to the result.
+    scale = np.maximum(np.abs(Jl_o), np.abs(Jd_o))
+    assert float(np.max(np.abs(Je_g - Je_o) / scale)) < 2 * J_TOL[precision]
+
+
+def test_one_statistical_equilibrium_step(pair):
+    name, model, orc, ctx = pair
+    ctx.set_populations(orc.populations())
+    orc.compute_radiation_field()
+    orc.compute_Jeff()
+    orc.update_using_statistical_equilibrium()
+    conv_o = orc.check_for_convergence()
+    ctx.compute_radiation_field("fp64")
+    ctx.compute_Jeff()
+    conv_g = ctx.compute_level_populations_from_stateq()
+    err = rel_err(ctx.populations(), orc.populations())
+    print(f"[parity] {name} stateq pops max rel err {err:.3e}", flush=True)
+    assert err < 1e-9
+    assert abs(conv_g[1] - conv_o[1]) <= 1e-6 * max(conv_o[1], 1e-30)
+
+
+def test_converged_populations(pair):
+    name, model, orc0, _ = pair
+    orc = Oracle(model).setup()
+    it_o = orc.compute_level_populations(True, 200, 1)
+    ctx = _ctx(model)
+    it_g = ctx.compute_level_populations(True, 200, "max", "fp32")
+    err = rel_err(ctx.populations(), orc.populations())
+    print(f"[parity] {name} converged pops: oracle {it_o} it, GPU fp32 {it_g} it, max rel err {err:.3e}",
+          flush=True)
+    assert err < 1e-5
+    assert abs(it_g - it_o) <= 2
