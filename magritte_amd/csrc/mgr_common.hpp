@@ -126,8 +126,11 @@ struct mgr_ctx {
     double  *W11 = nullptr;         // [N*L] 11 * width of sorted line l
     double  *IW64 = nullptr;        // [N*L] 1 / width
     float   *IW32 = nullptr;        // [N*L]
-    double  *E64 = nullptr, *K64 = nullptr;   // [N*L] emissivity/opacity prefactors (scaled)
-    float   *E32 = nullptr, *K32 = nullptr;   // [N*L]
+    // per (cell, sorted line) rows read by the radiation-field kernel:
+    // FP32 (1/width, E, K, 0), FP64 (11 width, 1/width, E, K); E and K are the
+    // scaled emissivity / opacity prefactors, refreshed with the populations.
+    float4  *row32 = nullptr;       // [N*L]
+    double4 *row64 = nullptr;       // [N*L]
     double   pw_max = 0.0;
 
     // device: spectral discretisation
@@ -161,8 +164,8 @@ struct mgr_ctx {
     int32_t *seg_cell = nullptr;    // [total segments]
     double  *seg_shift = nullptr, *seg_dZ = nullptr;
     int64_t  total_segments = 0;
-    uint64_t *shift_min_bits = nullptr;  // min shift over all points (FP64 bits)
-    double   shift_min = 1.0;
+    uint64_t *shift_bits = nullptr;  // [2] min and max shift over all steps (FP64 bits)
+    double   shift_min = 1.0, shift_max = 1.0;
     bool     traced = false;
     int64_t  sum_ndep = 0, n_pairs_solved = 0;
 

@@ -166,10 +166,9 @@ __global__ void k_emissivity_opacity(int64_t N, const LineTables *__restrict__ t
                                      const double *__restrict__ A, const double *__restrict__ Bs,
                                      const double *__restrict__ Ba, const int32_t *__restrict__ irad,
                                      const int32_t *__restrict__ jrad, const double *__restrict__ pop,
-                                     const double *__restrict__ IW64, double *__restrict__ emis,
-                                     double *__restrict__ opac, double *__restrict__ E64,
-                                     double *__restrict__ K64, float *__restrict__ E32,
-                                     float *__restrict__ K32)
+                                     const double *__restrict__ IW64, const double *__restrict__ W11,
+                                     double *__restrict__ emis, double *__restrict__ opac,
+                                     float4 *__restrict__ row32, double4 *__restrict__ row64)
 {
     const int L = tb->L, V = tb->V;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -185,10 +184,9 @@ __global__ void k_emissivity_opacity(int64_t N, const LineTables *__restrict__ t
     const double pref = INVERSE_SQRT_PI * IW64[p * L + l];
     const double es = e * (CHI_SCALE * I_SCALE) * pref;
     const double ks = o * CHI_SCALE * pref;
-    E64[p * L + l] = es;
-    K64[p * L + l] = ks;
-    E32[p * L + l] = (float)es;
-    K32[p * L + l] = (float)ks;
+    const double iw = IW64[p * L + l];
+    row64[p * L + l] = make_double4(W11[p * L + l], iw, es, ks);
+    row32[p * L + l] = make_float4((float)iw, (float)es, (float)ks, 0.0f);
 }
 
 namespace mgr {
@@ -208,8 +206,8 @@ int launch_emissivity_opacity(mgr_ctx *ctx)
 {
     hipLaunchKernelGGL(k_emissivity_opacity, dim3(grid_for(ctx->N * ctx->L, 256)), dim3(256), 0,
                        ctx->stream, ctx->N, ctx->tables, ctx->A, ctx->Bs, ctx->Ba, ctx->irad,
-                       ctx->jrad, ctx->pop, ctx->IW64, ctx->emis, ctx->opac, ctx->E64, ctx->K64,
-                       ctx->E32, ctx->K32);
+                       ctx->jrad, ctx->pop, ctx->IW64, ctx->W11, ctx->emis, ctx->opac, ctx->row32,
+                       ctx->row64);
     MGR_LAUNCH_CHECK();
     return MGR_OK;
 }
@@ -346,10 +344,8 @@ int upload_model(mgr_ctx *ctx, const mgr_model_view *m)
     if ((rc = dalloc(ctx, &ctx->W11, N * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->IW64, N * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->IW32, N * L))) return rc;
-    if ((rc = dalloc(ctx, &ctx->E64, N * L))) return rc;
-    if ((rc = dalloc(ctx, &ctx->K64, N * L))) return rc;
-    if ((rc = dalloc(ctx, &ctx->E32, N * L))) return rc;
-    if ((rc = dalloc(ctx, &ctx->K32, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->row32, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->row64, N * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->nu, N * F))) return rc;
     if ((rc = dalloc(ctx, &ctx->nr_line, N * L * Q))) return rc;
     if ((rc = dalloc(ctx, &ctx->I_bdy, NB * F))) return rc;
@@ -369,7 +365,7 @@ int upload_model(mgr_ctx *ctx, const mgr_model_view *m)
     if ((rc = dalloc(ctx, &ctx->err_flag, 1))) return rc;
     if ((rc = dalloc(ctx, &ctx->partials, (size_t)MAX_PARTIALS * 8))) return rc;
     if ((rc = dalloc(ctx, &ctx->reduced, 8))) return rc;
-    if ((rc = dalloc(ctx, &ctx->shift_min_bits, 1))) return rc;
+    if ((rc = dalloc(ctx, &ctx->shift_bits, 2))) return rc;
     MGR_HIP(hipHostMalloc((void **)&ctx->h_reduced, 8 * sizeof(double), hipHostMallocDefault));
     MGR_HIP(hipMemset(ctx->err_flag, 0, sizeof(int32_t)));
     MGR_HIP(hipMemset(ctx->prev1, 0, N * V * sizeof(double)));
@@ -402,14 +398,14 @@ int free_model(mgr_ctx *ctx)
     dfree(ctx->cp_koff); dfree(ctx->cp_toff); dfree(ctx->cp_coff); dfree(ctx->cp_op);
     dfree(ctx->icol); dfree(ctx->jcol); dfree(ctx->tmp); dfree(ctx->Cd); dfree(ctx->Ce);
     dfree(ctx->pw); dfree(ctx->dshift_max); dfree(ctx->W11); dfree(ctx->IW64); dfree(ctx->IW32);
-    dfree(ctx->E64); dfree(ctx->K64); dfree(ctx->E32); dfree(ctx->K32); dfree(ctx->nu);
+    dfree(ctx->row32); dfree(ctx->row64); dfree(ctx->nu);
     dfree(ctx->nr_line); dfree(ctx->f_appears); dfree(ctx->f_k); dfree(ctx->f_z);
     dfree(ctx->d_lam_f_off); dfree(ctx->d_lam_f); dfree(ctx->I_bdy); dfree(ctx->pop);
     dfree(ctx->prev1); dfree(ctx->prev2); dfree(ctx->prev3); dfree(ctx->pop_tot); dfree(ctx->emis);
     dfree(ctx->opac); dfree(ctx->J); dfree(ctx->Lam); dfree(ctx->Ls); dfree(ctx->Jlin);
     dfree(ctx->Jeff); dfree(ctx->Jdif); dfree(ctx->err_flag); dfree(ctx->seg_off); dfree(ctx->nseg);
     dfree(ctx->npts); dfree(ctx->seg_cell); dfree(ctx->seg_shift); dfree(ctx->seg_dZ);
-    dfree(ctx->shift_min_bits); dfree(ctx->partials); dfree(ctx->reduced);
+    dfree(ctx->shift_bits); dfree(ctx->partials); dfree(ctx->reduced);
     if (ctx->h_reduced) (void)hipHostFree(ctx->h_reduced);
     ctx->h_reduced = nullptr;
     if (ctx->graph_normal) (void)hipGraphExecDestroy(ctx->graph_normal);

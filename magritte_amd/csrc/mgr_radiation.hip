@@ -7,11 +7,17 @@
 // (raypair_solver.tpp:200-334, n_off_diag = 0) and RayPair::update_Lambda
 // (raypair_lambda.tpp:94-149).
 //
-// Work mapping (CDNA4, 64-wide wavefronts): one wavefront per (origin cell,
-// chunk of 64 frequency bins); lane = frequency bin.  The wavefront loops over
-// the nrays/2 ray pairs of its origin; the traced steps and per-cell line data
-// are uniform across lanes.  J and the Lambda* diagonal accumulate in FP64
-// registers over rays and are written once per (cell, bin): no atomics.
+// Work mapping (CDNA4, 64-wide wavefronts): one workgroup per origin cell,
+// one wavefront per chunk of 64 frequency bins (lane = bin).  The workgroup
+// loops over the nrays/2 ray pairs of its origin.  For every ray it stages the
+// cached traced steps, chunk by chunk, into LDS together with the line rows of
+// their cells (11 width, 1/width and the emissivity/opacity prefactors per
+// line), so that the sweep reads uniform LDS broadcasts instead of dependent
+// global gathers.  The interpolation split of every step (set_data,
+// geometry.tpp:100-207) is computed once while staging.  J and the Lambda*
+// diagonal accumulate in FP64 registers over rays and are written once per
+// (cell, bin): no atomics.  Workgroups are mapped to XCDs in contiguous origin
+// ranges so that neighbouring origins share an L2.
 //
 // The tridiagonal Feautrier system of a ray pair is solved two-sided: each ray
 // of the pair is eliminated from its boundary inwards in the Rybicki-Hummer
@@ -29,6 +35,15 @@
 // boundary cell) u = (S D^2/2 + I D + Z_A)/(D^2/2 + D + f_A),
 // Lambda = (D^2/2)/(D^2/2 + D + f_A).  Only ratios and products of optical
 // depths appear, so the FP32 variant neither overflows nor cancels.
+//
+// Line profiles.  FP64 variant: the window of get_eta_and_chi is decided in
+// FP64 exactly as the reference.  FP32 variant: each lane evaluates the lines
+// whose +-11-width window can contain its bin at any point of the model (its
+// candidate lines; lines outside contribute exactly zero in FP64 too, and
+// inside the window but beyond ~10 widths exp(-x^2) underflows FP32), with the
+// offset x = ((nu_f - nu_l) + delta nu_f) / width formed from the per-lane
+// difference nu_f - nu_l and the point's shift delta = shift - 1, which keeps
+// the profile argument accurate in FP32.
 #include <cfloat>
 
 #include "mgr_common.hpp"
@@ -40,88 +55,120 @@ namespace {
 
 constexpr int32_t ERR_NOT_BOUNDARY = 2;   // ray ends on a non-boundary cell
 constexpr int32_t ERR_EMPTY_SIDE   = 4;   // empty ray at a non-boundary origin
+constexpr int CH_MAX = 64;                // traced steps staged per chunk (at most)
+constexpr size_t LDS_LIMIT = 64 * 1024;   // dynamic LDS per workgroup
+constexpr int MAX_WAVES = 4;              // frequency chunks per workgroup
 
 template <typename real> struct Prec;
 template <> struct Prec<float> {
+    using row_t = float4;   // (1/width, E, K, 0)
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float expneg(float x2) { return __expf(-x2); }
 };
 template <> struct Prec<double> {
+    using row_t = double4;  // (11 width, 1/width, E, K)
     static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
     static __device__ __forceinline__ double expneg(double x2) { return exp(-x2); }
+};
+
+// One traced step in LDS.  Sub-point m = 1..n of the step lies at
+// shift = s0 + m ds (FP64: s0 = shift of the current point, FP32: s0 and ds
+// are shift - 1 and its increment) with path length dz (scaled) and belongs to
+// the current cell for m < half, to `cell` otherwise.
+template <typename real>
+struct Step {
+    int32_t cell, n, half, pad;
+    real    s0, ds, dz, pad2;
 };
 
 template <typename real>
 struct RFArgs {
     int64_t N;
-    int     R, F, L, nchunk;
+    int     R, F, L, nchunk, nwg, ngroups, grid_real, ch;
+    size_t  row_offset;                       // byte offset of the row region in LDS
     const LineTables *tb;
     const double  *nu;                        // [N*F]
-    const double  *W11;                       // [N*L] 11 * width
-    const real    *IW, *E, *K;                // [N*L] 1/width, scaled prefactors
+    const typename Prec<real>::row_t *rows;   // [N*L]
     const int64_t *seg_off;
     const int32_t *nseg, *npts, *seg_cell;
     const double  *seg_shift, *seg_dZ;
     const int32_t *cell2bdy, *antipod;
     const double  *I_bdy;                     // [NB*F]
     const double  *ray_w, *dshift_max;
-    double         lstart_threshold;          // line[l] * this < shift_min * nu_f => skipped everywhere
+    double         cand_lo, cand_hi;          // candidate-line bounds (FP32 variant)
+    double         lstart_threshold;          // first line that can be in a window (FP64 variant)
     double        *J, *Lam;                   // [N*F]
     int32_t       *err_flag;
 };
 
-// get_eta_and_chi (simulation.tpp:385-481) for one lane: the line window is
-// decided in FP64 exactly as the reference (skip while freq_diff > 11 width,
-// include while freq_diff >= -11 width, the last line always reachable);
-// profiles and sums in `real`, scaled by CHI_SCALE (and I_SCALE for eta).
-template <typename real>
-__device__ __forceinline__ void eta_chi(const RFArgs<real> &a, int32_t c, double fs, int lstart,
-                                        real &eta, real &chi)
+// ---- line profiles ----------------------------------------------------------
+
+// FP64: get_eta_and_chi (simulation.tpp:385-481) for one lane with the exact
+// window: skip while freq_diff > 11 width, include while freq_diff >= -11 width.
+__device__ __forceinline__ void eta_chi64(const RFArgs<double> &a, const double4 *__restrict__ row,
+                                          double fs, int lstart, double &eta, double &chi)
 {
+#pragma clang fp contract(off)
     const int L = a.L;
-    const double *__restrict__ W = a.W11 + (int64_t)c * L;
-    const real *__restrict__ IW = a.IW + (int64_t)c * L;
-    const real *__restrict__ E = a.E + (int64_t)c * L;
-    const real *__restrict__ K = a.K + (int64_t)c * L;
     int l = lstart;
-    double fd, w11;
-    {
-#pragma clang fp contract(off)
+    double fd = fs - a.tb->line[l];
+    double4 v = row[l];
+    while (fd > v.x) {
+        if (l >= L - 1) break;
+        l++;
         fd = fs - a.tb->line[l];
+        v = row[l];
     }
-    w11 = W[l];
-    while (fd > w11) {
+    double e_acc = 0.0, c_acc = CHI_FLOOR * CHI_SCALE;
+    while (fd >= -v.x) {
+        const double x = fd * v.y;
+        const double g = fs * exp(-(x * x));
+        e_acc += v.z * g;
+        c_acc += v.w * g;
         if (l >= L - 1) break;
         l++;
-        {
-#pragma clang fp contract(off)
-            fd = fs - a.tb->line[l];
-        }
-        w11 = W[l];
-    }
-    real e_acc = (real)0, c_acc = (real)(CHI_FLOOR * CHI_SCALE);
-    const real nus = (real)fs;
-    while (fd >= -w11) {
-        const real x = (real)fd * IW[l];
-        const real g = nus * Prec<real>::expneg(x * x);
-        e_acc += E[l] * g;
-        c_acc += K[l] * g;
-        if (l >= L - 1) break;
-        l++;
-        {
-#pragma clang fp contract(off)
-            fd = fs - a.tb->line[l];
-        }
-        w11 = W[l];
+        fd = fs - a.tb->line[l];
+        v = row[l];
     }
     eta = e_acc;
     chi = c_acc;
 }
 
-// Radiation::rescale_I_bdy, scalar branch (radiation.tpp:341-373): search the
-// boundary cell's own frequency grid for the first bin >= freq (the linear
-// search_with_notch from 0 is a lower bound), edge bins taken as they are,
-// otherwise linear interpolation between bins notch-1 and notch.
+// FP32: sum over the lane's candidate lines la .. la+nc-1.
+struct Lane32 {
+    float nuf;       // nu_f
+    float A0, A1;    // nu_f - nu_l for the first two candidates
+    int   la, nc;
+    double nu_f64;
+};
+
+__device__ __forceinline__ void line_term32(const float4 v, float A, float delta, float nuf, float nus,
+                                            float &e_acc, float &c_acc)
+{
+    const float x = fmaf(delta, nuf, A) * v.x;
+    const float g = nus * __expf(-(x * x));
+    e_acc = fmaf(v.y, g, e_acc);
+    c_acc = fmaf(v.z, g, c_acc);
+}
+
+__device__ __forceinline__ void eta_chi32(const RFArgs<float> &a, const float4 *__restrict__ row,
+                                          const Lane32 &ln, float delta, float &eta, float &chi)
+{
+    const float nus = fmaf(delta, ln.nuf, ln.nuf);
+    float e_acc = 0.0f, c_acc = (float)(CHI_FLOOR * CHI_SCALE);
+    if (ln.nc > 0) line_term32(row[ln.la], ln.A0, delta, ln.nuf, nus, e_acc, c_acc);
+    if (ln.nc > 1) line_term32(row[ln.la + 1], ln.A1, delta, ln.nuf, nus, e_acc, c_acc);
+    for (int c = 2; c < ln.nc; c++) {
+        const int l = ln.la + c;
+        line_term32(row[l], (float)(ln.nu_f64 - a.tb->line[l]), delta, ln.nuf, nus, e_acc, c_acc);
+    }
+    eta = e_acc;
+    chi = c_acc;
+}
+
+// Radiation::rescale_I_bdy, scalar branch (radiation.tpp:341-373): first bin
+// >= freq of the boundary cell's own grid (search_with_notch from 0 is a lower
+// bound), edge bins as they are, otherwise linear interpolation.
 __device__ __forceinline__ double rescale_I_bdy(const double *__restrict__ nup,
                                                 const double *__restrict__ Ib, int F, double fs)
 {
@@ -131,137 +178,278 @@ __device__ __forceinline__ double rescale_I_bdy(const double *__restrict__ nup,
         const int mid = (lo + hi) >> 1;
         if (nup[mid] < fs) lo = mid + 1; else hi = mid;
     }
-    int notch = lo < F ? lo : F - 1;
+    const int notch = lo < F ? lo : F - 1;
     if (notch == 0 || notch == F - 1) return Ib[notch];
     const double nu1 = nup[notch], nu2 = nup[notch - 1];
     const double I1 = Ib[notch], I2 = Ib[notch - 1];
     return (I2 - I1) / (nu2 - nu1) * (fs - nu1) + I1;
 }
 
-// One ray of a pair, eliminated from its boundary end inwards up to the
-// origin.  Returns f = F/(1+F) and Z of the point next to the origin and the
-// optical depth D between that point and the origin (plus 1/D).
+// ---- Feautrier elimination state of one side ----------------------------------
+
 template <typename real>
-struct SideState {
+struct Sweep {
+    bool have_q, first_row;
+    real eta_q, chi_q, dz_q, Ib;
     real f, Z, D, invD;
-};
 
-template <typename real>
-__device__ __forceinline__ SideState<real> sweep_side(const RFArgs<real> &a, int32_t o, int64_t q,
-                                                      double nu_f, int lstart, real chi_o,
-                                                      real S_o, double dsm, int F,
-                                                      int32_t &err)
-{
-    const int64_t s0 = a.seg_off[q];
-    const int ns = a.nseg[q];
-    bool have_q = false, first_row = true;
-    real chi_q = 0, S_q = 0, dZq = 0, Ib = 0;
-    real f_prev = 0, Z_prev = 0, D_prev = 0, invD_prev = 0;
+    __device__ __forceinline__ void init(real I_far)
+    {
+        have_q = false;
+        first_row = true;
+        Ib = I_far;
+        eta_q = chi_q = dz_q = 0;
+        f = Z = D = invD = 0;
+    }
 
-    auto step = [&](real chi, real S, real dZs) {
+    // New point (eta, chi) inwards of the pending one: close the pending row.
+    __device__ __forceinline__ void push(real eta, real chi, real dz)
+    {
         if (have_q) {
-            const real D = (real)0.5 * (chi_q + chi) * dZq;
-            real Fn, Zn;
+            const real Dn = (real)0.5 * (chi_q + chi) * dz_q;
+            const real S_q = eta_q * Prec<real>::rcp(chi_q);
+            real Fn, cc, rr, Zs;
             if (first_row) {
-                Fn = D * ((real)1 + (real)0.5 * D);
-                const real Dn = Prec<real>::rcp((real)1 + Fn);
-                Zn = (S_q * ((real)0.5 * D * D) + Ib * D) * Dn;
-                f_prev = Fn * Dn;
+                cc = (real)0.5 * Dn * Dn;
+                Fn = Dn + cc;
+                Zs = S_q * cc + Ib * Dn;
                 first_row = false;
             } else {
-                const real rr = D * invD_prev;
-                const real cc = (real)0.5 * (D_prev + D) * D;
-                Fn = cc + rr * f_prev;
-                const real Dn = Prec<real>::rcp((real)1 + Fn);
-                Zn = (S_q * cc + rr * Z_prev) * Dn;
-                f_prev = Fn * Dn;
+                rr = Dn * invD;
+                cc = (real)0.5 * (D + Dn) * Dn;
+                Fn = cc + rr * f;
+                Zs = S_q * cc + rr * Z;
             }
-            Z_prev = Zn;
-            D_prev = D;
-            invD_prev = Prec<real>::rcp(D);
+            const real q = Prec<real>::rcp((real)1 + Fn);
+            Z = Zs * q;
+            f = Fn * q;
+            D = Dn;
+            invD = Prec<real>::rcp(Dn);
         }
+        eta_q = eta;
         chi_q = chi;
-        S_q = S;
-        dZq = dZs;
+        dz_q = dz;
         have_q = true;
-    };
-
-    auto point = [&](int32_t c, double sh, double dz) {
-        double fs;
-        {
-#pragma clang fp contract(off)
-            fs = sh * nu_f;
-        }
-        real eta, chi;
-        eta_chi(a, c, fs, lstart, eta, chi);
-        const real S = eta * Prec<real>::rcp(chi);
-        if (!have_q) {
-            // far end of the ray: incoming boundary intensity
-            const int32_t b = a.cell2bdy[c];
-            if (b < 0) {
-                err |= ERR_NOT_BOUNDARY;
-                Ib = (real)0;
-            } else {
-                Ib = (real)(rescale_I_bdy(a.nu + (int64_t)c * F, a.I_bdy + (int64_t)b * F, F, fs) * I_SCALE);
-            }
-        }
-        step(chi, S, (real)(dz * (1.0 / CHI_SCALE)));
-    };
-
-    for (int j = ns - 1; j >= 0; --j) {
-        const int32_t cj = a.seg_cell[s0 + j];
-        const double sj = a.seg_shift[s0 + j];
-        const double zj = a.seg_dZ[s0 + j];
-        int32_t crt;
-        double s_crt;
-        if (j > 0) {
-            crt = a.seg_cell[s0 + j - 1];
-            s_crt = a.seg_shift[s0 + j - 1];
-        } else {
-            crt = o;
-            s_crt = 1.0;
-        }
-        const StepSplit sp = split_step(s_crt, sj, zj, dsm);
-        if (!sp.interp) {
-            point(cj, sj, zj);
-        } else {
-            for (int m = sp.n; m >= 1; --m)
-                point((m < sp.half) ? crt : cj, sub_shift(s_crt, m, sp.dshift), sp.dZ);
-        }
     }
-    // the origin closes the innermost row
-    step(chi_o, S_o, (real)0);
-    SideState<real> st;
-    st.f = f_prev;
-    st.Z = Z_prev;
-    st.D = D_prev;
-    st.invD = invD_prev;
-    return st;
+};
+
+// ---- staging ----------------------------------------------------------------
+
+// Stage the entries e = 0..cnt-1 of chunk [a, b): entry e holds step j = a-1+e
+// (j = -1 is the origin) and the line rows of its cell.
+template <typename real>
+__device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, int64_t q, int j0,
+                                            int cnt, double dsm, Step<real> *__restrict__ st,
+                                            typename Prec<real>::row_t *__restrict__ rw)
+{
+    const int64_t s0 = a.seg_off[q];
+    const int nthr = blockDim.x;
+    for (int e = threadIdx.x; e < cnt; e += nthr) {
+        const int j = j0 + e;
+        Step<real> s;
+        s.pad = 0;
+        s.pad2 = 0;
+        if (j < 0) {
+            s.cell = o; s.n = 0; s.half = 0; s.s0 = 0; s.ds = 0; s.dz = 0;
+        } else {
+            const int32_t c = a.seg_cell[s0 + j];
+            const double sn = a.seg_shift[s0 + j];
+            const double z = a.seg_dZ[s0 + j];
+            const double sc = (j > 0) ? a.seg_shift[s0 + j - 1] : 1.0;
+            const StepSplit sp = split_step(sc, sn, z, dsm);
+            s.cell = c;
+            s.n = sp.n;
+            s.half = sp.interp ? sp.half : 0;
+            if (sizeof(real) == 8) {
+                s.s0 = (real)(sp.interp ? sc : sn);
+                s.ds = (real)sp.dshift;
+            } else {
+                s.s0 = (real)((sp.interp ? sc : sn) - 1.0);
+                s.ds = (real)sp.dshift;
+            }
+            s.dz = (real)(sp.dZ * (1.0 / CHI_SCALE));
+        }
+        st[e] = s;
+    }
+    __syncthreads();
+    const int L = a.L;
+    for (int idx = threadIdx.x; idx < cnt * L; idx += nthr) {
+        const int e = idx / L, l = idx - e * L;
+        rw[idx] = a.rows[(int64_t)st[e].cell * L + l];
+    }
+    __syncthreads();
+}
+
+// Far end of a ray (last emitted point): its cell and exact FP64 shift.
+__device__ __forceinline__ bool far_end(const int64_t *__restrict__ seg_off,
+                                        const int32_t *__restrict__ nseg,
+                                        const int32_t *__restrict__ seg_cell,
+                                        const double *__restrict__ seg_shift,
+                                        const double *__restrict__ seg_dZ, int64_t q, double dsm,
+                                        int32_t &cell, double &shift)
+{
+    const int64_t s0 = seg_off[q];
+    for (int j = nseg[q] - 1; j >= 0; --j) {
+        const double sn = seg_shift[s0 + j];
+        const double sc = (j > 0) ? seg_shift[s0 + j - 1] : 1.0;
+        const StepSplit sp = split_step(sc, sn, seg_dZ[s0 + j], dsm);
+        if (sp.n == 0) continue;
+        cell = seg_cell[s0 + j];   // sub-point m = n >= half belongs to the next cell
+        shift = sp.interp ? sub_shift(sc, sp.n, sp.dshift) : sn;
+        return true;
+    }
+    return false;
 }
 
 template <typename real>
-__global__ __launch_bounds__(WAVE) void k_radiation_field(RFArgs<real> a)
+struct LaneCtx;   // per-lane constants of the origin
+
+template <>
+struct LaneCtx<float> {
+    Lane32 ln;
+};
+template <>
+struct LaneCtx<double> {
+    double nu_f;
+    int    lstart;
+};
+
+template <typename real>
+__device__ __forceinline__ void eval_point(const RFArgs<real> &a, const LaneCtx<real> &lc,
+                                           const typename Prec<real>::row_t *__restrict__ row,
+                                           real s, real &eta, real &chi);
+
+template <>
+__device__ __forceinline__ void eval_point<float>(const RFArgs<float> &a, const LaneCtx<float> &lc,
+                                                  const float4 *__restrict__ row, float s,
+                                                  float &eta, float &chi)
 {
-    const int64_t wid = blockIdx.x;
-    const int32_t o = (int32_t)(wid / a.nchunk);
-    const int chunk = (int)(wid % a.nchunk);
-    if (o >= a.N) return;
+    eta_chi32(a, row, lc.ln, s, eta, chi);
+}
+
+template <>
+__device__ __forceinline__ void eval_point<double>(const RFArgs<double> &a, const LaneCtx<double> &lc,
+                                                   const double4 *__restrict__ row, double s,
+                                                   double &eta, double &chi)
+{
+    double fs;
+    {
+#pragma clang fp contract(off)
+        fs = s * lc.nu_f;
+    }
+    eta_chi64(a, row, fs, lc.lstart, eta, chi);
+}
+
+// One ray of a pair, eliminated from its boundary end inwards to the origin.
+template <typename real>
+__device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const LaneCtx<real> &lc,
+                                                  int32_t o, int64_t q, double nu_f, real eta_o,
+                                                  real chi_o, double dsm, int fa, int F,
+                                                  Step<real> *__restrict__ st,
+                                                  typename Prec<real>::row_t *__restrict__ rw,
+                                                  int32_t &err)
+{
+    using row_t = typename Prec<real>::row_t;
+    Sweep<real> sw;
+    // incoming intensity at the far end
+    int32_t cf = o;
+    double shf = 1.0;
+    real Ib = 0;
+    if (far_end(a.seg_off, a.nseg, a.seg_cell, a.seg_shift, a.seg_dZ, q, dsm, cf, shf)) {
+        const int32_t b = a.cell2bdy[cf];
+        if (b < 0) {
+            err |= ERR_NOT_BOUNDARY;
+        } else {
+            double fs;
+            {
+#pragma clang fp contract(off)
+                fs = shf * nu_f;
+            }
+            Ib = (real)(rescale_I_bdy(a.nu + (int64_t)cf * F, a.I_bdy + (int64_t)b * F, F, fs) * I_SCALE);
+        }
+    }
+    sw.init(Ib);
+    const int ns = a.nseg[q];
+    const int L = a.L;
+    const int CH = a.ch;
+    for (int b_ = ns; b_ > 0; b_ -= CH) {
+        const int a_ = b_ > CH ? b_ - CH : 0;
+        const int cnt = b_ - a_ + 1;
+        __syncthreads();   // the previous chunk is consumed by every wave
+        stage_chunk<real>(a, o, q, a_ - 1, cnt, dsm, st, rw);
+        for (int e = cnt - 1; e >= 1; --e) {
+            const Step<real> s = st[e];
+            const row_t *rn = rw + e * L;
+            const row_t *rc = rw + (e - 1) * L;
+            for (int m = s.n; m >= 1; --m) {
+                const row_t *r = (m < s.half) ? rc : rn;
+                real sh;
+                {
+#pragma clang fp contract(off)
+                    sh = s.s0 + (real)m * s.ds;
+                }
+                real eta, chi;
+                eval_point<real>(a, lc, r, sh, eta, chi);
+                sw.push(eta, chi, s.dz);
+            }
+        }
+    }
+    sw.push(eta_o, chi_o, (real)0);   // the origin closes the innermost row
+    (void)fa;
+    return sw;
+}
+
+template <typename real>
+__global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<real> a)
+{
+    using row_t = typename Prec<real>::row_t;
+    extern __shared__ __align__(16) unsigned char smem[];
+    Step<real> *st = reinterpret_cast<Step<real> *>(smem);
+    row_t *rw = reinterpret_cast<row_t *>(smem + a.row_offset);
+
+    // XCD-aware order: consecutive workgroups go to different XCDs, so hand
+    // each XCD a contiguous range of origins.
+    const int G = gridDim.x;
+    const int per = G >> 3;
+    const int bid = blockIdx.x;
+    const int logical = (bid & 7) * per + (bid >> 3);
+    if (logical >= a.grid_real) return;   // whole workgroup: before any barrier
+    const int32_t o = (int32_t)(logical / a.ngroups);
+    const int group = logical - o * a.ngroups;
+    const int wave = threadIdx.x / WAVE;
+    const int chunk = group * a.nwg + wave;
     const int F = a.F;
-    const int f = chunk * WAVE + (int)threadIdx.x;
+    const int f = chunk * WAVE + (int)(threadIdx.x & (WAVE - 1));
     const bool active = f < F;
     const int fa = active ? f : F - 1;
     const double nu_f = a.nu[(int64_t)o * F + fa];
-
-    // first line that can be in the window anywhere on this cell's rays
-    int lstart = 0;
-    while (lstart < a.L - 1 && a.tb->line[lstart] * a.lstart_threshold < nu_f) lstart++;
-
-    // the origin point: unshifted frequency (simulation.tpp:210-221)
-    real eta_o, chi_o;
-    eta_chi(a, o, nu_f, lstart, eta_o, chi_o);
-    const real S_o = eta_o * Prec<real>::rcp(chi_o);
     const double dsm = a.dshift_max[o];
     const int32_t bo = a.cell2bdy[o];
+    const row_t *orow = a.rows + (int64_t)o * a.L;
+
+    LaneCtx<real> lc;
+    real eta_o, chi_o;
+    if constexpr (sizeof(real) == 4) {
+        // candidate lines: nu_l (1 - w) <= s_max nu_f and nu_l (1 + w) >= s_min nu_f
+        int la = 0;
+        while (la < a.L && a.tb->line[la] * a.cand_lo < nu_f) la++;
+        int lb = la;
+        while (lb < a.L && a.tb->line[lb] * a.cand_hi <= nu_f) lb++;
+        lc.ln.la = la < a.L ? la : a.L - 1;
+        lc.ln.nc = lb - la;
+        lc.ln.nuf = (float)nu_f;
+        lc.ln.nu_f64 = nu_f;
+        lc.ln.A0 = (lc.ln.nc > 0) ? (float)(nu_f - a.tb->line[la]) : 0.0f;
+        lc.ln.A1 = (lc.ln.nc > 1) ? (float)(nu_f - a.tb->line[la + 1]) : 0.0f;
+        eval_point<real>(a, lc, orow, (real)0, eta_o, chi_o);
+    } else {
+        int ls = 0;
+        while (ls < a.L - 1 && a.tb->line[ls] * a.lstart_threshold < nu_f) ls++;
+        lc.nu_f = nu_f;
+        lc.lstart = ls;
+        eval_point<real>(a, lc, orow, (real)1, eta_o, chi_o);
+    }
+    const real S_o = eta_o * Prec<real>::rcp(chi_o);
 
     int32_t err = 0;
     double J_acc = 0.0, Lam_acc = 0.0;
@@ -281,15 +469,15 @@ __global__ __launch_bounds__(WAVE) void k_radiation_field(RFArgs<real> a)
         }
         real u, lam;
         if (n_r > 0 && n_a > 0) {
-            const SideState<real> A = sweep_side(a, o, qa, nu_f, lstart, chi_o, S_o, dsm, F, err);
-            const SideState<real> B = sweep_side(a, o, qr, nu_f, lstart, chi_o, S_o, dsm, F, err);
+            const Sweep<real> A = sweep_side<real>(a, lc, o, qa, nu_f, eta_o, chi_o, dsm, fa, F, st, rw, err);
+            const Sweep<real> B = sweep_side<real>(a, lc, o, qr, nu_f, eta_o, chi_o, dsm, fa, F, st, rw, err);
             const real h = (real)0.5 * (A.D + B.D);
             const real den = Prec<real>::rcp(h + A.f * A.invD + B.f * B.invD);
             u = (S_o * h + A.Z * A.invD + B.Z * B.invD) * den;
             lam = h * den;
         } else {
             const int64_t qs = (n_a > 0) ? qa : qr;
-            const SideState<real> A = sweep_side(a, o, qs, nu_f, lstart, chi_o, S_o, dsm, F, err);
+            const Sweep<real> A = sweep_side<real>(a, lc, o, qs, nu_f, eta_o, chi_o, dsm, fa, F, st, rw, err);
             real I_o = 0;
             if (bo < 0) err |= ERR_EMPTY_SIDE;
             else I_o = (real)(a.I_bdy[(int64_t)bo * F + fa] * I_SCALE);
@@ -309,7 +497,7 @@ __global__ __launch_bounds__(WAVE) void k_radiation_field(RFArgs<real> a)
 }
 
 template <typename real>
-int launch_rf(mgr_ctx *ctx, const real *IW, const real *E, const real *K)
+int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows)
 {
     RFArgs<real> a;
     a.N = ctx->N;
@@ -317,12 +505,11 @@ int launch_rf(mgr_ctx *ctx, const real *IW, const real *E, const real *K)
     a.F = (int)ctx->F;
     a.L = (int)ctx->L;
     a.nchunk = (int)((ctx->F + WAVE - 1) / WAVE);
+    a.nwg = a.nchunk < MAX_WAVES ? a.nchunk : MAX_WAVES;
+    a.ngroups = (a.nchunk + a.nwg - 1) / a.nwg;
     a.tb = ctx->tables;
     a.nu = ctx->nu;
-    a.W11 = ctx->W11;
-    a.IW = IW;
-    a.E = E;
-    a.K = K;
+    a.rows = rows;
     a.seg_off = ctx->seg_off;
     a.nseg = ctx->nseg;
     a.npts = ctx->npts;
@@ -334,13 +521,30 @@ int launch_rf(mgr_ctx *ctx, const real *IW, const real *E, const real *K)
     a.I_bdy = ctx->I_bdy;
     a.ray_w = ctx->ray_w;
     a.dshift_max = ctx->dshift_max;
-    // line l is skipped at every point when line[l] (1 + 11 pw_max) (1 + 1e-9) < shift_min nu_f
-    a.lstart_threshold = (1.0 + LINE_WINDOW * ctx->pw_max) * (1.0 + 1.0e-9) / ctx->shift_min;
+    // A line can be inside the window of bin nu_f at some point only if
+    // nu_l (1 - 11 pw_max) <= s_max nu_f and nu_l (1 + 11 pw_max) >= s_min nu_f;
+    // the margins cover rounding of the shifts and widths.
+    const double w = LINE_WINDOW * ctx->pw_max;
+    a.cand_lo = (1.0 + w) * (1.0 + 1.0e-9) / ctx->shift_min;   // line * cand_lo < nu_f: below
+    a.cand_hi = (1.0 - w) * (1.0 - 1.0e-9) / ctx->shift_max;   // line * cand_hi <= nu_f: candidate
+    a.lstart_threshold = a.cand_lo;
     a.J = ctx->J;
     a.Lam = ctx->Lam;
     a.err_flag = ctx->err_flag;
-    const int64_t nblocks = ctx->N * a.nchunk;
-    hipLaunchKernelGGL(k_radiation_field<real>, dim3((unsigned)nblocks), dim3(WAVE), 0, ctx->stream, a);
+    const int64_t groups = ctx->N * a.ngroups;
+    if (groups > INT32_MAX - 8) return ctx->fail(MGR_ERR_UNSUPPORTED, "too many workgroups");
+    a.grid_real = (int)groups;
+    const int grid = (int)((groups + 7) / 8) * 8;
+    // chunk length: as many traced steps as fit the LDS budget, at most CH_MAX
+    const size_t per_entry = sizeof(Step<real>) + sizeof(typename Prec<real>::row_t) * ctx->L + 64;
+    int ch = (int)(LDS_LIMIT / per_entry) - 1;
+    ch = ch < CH_MAX ? ch : CH_MAX;
+    if (ch < 2) return ctx->fail(MGR_ERR_UNSUPPORTED, "line rows do not fit the LDS budget");
+    a.ch = ch;
+    a.row_offset = ((sizeof(Step<real>) * (ch + 1) + 63) / 64) * 64;
+    const size_t lds = a.row_offset + sizeof(typename Prec<real>::row_t) * (ch + 1) * ctx->L;
+    hipLaunchKernelGGL(k_radiation_field<real>, dim3((unsigned)grid), dim3(WAVE * a.nwg), lds,
+                       ctx->stream, a);
     MGR_LAUNCH_CHECK();
     return MGR_OK;
 }
@@ -351,8 +555,8 @@ namespace mgr {
 
 int launch_radiation_field(mgr_ctx *ctx, int precision)
 {
-    if (precision == MGR_FP64) return launch_rf<double>(ctx, ctx->IW64, ctx->E64, ctx->K64);
-    return launch_rf<float>(ctx, ctx->IW32, ctx->E32, ctx->K32);
+    if (precision == MGR_FP64) return launch_rf<double>(ctx, ctx->row64);
+    return launch_rf<float>(ctx, ctx->row32);
 }
 
 }  // namespace mgr

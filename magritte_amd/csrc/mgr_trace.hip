@@ -24,10 +24,11 @@ namespace {
 constexpr int32_t ERR_NO_NEIGHBOUR = 1;
 constexpr int32_t ERR_TOO_LONG     = 8;
 
-__device__ __forceinline__ void atomic_min_pos_double(uint64_t *addr, double v)
+// positive doubles order like their bit patterns
+__device__ __forceinline__ void atomic_minmax_pos_double(uint64_t *addr, double lo, double hi)
 {
-    // positive doubles order like their bit patterns
-    if (v > 0.0) atomicMin((unsigned long long *)addr, (unsigned long long)__double_as_longlong(v));
+    if (lo > 0.0) atomicMin((unsigned long long *)addr, (unsigned long long)__double_as_longlong(lo));
+    if (hi > 0.0) atomicMax((unsigned long long *)addr + 1, (unsigned long long)__double_as_longlong(hi));
 }
 
 template <bool FILL>
@@ -38,7 +39,7 @@ __global__ void k_trace(int64_t N, int R, const double *__restrict__ pos,
                         int32_t *__restrict__ nseg, int32_t *__restrict__ npts,
                         const int64_t *__restrict__ seg_off, int32_t *__restrict__ seg_cell,
                         double *__restrict__ seg_shift, double *__restrict__ seg_dZ,
-                        uint64_t *__restrict__ shift_min_bits, int32_t *__restrict__ err_flag)
+                        uint64_t *__restrict__ shift_bits, int32_t *__restrict__ err_flag)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= N * R) return;
@@ -48,7 +49,7 @@ __global__ void k_trace(int64_t N, int R, const double *__restrict__ pos,
     const double dsm = dshift_max[o];
     int64_t w = FILL ? seg_off[q] : 0;
     int32_t steps = 0, points = 0;
-    double smin = 1.0;
+    double smin = 1.0, smax = 1.0;
 
     double Z = 0.0, dZ = 0.0;
     int32_t nxt = get_next(pos, nb_off, nb, o, nx, ny, nz, o, Z, dZ);
@@ -59,6 +60,7 @@ __global__ void k_trace(int64_t N, int R, const double *__restrict__ pos,
             const StepSplit sp = split_step(shift_crt, shift_nxt, dZ, dsm);
             points += sp.n;
             smin = fmin(smin, shift_nxt);
+            smax = fmax(smax, shift_nxt);
             if (FILL) {
                 seg_cell[w] = nxt;
                 seg_shift[w] = shift_nxt;
@@ -77,7 +79,7 @@ __global__ void k_trace(int64_t N, int R, const double *__restrict__ pos,
     if (!FILL) {
         nseg[q] = steps;
         npts[q] = points;
-        atomic_min_pos_double(shift_min_bits, smin);
+        atomic_minmax_pos_double(shift_bits, smin, smax);
     }
 }
 
@@ -134,17 +136,17 @@ int launch_trace(mgr_ctx *ctx)
         MGR_HIP(hipMalloc((void **)&ctx->npts, NR * sizeof(int32_t)));
         MGR_HIP(hipMalloc((void **)&ctx->seg_off, (NR + 1) * sizeof(int64_t)));
     }
-    const uint64_t one_bits = 0x3FF0000000000000ull;   // 1.0
-    MGR_HIP(hipMemcpyAsync(ctx->shift_min_bits, &one_bits, sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+    static const uint64_t one_bits[2] = {0x3FF0000000000000ull, 0x3FF0000000000000ull};   // 1.0, 1.0
+    MGR_HIP(hipMemcpyAsync(ctx->shift_bits, one_bits, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
     MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
     hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->pos,
                        ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->dshift_max,
-                       ctx->nseg, ctx->npts, nullptr, nullptr, nullptr, nullptr, ctx->shift_min_bits,
+                       ctx->nseg, ctx->npts, nullptr, nullptr, nullptr, nullptr, ctx->shift_bits,
                        ctx->err_flag);
     MGR_LAUNCH_CHECK();
     std::vector<int32_t> ns(NR), np(NR);
     int32_t errv = 0;
-    uint64_t smin_bits = 0;
+    uint64_t sbits[2] = {0, 0};
     MGR_HIP(hipStreamSynchronize(ctx->stream));
     MGR_HIP(hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (errv) {
@@ -154,13 +156,15 @@ int launch_trace(mgr_ctx *ctx)
     }
     MGR_HIP(hipMemcpy(ns.data(), ctx->nseg, NR * sizeof(int32_t), hipMemcpyDeviceToHost));
     MGR_HIP(hipMemcpy(np.data(), ctx->npts, NR * sizeof(int32_t), hipMemcpyDeviceToHost));
-    MGR_HIP(hipMemcpy(&smin_bits, ctx->shift_min_bits, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    MGR_HIP(hipMemcpy(sbits, ctx->shift_bits, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost));
     std::vector<int64_t> off(NR + 1, 0);
     for (int64_t q = 0; q < NR; q++) off[q + 1] = off[q] + ns[q];
     ctx->total_segments = off[NR];
-    double smin;
-    std::memcpy(&smin, &smin_bits, sizeof(double));
+    double smin, smax;
+    std::memcpy(&smin, &sbits[0], sizeof(double));
+    std::memcpy(&smax, &sbits[1], sizeof(double));
     ctx->shift_min = std::min(1.0, smin);
+    ctx->shift_max = std::max(1.0, smax);
 
     // ray-pair statistics: ndep = n_ar + n_r + 1 (sim_radiation.tpp:164-178)
     int64_t sum_ndep = 0, solved = 0;

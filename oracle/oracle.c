@@ -410,10 +410,13 @@ static int64_t o_get_next(const orc_state *s, int64_t origin, int64_t ray, int64
 }
 
 /* Geometry::set_data (geometry.tpp:100-207).  Appends to the output arrays;
- * returns the new count (unchanged when more than 10000 points are needed). */
+ * returns the new count (unchanged when more than 10000 points are needed).
+ * o_crt receives crt for a plain point and -2 - step (the index of the traced
+ * step) for interpolation points, which the reference leaves unset. */
 static int64_t o_set_data(int64_t crt, int64_t nxt, double shift_crt, double shift_nxt,
                           double dZ_loc, double dshift_max, int64_t cnt, int64_t cap,
-                          int64_t *o_crt, int64_t *o_cell, double *o_shift, double *o_dZ)
+                          int64_t *o_crt, int64_t *o_cell, double *o_shift, double *o_dZ,
+                          int64_t step)
 {
     const double dshift     = shift_nxt - shift_crt;
     const double dshift_abs = fabs(dshift);
@@ -425,14 +428,14 @@ static int64_t o_set_data(int64_t crt, int64_t nxt, double shift_crt, double shi
         if ((n_interpl > 10000) || (n_interpl < 0)) return cnt;
         for (long m = 1; m < half_n_interpl; m++) {
             if (cnt < cap) {
-                if (o_crt) o_crt[cnt] = -1;
+                if (o_crt) o_crt[cnt] = -2 - step;
                 o_cell[cnt] = crt; o_shift[cnt] = shift_crt + m * dshift_interpl; o_dZ[cnt] = dZ_interpl;
             }
             cnt++;
         }
         for (long m = half_n_interpl; m <= n_interpl; m++) {
             if (cnt < cap) {
-                if (o_crt) o_crt[cnt] = -1;
+                if (o_crt) o_crt[cnt] = -2 - step;
                 o_cell[cnt] = nxt; o_shift[cnt] = shift_crt + m * dshift_interpl; o_dZ[cnt] = dZ_interpl;
             }
             cnt++;
@@ -458,16 +461,18 @@ int64_t orc_trace_ray(const orc_state *s, int64_t o, int64_t r, double dshift_ma
         int64_t crt = o;
         double shift_crt = orc_get_doppler_shift(s, o, r, crt, 0);
         double shift_nxt = orc_get_doppler_shift(s, o, r, nxt, 0);
+        int64_t step = 0;
         cnt = o_set_data(crt, nxt, shift_crt, shift_nxt, dZ, dshift_max, cnt, cap,
-                         crt_out, cellNr, shift, dZs);
+                         crt_out, cellNr, shift, dZs, step);
         while (!s->boundary[nxt]) {
             shift_crt = shift_nxt;
             crt = nxt;
             nxt = o_get_next(s, o, r, nxt, &Z, &dZ);
             if (nxt < 0) return -1;
             shift_nxt = orc_get_doppler_shift(s, o, r, nxt, 0);
+            step++;
             cnt = o_set_data(crt, nxt, shift_crt, shift_nxt, dZ, dshift_max, cnt, cap,
-                             crt_out, cellNr, shift, dZs);
+                             crt_out, cellNr, shift, dZs, step);
         }
     }
     return cnt;
@@ -738,12 +743,28 @@ static double o_setup_and_solve(const orc_state *s, int64_t R, int64_t origin, i
 /* Simulation::compute_radiation_field (src/Simulation/sim_radiation.tpp:56-338),
  * RayPair branch, n_off_diag = 0, with RayPair::update_Lambda
  * (raypair_lambda.tpp:94-149) and Lambda::add_element (lambda.tpp:130-143). */
+static int o_radiation_field(orc_state *s, int nthreads, const int64_t *list, int64_t o_begin,
+                             int64_t o_end);
+
 int orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, int64_t o_end)
 {
-    const int64_t F = s->F, L = s->L;
     if (o_begin < 0) o_begin = 0;
     if (o_end > s->N || o_end < 0) o_end = s->N;
-    for (int64_t p = o_begin; p < o_end; p++) {
+    return o_radiation_field(s, nthreads, NULL, o_begin, o_end);
+}
+
+int orc_compute_radiation_field_list(orc_state *s, int nthreads, const int64_t *origins, int64_t n)
+{
+    return o_radiation_field(s, nthreads, origins, 0, n);
+}
+
+/* list == NULL: origins o_begin..o_end-1; otherwise origins list[o_begin..o_end-1] */
+static int o_radiation_field(orc_state *s, int nthreads, const int64_t *list, int64_t o_begin,
+                             int64_t o_end)
+{
+    const int64_t F = s->F, L = s->L;
+    for (int64_t i = o_begin; i < o_end; i++) {
+        const int64_t p = list ? list[i] : i;
         for (int64_t f = 0; f < F; f++) s->J[p*F + f] = 0.0;    /* initialize_J */
         for (int64_t k = 0; k < L; k++) s->Ls[p*L + k] = 0.0;   /* lambda.clear */
     }
@@ -760,7 +781,8 @@ int orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, int
 #ifdef _OPENMP
             tid = omp_get_thread_num(); nth = omp_get_num_threads();
 #endif
-            for (int64_t o = o_begin + tid; o < o_end; o += nth) {
+            for (int64_t oi = o_begin + tid; oi < o_end; oi += nth) {
+                const int64_t o = list ? list[oi] : oi;
                 const int64_t ar = s->antipod[r];
                 const double weight_ang = s->m.weights[r];
                 const double dshift_max = orc_get_dshift_max(s, o);

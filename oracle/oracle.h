@@ -60,6 +60,8 @@ void orc_raypair_solve(int64_t ndep, int64_t n_ar, int64_t first, int64_t last,
  * every reduced ray; nthreads <= 0 uses all OpenMP threads.  J and Lambda of
  * the covered origins are reset first. */
 int  orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, int64_t o_end);
+/* the same for an explicit list of origins (bounded CPU-baseline samples) */
+int  orc_compute_radiation_field_list(orc_state *s, int nthreads, const int64_t *origins, int64_t n);
 /* src/Simulation/sim_lines.tpp:176-220 */
 void orc_compute_Jeff(orc_state *s);
 /* lineProducingSpecies.tpp:249-495 (per-cell dense LU, see oracle.c) */

@@ -64,6 +64,8 @@ def lib():
                                         vp, vp, C.c_double, C.c_double, vp, vp, vp, vp, vp, vp]
         L.orc_compute_radiation_field.argtypes = [vp, C.c_int, C.c_int64, C.c_int64]
         L.orc_compute_radiation_field.restype = C.c_int
+        L.orc_compute_radiation_field_list.argtypes = [vp, C.c_int, vp, C.c_int64]
+        L.orc_compute_radiation_field_list.restype = C.c_int
         L.orc_compute_Jeff.argtypes = [vp]
         L.orc_check_for_convergence.argtypes = [vp, C.c_double]
         L.orc_update_using_Ng_acceleration.argtypes = [vp]
@@ -163,6 +165,12 @@ class Oracle:
     # hot path
     def compute_radiation_field(self, nthreads=0, o_begin=0, o_end=-1):
         rc = lib().orc_compute_radiation_field(self._s, nthreads, o_begin, o_end)
+        if rc != 0:
+            raise RuntimeError("oracle radiation field failed (trace error)")
+
+    def compute_radiation_field_list(self, origins, nthreads=0):
+        o = np.ascontiguousarray(origins, dtype=np.int64)
+        rc = lib().orc_compute_radiation_field_list(self._s, nthreads, _p(o), o.shape[0])
         if rc != 0:
             raise RuntimeError("oracle radiation field failed (trace error)")
 
