@@ -181,6 +181,12 @@ int mgr_compute_level_populations(mgr_ctx *ctx, int use_Ng_acceleration,
                                   int64_t max_niterations, int criterion,
                                   int precision, int64_t *iterations_out);
 
+/* One iteration of that loop (sim_lines.tpp:101-132), replayed from the same
+ * hipGraphs: ng != 0 runs Lines::iteration_using_Ng_acceleration, otherwise
+ * compute_radiation_field + compute_Jeff + iteration_using_statistical_equilibrium.
+ * Lets a host driver keep the reference's per-iteration bookkeeping.          */
+int mgr_ali_iteration(mgr_ctx *ctx, int ng, int precision, mgr_conv *conv);
+
 /* ------------------------------------------------------------------------ */
 /* Data movement (the reference keeps these as Model members)                */
 /* ------------------------------------------------------------------------ */

@@ -1,0 +1,413 @@
+// simulation.cpp -- bodies of the drop-in Simulation methods over the C-ABI.
+#include "simulation.hpp"
+
+#include <cstdio>
+#include <stdexcept>
+
+namespace magritte {
+
+Simulation::~Simulation() { reset_device(); }
+
+void Simulation::reset_device()
+{
+    if (ctx_) mgr_destroy(ctx_);
+    ctx_ = nullptr;
+}
+
+void Simulation::check(int rc, const char *what)
+{
+    if (rc != MGR_OK) throw std::runtime_error(std::string(what) + " failed: " + mgr_last_error(ctx_));
+}
+
+// Upload of the Model (the arrays Model::read filled) into a device context.
+void Simulation::ensure_context()
+{
+    if (ctx_) return;
+    if (lines.lineProducingSpecies.size() != 1)
+        throw std::runtime_error("the GPU path supports one line-producing species");
+    const LineProducingSpecies &ls = lines.lineProducingSpecies[0];
+    const Linedata &ld = ls.linedata;
+    const Cells &cells = geometry.cells;
+    const size_t N = cells.position.size();
+    const size_t R = geometry.rays.rays.size();
+    auto need = [](bool ok, const char *what) {
+        if (!ok) throw std::runtime_error(std::string("model member has the wrong size: ") + what);
+    };
+    need(cells.velocity.size() == N, "geometry.cells.velocity");
+    need(cells.neighbors.size() == N, "geometry.cells.neighbors");
+    need(geometry.rays.weights.size() == R, "geometry.rays.weights");
+    need(thermodynamics.temperature.gas.size() == N, "thermodynamics.temperature.gas");
+    need(thermodynamics.turbulence.vturb2.size() == N, "thermodynamics.turbulence.vturb2");
+    need(chemistry.species.abundance.size() == N, "chemistry.species.abundance");
+    for (const auto &row : chemistry.species.abundance)
+        need(row.size() == chemistry.species.abundance[0].size() && (long)row.size() > ld.num,
+             "chemistry.species.abundance rows");
+    need(ld.irad.size() == (size_t)ld.nrad && ld.jrad.size() == (size_t)ld.nrad && ld.A.size() == (size_t)ld.nrad &&
+             ld.Bs.size() == (size_t)ld.nrad && ld.Ba.size() == (size_t)ld.nrad &&
+             ld.frequency.size() == (size_t)ld.nrad,
+         "linedata radiative arrays (nrad)");
+    need(ld.energy.size() == (size_t)ld.nlev && ld.weight.size() == (size_t)ld.nlev, "linedata level arrays (nlev)");
+    need(ls.quadrature.weights.size() == ls.quadrature.roots.size(), "quadrature");
+    for (const auto &cp : ld.colpar) {
+        need(cp.jcol.size() == cp.icol.size(), "collision partner jcol");
+        need(cp.Cd.size() == cp.tmp.size() && cp.Ce.size() == cp.tmp.size(), "collision partner Cd/Ce rows");
+        for (size_t t = 0; t < cp.tmp.size(); t++)
+            need(cp.Cd[t].size() == cp.icol.size() && cp.Ce[t].size() == cp.icol.size(), "collision partner Cd/Ce");
+    }
+
+    Double1 pos(3 * N), vel(3 * N), rays(3 * R), abundance;
+    Long1 n_nb(N), nbs;
+    for (size_t p = 0; p < N; p++)
+        for (int d = 0; d < 3; d++) {
+            pos[3 * p + d] = cells.position[p][d];
+            vel[3 * p + d] = cells.velocity[p][d];
+        }
+    for (size_t p = 0; p < N; p++) {
+        n_nb[p] = (long)cells.neighbors[p].size();
+        nbs.insert(nbs.end(), cells.neighbors[p].begin(), cells.neighbors[p].end());
+    }
+    for (size_t r = 0; r < R; r++)
+        for (int d = 0; d < 3; d++) rays[3 * r + d] = geometry.rays.rays[r][d];
+    const auto &ab = chemistry.species.abundance;
+    const size_t nspecs = ab.empty() ? 0 : ab[0].size();
+    for (size_t p = 0; p < N; p++) abundance.insert(abundance.end(), ab[p].begin(), ab[p].end());
+
+    std::vector<int64_t> n_nb64(n_nb.begin(), n_nb.end()), nbs64(nbs.begin(), nbs.end());
+    std::vector<int64_t> b2c(geometry.boundary.boundary2cell_nr.begin(), geometry.boundary.boundary2cell_nr.end());
+    std::vector<int64_t> irad(ld.irad.begin(), ld.irad.end()), jrad(ld.jrad.begin(), ld.jrad.end());
+    std::vector<int64_t> cp_num, cp_ncol, cp_ntmp, icol, jcol;
+    std::string cp_op;
+    Double1 tmp, Cd, Ce;
+    for (const auto &cp : ld.colpar) {
+        cp_num.push_back(cp.num_col_partner);
+        cp_ncol.push_back((int64_t)cp.icol.size());
+        cp_ntmp.push_back((int64_t)cp.tmp.size());
+        cp_op.push_back(cp.orth_or_para_H2.empty() ? 'n' : cp.orth_or_para_H2[0]);
+        icol.insert(icol.end(), cp.icol.begin(), cp.icol.end());
+        jcol.insert(jcol.end(), cp.jcol.begin(), cp.jcol.end());
+        tmp.insert(tmp.end(), cp.tmp.begin(), cp.tmp.end());
+        for (const auto &row : cp.Cd) Cd.insert(Cd.end(), row.begin(), row.end());
+        for (const auto &row : cp.Ce) Ce.insert(Ce.end(), row.begin(), row.end());
+    }
+
+    mgr_model_view v{};
+    v.ncells = (int64_t)N;
+    v.nrays = (int64_t)R;
+    v.nboundary = (int64_t)b2c.size();
+    v.nspecs = (int64_t)nspecs;
+    v.nquads = (int64_t)ls.quadrature.roots.size();
+    v.pop_prec = pop_prec();
+    v.max_width_fraction = parameters.max_width_fraction;
+    v.n_off_diag = parameters.n_off_diag;
+    v.position = pos.data();
+    v.velocity = vel.data();
+    v.n_neighbors = n_nb64.data();
+    v.neighbors = nbs64.data();
+    v.rays = rays.data();
+    v.weights = geometry.rays.weights.data();
+    v.boundary2cell_nr = b2c.data();
+    v.temperature = thermodynamics.temperature.gas.data();
+    v.vturb2 = thermodynamics.turbulence.vturb2.data();
+    v.abundance = abundance.data();
+    v.num = ld.num;
+    v.inverse_mass = ld.inverse_mass;
+    v.nlev = ld.nlev;
+    v.nrad = ld.nrad;
+    v.energy = ld.energy.data();
+    v.weight = ld.weight.data();
+    v.irad = irad.data();
+    v.jrad = jrad.data();
+    v.A = ld.A.data();
+    v.Bs = ld.Bs.data();
+    v.Ba = ld.Ba.data();
+    v.frequency = ld.frequency.data();
+    v.ncolpar = (int64_t)ld.colpar.size();
+    v.colpar_num = cp_num.data();
+    v.colpar_orth_or_para = cp_op.c_str();
+    v.colpar_ncol = cp_ncol.data();
+    v.colpar_ntmp = cp_ntmp.data();
+    v.icol = icol.data();
+    v.jcol = jcol.data();
+    v.tmp = tmp.data();
+    v.Cd = Cd.data();
+    v.Ce = Ce.data();
+    v.roots = ls.quadrature.roots.data();
+    v.qweights = ls.quadrature.weights.data();
+    const int rc = mgr_create(&v, device, &ctx_);
+    if (rc != MGR_OK) {
+        ctx_ = nullptr;
+        throw std::runtime_error(std::string("mgr_create failed: ") + mgr_last_error(nullptr));
+    }
+}
+
+// Simulation::compute_spectral_discretisation (sim_spectrum.tpp:13-196)
+int Simulation::compute_spectral_discretisation()
+{
+    ensure_context();
+    check(mgr_compute_spectral_discretisation(ctx_), "compute_spectral_discretisation");
+    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
+    const size_t N = geometry.cells.position.size();
+    const size_t K = ls.linedata.nrad, Q = ls.quadrature.roots.size(), F = K * Q;
+    Double1 nu(N * F);
+    std::vector<int64_t> nr(N * K * Q);
+    check(mgr_get_frequencies(ctx_, nu.data(), nr.data()), "get_frequencies");
+    Frequencies &fr = radiation.frequencies;
+    fr.nu.assign(N, Double1(F));
+    for (size_t p = 0; p < N; p++)
+        for (size_t f = 0; f < F; f++) fr.nu[p][f] = nu[p * F + f];
+    ls.nr_line.assign(N, Long2(K, Long1(Q)));
+    for (size_t p = 0; p < N; p++)
+        for (size_t k = 0; k < K; k++)
+            for (size_t z = 0; z < Q; z++) ls.nr_line[p][k][z] = (long)nr[(p * K + k) * Q + z];
+    fr.appears_in_line_integral.assign(F, false);
+    fr.corresponding_l_for_spec.assign(F, (long)F);
+    fr.corresponding_k_for_tran.assign(F, (long)F);
+    fr.corresponding_z_for_line.assign(F, (long)F);
+    for (size_t k = 0; k < K; k++)
+        for (size_t z = 0; z < Q; z++) {
+            fr.appears_in_line_integral[k * Q + z] = true;
+            fr.corresponding_l_for_spec[k * Q + z] = 0;
+            fr.corresponding_k_for_tran[k * Q + z] = (long)k;
+            fr.corresponding_z_for_line[k * Q + z] = (long)z;
+        }
+    specDiscSetting = LineSet;
+    return 0;
+}
+
+// Copies the device's boundary intensities into radiation.I_bdy; the
+// reference stores them per reduced ray, all rays hold the same values.
+void Simulation::pull_boundary_intensities()
+{
+    const size_t NB = geometry.boundary.boundary2cell_nr.size();
+    const size_t F = radiation.frequencies.nu.empty() ? 0 : radiation.frequencies.nu[0].size();
+    Double1 I(NB * F);
+    check(mgr_get_boundary_intensities(ctx_, I.data()), "get_boundary_intensities");
+    const size_t Rr = geometry.rays.rays.size() / 2;
+    radiation.I_bdy.assign(Rr, Double2(NB, Double1(F)));
+    for (size_t r = 0; r < Rr; r++)
+        for (size_t b = 0; b < NB; b++)
+            for (size_t f = 0; f < F; f++) radiation.I_bdy[r][b][f] = I[b * F + f];
+}
+
+// Simulation::compute_boundary_intensities (sim_radiation.tpp:6-14): CMB on
+// every boundary cell
+int Simulation::compute_boundary_intensities()
+{
+    ensure_context();
+    check(mgr_compute_boundary_intensities(ctx_, nullptr), "compute_boundary_intensities");
+    pull_boundary_intensities();
+    return 0;
+}
+
+// sim_radiation.tpp:17-40: Planck at the given temperatures; a size mismatch
+// returns -1
+int Simulation::compute_boundary_intensities(const Double1 &temperatures)
+{
+    if (temperatures.size() != geometry.boundary.boundary2cell_nr.size()) {
+        std::fprintf(stderr, "ERROR : temperatures.size() != nboundary\n");
+        return -1;
+    }
+    ensure_context();
+    check(mgr_compute_boundary_intensities(ctx_, temperatures.data()), "compute_boundary_intensities");
+    pull_boundary_intensities();
+    return 0;
+}
+
+void Simulation::pull_populations()
+{
+    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
+    const size_t N = geometry.cells.position.size();
+    ls.population.resize(N * ls.linedata.nlev);
+    ls.population_prev1.resize(ls.population.size());
+    ls.population_prev2.resize(ls.population.size());
+    ls.population_prev3.resize(ls.population.size());
+    check(mgr_get_populations(ctx_, ls.population.data()), "get_populations");
+    check(mgr_get_population_history(ctx_, ls.population_prev1.data(), ls.population_prev2.data(),
+                                     ls.population_prev3.data()),
+          "get_population_history");
+    lines.emissivity.resize(N * ls.linedata.nrad);
+    lines.opacity.resize(N * ls.linedata.nrad);
+    check(mgr_get_emissivity_opacity(ctx_, lines.emissivity.data(), lines.opacity.data()),
+          "get_emissivity_opacity");
+}
+
+void Simulation::push_populations()
+{
+    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
+    check(mgr_set_populations(ctx_, ls.population.data()), "set_populations");
+    check(mgr_set_population_history(ctx_, ls.population_prev1.data(), ls.population_prev2.data(),
+                                     ls.population_prev3.data()),
+          "set_population_history");
+}
+
+// Simulation::compute_LTE_level_populations (sim_lines.tpp:10-16)
+int Simulation::compute_LTE_level_populations()
+{
+    ensure_context();
+    check(mgr_compute_LTE_level_populations(ctx_), "compute_LTE_level_populations");
+    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
+    const size_t N = geometry.cells.position.size();
+    ls.population_tot.resize(N);
+    for (size_t p = 0; p < N; p++) ls.population_tot[p] = chemistry.species.abundance[p][ls.linedata.num];
+    pull_populations();
+    return 0;
+}
+
+void Simulation::pull_radiation()
+{
+    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
+    const size_t N = geometry.cells.position.size();
+    const size_t K = ls.linedata.nrad;
+    const size_t F = K * ls.quadrature.roots.size();
+    radiation.J.resize(N * F);
+    check(mgr_get_J(ctx_, radiation.J.data()), "get_J");
+    Double1 Ls(N * K);
+    check(mgr_get_lambda(ctx_, Ls.data()), "get_lambda");
+    // n_off_diag = 0: one ALO element per (p, k), at nr = p, when any was added
+    ls.lambda.Ls.assign(N, Double2(K));
+    ls.lambda.nr.assign(N, Long2(K));
+    ls.lambda.size.assign(N * K, 0);
+    for (size_t p = 0; p < N; p++)
+        for (size_t k = 0; k < K; k++)
+            if (Ls[p * K + k] != 0.0) {
+                ls.lambda.Ls[p][k] = {Ls[p * K + k]};
+                ls.lambda.nr[p][k] = {(long)p};
+                ls.lambda.size[k + K * p] = 1;
+            }
+}
+
+void Simulation::pull_jeff()
+{
+    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
+    const size_t N = geometry.cells.position.size();
+    const size_t K = ls.linedata.nrad;
+    Double1 a(N * K), b(N * K), c(N * K);
+    check(mgr_get_Jeff(ctx_, a.data(), b.data(), c.data()), "get_Jeff");
+    ls.Jlin.assign(N, Double1(K));
+    ls.Jeff.assign(N, Double1(K));
+    ls.Jdif.assign(N, Double1(K));
+    for (size_t p = 0; p < N; p++)
+        for (size_t k = 0; k < K; k++) {
+            ls.Jlin[p][k] = a[p * K + k];
+            ls.Jeff[p][k] = b[p * K + k];
+            ls.Jdif[p][k] = c[p * K + k];
+        }
+}
+
+int Simulation::run_radiation_field(int prec)
+{
+    ensure_context();
+    push_populations();
+    check(mgr_compute_radiation_field(ctx_, prec), "compute_radiation_field");
+    double ms = 0.0;
+    mgr_last_radiation_field_ms(ctx_, &ms);
+    int64_t sum_ndep = 0, pairs = 0;
+    mgr_trace_stats(ctx_, &sum_ndep, &pairs);
+    timings.push_back(ms);
+    nrpairs.push_back((long)pairs);
+    depths.push_back((long)sum_ndep);
+    pull_radiation();
+    return 0;
+}
+
+// Simulation::compute_radiation_field (sim_radiation.tpp:56-338)
+int Simulation::compute_radiation_field() { return run_radiation_field(precision); }
+// the reference's FP64 CPU path: here the FP64 variant of the GPU solve
+int Simulation::compute_radiation_field_cpu() { return run_radiation_field(MGR_FP64); }
+int Simulation::compute_radiation_field_gpu() { return run_radiation_field(MGR_FP32); }
+// The block and grid arguments of the reference's launch are not needed: the
+// kernel maps one workgroup per origin cell.
+int Simulation::gpu_compute_radiation_field(size_t, size_t, size_t, double) { return run_radiation_field(MGR_FP32); }
+int Simulation::cpu_compute_radiation_field(size_t, size_t, size_t, double) { return run_radiation_field(MGR_FP64); }
+
+int Simulation::gpu_get_device_properties()
+{
+    int n = 0;
+    mgr_device_count(&n);
+    std::printf("HIP devices: %d (gfx950 library)\n", n);
+    return 0;
+}
+
+// Simulation::compute_Jeff (sim_lines.tpp:176-220)
+void Simulation::compute_Jeff()
+{
+    ensure_context();
+    push_populations();
+    check(mgr_compute_Jeff(ctx_), "compute_Jeff");
+    pull_jeff();
+}
+
+// Simulation::compute_level_populations_from_stateq (sim_lines.tpp:24-30)
+void Simulation::compute_level_populations_from_stateq()
+{
+    ensure_context();
+    mgr_conv conv{};
+    check(mgr_compute_level_populations_from_stateq(ctx_, &conv), "compute_level_populations_from_stateq");
+    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
+    ls.fraction_not_converged = conv.fraction_not_converged;
+    ls.relative_change_mean = conv.relative_change_mean;
+    ls.relative_change_max = conv.relative_change_max;
+    pull_populations();
+}
+
+long Simulation::compute_level_populations(const Io &io)
+{
+    return compute_level_populations(io, true, 1000);   // sim_lines.tpp:42-48
+}
+
+// Simulation::compute_level_populations (sim_lines.tpp:61-164): the same
+// iteration count, Ng schedule, convergence test and per-iteration writes;
+// each iteration is one hipGraph replay on the device.
+long Simulation::compute_level_populations(const Io &io, bool use_Ng_acceleration, long max_niterations)
+{
+    if (specDiscSetting != LineSet) {
+        std::fprintf(stderr, "Error: Spectral discretisation was not set for Lines!\n");
+        return -1;
+    }
+    ensure_context();
+    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
+    ls.write_populations(io, 0, "_rank_0_iteration_0");
+    push_populations();
+    long iteration = 0, iteration_normal = 0;
+    error_mean.clear();
+    error_max.clear();
+    bool some_not_converged = true;
+    while (some_not_converged && iteration < max_niterations) {
+        iteration++;
+        some_not_converged = false;
+        const bool ng = use_Ng_acceleration && (iteration_normal == 4);
+        mgr_conv conv{};
+        check(mgr_ali_iteration(ctx_, ng ? 1 : 0, precision, &conv), "ALI iteration");
+        iteration_normal = ng ? 0 : iteration_normal + 1;
+        ls.fraction_not_converged = conv.fraction_not_converged;
+        ls.relative_change_mean = conv.relative_change_mean;
+        ls.relative_change_max = conv.relative_change_max;
+        error_mean.push_back(conv.relative_change_mean);
+        error_max.push_back(conv.relative_change_max);
+        if (convergence_criterion == MGR_CRIT_MAX) {
+            if (conv.relative_change_max >= pop_prec()) some_not_converged = true;
+        } else if (conv.fraction_not_converged > 0.005) {
+            some_not_converged = true;
+        }
+        pull_populations();
+        if (!ng) pull_jeff();
+        ls.write_populations(io, 0, "_rank_0_iteration_" + std::to_string(iteration));
+    }
+    pull_radiation();
+    return iteration;
+}
+
+Long2 Simulation::get_npoints_on_rays_comoving()
+{
+    ensure_context();
+    check(mgr_trace(ctx_), "trace");
+    const size_t N = geometry.cells.position.size(), Rr = geometry.rays.rays.size() / 2;
+    std::vector<int64_t> np(N * Rr);
+    check(mgr_get_npoints_on_rays(ctx_, np.data()), "get_npoints_on_rays");
+    Long2 out(Rr, Long1(N));
+    for (size_t r = 0; r < Rr; r++)
+        for (size_t o = 0; o < N; o++) out[r][o] = (long)np[r * N + o];
+    return out;
+}
+
+}  // namespace magritte

@@ -42,6 +42,8 @@ constexpr double CHI_FLOOR   = 1.0E-26;
 constexpr double CHI_SCALE = 73786976294838206464.0;     // 2^66 ~ 7.4e19
 constexpr double I_SCALE   = 1125899906842624.0;         // 2^50 ~ 1.1e15
 
+constexpr double SQRT_LOG2E = 1.2011224087864498;   // sqrt(log2(e))
+
 constexpr int WAVE = 64;                 // CDNA wavefront width
 constexpr int MAX_INTERPOLATION = 10000; // geometry.tpp:138
 
@@ -195,6 +197,7 @@ int launch_cell_tables(mgr_ctx *ctx);
 int launch_emissivity_opacity(mgr_ctx *ctx);
 int launch_trace(mgr_ctx *ctx);
 int launch_radiation_field(mgr_ctx *ctx, int precision);
+int launch_lambda(mgr_ctx *ctx);
 int launch_jeff(mgr_ctx *ctx);
 int launch_stateq(mgr_ctx *ctx);
 int launch_convergence(mgr_ctx *ctx);

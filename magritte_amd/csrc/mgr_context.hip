@@ -186,7 +186,8 @@ __global__ void k_emissivity_opacity(int64_t N, const LineTables *__restrict__ t
     const double ks = o * CHI_SCALE * pref;
     const double iw = IW64[p * L + l];
     row64[p * L + l] = make_double4(W11[p * L + l], iw, es, ks);
-    row32[p * L + l] = make_float4((float)iw, (float)es, (float)ks, 0.0f);
+    // FP32 row: 1/width scaled by sqrt(log2 e) so that exp(-x^2) = exp2(-x'^2)
+    row32[p * L + l] = make_float4((float)(iw * SQRT_LOG2E), (float)es, (float)ks, 0.0f);
 }
 
 namespace mgr {

@@ -22,25 +22,20 @@ namespace {
 
 constexpr int32_t ERR_SINGULAR = 16;
 
-// Simulation::compute_Jeff with the Lambda* diagonal of update_Lambda
-// (raypair_lambda.tpp:94-149): Ls(p,k) = sum over bins f whose back-map names
-// transition k of A_k w_z nu phi_k(nu) * [sum_r 2 w_r L_diag / chi](p,f).
-__global__ void k_jeff(int64_t N, int L, int Q, int F, int V, const double *__restrict__ J,
-                       const double *__restrict__ Lam, const int32_t *__restrict__ nr_line,
-                       const double *__restrict__ qw, const double *__restrict__ nu,
-                       const double *__restrict__ pw, const double *__restrict__ A,
-                       const double *__restrict__ frequency, const int32_t *__restrict__ irad,
-                       const int32_t *__restrict__ lam_f_off, const int32_t *__restrict__ lam_f,
-                       const int32_t *__restrict__ f_z, const double *__restrict__ pop,
-                       double *__restrict__ Ls, double *__restrict__ Jlin,
-                       double *__restrict__ Jeff, double *__restrict__ Jdif)
+// The Lambda* diagonal of RayPair::update_Lambda (raypair_lambda.tpp:94-149),
+// n_off_diag = 0: Ls(p,k) = sum over bins f whose back-map names transition k
+// of A_k w_z nu phi_k(nu) * [sum_r 2 w_r L_diag / chi](p,f).
+__global__ void k_lambda(int64_t N, int L, int F, const double *__restrict__ Lam,
+                         const double *__restrict__ qw, const double *__restrict__ nu,
+                         const double *__restrict__ pw, const double *__restrict__ A,
+                         const double *__restrict__ frequency, const int32_t *__restrict__ lam_f_off,
+                         const int32_t *__restrict__ lam_f, const int32_t *__restrict__ f_z,
+                         double *__restrict__ Ls)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N * L) return;
     const int64_t p = t / L;
     const int k = (int)(t % L);
-    double jl = 0.0;
-    for (int z = 0; z < Q; z++) jl += qw[z] * J[p * F + nr_line[(p * L + k) * Q + z]];
     const double freq_line = frequency[k];
     const double width = freq_line * pw[p];
     const double iw = 1.0 / width;
@@ -53,7 +48,23 @@ __global__ void k_jeff(int64_t N, int L, int Q, int F, int V, const double *__re
         ls += A[k] * qw[f_z[f]] * (nuf * prof * Lam[p * F + f]);
     }
     Ls[p * L + k] = ls;
-    const double diff = ls * pop[p * V + irad[k]];
+}
+
+// Simulation::compute_Jeff (sim_lines.tpp:176-220): Jlin = sum_z w_z J(p, nr_line),
+// Jeff = Jlin - h/(4 pi) Ls n_i, Jdif = h/(4 pi) Ls n_i.
+__global__ void k_jeff(int64_t N, int L, int Q, int F, int V, const double *__restrict__ J,
+                       const int32_t *__restrict__ nr_line, const double *__restrict__ qw,
+                       const int32_t *__restrict__ irad, const double *__restrict__ pop,
+                       const double *__restrict__ Ls, double *__restrict__ Jlin,
+                       double *__restrict__ Jeff, double *__restrict__ Jdif)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * L) return;
+    const int64_t p = t / L;
+    const int k = (int)(t % L);
+    double jl = 0.0;
+    for (int z = 0; z < Q; z++) jl += qw[z] * J[p * F + nr_line[(p * L + k) * Q + z]];
+    const double diff = Ls[p * L + k] * pop[p * V + irad[k]];
     Jlin[p * L + k] = jl;
     Jeff[p * L + k] = jl - HH_OVER_FOUR_PI * diff;
     Jdif[p * L + k] = HH_OVER_FOUR_PI * diff;
@@ -356,12 +367,20 @@ int conv_blocks(mgr_ctx *ctx)
 
 namespace mgr {
 
+int launch_lambda(mgr_ctx *ctx)
+{
+    hipLaunchKernelGGL(k_lambda, dim3(grid_for(ctx->N * ctx->L, 256)), dim3(256), 0, ctx->stream,
+                       ctx->N, (int)ctx->L, (int)ctx->F, ctx->Lam, ctx->qweights, ctx->nu, ctx->pw,
+                       ctx->A, ctx->frequency, ctx->d_lam_f_off, ctx->d_lam_f, ctx->f_z, ctx->Ls);
+    MGR_LAUNCH_CHECK();
+    return MGR_OK;
+}
+
 int launch_jeff(mgr_ctx *ctx)
 {
     hipLaunchKernelGGL(k_jeff, dim3(grid_for(ctx->N * ctx->L, 256)), dim3(256), 0, ctx->stream,
-                       ctx->N, (int)ctx->L, (int)ctx->Q, (int)ctx->F, (int)ctx->V, ctx->J, ctx->Lam,
-                       ctx->nr_line, ctx->qweights, ctx->nu, ctx->pw, ctx->A, ctx->frequency, ctx->irad,
-                       ctx->d_lam_f_off, ctx->d_lam_f, ctx->f_z, ctx->pop, ctx->Ls, ctx->Jlin,
+                       ctx->N, (int)ctx->L, (int)ctx->Q, (int)ctx->F, (int)ctx->V, ctx->J,
+                       ctx->nr_line, ctx->qweights, ctx->irad, ctx->pop, ctx->Ls, ctx->Jlin,
                        ctx->Jeff, ctx->Jdif);
     MGR_LAUNCH_CHECK();
     return MGR_OK;
@@ -524,6 +543,38 @@ int mgr_iteration_using_Ng_acceleration(mgr_ctx *ctx, mgr_conv *conv)
     return read_convergence(ctx, conv);
 }
 
+static int ensure_graphs(mgr_ctx *ctx, int precision)
+{
+    int rc;
+    if (!ctx->traced && (rc = launch_trace(ctx))) return rc;
+    if (ctx->graph_precision != precision) {
+        if (ctx->graph_normal) (void)hipGraphExecDestroy(ctx->graph_normal);
+        if (ctx->graph_ng) (void)hipGraphExecDestroy(ctx->graph_ng);
+        ctx->graph_normal = ctx->graph_ng = nullptr;
+        ctx->graph_precision = -1;
+        if ((rc = build_graph(ctx, false, precision, &ctx->graph_normal))) return rc;
+        if ((rc = build_graph(ctx, true, precision, &ctx->graph_ng))) return rc;
+        ctx->graph_precision = precision;
+    }
+    return MGR_OK;
+}
+
+int mgr_ali_iteration(mgr_ctx *ctx, int ng, int precision, mgr_conv *conv)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (!ctx->spectral_set) return ctx->fail(MGR_ERR_STATE, "Spectral discretisation was not set for Lines");
+    if (!ctx->bdy_set) return ctx->fail(MGR_ERR_STATE, "compute_boundary_intensities first");
+    if (!ctx->pops_set) return ctx->fail(MGR_ERR_STATE, "level populations not set");
+    if (precision != MGR_FP32 && precision != MGR_FP64) return ctx->fail(MGR_ERR_ARGUMENT, "bad precision");
+    MGR_HIP(hipSetDevice(ctx->device));
+    int rc = ensure_graphs(ctx, precision);
+    if (rc) return rc;
+    MGR_HIP(hipGraphLaunch(ng ? ctx->graph_ng : ctx->graph_normal, ctx->stream));
+    rc = read_convergence(ctx, conv);
+    if (rc == MGR_OK && !ng) ctx->J_set = true;
+    return rc;
+}
+
 int mgr_compute_level_populations(mgr_ctx *ctx, int use_Ng_acceleration, int64_t max_niterations,
                                   int criterion, int precision, int64_t *iterations_out)
 {
@@ -536,16 +587,8 @@ int mgr_compute_level_populations(mgr_ctx *ctx, int use_Ng_acceleration, int64_t
     if (precision != MGR_FP32 && precision != MGR_FP64) return ctx->fail(MGR_ERR_ARGUMENT, "bad precision");
     if (criterion != MGR_CRIT_FRACTION && criterion != MGR_CRIT_MAX) return ctx->fail(MGR_ERR_ARGUMENT, "bad criterion");
     MGR_HIP(hipSetDevice(ctx->device));
-    int rc;
-    if (!ctx->traced && (rc = launch_trace(ctx))) return rc;
-    if (ctx->graph_precision != precision) {
-        if (ctx->graph_normal) (void)hipGraphExecDestroy(ctx->graph_normal);
-        if (ctx->graph_ng) (void)hipGraphExecDestroy(ctx->graph_ng);
-        ctx->graph_normal = ctx->graph_ng = nullptr;
-        if ((rc = build_graph(ctx, false, precision, &ctx->graph_normal))) return rc;
-        if ((rc = build_graph(ctx, true, precision, &ctx->graph_ng))) return rc;
-        ctx->graph_precision = precision;
-    }
+    int rc = ensure_graphs(ctx, precision);
+    if (rc) return rc;
     int64_t iteration = 0, iteration_normal = 0;
     bool some_not_converged = true;
     while (some_not_converged && iteration < max_niterations) {

@@ -73,8 +73,8 @@ template <> struct Prec<double> {
 
 // One traced step in LDS.  Sub-point m = 1..n of the step lies at
 // shift = s0 + m ds (FP64: s0 = shift of the current point, FP32: s0 and ds
-// are shift - 1 and its increment) with path length dz (scaled) and belongs to
-// the current cell for m < half, to `cell` otherwise.
+// are shift - 1 and its increment) with half path length dz (scaled) and
+// belongs to the current cell for m < half, to `cell` otherwise.
 template <typename real>
 struct Step {
     int32_t cell, n, half, pad;
@@ -134,33 +134,52 @@ __device__ __forceinline__ void eta_chi64(const RFArgs<double> &a, const double4
     chi = c_acc;
 }
 
-// FP32: sum over the lane's candidate lines la .. la+nc-1.
+// FP32: the lane's candidate lines la .. la+nc-1.  The first candidate is
+// evaluated unconditionally (a lane without candidates carries an offset so
+// large that its term is exactly zero); further candidates behind
+// wave-uniform flags.  Rows hold 1/width scaled by sqrt(log2 e), so
+// exp(-x^2) = exp2(-x'^2).
 struct Lane32 {
-    float nuf;       // nu_f
-    float A0, A1;    // nu_f - nu_l for the first two candidates
-    int   la, nc;
+    float  nuf;        // nu_f
+    float  A0, A1;     // nu_f - nu_l of the first two candidates (or 1e30)
+    int    off0, off1; // byte offsets of their rows
+    int    la, nc;
+    bool   has2, has3; // wave-uniform: some lane has 2 / more than 2 candidates
     double nu_f64;
 };
 
-__device__ __forceinline__ void line_term32(const float4 v, float A, float delta, float nuf, float nus,
-                                            float &e_acc, float &c_acc)
+__device__ __forceinline__ float rfl(float x)
 {
-    const float x = fmaf(delta, nuf, A) * v.x;
-    const float g = nus * __expf(-(x * x));
-    e_acc = fmaf(v.y, g, e_acc);
-    c_acc = fmaf(v.z, g, c_acc);
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
 }
 
 __device__ __forceinline__ void eta_chi32(const RFArgs<float> &a, const float4 *__restrict__ row,
                                           const Lane32 &ln, float delta, float &eta, float &chi)
 {
-    const float nus = fmaf(delta, ln.nuf, ln.nuf);
-    float e_acc = 0.0f, c_acc = (float)(CHI_FLOOR * CHI_SCALE);
-    if (ln.nc > 0) line_term32(row[ln.la], ln.A0, delta, ln.nuf, nus, e_acc, c_acc);
-    if (ln.nc > 1) line_term32(row[ln.la + 1], ln.A1, delta, ln.nuf, nus, e_acc, c_acc);
-    for (int c = 2; c < ln.nc; c++) {
-        const int l = ln.la + c;
-        line_term32(row[l], (float)(ln.nu_f64 - a.tb->line[l]), delta, ln.nuf, nus, e_acc, c_acc);
+    const char *base = reinterpret_cast<const char *>(row);
+    const float t = delta * ln.nuf;
+    const float nus = t + ln.nuf;
+    const float4 v0 = *reinterpret_cast<const float4 *>(base + ln.off0);
+    const float x0 = (t + ln.A0) * v0.x;
+    const float g0 = nus * __builtin_amdgcn_exp2f(-(x0 * x0));
+    float e_acc = v0.y * g0;
+    float c_acc = fmaf(v0.z, g0, (float)(CHI_FLOOR * CHI_SCALE));
+    if (ln.has2) {
+        const float4 v1 = *reinterpret_cast<const float4 *>(base + ln.off1);
+        const float x1 = (t + ln.A1) * v1.x;
+        const float g1 = nus * __builtin_amdgcn_exp2f(-(x1 * x1));
+        e_acc = fmaf(v1.y, g1, e_acc);
+        c_acc = fmaf(v1.z, g1, c_acc);
+        if (ln.has3) {
+            for (int c = 2; c < ln.nc; c++) {
+                const int l = ln.la + c;
+                const float4 v = row[l];
+                const float x = (t + (float)(ln.nu_f64 - a.tb->line[l])) * v.x;
+                const float g = nus * __builtin_amdgcn_exp2f(-(x * x));
+                e_acc = fmaf(v.y, g, e_acc);
+                c_acc = fmaf(v.z, g, c_acc);
+            }
+        }
     }
     eta = e_acc;
     chi = c_acc;
@@ -186,50 +205,45 @@ __device__ __forceinline__ double rescale_I_bdy(const double *__restrict__ nup,
 }
 
 // ---- Feautrier elimination state of one side ----------------------------------
+//
+// The sweep starts from a virtual outer state D = 0, 1/D = 1, f = 1, Z = I_bdy
+// with which the interior recurrence reproduces the boundary row exactly:
+// c = D_n^2/2, r = D_n, F = D_n + D_n^2/2, Z = (S D_n^2/2 + I D_n)/(1+F).
 
 template <typename real>
 struct Sweep {
-    bool have_q, first_row;
-    real eta_q, chi_q, dz_q, Ib;
+    real S_q, chi_q, hdz_q;   // pending (outer) point: source function, opacity, dz/2
     real f, Z, D, invD;
 
     __device__ __forceinline__ void init(real I_far)
     {
-        have_q = false;
-        first_row = true;
-        Ib = I_far;
-        eta_q = chi_q = dz_q = 0;
-        f = Z = D = invD = 0;
+        S_q = chi_q = hdz_q = 0;
+        f = 1;
+        Z = I_far;
+        D = 0;
+        invD = 1;
     }
 
-    // New point (eta, chi) inwards of the pending one: close the pending row.
-    __device__ __forceinline__ void push(real eta, real chi, real dz)
+    __device__ __forceinline__ void pend(real S, real chi, real hdz)
     {
-        if (have_q) {
-            const real Dn = (real)0.5 * (chi_q + chi) * dz_q;
-            const real S_q = eta_q * Prec<real>::rcp(chi_q);
-            real Fn, cc, rr, Zs;
-            if (first_row) {
-                cc = (real)0.5 * Dn * Dn;
-                Fn = Dn + cc;
-                Zs = S_q * cc + Ib * Dn;
-                first_row = false;
-            } else {
-                rr = Dn * invD;
-                cc = (real)0.5 * (D + Dn) * Dn;
-                Fn = cc + rr * f;
-                Zs = S_q * cc + rr * Z;
-            }
-            const real q = Prec<real>::rcp((real)1 + Fn);
-            Z = Zs * q;
-            f = Fn * q;
-            D = Dn;
-            invD = Prec<real>::rcp(Dn);
-        }
-        eta_q = eta;
+        S_q = S;
         chi_q = chi;
-        dz_q = dz;
-        have_q = true;
+        hdz_q = hdz;
+    }
+
+    // close the pending row with the next point (S, chi) inwards
+    __device__ __forceinline__ void step(real S, real chi)
+    {
+        const real Dn = (chi_q + chi) * hdz_q;
+        const real rr = Dn * invD;
+        const real cc = (D + Dn) * ((real)0.5 * Dn);
+        const real Fn = rr * f + cc;
+        const real q = Prec<real>::rcp((real)1 + Fn);
+        const real Zs = S_q * cc + rr * Z;
+        Z = Zs * q;
+        f = Fn * q;
+        D = Dn;
+        invD = Prec<real>::rcp(Dn);
     }
 };
 
@@ -267,7 +281,7 @@ __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, in
                 s.s0 = (real)((sp.interp ? sc : sn) - 1.0);
                 s.ds = (real)sp.dshift;
             }
-            s.dz = (real)(sp.dZ * (1.0 / CHI_SCALE));
+            s.dz = (real)(sp.dZ * (0.5 / CHI_SCALE));   // dz/2
         }
         st[e] = s;
     }
@@ -340,11 +354,27 @@ __device__ __forceinline__ void eval_point<double>(const RFArgs<double> &a, cons
     eta_chi64(a, row, fs, lc.lstart, eta, chi);
 }
 
+// shift of sub-point m: FP64 exactly s0 + m ds as Geometry::set_data;
+// FP32 the offset shift - 1 in one fused multiply-add.
+__device__ __forceinline__ double sub_point_shift(double s0, int m, double ds)
+{
+#pragma clang fp contract(off)
+    return s0 + (double)m * ds;
+}
+__device__ __forceinline__ float sub_point_shift(float s0, int m, float ds)
+{
+    return fmaf((float)m, ds, s0);
+}
+
+__device__ __forceinline__ int rfl_int(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float rfl_real(float x) { return rfl(x); }
+__device__ __forceinline__ double rfl_real(double x) { return x; }
+
 // One ray of a pair, eliminated from its boundary end inwards to the origin.
 template <typename real>
 __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const LaneCtx<real> &lc,
-                                                  int32_t o, int64_t q, double nu_f, real eta_o,
-                                                  real chi_o, double dsm, int fa, int F,
+                                                  int32_t o, int64_t q, double nu_f, real S_o,
+                                                  real chi_o, double dsm, int F,
                                                   Step<real> *__restrict__ st,
                                                   typename Prec<real>::row_t *__restrict__ rw,
                                                   int32_t &err)
@@ -369,6 +399,7 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
         }
     }
     sw.init(Ib);
+    bool have_q = false;   // wave-uniform
     const int ns = a.nseg[q];
     const int L = a.L;
     const int CH = a.ch;
@@ -378,24 +409,25 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
         __syncthreads();   // the previous chunk is consumed by every wave
         stage_chunk<real>(a, o, q, a_ - 1, cnt, dsm, st, rw);
         for (int e = cnt - 1; e >= 1; --e) {
-            const Step<real> s = st[e];
+            const int n = rfl_int(st[e].n);
+            const int half = rfl_int(st[e].half);
+            const real s0 = st[e].s0;   // stays in a VGPR: one scalar operand per FMA
+            const real ds = rfl_real(st[e].ds);
+            const real hdz = rfl_real(st[e].dz);
             const row_t *rn = rw + e * L;
             const row_t *rc = rw + (e - 1) * L;
-            for (int m = s.n; m >= 1; --m) {
-                const row_t *r = (m < s.half) ? rc : rn;
-                real sh;
-                {
-#pragma clang fp contract(off)
-                    sh = s.s0 + (real)m * s.ds;
-                }
+            for (int m = n; m >= 1; --m) {
+                const row_t *r = (m < half) ? rc : rn;
                 real eta, chi;
-                eval_point<real>(a, lc, r, sh, eta, chi);
-                sw.push(eta, chi, s.dz);
+                eval_point<real>(a, lc, r, sub_point_shift(s0, m, ds), eta, chi);
+                const real S = eta * Prec<real>::rcp(chi);
+                if (have_q) sw.step(S, chi);
+                have_q = true;
+                sw.pend(S, chi, hdz);
             }
         }
     }
-    sw.push(eta_o, chi_o, (real)0);   // the origin closes the innermost row
-    (void)fa;
+    sw.step(S_o, chi_o);   // the origin closes the innermost row
     return sw;
 }
 
@@ -435,12 +467,19 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
         while (la < a.L && a.tb->line[la] * a.cand_lo < nu_f) la++;
         int lb = la;
         while (lb < a.L && a.tb->line[lb] * a.cand_hi <= nu_f) lb++;
-        lc.ln.la = la < a.L ? la : a.L - 1;
-        lc.ln.nc = lb - la;
+        const int nc = lb - la;
+        const int l0 = la < a.L ? la : a.L - 1;
+        const int l1 = la + 1 < a.L ? la + 1 : a.L - 1;
+        lc.ln.la = l0;
+        lc.ln.nc = nc;
         lc.ln.nuf = (float)nu_f;
         lc.ln.nu_f64 = nu_f;
-        lc.ln.A0 = (lc.ln.nc > 0) ? (float)(nu_f - a.tb->line[la]) : 0.0f;
-        lc.ln.A1 = (lc.ln.nc > 1) ? (float)(nu_f - a.tb->line[la + 1]) : 0.0f;
+        lc.ln.A0 = (nc > 0) ? (float)(nu_f - a.tb->line[l0]) : 1.0e30f;
+        lc.ln.A1 = (nc > 1) ? (float)(nu_f - a.tb->line[l1]) : 1.0e30f;
+        lc.ln.off0 = l0 * (int)sizeof(float4);
+        lc.ln.off1 = l1 * (int)sizeof(float4);
+        lc.ln.has2 = __any(nc > 1);
+        lc.ln.has3 = __any(nc > 2);
         eval_point<real>(a, lc, orow, (real)0, eta_o, chi_o);
     } else {
         int ls = 0;
@@ -469,15 +508,15 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
         }
         real u, lam;
         if (n_r > 0 && n_a > 0) {
-            const Sweep<real> A = sweep_side<real>(a, lc, o, qa, nu_f, eta_o, chi_o, dsm, fa, F, st, rw, err);
-            const Sweep<real> B = sweep_side<real>(a, lc, o, qr, nu_f, eta_o, chi_o, dsm, fa, F, st, rw, err);
+            const Sweep<real> A = sweep_side<real>(a, lc, o, qa, nu_f, S_o, chi_o, dsm, F, st, rw, err);
+            const Sweep<real> B = sweep_side<real>(a, lc, o, qr, nu_f, S_o, chi_o, dsm, F, st, rw, err);
             const real h = (real)0.5 * (A.D + B.D);
             const real den = Prec<real>::rcp(h + A.f * A.invD + B.f * B.invD);
             u = (S_o * h + A.Z * A.invD + B.Z * B.invD) * den;
             lam = h * den;
         } else {
             const int64_t qs = (n_a > 0) ? qa : qr;
-            const Sweep<real> A = sweep_side<real>(a, lc, o, qs, nu_f, eta_o, chi_o, dsm, fa, F, st, rw, err);
+            const Sweep<real> A = sweep_side<real>(a, lc, o, qs, nu_f, S_o, chi_o, dsm, F, st, rw, err);
             real I_o = 0;
             if (bo < 0) err |= ERR_EMPTY_SIDE;
             else I_o = (real)(a.I_bdy[(int64_t)bo * F + fa] * I_SCALE);
@@ -555,8 +594,10 @@ namespace mgr {
 
 int launch_radiation_field(mgr_ctx *ctx, int precision)
 {
-    if (precision == MGR_FP64) return launch_rf<double>(ctx, ctx->row64);
-    return launch_rf<float>(ctx, ctx->row32);
+    const int rc = (precision == MGR_FP64) ? launch_rf<double>(ctx, ctx->row64)
+                                           : launch_rf<float>(ctx, ctx->row32);
+    if (rc) return rc;
+    return launch_lambda(ctx);   // update_Lambda's sums over the line bins
 }
 
 }  // namespace mgr

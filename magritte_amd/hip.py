@@ -36,6 +36,7 @@ SIGNATURES = {
     "mgr_compute_level_populations_from_stateq": ([_vp, _vp], _i),
     "mgr_iteration_using_Ng_acceleration": ([_vp, _vp], _i),
     "mgr_compute_level_populations": ([_vp, _i, _i64, _i, _i, _vp], _i),
+    "mgr_ali_iteration": ([_vp, _i, _i, _vp], _i),
     "mgr_set_populations": ([_vp, _vp], _i),
     "mgr_get_populations": ([_vp, _vp], _i),
     "mgr_get_population_history": ([_vp, _vp, _vp, _vp], _i),
@@ -169,6 +170,12 @@ class Context:
                                                            crit, _prec(precision), C.byref(it)),
                     "compute_level_populations")
         return it.value
+
+    def ali_iteration(self, ng=False, precision="fp32"):
+        c = ConvStats()
+        self._check(self.lib.mgr_ali_iteration(self.h, 1 if ng else 0, _prec(precision), C.byref(c)),
+                    "ali_iteration")
+        return c.fraction_not_converged, c.relative_change_mean, c.relative_change_max
 
     # ---- data ---------------------------------------------------------------
     def J(self):

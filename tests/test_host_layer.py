@@ -1,0 +1,126 @@
+"""Drop-in host layer (magritte_amd.core: Io, Model, Simulation).
+
+CPU: a model written with IoText and read back into a fresh Model carries the
+same members (the io layout of src/Model/model.cpp:54-86 and the component
+read()/write() functions); the SetOnce parameters raise as the reference's
+(src/Tools/setOnce.hpp:15-85); a Simulation without a spectral discretisation
+returns -1 from compute_level_populations (sim_lines.tpp:66-71) without
+touching a GPU.
+
+GPU: the Simulation methods, called in the order of the reference's own
+tests (compute_spectral_discretisation, compute_boundary_intensities,
+compute_LTE_level_populations, compute_radiation_field,
+compute_level_populations), reproduce the oracle's J within 1e-5 and its
+converged populations within 1e-5.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from magritte_amd.setup import synthetic
+
+core = pytest.importorskip("magritte_amd.core", reason="host layer not built")
+from magritte_amd.host import to_simulation  # noqa: E402
+
+
+def _small():
+    return synthetic.envelope_model(n=6, seed=3)
+
+
+def test_set_once_semantics():
+    p = core.Parameters()
+    with pytest.raises(RuntimeError, match="GetBeforeSet"):
+        p.ncells()
+    p.set_ncells(10)
+    p.set_ncells(10)
+    with pytest.raises(RuntimeError, match="DoubleSet"):
+        p.set_ncells(11)
+    assert p.ncells() == 10
+
+
+def test_io_text_round_trip(tmp_path):
+    m = _small()
+    sim = to_simulation(m)
+    io = core.IoText(str(tmp_path) + "/model/")
+    sim.write(io)
+    assert os.path.exists(str(tmp_path) + "/model/parameters.txt") or os.listdir(str(tmp_path) + "/model")
+    back = core.Model()
+    back.read(io)
+    assert back.parameters.ncells() == m.ncells
+    assert back.parameters.nrays() == m.nrays
+    assert back.parameters.nboundary() == m.nboundary
+    np.testing.assert_array_equal(np.array(back.geometry.cells.position), m.position)
+    np.testing.assert_array_equal(np.array(back.geometry.cells.velocity), m.velocity)
+    nb = np.concatenate([np.asarray(x, dtype=np.int64) for x in back.geometry.cells.neighbors])
+    np.testing.assert_array_equal(nb, m.neighbors)
+    np.testing.assert_array_equal(np.array(back.geometry.cells.n_neighbors), m.n_neighbors)
+    np.testing.assert_array_equal(np.array(back.geometry.rays.rays), m.rays)
+    np.testing.assert_array_equal(np.array(back.geometry.rays.weights), m.weights)
+    np.testing.assert_array_equal(np.array(back.geometry.boundary.boundary2cell_nr), m.boundary2cell_nr)
+    np.testing.assert_array_equal(np.array(back.thermodynamics.temperature.gas), m.temperature)
+    np.testing.assert_array_equal(np.array(back.thermodynamics.turbulence.vturb2), m.vturb2)
+    np.testing.assert_array_equal(np.array(back.chemistry.species.abundance), m.abundance)
+    ls = back.lines.lineProducingSpecies[0]
+    ld = m.linedata
+    assert ls.linedata.nlev == ld.nlev and ls.linedata.nrad == ld.nrad
+    np.testing.assert_array_equal(np.array(ls.linedata.energy), ld.energy)
+    np.testing.assert_array_equal(np.array(ls.linedata.A), ld.A)
+    np.testing.assert_array_equal(np.array(ls.linedata.irad), ld.irad)
+    assert len(ls.linedata.colpar) == len(ld.colpar)
+    for a, b in zip(ls.linedata.colpar, ld.colpar):
+        np.testing.assert_array_equal(np.array(a.Cd), b.Cd)
+        np.testing.assert_array_equal(np.array(a.Ce), b.Ce)
+        np.testing.assert_array_equal(np.array(a.tmp), b.tmp)
+        assert a.num_col_partner == b.num
+    np.testing.assert_array_equal(np.array(ls.quadrature.roots), m.roots)
+    np.testing.assert_array_equal(np.array(ls.quadrature.weights), m.qweights)
+    # the antipodes are set up on read (rays.cpp:31-56)
+    rays = np.array(back.geometry.rays.rays)
+    for r, a in enumerate(back.geometry.rays.antipod):
+        np.testing.assert_allclose(rays[a], -rays[r], atol=1e-12)
+
+
+def test_level_populations_without_spectral_discretisation_returns_minus_one(tmp_path):
+    sim = to_simulation(_small())
+    assert sim.compute_level_populations(core.IoText(str(tmp_path) + "/"), True, 10) == -1
+
+
+def test_boundary_intensities_size_mismatch_returns_minus_one():
+    sim = to_simulation(_small())
+    assert sim.compute_boundary_intensities([1.0, 2.0]) == -1
+
+
+@pytest.mark.gpu
+def test_simulation_matches_oracle(tmp_path):
+    from oracle.py_oracle import Oracle
+    m = synthetic.envelope_model(n=8, seed=7)
+    print("\n[host] envelope8 seed 7: Simulation vs oracle", flush=True)
+    orc = Oracle(m).setup()
+    sim = to_simulation(m)
+    assert sim.compute_spectral_discretisation() == 0
+    assert sim.compute_boundary_intensities() == 0
+    assert sim.compute_LTE_level_populations() == 0
+    nu_o, _ = orc.frequencies()
+    np.testing.assert_array_equal(np.array(sim.radiation.frequencies.nu).ravel(), nu_o.ravel())
+    assert sim.compute_radiation_field() == 0
+    orc.compute_radiation_field()
+    J_o = orc.J().ravel()
+    J_g = np.array(sim.radiation.J)
+    assert float(np.max(np.abs(J_g - J_o) / np.abs(J_o))) < 1e-5
+    assert len(sim.timings) == 1 and sim.timings[0] > 0.0
+    ls = sim.lines.lineProducingSpecies[0]
+    Ls = np.array([[x[0] if len(x) else 0.0 for x in row] for row in ls.Lambda.Ls])
+    assert Ls.shape == (m.ncells, m.nrad) and np.all(Ls > 0.0)
+
+    sim.convergence_criterion = core.CRIT_MAX
+    it = sim.compute_level_populations(core.IoText(str(tmp_path) + "/"), True, 60)
+    it_o = orc.compute_level_populations(True, 60, 1)
+    assert 0 < it < 60 and abs(it - it_o) <= 2
+    pop_g = np.array(ls.population)
+    pop_o = orc.populations().ravel()
+    assert float(np.max(np.abs(pop_g - pop_o) / np.abs(pop_o))) < 1e-5
+    assert len(sim.error_max) == it
+    # per-iteration population files, tag "_rank_0_iteration_<k>" (sim_lines.tpp:149-151)
+    written = os.listdir(str(tmp_path) + "/Lines/LineProducingSpecies_0")
+    assert f"population_rank_0_iteration_{it}.txt" in written
