@@ -23,18 +23,21 @@
 // of the pair is eliminated from its boundary inwards in the Rybicki-Hummer
 // (1+F) form and both eliminations meet at the origin, which yields u and the
 // diagonal of the inverse there without storing or back-substituting the pair.
-// With a = Dtau_{n-1}, b = Dtau_n the interior row reads, per sweep step,
-//   r = b/a, c = (a+b) b / 2, F_n = c + r F_{n-1}/(1+F_{n-1}),
-//   Z_n = (S_n c + r Z_{n-1}) / (1+F_n)
-// (the reference's F and Su recurrences divided through by C_n), the boundary
-// row F_0 = D(1 + D/2), Z_0 = (S D^2/2 + I D)/(1+F_0), and at the origin with
-// both sides present, h = (D_A + D_B)/2,
-//   u = (S h + Z_A/D_A + Z_B/D_B) / (h + f_A/D_A + f_B/D_B),  f = F/(1+F),
-//   Lambda = h / (h + f_A/D_A + f_B/D_B);
-// with one side empty (origin on the boundary, incoming I_bdy of the origin's
-// boundary cell) u = (S D^2/2 + I D + Z_A)/(D^2/2 + D + f_A),
-// Lambda = (D^2/2)/(D^2/2 + D + f_A).  Only ratios and products of optical
-// depths appear, so the FP32 variant neither overflows nor cancels.
+// With a = Dtau_{n-1}, b = Dtau_n the reference's F and Su recurrences, divided
+// through by C_n, read F_n = (a+b) b/2 + (b/a) F_{n-1}/(1+F_{n-1}) and
+// Z_n = (S_n (a+b) b/2 + (b/a) Z_{n-1}) / (1+F_n).  In the variables
+// phi_n = F_n / b and zeta_n = Z_n / b every 1/a cancels:
+//   h = (a+b)/2,  phi_n = h + phi_{n-1} q_{n-1},  q_n = 1/(1 + b phi_n),
+//   zeta_n = (S_n h + zeta_{n-1}) q_n,
+// one reciprocal per point.  The virtual start phi = q = 1, zeta = I_bdy, a = 0
+// reproduces the boundary row F_0 = b(1 + b/2), Z_0 = (S b^2/2 + I b)/(1+F_0).
+// At the origin, with h_o = (b_A + b_B)/2,
+//   u = (S h_o + zeta_A + zeta_B) / (h_o + phi_A q_A + phi_B q_B),
+//   Lambda = h_o / (h_o + phi_A q_A + phi_B q_B),
+// and an empty side (origin on the boundary, incoming I_bdy of the origin's
+// boundary cell) enters as the virtual start: phi q = 1, zeta = I, b = 0.
+// phi stays below 1 + tau and q in (0, 1], so the FP32 variant neither
+// overflows nor loses the optically thin limit.
 //
 // Line profiles.  FP64 variant: the window of get_eta_and_chi is decided in
 // FP64 exactly as the reference.  FP32 variant: each lane evaluates the lines
@@ -205,23 +208,19 @@ __device__ __forceinline__ double rescale_I_bdy(const double *__restrict__ nup,
 }
 
 // ---- Feautrier elimination state of one side ----------------------------------
-//
-// The sweep starts from a virtual outer state D = 0, 1/D = 1, f = 1, Z = I_bdy
-// with which the interior recurrence reproduces the boundary row exactly:
-// c = D_n^2/2, r = D_n, F = D_n + D_n^2/2, Z = (S D_n^2/2 + I D_n)/(1+F).
 
 template <typename real>
 struct Sweep {
     real S_q, chi_q, hdz_q;   // pending (outer) point: source function, opacity, dz/2
-    real f, Z, D, invD;
+    real phi, q, zeta, bh;    // elimination state; bh = half the last closed dtau
 
     __device__ __forceinline__ void init(real I_far)
     {
         S_q = chi_q = hdz_q = 0;
-        f = 1;
-        Z = I_far;
-        D = 0;
-        invD = 1;
+        phi = 1;
+        q = 1;
+        zeta = I_far;
+        bh = 0;
     }
 
     __device__ __forceinline__ void pend(real S, real chi, real hdz)
@@ -234,17 +233,15 @@ struct Sweep {
     // close the pending row with the next point (S, chi) inwards
     __device__ __forceinline__ void step(real S, real chi)
     {
-        const real Dn = (chi_q + chi) * hdz_q;
-        const real rr = Dn * invD;
-        const real cc = (D + Dn) * ((real)0.5 * Dn);
-        const real Fn = rr * f + cc;
-        const real q = Prec<real>::rcp((real)1 + Fn);
-        const real Zs = S_q * cc + rr * Z;
-        Z = Zs * q;
-        f = Fn * q;
-        D = Dn;
-        invD = Prec<real>::rcp(Dn);
+        const real b = (chi_q + chi) * hdz_q;
+        const real h = b * (real)0.5 + bh;
+        bh = b * (real)0.5;
+        phi = phi * q + h;
+        q = Prec<real>::rcp(b * phi + (real)1);
+        zeta = (S_q * h + zeta) * q;
     }
+
+    __device__ __forceinline__ real fq() const { return phi * q; }
 };
 
 // ---- staging ----------------------------------------------------------------
@@ -510,9 +507,9 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
         if (n_r > 0 && n_a > 0) {
             const Sweep<real> A = sweep_side<real>(a, lc, o, qa, nu_f, S_o, chi_o, dsm, F, st, rw, err);
             const Sweep<real> B = sweep_side<real>(a, lc, o, qr, nu_f, S_o, chi_o, dsm, F, st, rw, err);
-            const real h = (real)0.5 * (A.D + B.D);
-            const real den = Prec<real>::rcp(h + A.f * A.invD + B.f * B.invD);
-            u = (S_o * h + A.Z * A.invD + B.Z * B.invD) * den;
+            const real h = A.bh + B.bh;
+            const real den = Prec<real>::rcp(h + A.fq() + B.fq());
+            u = (S_o * h + A.zeta + B.zeta) * den;
             lam = h * den;
         } else {
             const int64_t qs = (n_a > 0) ? qa : qr;
@@ -520,10 +517,10 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
             real I_o = 0;
             if (bo < 0) err |= ERR_EMPTY_SIDE;
             else I_o = (real)(a.I_bdy[(int64_t)bo * F + fa] * I_SCALE);
-            const real h2 = (real)0.5 * A.D * A.D;
-            const real den = Prec<real>::rcp(h2 + A.D + A.f);
-            u = (S_o * h2 + I_o * A.D + A.Z) * den;
-            lam = h2 * den;
+            const real h = A.bh;
+            const real den = Prec<real>::rcp(h + (real)1 + A.fq());
+            u = (S_o * h + I_o + A.zeta) * den;
+            lam = h * den;
         }
         J_acc += w2 * ((double)u * (1.0 / I_SCALE));
         Lam_acc += w2 * (double)lam;

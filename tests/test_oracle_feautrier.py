@@ -109,38 +109,39 @@ def test_wrong_matrix_is_detected():
 
 def two_sided(term1, dtau, I0, In, n_ar):
     """The two-sided elimination of the GPU kernel (see
-    magritte_amd/csrc/mgr_radiation.hip), in FP64 numpy."""
+    magritte_amd/csrc/mgr_radiation.hip), in FP64 numpy.
+
+    Each side is eliminated from its far end towards the origin in the
+    variables phi = F_n / b_n and zeta = Z_n / b_n, where b_n is the optical
+    depth between point n and the next point inwards, so that no 1/b appears:
+        h = (a + b)/2,  phi_n = h + phi_{n-1} q_{n-1},  q_n = 1/(1 + b phi_n),
+        zeta_n = (S_n h + zeta_{n-1}) q_n,
+    from the virtual start phi = q = 1, zeta = I_far, a = 0.  At the origin
+        u = (S h_o + zeta_A + zeta_B) / (h_o + phi_A q_A + phi_B q_B),
+        Lambda = h_o / (h_o + phi_A q_A + phi_B q_B),  h_o = (b_A + b_B)/2,
+    and an empty side enters as the virtual start (phi q = 1, zeta = I, b = 0).
+    """
     ndep = len(term1)
 
-    def side(points):      # points ordered from the far end inwards: (S, D_to_inner)
-        F = Z = None
-        Dprev = None
-        for i, (S, D) in enumerate(points):
-            if i == 0:
-                F = D * (1 + 0.5 * D)
-                Z = (S * 0.5 * D * D + I_far * D) / (1 + F)
-            else:
-                r = D / Dprev
-                c = 0.5 * (Dprev + D) * D
-                Fn = c + r * F / (1 + F)
-                Z = (S * c + r * Z) / (1 + Fn)
-                F = Fn
-            Dprev = D
-        return F / (1 + F), Z, Dprev
+    def side(points, I_far):   # points from the far end inwards: (S, b to the next point inwards)
+        phi, q, zeta, a = 1.0, 1.0, I_far, 0.0
+        for S, b in points:
+            h = 0.5 * (a + b)
+            phi = h + phi * q
+            q = 1.0 / (1.0 + b * phi)
+            zeta = (S * h + zeta) * q
+            a = b
+        return phi * q, zeta, a
 
     So = term1[n_ar]
-    I_far = I0
-    A = side([(term1[n], dtau[n]) for n in range(0, n_ar)])
+    A = side([(term1[n], dtau[n]) for n in range(0, n_ar)], I0)
     if n_ar == ndep - 1:
-        D = A[2]
-        h2 = 0.5 * D * D
-        den = h2 + D + A[0]
-        return (So * h2 + In * D + A[1]) / den, h2 / den
-    I_far = In
-    B = side([(term1[n], dtau[n - 1]) for n in range(ndep - 1, n_ar, -1)])
+        B = (1.0, In, 0.0)
+    else:
+        B = side([(term1[n], dtau[n - 1]) for n in range(ndep - 1, n_ar, -1)], In)
     h = 0.5 * (A[2] + B[2])
-    den = h + A[0] / A[2] + B[0] / B[2]
-    return (So * h + A[1] / A[2] + B[1] / B[2]) / den, h / den
+    den = h + A[0] + B[0]
+    return (So * h + A[1] + B[1]) / den, h / den
 
 
 def test_two_sided_elimination_matches_reference():

@@ -1,0 +1,108 @@
+"""Precision stress (optically thick envelope, n0 = 1e17 m^-3, 20 % of cells
+with +-30 km/s radial velocity jumps): errors of the FP32 radiation-field
+kernel against the FP64 kernel and the FP64 oracle, for J and for the
+converged level populations, per level.
+
+The size is MGR_STRESS_N cells per side (default 16; the full 64^3 case is
+run as MGR_STRESS_N=64).  The oracle's J is computed on a random sample of
+origins (all ray pairs and bins of each), its converged populations only when
+N <= 16^3, where its ALI finishes in about half a minute.  With MGR_REPORT set the tables
+are written there as JSON.
+
+Bars (the precision targets of the FP32 path): J within 1e-5 relative of the
+FP64 solutions; converged populations within 1e-5 relative.
+"""
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+from magritte_amd.setup import synthetic
+
+pytestmark = pytest.mark.gpu
+
+N_SIDE = int(os.environ.get("MGR_STRESS_N", "16"))
+SAMPLE = int(os.environ.get("MGR_STRESS_SAMPLE", "256"))
+
+
+def _stats(a, b, mask=None):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+    if mask is not None:
+        rel = rel[mask]
+    return {"max": float(rel.max()), "rms": float(np.sqrt(np.mean(rel ** 2)))}
+
+
+def test_precision_stress_report():
+    from magritte_amd.hip import Context
+    from oracle.py_oracle import Oracle
+
+    print(f"\n[stress] n={N_SIDE}^3, seed 4321, sample {SAMPLE} origins", flush=True)
+    m = synthetic.stress_model(n=N_SIDE)
+    report = {"model": f"stress envelope {N_SIDE}^3 (n0 1e17, 20% +-30 km/s jumps, seed 4321)",
+              "ncells": m.ncells}
+
+    c32 = Context(m).setup()
+    c32.trace()
+    c64 = Context(m).setup()
+    c64.trace()
+    c32.compute_radiation_field("fp32")
+    c64.compute_radiation_field("fp64")
+    J32, J64 = c32.J(), c64.J()
+    L32, L64 = c32.lambda_diag(), c64.lambda_diag()
+    rng = np.random.default_rng(0)
+    sample = np.sort(rng.choice(m.ncells, size=min(SAMPLE, m.ncells), replace=False))
+    orc = Oracle(m).setup()
+    t0 = time.perf_counter()
+    orc.compute_radiation_field_list(sample)
+    Jo = orc.J()[sample]
+    report["J"] = {"fp32_vs_fp64_gpu": _stats(J32, J64),
+                   "fp32_vs_fp64_oracle_sample": _stats(J32[sample], Jo),
+                   "fp64_gpu_vs_fp64_oracle_sample": _stats(J64[sample], Jo),
+                   "oracle_seconds": round(time.perf_counter() - t0, 2)}
+    report["lambda_star"] = {"fp32_vs_fp64_gpu": _stats(L32, L64, L64 > 0)}
+    print("[stress] J", json.dumps(report["J"]), flush=True)
+
+    t0 = time.perf_counter()
+    it32 = c32.compute_level_populations(True, 400, "max", "fp32")
+    t32 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    it64 = c64.compute_level_populations(True, 400, "max", "fp64")
+    t64 = time.perf_counter() - t0
+    P32, P64 = c32.populations(), c64.populations()
+    ntot = m.abundance[:, m.linedata.num]
+    per_level = []
+    for i in range(m.nlev):
+        mask = P64[:, i] > 1e-10 * ntot
+        per_level.append({"level": i, **_stats(P32[:, i], P64[:, i], mask)})
+    report["populations"] = {"iterations_fp32": int(it32), "iterations_fp64": int(it64),
+                             "seconds_fp32": round(t32, 2), "seconds_fp64": round(t64, 2),
+                             "fp32_vs_fp64_gpu_per_level": per_level}
+    if m.ncells <= 16 ** 3:
+        t0 = time.perf_counter()
+        ito = orc.compute_level_populations(True, 400, 1)
+        Po = orc.populations()
+        report["populations"]["iterations_oracle"] = int(ito)
+        report["populations"]["oracle_seconds"] = round(time.perf_counter() - t0, 2)
+        report["populations"]["fp32_vs_oracle_per_level"] = [
+            {"level": i, **_stats(P32[:, i], Po[:, i], Po[:, i] > 1e-10 * ntot)} for i in range(m.nlev)]
+        report["populations"]["fp64_gpu_vs_oracle_per_level"] = [
+            {"level": i, **_stats(P64[:, i], Po[:, i], Po[:, i] > 1e-10 * ntot)} for i in range(m.nlev)]
+    print("[stress] populations", json.dumps(report["populations"]), flush=True)
+    out = os.environ.get("MGR_REPORT")
+    if out:
+        with open(out, "w") as fh:
+            json.dump(report, fh, indent=1)
+    c32.close()
+    c64.close()
+
+    assert report["J"]["fp32_vs_fp64_gpu"]["max"] < 1e-5
+    assert report["J"]["fp64_gpu_vs_fp64_oracle_sample"]["max"] < 1e-10
+    assert report["J"]["fp32_vs_fp64_oracle_sample"]["max"] < 1e-5
+    assert it32 < 400 and it64 < 400
+    assert max(x["max"] for x in per_level) < 1e-5
+    if "fp32_vs_oracle_per_level" in report["populations"]:
+        assert max(x["max"] for x in report["populations"]["fp32_vs_oracle_per_level"]) < 1e-5
