@@ -132,6 +132,21 @@ typedef struct mgr_ctx mgr_ctx;   /* opaque: device-resident model + state      
  * static tables.  Replaces Model::read (src/Model/model.cpp:54-86) plus the
  * antipodal-ray set-up Rays::setup_antipodal_rays (Rays/rays.cpp:90-103).     */
 int  mgr_create(const mgr_model_view *model, int device, mgr_ctx **out);
+/* A batch of independent models in one context (no counterpart in the
+ * reference, which solves one model per process).  The models share the
+ * species, line data, collision tables, quadrature and parameters; geometry,
+ * rays, thermodynamics and abundances differ.  Cells, boundary cells and all
+ * per-cell arrays of the getters and setters are concatenated in model order
+ * (model m owns cells [cell_offsets[m], cell_offsets[m+1])).  Every kernel
+ * covers the whole batch in one launch; mgr_compute_level_populations
+ * iterates each model until it meets the criterion, freezing converged ones,
+ * and Ng acceleration and convergence statistics are per model, so each
+ * model follows the iteration it would follow alone.                        */
+int  mgr_create_batch(const mgr_model_view *models, int64_t nmodels, int device, mgr_ctx **out);
+int  mgr_batch_info(mgr_ctx *ctx, int64_t *nmodels, int64_t *cell_offsets /* [nmodels+1] or NULL */);
+/* Per model, after mgr_compute_level_populations: the iteration at which it
+ * converged (or the last one) and its last convergence statistics.          */
+int  mgr_batch_convergence(mgr_ctx *ctx, int64_t *iterations, mgr_conv *conv);
 void mgr_destroy(mgr_ctx *ctx);
 const char *mgr_last_error(const mgr_ctx *ctx);   /* ctx may be NULL (create errors) */
 int  mgr_device_count(int *count);
@@ -228,6 +243,49 @@ int mgr_last_radiation_field_ms(mgr_ctx *ctx, double *ms);
 /* Sum over the cached traces of the expanded points of every ray pair
  * (sum of ndep) and the number of ray pairs with ndep > 1.                    */
 int mgr_trace_stats(mgr_ctx *ctx, int64_t *sum_ndep, int64_t *n_pairs_solved);
+
+/* ---- learned warm start --------------------------------------------------
+ * No counterpart in the reference, whose iteration starts from LTE
+ * (Simulation::compute_LTE_level_populations, sim_lines.tpp:10-16).  A
+ * multilayer perceptron 8 -> 128 -> 128 -> 7 (GELU; bfloat16 matrix-core
+ * products accumulated in FP32; FP32 master weights and Adam) predicts
+ * per-cell log10 level-population fractions; mgr_warm_start writes the
+ * resulting populations into a context, after which the ALI loop runs
+ * unchanged.  Parameters are packed as W1 [8][128], b1 [128], W2 [128][128],
+ * b2 [128], W3 [128][7], b3 [7] (weights [in][out], row-major FP32).        */
+#define MGR_MLP_IN     8
+#define MGR_MLP_HIDDEN 128
+#define MGR_MLP_OUT    7
+typedef struct mgr_mlp mgr_mlp;   /* opaque: device-resident network          */
+
+int64_t     mgr_mlp_num_params(void);
+int         mgr_mlp_create(int device, uint64_t seed, mgr_mlp **out);
+void        mgr_mlp_destroy(mgr_mlp *mlp);
+const char *mgr_mlp_last_error(const mgr_mlp *mlp);
+int         mgr_mlp_set_params(mgr_mlp *mlp, const float *params);   /* resets Adam */
+int         mgr_mlp_get_params(mgr_mlp *mlp, float *params);
+/* Inputs are normalised as (x - x_mean)/x_std, outputs de-normalised as
+ * y' y_std + y_mean; the defaults are 0 and 1.                               */
+int mgr_mlp_set_normalisation(mgr_mlp *mlp, const float *x_mean, const float *x_std,
+                              const float *y_mean, const float *y_std);
+/* Y [n][7] = network(X [n][8]); host buffers.                               */
+int mgr_mlp_forward(mgr_mlp *mlp, const float *X, int64_t n, float *Y);
+/* Mean squared error over rows and outputs of the normalised targets T [n][7]
+ * and its gradient, packed like the parameters (grad may be NULL).           */
+int mgr_mlp_loss_grad(mgr_mlp *mlp, const float *X, const float *T, int64_t n, float *loss,
+                      float *grad);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over shuffled minibatches on the device;
+ * loss_per_epoch [epochs] may be NULL.                                        */
+int mgr_mlp_train(mgr_mlp *mlp, const float *X, const float *T, int64_t n, int64_t epochs,
+                  int64_t batch, float lr, uint64_t seed, float *loss_per_epoch);
+/* Per-cell features [ncells][8] after a radiation field and J_eff:
+ * log10 n(collision partner), log10 T, log10 n(species), log10 of the
+ * largest neighbour velocity gradient (s^-1), log10 Jlin_k / B_nu_k(T) for
+ * transitions k = 0..3.                                                       */
+int mgr_features(mgr_ctx *ctx, float *X);
+/* Features -> network -> populations (pop_tot 10^y_i / sum 10^y) in place of
+ * the current populations; needs 7 levels.                                    */
+int mgr_warm_start(mgr_ctx *ctx, mgr_mlp *mlp);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,7 @@ UNITS = {
     "mgr_trace.hip": ["-ffp-contract=off"],
     "mgr_radiation.hip": [],
     "mgr_levels.hip": [],
+    "mgr_mlp.hip": [],
 }
 
 

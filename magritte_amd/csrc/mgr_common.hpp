@@ -88,8 +88,32 @@ struct mgr_ctx {
     std::string err;
     int         status_sticky = MGR_OK;
 
-    // sizes
+    // sizes (N, NB: totals over the models of a batch; R: the largest ray count)
     int64_t N = 0, R = 0, NB = 0, F = 0, L = 0, V = 0, Q = 0, nspecs = 0;
+
+    // batch of models: model m owns cells [m_cell_off[m], m_cell_off[m+1]) and
+    // the (origin, ray) entries [m_q_off[m], m_q_off[m+1]); its rays are the
+    // global rays m_rbase[m] .. m_rbase[m] + m_R[m] - 1 (one set per distinct
+    // ray list).  A single model is a batch of one.
+    int64_t M = 1, NQ = 0;
+    std::vector<int64_t> m_cell_off, m_q_off, h_cell_qbase;
+    std::vector<int32_t> m_R, m_rbase;
+    int32_t *d_cell_model = nullptr;   // [N]
+    int64_t *d_cell_qbase = nullptr;   // [N] first (origin, ray) entry of the cell
+    int32_t *d_m_R = nullptr, *d_m_rbase = nullptr, *d_m_active = nullptr;   // [M]
+    int32_t *h_m_active = nullptr;     // pinned [M]
+    // segmented reductions: chunk c covers cells [chunk_cell[c], chunk_cell[c+1])
+    // of one model; the chunks of model m are m_chunk_off[m] .. m_chunk_off[m+1]-1
+    int64_t nchunks = 0;
+    std::vector<int64_t> m_chunk_off;
+    int64_t *d_chunk_cell = nullptr;   // [nchunks + 1]
+    int64_t *d_m_chunk_off = nullptr;  // [M + 1]
+    int64_t *d_m_cell_off = nullptr;   // [M + 1]
+    double  *d_mconv = nullptr;        // [M * 4] per-model fraction, mean, max, error word
+    double  *h_mconv = nullptr;        // pinned copy
+    double  *d_mng = nullptr;          // [M * 4] per-model Ng determinant and coefficients
+    std::vector<int64_t> m_iterations; // per model, of the last level-population run
+    std::vector<mgr_conv> m_last_conv;
     int64_t ncolpar = 0, ncol_tot = 0, ntmp_tot = 0, ctab_tot = 0;
     double  pop_prec = 1e-6, max_width_fraction = 0.5;
     int64_t num = 0;
@@ -98,6 +122,7 @@ struct mgr_ctx {
     std::vector<int64_t> antipod;          // [R]
     std::vector<int64_t> boundary2cell;    // [NB]
     std::vector<int64_t> lam_f_off, lam_f; // CSR: sorted f of each transition k
+    std::vector<int64_t> h_cp_num;         // [ncolpar] species of each collision partner
 
     // device: static model
     double  *pos = nullptr, *vel = nullptr;        // [N*3]
@@ -172,7 +197,7 @@ struct mgr_ctx {
     int64_t  sum_ndep = 0, n_pairs_solved = 0;
 
     // reductions
-    double  *partials = nullptr;    // [MAX_PARTIALS * 8]
+    double  *partials = nullptr;    // [nchunks * 8]
     double  *reduced = nullptr;     // [8]
     double  *h_reduced = nullptr;   // pinned [8]
 
@@ -188,10 +213,10 @@ struct mgr_ctx {
 
 namespace mgr {
 constexpr int RED_BLOCK = 256;
-constexpr int MAX_PARTIALS = 4096;
+constexpr int CHUNK_CELLS = 1024;   // cells per reduction chunk
 
 // entry points shared between translation units
-int upload_model(mgr_ctx *ctx, const mgr_model_view *m);
+int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels);
 int free_model(mgr_ctx *ctx);
 int launch_cell_tables(mgr_ctx *ctx);
 int launch_emissivity_opacity(mgr_ctx *ctx);
@@ -204,4 +229,5 @@ int launch_stateq(mgr_ctx *ctx);
 int launch_convergence(mgr_ctx *ctx);
 int launch_ng(mgr_ctx *ctx);
 int read_convergence(mgr_ctx *ctx, mgr_conv *conv);
+int set_active(mgr_ctx *ctx, bool all);   // all: every model of the batch iterates
 }  // namespace mgr

@@ -213,21 +213,189 @@ int launch_emissivity_opacity(mgr_ctx *ctx)
     return MGR_OK;
 }
 
-int upload_model(mgr_ctx *ctx, const mgr_model_view *m)
+// Validation of one model view on its own (sizes and index ranges).
+static int check_view(mgr_ctx *ctx, const mgr_model_view *m, int64_t idx)
+{
+    const std::string who = "model " + std::to_string(idx) + ": ";
+    if (m->ncells <= 0 || m->nrays <= 0 || (m->nrays % 2) != 0 || m->nrad <= 0 || m->nlev < 2 ||
+        m->nquads <= 0 || m->nboundary < 0)
+        return ctx->fail(MGR_ERR_ARGUMENT, who + "model sizes out of range (need ncells, nrad, nquads > 0, even nrays, nlev >= 2)");
+    if (!m->position || !m->velocity || !m->n_neighbors || !m->rays || !m->weights || !m->temperature ||
+        !m->vturb2 || !m->abundance || !m->energy || !m->weight || !m->irad || !m->jrad || !m->A || !m->Bs ||
+        !m->Ba || !m->frequency || !m->roots || !m->qweights || (m->nboundary > 0 && !m->boundary2cell_nr))
+        return ctx->fail(MGR_ERR_ARGUMENT, who + "null array in the model view");
+    for (int64_t p = 0; p < m->ncells; p++)
+        if (m->n_neighbors[p] < 0) return ctx->fail(MGR_ERR_ARGUMENT, who + "negative n_neighbors");
+    int64_t M = 0;
+    for (int64_t p = 0; p < m->ncells; p++) M += m->n_neighbors[p];
+    if (M > 0 && !m->neighbors) return ctx->fail(MGR_ERR_ARGUMENT, who + "null neighbours");
+    for (int64_t q = 0; q < M; q++)
+        if (m->neighbors[q] < 0 || m->neighbors[q] >= m->ncells)
+            return ctx->fail(MGR_ERR_ARGUMENT, who + "neighbour index out of range");
+    for (int64_t b = 0; b < m->nboundary; b++)
+        if (m->boundary2cell_nr[b] < 0 || m->boundary2cell_nr[b] >= m->ncells)
+            return ctx->fail(MGR_ERR_ARGUMENT, who + "boundary2cell_nr out of range");
+    return MGR_OK;
+}
+
+template <typename T>
+static bool same(const T *a, const T *b, int64_t n)
+{
+    if (n <= 0) return true;
+    if (!a || !b) return a == b;
+    return std::memcmp(a, b, (size_t)n * sizeof(T)) == 0;
+}
+
+// Models of one batch must share the species, line data, collision tables,
+// quadrature and iteration parameters (only the geometry, the thermodynamics
+// and the abundances differ).
+static int check_same_physics(mgr_ctx *ctx, const mgr_model_view *a, const mgr_model_view *b, int64_t idx)
+{
+    const std::string who = "model " + std::to_string(idx) + ": ";
+    bool ok = a->nspecs == b->nspecs && a->nquads == b->nquads && a->num == b->num && a->nlev == b->nlev &&
+              a->nrad == b->nrad && a->ncolpar == b->ncolpar && a->pop_prec == b->pop_prec &&
+              a->max_width_fraction == b->max_width_fraction && a->n_off_diag == b->n_off_diag &&
+              a->inverse_mass == b->inverse_mass;
+    if (ok) {
+        const int64_t V = a->nlev, L = a->nrad, Q = a->nquads;
+        ok = same(a->energy, b->energy, V) && same(a->weight, b->weight, V) && same(a->irad, b->irad, L) &&
+             same(a->jrad, b->jrad, L) && same(a->A, b->A, L) && same(a->Bs, b->Bs, L) && same(a->Ba, b->Ba, L) &&
+             same(a->frequency, b->frequency, L) && same(a->roots, b->roots, Q) &&
+             same(a->qweights, b->qweights, Q) && same(a->colpar_num, b->colpar_num, a->ncolpar) &&
+             same(a->colpar_ncol, b->colpar_ncol, a->ncolpar) && same(a->colpar_ntmp, b->colpar_ntmp, a->ncolpar) &&
+             same(a->colpar_orth_or_para, b->colpar_orth_or_para, a->ncolpar);
+        if (ok) {
+            int64_t nk = 0, nt = 0, nc = 0;
+            for (int64_t c = 0; c < a->ncolpar; c++) {
+                nk += a->colpar_ncol[c];
+                nt += a->colpar_ntmp[c];
+                nc += a->colpar_ncol[c] * a->colpar_ntmp[c];
+            }
+            ok = same(a->icol, b->icol, nk) && same(a->jcol, b->jcol, nk) && same(a->tmp, b->tmp, nt) &&
+                 same(a->Cd, b->Cd, nc) && same(a->Ce, b->Ce, nc);
+        }
+    }
+    if (!ok) return ctx->fail(MGR_ERR_ARGUMENT, who + "a batch needs the same species, line data and quadrature in every model");
+    return MGR_OK;
+}
+
+int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
 {
     int rc;
-    ctx->N = m->ncells; ctx->R = m->nrays; ctx->NB = m->nboundary; ctx->nspecs = m->nspecs;
+    if (nmodels < 1) return ctx->fail(MGR_ERR_ARGUMENT, "a batch needs at least one model");
+    for (int64_t k = 0; k < nmodels; k++) {
+        if ((rc = check_view(ctx, &models[k], k))) return rc;
+        if (k > 0 && (rc = check_same_physics(ctx, &models[0], &models[k], k))) return rc;
+    }
+    const mgr_model_view *m0 = &models[0];
+    const int64_t Mm = nmodels;
+    ctx->M = Mm;
+
+    // merged geometry, thermodynamics and abundances (cell indices offset per model)
+    ctx->m_cell_off.assign(Mm + 1, 0);
+    int64_t nb_tot = 0, nbd_tot = 0;
+    for (int64_t k = 0; k < Mm; k++) {
+        ctx->m_cell_off[k + 1] = ctx->m_cell_off[k] + models[k].ncells;
+        for (int64_t p = 0; p < models[k].ncells; p++) nb_tot += models[k].n_neighbors[p];
+        nbd_tot += models[k].nboundary;
+    }
+    const int64_t N = ctx->m_cell_off[Mm];
+    if (N > INT32_MAX / 2) return ctx->fail(MGR_ERR_UNSUPPORTED, "ncells exceeds the 32-bit cell index");
+    std::vector<double> pos, vel, temp, vt2, abund;
+    std::vector<int64_t> nnb, nbs, b2c;
+    pos.reserve(3 * N); vel.reserve(3 * N); temp.reserve(N); vt2.reserve(N);
+    abund.reserve(N * m0->nspecs); nnb.reserve(N); nbs.reserve(nb_tot); b2c.reserve(nbd_tot);
+    // distinct ray lists become ray sets
+    std::vector<double> rays, rw;
+    std::vector<int64_t> set_off{0};
+    std::vector<int64_t> model_set(Mm);
+    for (int64_t k = 0; k < Mm; k++) {
+        const mgr_model_view &v = models[k];
+        const int64_t c0 = ctx->m_cell_off[k];
+        pos.insert(pos.end(), v.position, v.position + 3 * v.ncells);
+        vel.insert(vel.end(), v.velocity, v.velocity + 3 * v.ncells);
+        temp.insert(temp.end(), v.temperature, v.temperature + v.ncells);
+        vt2.insert(vt2.end(), v.vturb2, v.vturb2 + v.ncells);
+        abund.insert(abund.end(), v.abundance, v.abundance + v.ncells * v.nspecs);
+        int64_t e = 0;
+        for (int64_t p = 0; p < v.ncells; p++) {
+            nnb.push_back(v.n_neighbors[p]);
+            for (int64_t j = 0; j < v.n_neighbors[p]; j++) nbs.push_back(v.neighbors[e + j] + c0);
+            e += v.n_neighbors[p];
+        }
+        for (int64_t b = 0; b < v.nboundary; b++) b2c.push_back(v.boundary2cell_nr[b] + c0);
+        int64_t found = -1;
+        for (size_t sidx = 0; sidx + 1 < set_off.size() && found < 0; sidx++) {
+            const int64_t R0 = set_off[sidx + 1] - set_off[sidx];
+            if (R0 == v.nrays && same(rays.data() + 3 * set_off[sidx], v.rays, 3 * R0) &&
+                same(rw.data() + set_off[sidx], v.weights, R0))
+                found = (int64_t)sidx;
+        }
+        if (found < 0) {
+            found = (int64_t)set_off.size() - 1;
+            rays.insert(rays.end(), v.rays, v.rays + 3 * v.nrays);
+            rw.insert(rw.end(), v.weights, v.weights + v.nrays);
+            set_off.push_back(set_off.back() + v.nrays);
+        }
+        model_set[k] = found;
+    }
+    const int64_t Rtot = set_off.back();
+
+    mgr_model_view mv = *m0;
+    mv.ncells = N;
+    mv.nboundary = nbd_tot;
+    mv.nrays = Rtot;
+    mv.position = pos.data();
+    mv.velocity = vel.data();
+    mv.n_neighbors = nnb.data();
+    mv.neighbors = nbs.data();
+    mv.rays = rays.data();
+    mv.weights = rw.data();
+    mv.boundary2cell_nr = b2c.data();
+    mv.temperature = temp.data();
+    mv.vturb2 = vt2.data();
+    mv.abundance = abund.data();
+    const mgr_model_view *m = &mv;
+
+    // per-model ray sets and (origin, ray) offsets
+    ctx->m_R.assign(Mm, 0);
+    ctx->m_rbase.assign(Mm, 0);
+    ctx->m_q_off.assign(Mm + 1, 0);
+    ctx->h_cell_qbase.assign(N, 0);
+    std::vector<int32_t> cell_model(N);
+    int64_t Rmax = 0;
+    for (int64_t k = 0; k < Mm; k++) {
+        const int64_t sidx = model_set[k];
+        ctx->m_R[k] = (int32_t)(set_off[sidx + 1] - set_off[sidx]);
+        ctx->m_rbase[k] = (int32_t)set_off[sidx];
+        Rmax = std::max<int64_t>(Rmax, ctx->m_R[k]);
+        ctx->m_q_off[k + 1] = ctx->m_q_off[k] + models[k].ncells * ctx->m_R[k];
+        for (int64_t p = ctx->m_cell_off[k]; p < ctx->m_cell_off[k + 1]; p++) {
+            cell_model[p] = (int32_t)k;
+            ctx->h_cell_qbase[p] = ctx->m_q_off[k] + (p - ctx->m_cell_off[k]) * ctx->m_R[k];
+        }
+    }
+    ctx->NQ = ctx->m_q_off[Mm];
+
+    // reduction chunks: CHUNK_CELLS cells of one model each
+    ctx->m_chunk_off.assign(Mm + 1, 0);
+    std::vector<int64_t> chunk_cell{0};
+    for (int64_t k = 0; k < Mm; k++) {
+        for (int64_t c = ctx->m_cell_off[k]; c < ctx->m_cell_off[k + 1]; c += CHUNK_CELLS)
+            chunk_cell.push_back(std::min<int64_t>(c + CHUNK_CELLS, ctx->m_cell_off[k + 1]));
+        ctx->m_chunk_off[k + 1] = (int64_t)chunk_cell.size() - 1;
+    }
+    ctx->nchunks = (int64_t)chunk_cell.size() - 1;
+
+    ctx->N = N; ctx->R = Rmax; ctx->NB = nbd_tot; ctx->nspecs = m->nspecs;
     ctx->L = m->nrad; ctx->V = m->nlev; ctx->Q = m->nquads; ctx->F = m->nrad * m->nquads;
     ctx->pop_prec = m->pop_prec; ctx->max_width_fraction = m->max_width_fraction;
     ctx->num = m->num; ctx->ncolpar = m->ncolpar;
-    const int64_t N = ctx->N, R = ctx->R, NB = ctx->NB, L = ctx->L, V = ctx->V, Q = ctx->Q;
+    const int64_t R = Rtot, NB = ctx->NB, L = ctx->L, V = ctx->V, Q = ctx->Q;
 
-    if (N <= 0 || R <= 0 || (R % 2) != 0 || L <= 0 || V < 2 || Q <= 0 || NB < 0)
-        return ctx->fail(MGR_ERR_ARGUMENT, "model sizes out of range (need ncells, nrad, nquads > 0, even nrays, nlev >= 2)");
     if (L > MAX_LINES) return ctx->fail(MGR_ERR_UNSUPPORTED, "more than 64 line transitions");
     if (m->n_off_diag != 0)
         return ctx->fail(MGR_ERR_UNSUPPORTED, "n_off_diag > 0 (banded ALO) is not supported");
-    if (N > INT32_MAX / 2) return ctx->fail(MGR_ERR_UNSUPPORTED, "ncells exceeds the 32-bit cell index");
     if (m->num < 0 || m->num >= m->nspecs) return ctx->fail(MGR_ERR_ARGUMENT, "line species index out of range");
 
     // Lines::setup (src/Model/Lines/lines.cpp:43-87): sorted line list.
@@ -246,16 +414,11 @@ int upload_model(mgr_ctx *ctx, const mgr_model_view *m)
             return ctx->fail(MGR_ERR_ARGUMENT, "radiative transition level out of range");
     }
 
-    // neighbours (CSR) with bounds checks: the tracer trusts these indices
+    // neighbours (CSR); every index was checked against its own model above,
+    // so a ray never leaves its model: the tracer trusts these indices
     std::vector<int64_t> off(N + 1, 0);
-    for (int64_t p = 0; p < N; p++) {
-        if (m->n_neighbors[p] < 0) return ctx->fail(MGR_ERR_ARGUMENT, "negative n_neighbors");
-        off[p + 1] = off[p] + m->n_neighbors[p];
-    }
+    for (int64_t p = 0; p < N; p++) off[p + 1] = off[p] + m->n_neighbors[p];
     const int64_t M = off[N];
-    for (int64_t q = 0; q < M; q++)
-        if (m->neighbors[q] < 0 || m->neighbors[q] >= N)
-            return ctx->fail(MGR_ERR_ARGUMENT, "neighbour index out of range");
 
     // Boundary::read (boundary.cpp:19-51)
     std::vector<int32_t> c2b(N, -1), isb_i(N, 0);
@@ -263,22 +426,25 @@ int upload_model(mgr_ctx *ctx, const mgr_model_view *m)
     ctx->boundary2cell.assign(m->boundary2cell_nr, m->boundary2cell_nr + NB);
     for (int64_t b = 0; b < NB; b++) {
         const int64_t c = m->boundary2cell_nr[b];
-        if (c < 0 || c >= N) return ctx->fail(MGR_ERR_ARGUMENT, "boundary2cell_nr out of range");
         c2b[c] = (int32_t)b;
         isb[c] = 1;
     }
 
-    // Rays::setup_antipodal_rays (rays.cpp:90-103): the last match wins
+    // Rays::setup_antipodal_rays (rays.cpp:90-103), within each ray set: the
+    // last match wins; antipod holds global ray indices
     ctx->antipod.assign(R, -1);
-    for (int64_t r1 = 0; r1 < R; r1++)
-        for (int64_t r2 = 0; r2 < R; r2++) {
-            const double x = m->rays[3 * r1 + 0] + m->rays[3 * r2 + 0];
-            const double y = m->rays[3 * r1 + 1] + m->rays[3 * r2 + 1];
-            const double z = m->rays[3 * r1 + 2] + m->rays[3 * r2 + 2];
-            if (((x * x + y * y) + z * z) < 1.0E-9) ctx->antipod[r1] = r2;
-        }
-    for (int64_t r = 0; r < R / 2; r++)
-        if (ctx->antipod[r] < 0) return ctx->fail(MGR_ERR_ARGUMENT, "a ray has no antipode");
+    for (size_t sidx = 0; sidx + 1 < set_off.size(); sidx++) {
+        const int64_t a0 = set_off[sidx], a1 = set_off[sidx + 1];
+        for (int64_t r1 = a0; r1 < a1; r1++)
+            for (int64_t r2 = a0; r2 < a1; r2++) {
+                const double x = m->rays[3 * r1 + 0] + m->rays[3 * r2 + 0];
+                const double y = m->rays[3 * r1 + 1] + m->rays[3 * r2 + 1];
+                const double z = m->rays[3 * r1 + 2] + m->rays[3 * r2 + 2];
+                if (((x * x + y * y) + z * z) < 1.0E-9) ctx->antipod[r1] = r2;
+            }
+        for (int64_t r = a0; r < a0 + (a1 - a0) / 2; r++)
+            if (ctx->antipod[r] < 0) return ctx->fail(MGR_ERR_ARGUMENT, "a ray has no antipode");
+    }
 
     // collision partners: offsets into the concatenated tables
     std::vector<int32_t> cpn(ctx->ncolpar + 1), cpc(ctx->ncolpar + 1), cpt(ctx->ncolpar + 1),
@@ -311,6 +477,24 @@ int upload_model(mgr_ctx *ctx, const mgr_model_view *m)
     if ((rc = upload(ctx, &ctx->rays, m->rays, R * 3))) return rc;
     if ((rc = upload(ctx, &ctx->ray_w, m->weights, R))) return rc;
     if ((rc = upload_cast<int32_t>(ctx, &ctx->d_antipod, ctx->antipod.data(), R))) return rc;
+    if ((rc = upload(ctx, &ctx->d_cell_model, cell_model.data(), N))) return rc;
+    if ((rc = upload(ctx, &ctx->d_cell_qbase, ctx->h_cell_qbase.data(), N))) return rc;
+    if ((rc = upload(ctx, &ctx->d_m_R, ctx->m_R.data(), Mm))) return rc;
+    if ((rc = upload(ctx, &ctx->d_m_rbase, ctx->m_rbase.data(), Mm))) return rc;
+    if ((rc = upload(ctx, &ctx->d_chunk_cell, chunk_cell.data(), ctx->nchunks + 1))) return rc;
+    if ((rc = upload(ctx, &ctx->d_m_chunk_off, ctx->m_chunk_off.data(), Mm + 1))) return rc;
+    if ((rc = upload(ctx, &ctx->d_m_cell_off, ctx->m_cell_off.data(), Mm + 1))) return rc;
+    {
+        std::vector<int32_t> ones(Mm, 1);
+        if ((rc = upload(ctx, &ctx->d_m_active, ones.data(), Mm))) return rc;
+    }
+    if ((rc = dalloc(ctx, &ctx->d_mconv, Mm * 4))) return rc;
+    if ((rc = dalloc(ctx, &ctx->d_mng, Mm * 4))) return rc;
+    MGR_HIP(hipHostMalloc((void **)&ctx->h_m_active, Mm * sizeof(int32_t), hipHostMallocDefault));
+    MGR_HIP(hipHostMalloc((void **)&ctx->h_mconv, Mm * 4 * sizeof(double), hipHostMallocDefault));
+    for (int64_t k = 0; k < Mm; k++) ctx->h_m_active[k] = 1;
+    ctx->m_iterations.assign(Mm, 0);
+    ctx->m_last_conv.assign(Mm, mgr_conv{0.0, 0.0, 0.0});
     if ((rc = upload(ctx, &ctx->temperature, m->temperature, N))) return rc;
     if ((rc = upload(ctx, &ctx->vturb2, m->vturb2, N))) return rc;
     if ((rc = upload(ctx, &ctx->abundance, m->abundance, N * m->nspecs))) return rc;
@@ -326,6 +510,7 @@ int upload_model(mgr_ctx *ctx, const mgr_model_view *m)
     if ((rc = upload(ctx, &ctx->roots, m->roots, Q))) return rc;
     if ((rc = upload(ctx, &ctx->qweights, m->qweights, Q))) return rc;
     if ((rc = upload(ctx, &ctx->cp_num, cpn.data(), ctx->ncolpar))) return rc;
+    ctx->h_cp_num.assign(m->colpar_num, m->colpar_num + ctx->ncolpar);
     if ((rc = upload(ctx, &ctx->cp_ncol, cpc.data(), ctx->ncolpar))) return rc;
     if ((rc = upload(ctx, &ctx->cp_ntmp, cpt.data(), ctx->ncolpar))) return rc;
     if ((rc = upload(ctx, &ctx->cp_koff, cpk.data(), ctx->ncolpar))) return rc;
@@ -364,7 +549,7 @@ int upload_model(mgr_ctx *ctx, const mgr_model_view *m)
     if ((rc = dalloc(ctx, &ctx->Jeff, N * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->Jdif, N * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->err_flag, 1))) return rc;
-    if ((rc = dalloc(ctx, &ctx->partials, (size_t)MAX_PARTIALS * 8))) return rc;
+    if ((rc = dalloc(ctx, &ctx->partials, (size_t)ctx->nchunks * 8))) return rc;
     if ((rc = dalloc(ctx, &ctx->reduced, 8))) return rc;
     if ((rc = dalloc(ctx, &ctx->shift_bits, 2))) return rc;
     MGR_HIP(hipHostMalloc((void **)&ctx->h_reduced, 8 * sizeof(double), hipHostMallocDefault));
@@ -407,8 +592,16 @@ int free_model(mgr_ctx *ctx)
     dfree(ctx->Jeff); dfree(ctx->Jdif); dfree(ctx->err_flag); dfree(ctx->seg_off); dfree(ctx->nseg);
     dfree(ctx->npts); dfree(ctx->seg_cell); dfree(ctx->seg_shift); dfree(ctx->seg_dZ);
     dfree(ctx->shift_bits); dfree(ctx->partials); dfree(ctx->reduced);
+    dfree(ctx->d_cell_model); dfree(ctx->d_cell_qbase); dfree(ctx->d_m_R); dfree(ctx->d_m_rbase);
+    dfree(ctx->d_m_active); dfree(ctx->d_chunk_cell); dfree(ctx->d_m_chunk_off); dfree(ctx->d_mconv);
+    dfree(ctx->d_m_cell_off);
+    dfree(ctx->d_mng);
     if (ctx->h_reduced) (void)hipHostFree(ctx->h_reduced);
+    if (ctx->h_m_active) (void)hipHostFree(ctx->h_m_active);
+    if (ctx->h_mconv) (void)hipHostFree(ctx->h_mconv);
     ctx->h_reduced = nullptr;
+    ctx->h_m_active = nullptr;
+    ctx->h_mconv = nullptr;
     if (ctx->graph_normal) (void)hipGraphExecDestroy(ctx->graph_normal);
     if (ctx->graph_ng) (void)hipGraphExecDestroy(ctx->graph_ng);
     ctx->graph_normal = ctx->graph_ng = nullptr;
@@ -428,11 +621,11 @@ int mgr_device_count(int *count)
     return hipGetDeviceCount(count) == hipSuccess ? MGR_OK : MGR_ERR_DEVICE;
 }
 
-int mgr_create(const mgr_model_view *model, int device, mgr_ctx **out)
+int mgr_create_batch(const mgr_model_view *models, int64_t nmodels, int device, mgr_ctx **out)
 {
     if (!out) return MGR_ERR_ARGUMENT;
     *out = nullptr;
-    if (!model) { g_create_error = "null model"; return MGR_ERR_ARGUMENT; }
+    if (!models || nmodels < 1) { g_create_error = "null or empty model list"; return MGR_ERR_ARGUMENT; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         g_create_error = "no HIP device available";
@@ -446,7 +639,7 @@ int mgr_create(const mgr_model_view *model, int device, mgr_ctx **out)
         delete ctx;
         return MGR_ERR_DEVICE;
     }
-    const int rc = upload_model(ctx, model);
+    const int rc = upload_models(ctx, models, nmodels);
     if (rc != MGR_OK) {
         g_create_error = ctx->err;
         free_model(ctx);
@@ -455,6 +648,32 @@ int mgr_create(const mgr_model_view *model, int device, mgr_ctx **out)
         return rc;
     }
     *out = ctx;
+    return MGR_OK;
+}
+
+int mgr_create(const mgr_model_view *model, int device, mgr_ctx **out)
+{
+    if (!model) {
+        if (out) *out = nullptr;
+        g_create_error = "null model";
+        return MGR_ERR_ARGUMENT;
+    }
+    return mgr_create_batch(model, 1, device, out);
+}
+
+int mgr_batch_info(mgr_ctx *ctx, int64_t *nmodels, int64_t *cell_offsets)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (nmodels) *nmodels = ctx->M;
+    if (cell_offsets) std::memcpy(cell_offsets, ctx->m_cell_off.data(), (ctx->M + 1) * sizeof(int64_t));
+    return MGR_OK;
+}
+
+int mgr_batch_convergence(mgr_ctx *ctx, int64_t *iterations, mgr_conv *conv)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (iterations) std::memcpy(iterations, ctx->m_iterations.data(), ctx->M * sizeof(int64_t));
+    if (conv) std::memcpy(conv, ctx->m_last_conv.data(), ctx->M * sizeof(mgr_conv));
     return MGR_OK;
 }
 

@@ -94,6 +94,7 @@ struct StateqArgs {
     const double *tmp, *Cd, *Ce;
     double *pop, *prev1, *prev2, *prev3;
     int32_t *err_flag;
+    const int32_t *cell_model, *m_active;   // converged models of a batch keep their populations
 };
 
 template <int VV>
@@ -204,6 +205,7 @@ __global__ void k_stateq(StateqArgs a)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.N) return;
+    if (!a.m_active[a.cell_model[p]]) return;
     double M[VV * VV], y[VV];
     stateq_cell<VV>(a, p, VV, M, y);
 }
@@ -213,6 +215,7 @@ __global__ void k_stateq_generic(StateqArgs a)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.N) return;
+    if (!a.m_active[a.cell_model[p]]) return;
     double M[VMAX_GENERIC * VMAX_GENERIC], y[VMAX_GENERIC];
     stateq_cell<0>(a, p, a.V, M, y);
 }
@@ -246,15 +249,21 @@ __device__ __forceinline__ double block_max(double v, double *sh)
 // check_for_convergence (lineProducingSpecies.tpp:120-171): per block partial
 // sums of the not-converged weight and the mean relative change, and the
 // maximum relative change; levels with n <= 1e-10 n_tot are ignored.
-__global__ void k_conv_partial(int64_t N, int V, double pop_prec, const double *__restrict__ pop,
-                               const double *__restrict__ prev1, const double *__restrict__ pop_tot,
-                               double *__restrict__ partials)
+// check_for_convergence (lineProducingSpecies.tpp:120-171) per model: each
+// block reduces one chunk of cells of one model; the final kernel sums the
+// chunks of every model in order (deterministic) and reports, for the batch,
+// the worst model.
+__global__ void k_conv_partial(int V, double pop_prec, const int64_t *__restrict__ chunk_cell,
+                               const int32_t *__restrict__ cell_model, const int64_t *__restrict__ m_cell_off,
+                               const double *__restrict__ pop, const double *__restrict__ prev1,
+                               const double *__restrict__ pop_tot, double *__restrict__ partials)
 {
     __shared__ double sh[RED_BLOCK];
-    const double weight = 1.0 / (double)(N * V);
+    const int64_t c0 = chunk_cell[blockIdx.x], c1 = chunk_cell[blockIdx.x + 1];
+    const int32_t mdl = cell_model[c0];
+    const double weight = 1.0 / (double)((m_cell_off[mdl + 1] - m_cell_off[mdl]) * V);
     double fnc = 0.0, rcm = 0.0, rmax = 0.0;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N;
-         p += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
         const double min_pop = 1.0E-10 * pop_tot[p];
         for (int i = 0; i < V; i++) {
             const int64_t ind = p * V + i;
@@ -278,18 +287,31 @@ __global__ void k_conv_partial(int64_t N, int V, double pop_prec, const double *
     }
 }
 
-__global__ void k_conv_final(int nparts, const double *__restrict__ partials, double *__restrict__ out,
-                             const int32_t *__restrict__ err_flag)
+__global__ void k_conv_final(int64_t M, const int64_t *__restrict__ m_chunk_off, const double *__restrict__ partials,
+                             const int32_t *__restrict__ m_active, double *__restrict__ mconv,
+                             double *__restrict__ out, const int32_t *__restrict__ err_flag)
 {
     __shared__ double sh[RED_BLOCK];
     double a = 0.0, b = 0.0, c = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-        a += partials[3 * i + 0];
-        b += partials[3 * i + 1];
-        c = fmax(c, partials[3 * i + 2]);
+    for (int64_t m = threadIdx.x; m < M; m += blockDim.x) {
+        double fa = 0.0, fb = 0.0, fc = 0.0;
+        for (int64_t i = m_chunk_off[m]; i < m_chunk_off[m + 1]; i++) {
+            fa += partials[3 * i + 0];
+            fb += partials[3 * i + 1];
+            fc = fmax(fc, partials[3 * i + 2]);
+        }
+        mconv[4 * m + 0] = fa;
+        mconv[4 * m + 1] = fb;
+        mconv[4 * m + 2] = fc;
+        mconv[4 * m + 3] = (double)m_active[m];
+        if (m_active[m]) {
+            a = fmax(a, fa);
+            b = fmax(b, fb);
+            c = fmax(c, fc);
+        }
     }
-    a = block_sum(a, sh);
-    b = block_sum(b, sh);
+    a = block_max(a, sh);
+    b = block_max(b, sh);
     c = block_max(c, sh);
     if (threadIdx.x == 0) {
         out[0] = a;
@@ -299,17 +321,17 @@ __global__ void k_conv_final(int nparts, const double *__restrict__ partials, do
     }
 }
 
-// Ng acceleration (lineProducingSpecies.tpp:181-238): five global dot
-// products, then x <- (1-a-b) x + a x1 + b x2 and the history shift when the
-// determinant is non-zero.
-__global__ void k_ng_partial(int64_t n, const double *__restrict__ x, const double *__restrict__ x1,
-                             const double *__restrict__ x2, const double *__restrict__ x3,
-                             double *__restrict__ partials)
+// Ng acceleration (lineProducingSpecies.tpp:181-238) per model: five dot
+// products over the model's populations, then x <- (1-a-b) x + a x1 + b x2
+// and the history shift where the determinant is non-zero.
+__global__ void k_ng_partial(int V, const int64_t *__restrict__ chunk_cell, const double *__restrict__ x,
+                             const double *__restrict__ x1, const double *__restrict__ x2,
+                             const double *__restrict__ x3, double *__restrict__ partials)
 {
     __shared__ double sh[RED_BLOCK];
+    const int64_t i0 = chunk_cell[blockIdx.x] * V, i1 = chunk_cell[blockIdx.x + 1] * V;
     double A1 = 0, A2 = 0, B2 = 0, C1 = 0, C2 = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const double Q1 = x[i] - 2.0 * x1[i] + x2[i];
         const double Q2 = x[i] - x1[i] - x2[i] + x3[i];
         const double Q3 = x[i] - x1[i];
@@ -323,30 +345,32 @@ __global__ void k_ng_partial(int64_t n, const double *__restrict__ x, const doub
     }
 }
 
-__global__ void k_ng_final(int nparts, const double *__restrict__ partials, double *__restrict__ coef)
+__global__ void k_ng_final(int64_t M, const int64_t *__restrict__ m_chunk_off, const double *__restrict__ partials,
+                           double *__restrict__ coef)
 {
-    __shared__ double sh[RED_BLOCK];
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
     double s[5] = {0, 0, 0, 0, 0};
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+    for (int64_t i = m_chunk_off[m]; i < m_chunk_off[m + 1]; i++)
         for (int q = 0; q < 5; q++) s[q] += partials[5 * i + q];
-    for (int q = 0; q < 5; q++) s[q] = block_sum(s[q], sh);
-    if (threadIdx.x == 0) {
-        const double A1 = s[0], A2 = s[1], B2 = s[2], C1 = s[3], C2 = s[4];
-        const double B1 = A2;
-        const double denominator = A1 * B2 - A2 * B1;
-        coef[4] = denominator;
-        coef[5] = (denominator != 0.0) ? (C1 * B2 - C2 * B1) / denominator : 0.0;
-        coef[6] = (denominator != 0.0) ? (C2 * A1 - C1 * A2) / denominator : 0.0;
-    }
+    const double A1 = s[0], A2 = s[1], B2 = s[2], C1 = s[3], C2 = s[4];
+    const double B1 = A2;
+    const double denominator = A1 * B2 - A2 * B1;
+    coef[4 * m + 0] = denominator;
+    coef[4 * m + 1] = (denominator != 0.0) ? (C1 * B2 - C2 * B1) / denominator : 0.0;
+    coef[4 * m + 2] = (denominator != 0.0) ? (C2 * A1 - C1 * A2) / denominator : 0.0;
 }
 
-__global__ void k_ng_apply(int64_t n, const double *__restrict__ coef, double *__restrict__ x,
-                           double *__restrict__ x1, double *__restrict__ x2, double *__restrict__ x3)
+__global__ void k_ng_apply(int64_t N, int V, const int32_t *__restrict__ cell_model,
+                           const int32_t *__restrict__ m_active, const double *__restrict__ coef,
+                           double *__restrict__ x, double *__restrict__ x1, double *__restrict__ x2,
+                           double *__restrict__ x3)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (coef[4] == 0.0) return;
-    const double a = coef[5], b = coef[6];
+    if (i >= N * V) return;
+    const int32_t m = cell_model[i / V];
+    if (!m_active[m] || coef[4 * m] == 0.0) return;
+    const double a = coef[4 * m + 1], b = coef[4 * m + 2];
     const double xo = x[i];
     x[i] = (1.0 - a - b) * xo + a * x1[i] + b * x2[i];
     x3[i] = x2[i];
@@ -358,10 +382,6 @@ __global__ void k_zero_flag(int32_t *flag) { *flag = 0; }
 
 unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-int conv_blocks(mgr_ctx *ctx)
-{
-    return (int)std::min<int64_t>(MAX_PARTIALS, std::max<int64_t>(1, (ctx->N + RED_BLOCK - 1) / RED_BLOCK));
-}
 
 }  // namespace
 
@@ -399,6 +419,8 @@ int launch_stateq(mgr_ctx *ctx)
     a.icol = ctx->icol; a.jcol = ctx->jcol; a.tmp = ctx->tmp; a.Cd = ctx->Cd; a.Ce = ctx->Ce;
     a.pop = ctx->pop; a.prev1 = ctx->prev1; a.prev2 = ctx->prev2; a.prev3 = ctx->prev3;
     a.err_flag = ctx->err_flag;
+    a.cell_model = ctx->d_cell_model;
+    a.m_active = ctx->d_m_active;
     const dim3 g(grid_for(ctx->N, 128)), b(128);
     switch (ctx->V) {
         case 2: hipLaunchKernelGGL(k_stateq<2>, g, b, 0, ctx->stream, a); break;
@@ -419,27 +441,27 @@ int launch_stateq(mgr_ctx *ctx)
 
 int launch_convergence(mgr_ctx *ctx)
 {
-    const int nb = conv_blocks(ctx);
-    hipLaunchKernelGGL(k_conv_partial, dim3(nb), dim3(RED_BLOCK), 0, ctx->stream, ctx->N, (int)ctx->V,
-                       ctx->pop_prec, ctx->pop, ctx->prev1, ctx->pop_tot, ctx->partials);
+    hipLaunchKernelGGL(k_conv_partial, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, (int)ctx->V,
+                       ctx->pop_prec, ctx->d_chunk_cell, ctx->d_cell_model, ctx->d_m_cell_off,
+                       ctx->pop, ctx->prev1, ctx->pop_tot, ctx->partials);
     MGR_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_conv_final, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, nb, ctx->partials,
-                       ctx->reduced, ctx->err_flag);
+    hipLaunchKernelGGL(k_conv_final, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->M, ctx->d_m_chunk_off,
+                       ctx->partials, ctx->d_m_active, ctx->d_mconv, ctx->reduced, ctx->err_flag);
     MGR_LAUNCH_CHECK();
     return MGR_OK;
 }
 
 int launch_ng(mgr_ctx *ctx)
 {
-    const int64_t n = ctx->N * ctx->V;
-    const int nb = (int)std::min<int64_t>(MAX_PARTIALS, std::max<int64_t>(1, (n + RED_BLOCK - 1) / RED_BLOCK));
-    hipLaunchKernelGGL(k_ng_partial, dim3(nb), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->pop, ctx->prev1,
-                       ctx->prev2, ctx->prev3, ctx->partials);
+    hipLaunchKernelGGL(k_ng_partial, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, (int)ctx->V,
+                       ctx->d_chunk_cell, ctx->pop, ctx->prev1, ctx->prev2, ctx->prev3, ctx->partials);
     MGR_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_ng_final, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, nb, ctx->partials, ctx->reduced);
+    hipLaunchKernelGGL(k_ng_final, dim3(grid_for(ctx->M, 64)), dim3(64), 0, ctx->stream, ctx->M,
+                       ctx->d_m_chunk_off, ctx->partials, ctx->d_mng);
     MGR_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_ng_apply, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, ctx->reduced,
-                       ctx->pop, ctx->prev1, ctx->prev2, ctx->prev3);
+    hipLaunchKernelGGL(k_ng_apply, dim3(grid_for(ctx->N * ctx->V, 256)), dim3(256), 0, ctx->stream, ctx->N,
+                       (int)ctx->V, ctx->d_cell_model, ctx->d_m_active, ctx->d_mng, ctx->pop, ctx->prev1,
+                       ctx->prev2, ctx->prev3);
     MGR_LAUNCH_CHECK();
     return MGR_OK;
 }
@@ -448,8 +470,24 @@ int launch_ng(mgr_ctx *ctx)
 static int enqueue_conv_copy(mgr_ctx *ctx)
 {
     MGR_HIP(hipMemcpyAsync(ctx->h_reduced, ctx->reduced, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    MGR_HIP(hipMemcpyAsync(ctx->h_mconv, ctx->d_mconv, ctx->M * 4 * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
     return MGR_OK;
 }
+
+}  // namespace mgr
+
+// Copy the host's per-model activity flags to the device (outside graphs).
+int mgr::set_active(mgr_ctx *ctx, bool all)
+{
+    if (all)
+        for (int64_t m = 0; m < ctx->M; m++) ctx->h_m_active[m] = 1;
+    MGR_HIP(hipMemcpyAsync(ctx->d_m_active, ctx->h_m_active, ctx->M * sizeof(int32_t), hipMemcpyHostToDevice,
+                           ctx->stream));
+    return MGR_OK;
+}
+
+namespace mgr {
 
 int read_convergence(mgr_ctx *ctx, mgr_conv *conv)
 {
@@ -519,6 +557,7 @@ int mgr_compute_level_populations_from_stateq(mgr_ctx *ctx, mgr_conv *conv)
     if (!ctx->J_set) return ctx->fail(MGR_ERR_STATE, "compute_radiation_field and compute_Jeff first");
     MGR_HIP(hipSetDevice(ctx->device));
     int rc;
+    if ((rc = set_active(ctx, true))) return rc;
     hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, ctx->stream, ctx->err_flag);
     MGR_LAUNCH_CHECK();
     if ((rc = launch_stateq(ctx))) return rc;
@@ -534,6 +573,7 @@ int mgr_iteration_using_Ng_acceleration(mgr_ctx *ctx, mgr_conv *conv)
     if (!ctx->pops_set) return ctx->fail(MGR_ERR_STATE, "level populations not set");
     MGR_HIP(hipSetDevice(ctx->device));
     int rc;
+    if ((rc = set_active(ctx, true))) return rc;
     hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, ctx->stream, ctx->err_flag);
     MGR_LAUNCH_CHECK();
     if ((rc = launch_ng(ctx))) return rc;
@@ -569,6 +609,7 @@ int mgr_ali_iteration(mgr_ctx *ctx, int ng, int precision, mgr_conv *conv)
     MGR_HIP(hipSetDevice(ctx->device));
     int rc = ensure_graphs(ctx, precision);
     if (rc) return rc;
+    if ((rc = set_active(ctx, true))) return rc;
     MGR_HIP(hipGraphLaunch(ng ? ctx->graph_ng : ctx->graph_normal, ctx->stream));
     rc = read_convergence(ctx, conv);
     if (rc == MGR_OK && !ng) ctx->J_set = true;
@@ -589,6 +630,10 @@ int mgr_compute_level_populations(mgr_ctx *ctx, int use_Ng_acceleration, int64_t
     MGR_HIP(hipSetDevice(ctx->device));
     int rc = ensure_graphs(ctx, precision);
     if (rc) return rc;
+    // Every model of a batch iterates until it meets the criterion (its own
+    // iteration count, as if solved alone); converged models are frozen.
+    if ((rc = set_active(ctx, true))) return rc;
+    for (int64_t m = 0; m < ctx->M; m++) ctx->m_iterations[m] = 0;
     int64_t iteration = 0, iteration_normal = 0;
     bool some_not_converged = true;
     while (some_not_converged && iteration < max_niterations) {
@@ -596,12 +641,19 @@ int mgr_compute_level_populations(mgr_ctx *ctx, int use_Ng_acceleration, int64_t
         const bool ng = use_Ng_acceleration && (iteration_normal == 4);
         MGR_HIP(hipGraphLaunch(ng ? ctx->graph_ng : ctx->graph_normal, ctx->stream));
         iteration_normal = ng ? 0 : iteration_normal + 1;
-        mgr_conv conv;
-        if ((rc = read_convergence(ctx, &conv))) return rc;
-        if (criterion == MGR_CRIT_MAX)
-            some_not_converged = conv.relative_change_max >= ctx->pop_prec;
-        else
-            some_not_converged = conv.fraction_not_converged > 0.005;
+        if ((rc = read_convergence(ctx, nullptr))) return rc;
+        some_not_converged = false;
+        for (int64_t m = 0; m < ctx->M; m++) {
+            if (!ctx->h_m_active[m]) continue;
+            const mgr_conv c{ctx->h_mconv[4 * m + 0], ctx->h_mconv[4 * m + 1], ctx->h_mconv[4 * m + 2]};
+            ctx->m_last_conv[m] = c;
+            ctx->m_iterations[m] = iteration;
+            const bool not_conv = (criterion == MGR_CRIT_MAX) ? c.relative_change_max >= ctx->pop_prec
+                                                              : c.fraction_not_converged > 0.005;
+            if (not_conv) some_not_converged = true;
+            else ctx->h_m_active[m] = 0;
+        }
+        if (some_not_converged && ctx->M > 1 && (rc = set_active(ctx, false))) return rc;
     }
     ctx->J_set = true;
     if (iterations_out) *iterations_out = iteration;

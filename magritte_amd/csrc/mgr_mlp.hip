@@ -64,7 +64,6 @@ constexpr int BF_W2B = BF_W2F + HID * HID;
 constexpr int BF_W3F = BF_W2B + HID * HID;
 constexpr int BF_W3B = BF_W3F + NOUT * HID;
 constexpr int BF_TOTAL = BF_W3B + HID * NOUT;
-constexpr int BF_FWD = BF_W2B;   // W1f, W2f are contiguous; W3f follows W2b
 
 // LDS of the training kernel (bf16 elements)
 constexpr int L_X = BF_TOTAL;                 // [ROWS][KIN]
@@ -210,7 +209,7 @@ __global__ __launch_bounds__(THREADS) void k_mlp_grad(const float *__restrict__ 
     extern __shared__ __align__(16) uint16_t s[];
     uint16_t *sX = s + L_X, *sH1 = s + L_H1, *sH2 = s + L_H2, *sDY = s + L_DY, *sDZ2 = s + L_DZ2,
              *sDZ1 = s + L_DZ1;
-    const Norm nm = *norm;
+    const Norm &nm = *norm;
     load_weights(s, wbf, false);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     const int R0 = wave * 16;
@@ -365,7 +364,7 @@ __global__ __launch_bounds__(THREADS) void k_mlp_forward(const float *__restrict
 {
     extern __shared__ __align__(16) uint16_t s[];
     uint16_t *sX = s + L_X, *sH1 = s + L_H1, *sH2 = s + L_H2;
-    const Norm nm = *norm;
+    const Norm &nm = *norm;
     load_weights(s, wbf, true);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     const int R0 = wave * 16;

@@ -87,7 +87,7 @@ struct Step {
 template <typename real>
 struct RFArgs {
     int64_t N;
-    int     R, F, L, nchunk, nwg, ngroups, grid_real, ch;
+    int     F, L, nchunk, nwg, ngroups, grid_real, ch;
     size_t  row_offset;                       // byte offset of the row region in LDS
     const LineTables *tb;
     const double  *nu;                        // [N*F]
@@ -95,7 +95,10 @@ struct RFArgs {
     const int64_t *seg_off;
     const int32_t *nseg, *npts, *seg_cell;
     const double  *seg_shift, *seg_dZ;
-    const int32_t *cell2bdy, *antipod;
+    const int32_t *cell2bdy, *antipod;        // antipod: global ray indices
+    const int32_t *cell_model;                // [N] model of each cell
+    const int64_t *cell_qbase;                // [N] first (origin, ray) entry of the cell
+    const int32_t *m_R, *m_rbase, *m_active;  // per model: rays, first global ray, still iterating
     const double  *I_bdy;                     // [NB*F]
     const double  *ray_w, *dshift_max;
     double         cand_lo, cand_hi;          // candidate-line bounds (FP32 variant)
@@ -444,6 +447,8 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
     const int logical = (bid & 7) * per + (bid >> 3);
     if (logical >= a.grid_real) return;   // whole workgroup: before any barrier
     const int32_t o = (int32_t)(logical / a.ngroups);
+    const int32_t mdl = a.cell_model[o];
+    if (!a.m_active[mdl]) return;         // converged model of a batch: whole workgroup
     const int group = logical - o * a.ngroups;
     const int wave = threadIdx.x / WAVE;
     const int chunk = group * a.nwg + wave;
@@ -489,12 +494,14 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
 
     int32_t err = 0;
     double J_acc = 0.0, Lam_acc = 0.0;
-    const int Rh = a.R / 2;
+    const int rb = a.m_rbase[mdl];
+    const int Rh = a.m_R[mdl] / 2;
+    const int64_t qb = a.cell_qbase[o];
     for (int r = 0; r < Rh; r++) {
-        const int ar = a.antipod[r];
-        const double w2 = 2.0 * a.ray_w[r];
-        const int64_t qr = (int64_t)o * a.R + r;
-        const int64_t qa = (int64_t)o * a.R + ar;
+        const int ar = a.antipod[rb + r];
+        const double w2 = 2.0 * a.ray_w[rb + r];
+        const int64_t qr = qb + r;
+        const int64_t qa = qb + (ar - rb);
         const int n_r = a.npts[qr];
         const int n_a = a.npts[qa];
         if (n_r == 0 && n_a == 0) {
@@ -537,7 +544,11 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows)
 {
     RFArgs<real> a;
     a.N = ctx->N;
-    a.R = (int)ctx->R;
+    a.cell_model = ctx->d_cell_model;
+    a.cell_qbase = ctx->d_cell_qbase;
+    a.m_R = ctx->d_m_R;
+    a.m_rbase = ctx->d_m_rbase;
+    a.m_active = ctx->d_m_active;
     a.F = (int)ctx->F;
     a.L = (int)ctx->L;
     a.nchunk = (int)((ctx->F + WAVE - 1) / WAVE);
@@ -615,6 +626,8 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
         int rc = launch_trace(ctx);
         if (rc) return rc;
     }
+    int rc0 = set_active(ctx, true);
+    if (rc0) return rc0;
     hipEvent_t e0, e1;
     MGR_HIP(hipEventCreate(&e0));
     MGR_HIP(hipEventCreate(&e1));

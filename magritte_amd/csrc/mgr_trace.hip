@@ -31,8 +31,13 @@ __device__ __forceinline__ void atomic_minmax_pos_double(uint64_t *addr, double 
     if (hi > 0.0) atomicMax((unsigned long long *)addr + 1, (unsigned long long)__double_as_longlong(hi));
 }
 
+// One thread per (origin o, ray r < Rmax); r beyond the origin's own ray
+// count exits.  Entry q = cell_qbase[o] + r of the (origin, ray) arrays; the
+// ray direction is the global ray m_rbase[model] + r.
 template <bool FILL>
-__global__ void k_trace(int64_t N, int R, const double *__restrict__ pos,
+__global__ void k_trace(int64_t N, int Rmax, const int32_t *__restrict__ cell_model,
+                        const int64_t *__restrict__ cell_qbase, const int32_t *__restrict__ m_R,
+                        const int32_t *__restrict__ m_rbase, const double *__restrict__ pos,
                         const double *__restrict__ vel, const int64_t *__restrict__ nb_off,
                         const int32_t *__restrict__ nb, const uint8_t *__restrict__ is_boundary,
                         const double *__restrict__ rays, const double *__restrict__ dshift_max,
@@ -41,11 +46,15 @@ __global__ void k_trace(int64_t N, int R, const double *__restrict__ pos,
                         double *__restrict__ seg_shift, double *__restrict__ seg_dZ,
                         uint64_t *__restrict__ shift_bits, int32_t *__restrict__ err_flag)
 {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= N * R) return;
-    const int32_t o = (int32_t)(q / R);
-    const int r = (int)(q % R);
-    const double nx = rays[3 * r + 0], ny = rays[3 * r + 1], nz = rays[3 * r + 2];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * Rmax) return;
+    const int32_t o = (int32_t)(t / Rmax);
+    const int r = (int)(t % Rmax);
+    const int32_t mdl = cell_model[o];
+    if (r >= m_R[mdl]) return;
+    const int64_t q = cell_qbase[o] + r;
+    const int64_t rg = (int64_t)m_rbase[mdl] + r;
+    const double nx = rays[3 * rg + 0], ny = rays[3 * rg + 1], nz = rays[3 * rg + 2];
     const double dsm = dshift_max[o];
     int64_t w = FILL ? seg_off[q] : 0;
     int32_t steps = 0, points = 0;
@@ -129,8 +138,8 @@ namespace mgr {
 
 int launch_trace(mgr_ctx *ctx)
 {
-    const int64_t N = ctx->N, R = ctx->R, NR = N * R;
-    const unsigned grid = (unsigned)((NR + 127) / 128);
+    const int64_t N = ctx->N, R = ctx->R, NR = ctx->NQ;
+    const unsigned grid = (unsigned)((N * R + 127) / 128);
     if (!ctx->nseg) {
         MGR_HIP(hipMalloc((void **)&ctx->nseg, NR * sizeof(int32_t)));
         MGR_HIP(hipMalloc((void **)&ctx->npts, NR * sizeof(int32_t)));
@@ -139,7 +148,8 @@ int launch_trace(mgr_ctx *ctx)
     static const uint64_t one_bits[2] = {0x3FF0000000000000ull, 0x3FF0000000000000ull};   // 1.0, 1.0
     MGR_HIP(hipMemcpyAsync(ctx->shift_bits, one_bits, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
     MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
-    hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->pos,
+    hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->d_cell_model,
+                       ctx->d_cell_qbase, ctx->d_m_R, ctx->d_m_rbase, ctx->pos,
                        ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->dshift_max,
                        ctx->nseg, ctx->npts, nullptr, nullptr, nullptr, nullptr, ctx->shift_bits,
                        ctx->err_flag);
@@ -168,12 +178,17 @@ int launch_trace(mgr_ctx *ctx)
 
     // ray-pair statistics: ndep = n_ar + n_r + 1 (sim_radiation.tpp:164-178)
     int64_t sum_ndep = 0, solved = 0;
-    for (int64_t o = 0; o < N; o++)
-        for (int64_t r = 0; r < R / 2; r++) {
-            const int64_t nd = (int64_t)np[o * R + r] + np[o * R + ctx->antipod[r]] + 1;
-            sum_ndep += nd;
-            solved += (nd > 1);
+    for (int64_t k = 0; k < ctx->M; k++) {
+        const int64_t Rk = ctx->m_R[k], rb = ctx->m_rbase[k];
+        for (int64_t o = ctx->m_cell_off[k]; o < ctx->m_cell_off[k + 1]; o++) {
+            const int64_t qb = ctx->h_cell_qbase[o];
+            for (int64_t r = 0; r < Rk / 2; r++) {
+                const int64_t nd = (int64_t)np[qb + r] + np[qb + ctx->antipod[rb + r] - rb] + 1;
+                sum_ndep += nd;
+                solved += (nd > 1);
+            }
         }
+    }
     ctx->sum_ndep = sum_ndep;
     ctx->n_pairs_solved = solved;
 
@@ -186,7 +201,8 @@ int launch_trace(mgr_ctx *ctx)
     MGR_HIP(hipMalloc((void **)&ctx->seg_shift, T * sizeof(double)));
     MGR_HIP(hipMalloc((void **)&ctx->seg_dZ, T * sizeof(double)));
     MGR_HIP(hipMemcpy(ctx->seg_off, off.data(), (NR + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->pos,
+    hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->d_cell_model,
+                       ctx->d_cell_qbase, ctx->d_m_R, ctx->d_m_rbase, ctx->pos,
                        ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->dshift_max,
                        nullptr, nullptr, ctx->seg_off, ctx->seg_cell, ctx->seg_shift, ctx->seg_dZ,
                        nullptr, ctx->err_flag);
@@ -212,8 +228,12 @@ int mgr_get_ray(mgr_ctx *ctx, int64_t origin, int64_t ray, int64_t capacity, int
 {
     if (!ctx || !n_out || capacity < 0) return MGR_ERR_ARGUMENT;
     if (!ctx->traced) return ctx->fail(MGR_ERR_STATE, "mgr_trace first");
-    if (origin < 0 || origin >= ctx->N || ray < 0 || ray >= ctx->R)
-        return ctx->fail(MGR_ERR_ARGUMENT, "origin or ray out of range");
+    if (origin < 0 || origin >= ctx->N) return ctx->fail(MGR_ERR_ARGUMENT, "origin out of range");
+    {
+        const int64_t k = std::upper_bound(ctx->m_cell_off.begin(), ctx->m_cell_off.end(), origin) -
+                          ctx->m_cell_off.begin() - 1;
+        if (ray < 0 || ray >= ctx->m_R[k]) return ctx->fail(MGR_ERR_ARGUMENT, "ray out of range");
+    }
     if (capacity > 0 && (!cellNr || !shift || !dZ)) return MGR_ERR_ARGUMENT;
     MGR_HIP(hipSetDevice(ctx->device));
     const int64_t cap = std::max<int64_t>(capacity, 1);
@@ -224,7 +244,7 @@ int mgr_get_ray(mgr_ctx *ctx, int64_t origin, int64_t ray, int64_t capacity, int
     MGR_HIP(hipMalloc((void **)&dz, cap * sizeof(double)));
     MGR_HIP(hipMalloc((void **)&dn, sizeof(int64_t)));
     hipLaunchKernelGGL(k_expand_ray, dim3(1), dim3(64), 0, ctx->stream, (int32_t)origin,
-                       origin * ctx->R + ray, ctx->seg_off, ctx->nseg, ctx->seg_cell, ctx->seg_shift,
+                       ctx->h_cell_qbase[origin] + ray, ctx->seg_off, ctx->nseg, ctx->seg_cell, ctx->seg_shift,
                        ctx->seg_dZ, ctx->dshift_max, capacity, dc, ds, dz, dn);
     hipError_t le = hipGetLastError();
     hipError_t se = hipStreamSynchronize(ctx->stream);
@@ -249,11 +269,19 @@ int mgr_get_npoints_on_rays(mgr_ctx *ctx, int64_t *npoints)
     if (!ctx->traced) return ctx->fail(MGR_ERR_STATE, "mgr_trace first");
     MGR_HIP(hipSetDevice(ctx->device));
     const int64_t N = ctx->N, R = ctx->R;
-    std::vector<int32_t> np(N * R);
-    MGR_HIP(hipMemcpy(np.data(), ctx->npts, N * R * sizeof(int32_t), hipMemcpyDeviceToHost));
-    for (int64_t r = 0; r < R / 2; r++)
-        for (int64_t o = 0; o < N; o++)
-            npoints[r * N + o] = (int64_t)np[o * R + r] + np[o * R + ctx->antipod[r]] + 1;
+    for (int64_t k = 0; k < ctx->M; k++)
+        if (ctx->m_R[k] != R)
+            return ctx->fail(MGR_ERR_UNSUPPORTED, "npoints_on_rays needs the same ray count in every model");
+    std::vector<int32_t> np(ctx->NQ);
+    MGR_HIP(hipMemcpy(np.data(), ctx->npts, ctx->NQ * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int64_t o = 0; o < N; o++) {
+        const int64_t qb = ctx->h_cell_qbase[o];
+        const int64_t k = std::upper_bound(ctx->m_cell_off.begin(), ctx->m_cell_off.end(), o) -
+                          ctx->m_cell_off.begin() - 1;
+        const int64_t rb = ctx->m_rbase[k];
+        for (int64_t r = 0; r < R / 2; r++)
+            npoints[r * N + o] = (int64_t)np[qb + r] + np[qb + ctx->antipod[rb + r] - rb] + 1;
+    }
     return MGR_OK;
 }
 

@@ -24,6 +24,9 @@ MGR_CRIT_FRACTION, MGR_CRIT_MAX = 0, 1
 _vp, _i, _i64, _d = C.c_void_p, C.c_int, C.c_int64, C.c_double
 SIGNATURES = {
     "mgr_create": ([_vp, _i, _vp], _i),
+    "mgr_create_batch": ([_vp, _i64, _i, _vp], _i),
+    "mgr_batch_info": ([_vp, _vp, _vp], _i),
+    "mgr_batch_convergence": ([_vp, _vp, _vp], _i),
     "mgr_destroy": ([_vp], None),
     "mgr_last_error": ([_vp], C.c_char_p),
     "mgr_device_count": ([_vp], _i),
@@ -51,6 +54,18 @@ SIGNATURES = {
     "mgr_get_npoints_on_rays": ([_vp, _vp], _i),
     "mgr_last_radiation_field_ms": ([_vp, _vp], _i),
     "mgr_trace_stats": ([_vp, _vp, _vp], _i),
+    "mgr_mlp_num_params": ([], _i64),
+    "mgr_mlp_create": ([_i, C.c_uint64, _vp], _i),
+    "mgr_mlp_destroy": ([_vp], None),
+    "mgr_mlp_last_error": ([_vp], C.c_char_p),
+    "mgr_mlp_set_params": ([_vp, _vp], _i),
+    "mgr_mlp_get_params": ([_vp, _vp], _i),
+    "mgr_mlp_set_normalisation": ([_vp, _vp, _vp, _vp, _vp], _i),
+    "mgr_mlp_forward": ([_vp, _vp, _i64, _vp], _i),
+    "mgr_mlp_loss_grad": ([_vp, _vp, _vp, _i64, _vp, _vp], _i),
+    "mgr_mlp_train": ([_vp, _vp, _vp, _i64, _i64, _i64, C.c_float, C.c_uint64, _vp], _i),
+    "mgr_features": ([_vp, _vp], _i),
+    "mgr_warm_start": ([_vp, _vp], _i),
 }
 
 _LIB = None
@@ -252,3 +267,136 @@ class Context:
         a, b = C.c_int64(0), C.c_int64(0)
         self._check(self.lib.mgr_trace_stats(self.h, C.byref(a), C.byref(b)), "trace_stats")
         return a.value, b.value
+
+    # ---- learned warm start ----------------------------------------------------
+    def features(self):
+        """Per-cell warm-start features [N, 8] (after a radiation field and J_eff)."""
+        a = np.zeros((self.N, MLP_IN), np.float32)
+        self._check(self.lib.mgr_features(self.h, _p(a)), "features")
+        return a
+
+    def warm_start(self, mlp):
+        """Replace the populations by the network's prediction."""
+        self._check(self.lib.mgr_warm_start(self.h, mlp.h), "warm_start")
+
+
+class BatchContext(Context):
+    """Many independent models of the same species in one context
+    (mgr_create_batch): concatenated cells, one launch per kernel."""
+
+    def __init__(self, models, device=0):
+        from .model import ModelView
+        self.lib = load()
+        self.models = list(models)
+        views, keeps = zip(*[m.view() for m in self.models])
+        self._keep = keeps
+        arr = (ModelView * len(views))(*views)
+        self._views = arr
+        h = C.c_void_p()
+        rc = self.lib.mgr_create_batch(C.cast(arr, C.c_void_p), len(views), device, C.byref(h))
+        if rc != 0:
+            raise HipError(f"mgr_create_batch failed ({rc}): {self.lib.mgr_last_error(None).decode()}")
+        self.h = h
+        m0 = self.models[0]
+        self.model = m0
+        self.N = sum(m.ncells for m in self.models)
+        self.R = max(m.nrays for m in self.models)
+        self.F, self.L, self.V, self.Q = m0.nfreqs, m0.nrad, m0.nlev, m0.nquads
+        self.NB = sum(m.nboundary for m in self.models)
+        self.M = len(self.models)
+        off = np.zeros(self.M + 1, np.int64)
+        n = C.c_int64(0)
+        self._check(self.lib.mgr_batch_info(self.h, C.byref(n), _p(off)), "batch_info")
+        self.cell_offsets = off
+
+    def split(self, a):
+        """Per-model views of a concatenated per-cell array."""
+        return [a[self.cell_offsets[k]:self.cell_offsets[k + 1]] for k in range(self.M)]
+
+    def convergence(self):
+        its = np.zeros(self.M, np.int64)
+        conv = (ConvStats * self.M)()
+        self._check(self.lib.mgr_batch_convergence(self.h, _p(its), C.cast(conv, C.c_void_p)),
+                    "batch_convergence")
+        return its, [(c.fraction_not_converged, c.relative_change_mean, c.relative_change_max) for c in conv]
+
+
+MLP_IN, MLP_HIDDEN, MLP_OUT = 8, 128, 7
+
+
+def _unpack_params(flat):
+    """Packed FP32 parameters -> dict W1 b1 W2 b2 W3 b3 ([in, out] weights)."""
+    shapes = [("W1", (MLP_IN, MLP_HIDDEN)), ("b1", (MLP_HIDDEN,)), ("W2", (MLP_HIDDEN, MLP_HIDDEN)),
+              ("b2", (MLP_HIDDEN,)), ("W3", (MLP_HIDDEN, MLP_OUT)), ("b3", (MLP_OUT,))]
+    out, o = {}, 0
+    for k, sh in shapes:
+        n = int(np.prod(sh))
+        out[k] = flat[o:o + n].reshape(sh)
+        o += n
+    return out
+
+
+def _pack_params(p):
+    return np.concatenate([np.asarray(p[k], np.float32).ravel() for k in ("W1", "b1", "W2", "b2", "W3", "b3")])
+
+
+class Mlp:
+    """The warm-start network (mgr_mlp) on one device."""
+
+    def __init__(self, seed=0, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.mgr_mlp_create(device, seed, C.byref(h))
+        if rc != 0:
+            raise HipError(f"mgr_mlp_create failed ({rc})")
+        self.h = h
+        self.nparams = int(self.lib.mgr_mlp_num_params())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mgr_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise HipError(f"{what} failed ({rc}): {self.lib.mgr_mlp_last_error(self.h).decode()}")
+
+    def params(self):
+        a = np.zeros(self.nparams, np.float32)
+        self._check(self.lib.mgr_mlp_get_params(self.h, _p(a)), "get_params")
+        return _unpack_params(a)
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(_pack_params(p), dtype=np.float32)
+        assert a.size == self.nparams
+        self._check(self.lib.mgr_mlp_set_params(self.h, _p(a)), "set_params")
+
+    def set_normalisation(self, x_mean, x_std, y_mean, y_std):
+        arrs = [np.ascontiguousarray(v, dtype=np.float32) for v in (x_mean, x_std, y_mean, y_std)]
+        self._check(self.lib.mgr_mlp_set_normalisation(self.h, *[_p(v) for v in arrs]), "set_normalisation")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], MLP_OUT), np.float32)
+        self._check(self.lib.mgr_mlp_forward(self.h, _p(X), X.shape[0], _p(Y)), "forward")
+        return Y
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_float(0.0)
+        self._check(self.lib.mgr_mlp_loss_grad(self.h, _p(X), _p(T), X.shape[0], C.byref(loss), _p(g)),
+                    "loss_grad")
+        return loss.value, _unpack_params(g)
+
+    def train(self, X, T, epochs, batch, lr, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        losses = np.zeros(epochs, np.float32)
+        self._check(self.lib.mgr_mlp_train(self.h, _p(X), _p(T), X.shape[0], epochs, batch, lr, seed,
+                                           _p(losses)), "train")
+        return losses

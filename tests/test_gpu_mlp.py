@@ -1,0 +1,160 @@
+"""GPU checks of the warm-start network's HIP kernels against the CPU
+reference (oracle/mlp_oracle.py, itself pinned by finite differences), and of
+what a trained network gains over the plain LTE start.
+
+Bars:
+  * forward: within 1e-2 of the reference relative to max |Y| (bf16 operands;
+    a GELU output rounding to the other bf16 neighbour moves single values);
+  * loss within 1e-4 relative; each gradient block within 3e-2 in relative
+    L2 norm of the reference's bf16-rounded gradients;
+  * one Adam step: parameters within 1e-5 of the reference step (where the
+    gradient is not vanishingly small);
+  * training lowers the loss of a smooth regression task 10x;
+  * training on solved envelopes lowers the loss; a warm start converges to
+    the LTE-started populations within 1e-5;
+  * the committed trained network: the same converged populations in fewer
+    ALI iterations than the LTE start on held-out envelopes.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(rng, scale=0.3):
+    return {"W1": rng.normal(0, scale, (8, 128)), "b1": rng.normal(0, 0.1, 128),
+            "W2": rng.normal(0, scale / 4, (128, 128)), "b2": rng.normal(0, 0.1, 128),
+            "W3": rng.normal(0, scale / 4, (128, 7)), "b3": rng.normal(0, 0.1, 7)}
+
+
+def _f32(p):
+    return {k: np.asarray(v, np.float32).astype(np.float64) for k, v in p.items()}
+
+
+@pytest.fixture(scope="module")
+def mlp():
+    from magritte_amd.hip import Mlp
+    return Mlp(seed=1)
+
+
+@pytest.mark.parametrize("n", [1, 16, 100, 1000])
+def test_forward_matches_reference(mlp, n):
+    print(f"\n[mlp] forward n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    p = _f32(_params(rng))
+    mlp.set_params(p)
+    mlp.set_normalisation(np.zeros(8), np.ones(8), np.zeros(7), np.ones(7))
+    X = rng.normal(0, 1, (n, 8)).astype(np.float32)
+    Y = mlp.forward(X)
+    Yo, _ = mo.forward(p, X.astype(np.float64))
+    err = np.abs(Y - Yo).max() / np.abs(Yo).max()
+    assert err < 1e-2, err
+
+
+def test_normalisation_applied(mlp):
+    rng = np.random.default_rng(5)
+    p = _f32(_params(rng))
+    mlp.set_params(p)
+    xm, xs = rng.normal(0, 1, 8), rng.uniform(0.5, 2, 8)
+    ym, ys = rng.normal(0, 1, 7), rng.uniform(0.5, 2, 7)
+    mlp.set_normalisation(xm, xs, ym, ys)
+    X = rng.normal(0, 1, (200, 8)).astype(np.float32)
+    Y = mlp.forward(X)
+    Xn = (X.astype(np.float64) - np.float32(xm)) / np.float32(xs)
+    Yo, _ = mo.forward(p, Xn)
+    Yo = Yo * np.float32(ys) + np.float32(ym)
+    assert np.abs(Y - Yo).max() / np.abs(Yo).max() < 1e-2
+
+
+@pytest.mark.parametrize("n", [64, 300, 5000])
+def test_gradients_match_reference(mlp, n):
+    print(f"\n[mlp] gradients n={n}", flush=True)
+    rng = np.random.default_rng(100 + n)
+    p = _f32(_params(rng))
+    mlp.set_params(p)
+    mlp.set_normalisation(np.zeros(8), np.ones(8), np.zeros(7), np.ones(7))
+    X = rng.normal(0, 1, (n, 8)).astype(np.float32)
+    T = rng.normal(0, 1, (n, 7)).astype(np.float32)
+    loss, g = mlp.loss_grad(X, T)
+    lo, go = mo.loss_and_grads(p, X.astype(np.float64), T.astype(np.float64))
+    assert abs(loss - lo) <= 1e-4 * lo
+    for k in go:
+        err = np.linalg.norm(g[k] - go[k]) / np.linalg.norm(go[k])
+        assert err < 3e-2, (k, err)
+
+
+def test_adam_step_matches_reference(mlp):
+    rng = np.random.default_rng(7)
+    p = _f32(_params(rng))
+    mlp.set_params(p)
+    mlp.set_normalisation(np.zeros(8), np.ones(8), np.zeros(7), np.ones(7))
+    X = rng.normal(0, 1, (256, 8)).astype(np.float32)
+    T = rng.normal(0, 1, (256, 7)).astype(np.float32)
+    _, g = mlp.loss_grad(X, T)             # the device's own gradient
+    lr = 1e-3
+    mlp.train(X, T, epochs=1, batch=256, lr=lr, seed=0)   # one step over the whole set
+    got = mlp.params()
+    ref = {k: v.copy() for k, v in p.items()}
+    m = {k: np.zeros_like(v) for k, v in p.items()}
+    v = {k: np.zeros_like(x) for k, x in p.items()}
+    mo.adam_step(ref, {k: g[k].astype(np.float64) for k in g}, m, v, 1, lr)
+    for k in ref:
+        # where |g| is tiny the step's sign follows the summation order of g
+        # (the training pass sums the shuffled rows): allow the full 2 lr there
+        gk = np.abs(g[k])
+        tol = 1e-5 + 2.0 * lr * (gk < 1e-3 * gk.max())
+        assert np.all(np.abs(got[k] - ref[k]) <= tol), k
+
+
+def test_training_reduces_loss():
+    from magritte_amd.hip import Mlp
+    rng = np.random.default_rng(11)
+    X = rng.normal(0, 1, (20000, 8)).astype(np.float32)
+    W = rng.normal(0, 0.5, (8, 7))
+    T = np.tanh(X @ W).astype(np.float32) + 0.1 * np.sin(3 * X[:, :1])
+    net = Mlp(seed=3)
+    losses = net.train(X, T, epochs=20, batch=1024, lr=3e-3, seed=1)
+    print(f"\n[mlp] training loss {losses[0]:.4g} -> {losses[-1]:.4g}", flush=True)
+    assert losses[-1] < 0.1 * losses[0]
+    net.close()
+
+
+def test_warm_start_training_on_solved_models():
+    """Features and targets from populations the plain method converged on a
+    few small envelopes: training lowers the loss, and starting a held-out
+    envelope from the network converges to the LTE-started populations."""
+    from magritte_amd import warmstart
+    from magritte_amd.setup import synthetic
+    X, Y, its = warmstart.build_dataset(12, seed=21, log=lambda s: print(s, flush=True))
+    assert np.all(np.isfinite(X)) and np.all(np.isfinite(Y))
+    net, norm, losses = warmstart.fit(X, Y, epochs=40, batch=2048, lr=3e-3, seed=0,
+                                      log=lambda s: print(s, flush=True))
+    assert losses[-1] < 0.7 * losses[0]
+    m = synthetic.envelope_model(n=16, n0=1e13, T0=1500.0, seed=901)
+    r = warmstart.evaluate_model(m, net)
+    print(f"[mlp] held-out: {r}", flush=True)
+    assert r["pops_max_rel_diff"] < 1e-5
+    net.close()
+
+
+def test_pretrained_warm_start_gains_over_lte():
+    """The committed network (trained on the GPU by these kernels on
+    envelopes the plain method solved) against the LTE start on held-out
+    envelopes: the same converged populations in fewer ALI iterations."""
+    import os
+    from magritte_amd import warmstart
+    from magritte_amd.setup import synthetic
+    if not os.path.exists(warmstart.DEFAULT_WEIGHTS):
+        pytest.skip("no trained warm-start weights in magritte_amd/data")
+    net = warmstart.load(warmstart.DEFAULT_WEIGHTS)
+    gains = []
+    for seed, n0, T0 in ((911, 3e12, 900.0), (912, 3e13, 2000.0)):
+        m = synthetic.envelope_model(n=24, n0=n0, T0=T0, seed=seed)
+        r = warmstart.evaluate_model(m, net)
+        print(f"[mlp] held-out n0={n0:g} T0={T0:g}: {r}", flush=True)
+        assert r["pops_max_rel_diff"] < 1e-5
+        gains.append(r["lte_iterations"] - r["nn_iterations"])
+    assert max(gains) > 0 and min(gains) >= 0
+    net.close()
