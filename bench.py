@@ -17,7 +17,12 @@ per second over the timed steps.  The JSON line also carries:
                    launch); its limiter is VALU issue, reported beside it;
   cpu_baseline  -- the FP64 OpenMP oracle (restatement of the reference's
                    RayPair path) timed on a bounded random sample of origins;
-  time_to_converged -- the ALI loop from LTE to max relative change < 1e-6.
+  time_to_converged -- the ALI loop to max relative change < 1e-6, from LTE
+                   and from the learned warm start (trained weights in
+                   magritte_amd/data), with the converged populations' match.
+
+--workload batch measures the same on 512 small envelopes of uneven sizes
+and ray counts solved together in one context (one launch per kernel).
 """
 import argparse
 import json
@@ -96,69 +101,41 @@ def cpu_baseline(model, target_s, seed=0):
                        f"FP64 oracle, {dt:.1f} s")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=64, help="cells per side (64 = full workload)")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-converge", action="store_true")
-    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
-                    help="first run a plain device copy of this size (counter calibration)")
-    args = ap.parse_args()
+def copy_probe(gib):
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] copy probe: {4 * n} bytes read, {4 * n} bytes written")
+    del src, dst
 
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
-                          "n_gpus": args.gpus}))
-        return 2
 
-    if args.copy_probe_gib > 0:
-        import torch
-        n = int(args.copy_probe_gib * (1 << 30)) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"[bench] copy probe: {4 * n} bytes read, {4 * n} bytes written")
-        del src, dst
-
-    from magritte_amd.hip import Context
-    t0 = time.perf_counter()
-    model = build_workload(args.n)
-    log(f"[bench] model: {model.ncells} cells, {model.nrays} rays, {model.nfreqs} bins "
-        f"({time.perf_counter() - t0:.1f} s)")
-    t0 = time.perf_counter()
-    ctx = Context(model).setup()
-    ctx.trace()
-    sum_ndep, pairs = ctx.trace_stats()
-    log(f"[bench] set-up + trace {time.perf_counter() - t0:.1f} s; {pairs} ray pairs solved, "
-        f"mean ndep {sum_ndep / (model.ncells * model.nrays // 2):.1f}")
-    solves_per_step = pairs * model.nfreqs
-
-    for _ in range(args.warmup):
-        ctx.compute_radiation_field(args.precision)
+def time_radiation_field(ctx, steps, warmup, precision):
+    """Wall time of `steps` radiation fields after `warmup` untimed ones, and
+    the kernel's own HIP-event time per launch."""
+    for _ in range(warmup):
+        ctx.compute_radiation_field(precision)
     kernel_ms = []
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        ctx.compute_radiation_field(args.precision)
+    for i in range(steps):
+        ctx.compute_radiation_field(precision)
         kernel_ms.append(ctx.last_radiation_field_ms())
         log(f"[bench] step {i}: kernel {kernel_ms[-1]:.2f} ms")
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
-    value = solves_per_step * args.steps / wall
-    k_ms = float(np.mean(kernel_ms))
-    points = sum_ndep * model.nfreqs                 # sum over solves of their points
-    alg_bytes = BYTES_PER_POINT_BIN * points + BYTES_PER_SOLVE * solves_per_step
+    return time.perf_counter() - t0, float(np.mean(kernel_ms))
+
+
+def roofline_entry(sum_ndep, nfreqs, solves, k_ms, precision, counters_workload):
+    points = sum_ndep * nfreqs                 # sum over solves of their points
+    alg_bytes = BYTES_PER_POINT_BIN * points + BYTES_PER_SOLVE * solves
     flops = FLOPS_PER_POINT_BIN * points
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
     prof, pw = profile_counters(os.path.join(ROOT, "profiles", "radiation_field_envelope64_counters.json"))
-    same = prof is not None and pw == f"envelope{args.n}" and args.precision == "fp32"
-    roofline = {
+    same = prof is not None and pw == counters_workload and precision == "fp32"
+    return {
         "bound": "hbm",
-        "kernel": "k_radiation_field<float>" if args.precision == "fp32" else "k_radiation_field<double>",
+        "kernel": "k_radiation_field<float>" if precision == "fp32" else "k_radiation_field<double>",
         "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
         "frac": round(achieved / PEAK_HBM_GBS, 4),
         "traffic": prof.get("hbm_bytes_per_launch") if same else None,
@@ -169,37 +146,127 @@ def main():
         "fp32_tflops": round(flops / (k_ms * 1e-3) / 1e12, 3),
     }
 
+
+def time_to_converged(make_ctx, precision, weights):
+    """ALI from LTE, and from the learned warm start when trained weights are
+    present; the clock starts after the one-off set-up and trace."""
+    out = {"criterion": "max relative change < 1e-6, Ng every 5th"}
+    ctx = make_ctx()
+    t0 = time.perf_counter()
+    it = ctx.compute_level_populations(True, 400, "max", precision)
+    out["lte"] = {"seconds": round(time.perf_counter() - t0, 3), "iterations": int(it)}
+    P_lte = ctx.populations()
+    ctx.close()
+    log(f"[bench] converged from LTE in {it} iterations, {out['lte']['seconds']:.2f} s")
+    if weights and os.path.exists(weights):
+        from magritte_amd import warmstart
+        net = warmstart.load(weights)
+        ctx = make_ctx()
+        t0 = time.perf_counter()
+        ctx.compute_radiation_field(precision)
+        ctx.compute_Jeff()
+        ctx.warm_start(net)
+        it = ctx.compute_level_populations(True, 400, "max", precision)
+        dt = time.perf_counter() - t0
+        P_nn = ctx.populations()
+        ctx.close()
+        net.close()
+        mask = P_lte > 1e-10 * P_lte.sum(axis=1, keepdims=True)
+        diff = float(np.max(np.abs(P_nn - P_lte)[mask] / P_lte[mask]))
+        out["nn"] = {"seconds": round(dt, 3), "iterations": int(it), "pops_max_rel_diff_vs_lte": diff}
+        out["speedup_nn_vs_lte"] = round(out["lte"]["seconds"] / dt, 3)
+        log(f"[bench] converged from the warm start in {it} iterations, {dt:.2f} s")
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="envelope", choices=["envelope", "batch"],
+                    help="envelope: one n^3 model; batch: many small models in one context")
+    ap.add_argument("--n", type=int, default=64, help="cells per side of the envelope (64 = full workload)")
+    ap.add_argument("--models", type=int, default=512, help="models of the batch workload")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "magritte_amd", "data", "warmstart_mlp.npz"))
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="first run a plain device copy of this size (counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        return 2
+    if args.copy_probe_gib > 0:
+        copy_probe(args.copy_probe_gib)
+
+    from magritte_amd.hip import BatchContext, Context
+    t0 = time.perf_counter()
+    if args.workload == "envelope":
+        model = build_workload(args.n)
+        models = [model]
+        make_ctx = lambda: Context(model).setup()   # noqa: E731
+        workload = (f"envelope point cloud {args.n}^3 jittered, 26-neighbour lists, {model.nrays} rays, "
+                    f"7-level rotor, 6 lines x 31 bins")
+        tag = f"envelope{args.n}"
+    else:
+        from magritte_amd.warmstart import training_model
+        rng = np.random.default_rng(2024)
+        models = [training_model(rng) for _ in range(args.models)]
+        model = models[0]
+        make_ctx = lambda: BatchContext(models).setup()   # noqa: E731
+        workload = (f"batch of {args.models} envelopes, cells log-uniform 1000-32768, n0 1e12-1e16, "
+                    f"T0 500-3000 K, 12 or 48 rays, one context")
+        tag = f"batch{args.models}"
+    ncells = sum(m.ncells for m in models)
+    log(f"[bench] {len(models)} model(s), {ncells} cells, {model.nfreqs} bins ({time.perf_counter() - t0:.1f} s)")
+    t0 = time.perf_counter()
+    ctx = make_ctx()
+    ctx.trace()
+    sum_ndep, pairs = ctx.trace_stats()
+    log(f"[bench] set-up + trace {time.perf_counter() - t0:.1f} s; {pairs} ray pairs solved, sum ndep {sum_ndep}")
+    solves_per_step = pairs * model.nfreqs
+    wall, k_ms = time_radiation_field(ctx, args.steps, args.warmup, args.precision)
+    shape, tried = ctx.radiation_field_shape(args.precision)
+    log(f"[bench] launch shape (waves per workgroup, steps per chunk) {shape}; tried {tried}")
+    ms_per_step = 1e3 * wall / args.steps
+    value = solves_per_step * args.steps / wall
+
     out = {
         "metric": METRIC, "value": value, "unit": "ray-pair·freq solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
-        "config": {"workload": f"envelope point cloud {args.n}^3 jittered, 26-neighbour lists, "
-                               f"{model.nrays} rays, 7-level rotor, 6 lines x 31 bins",
-                   "ncells": model.ncells, "nrays": model.nrays, "nfreqs": model.nfreqs,
-                   "solves_per_step": int(solves_per_step), "sum_ndep": int(sum_ndep)},
-        "roofline": roofline,
+        "config": {"workload": workload, "models": len(models), "ncells": ncells,
+                   "nrays": max(m.nrays for m in models), "nfreqs": model.nfreqs,
+                   "solves_per_step": int(solves_per_step), "sum_ndep": int(sum_ndep),
+                   "launch_shape": {"waves_per_workgroup": shape[0], "steps_per_chunk": shape[1]}},
+        "roofline": roofline_entry(sum_ndep, model.nfreqs, solves_per_step, k_ms, args.precision, tag),
     }
+    ctx.close()
 
     if not args.no_converge:
-        ctx2 = Context(model).setup()
-        ctx2.trace()
-        t0 = time.perf_counter()
-        it = ctx2.compute_level_populations(True, 100, "max", args.precision)
-        tc = time.perf_counter() - t0
-        out["time_to_converged"] = {"start": "LTE", "seconds": round(tc, 3), "iterations": int(it),
-                                    "criterion": "max relative change < 1e-6, Ng every 5th"}
-        log(f"[bench] converged from LTE in {it} iterations, {tc:.2f} s")
-        ctx2.close()
+        out["time_to_converged"] = time_to_converged(make_ctx_traced(make_ctx), args.precision, args.weights)
 
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "envelope":
         log("[bench] CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(model, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
-    ctx.close()
     print(json.dumps(out))
     return 0
+
+
+def make_ctx_traced(make_ctx):
+    def f():
+        c = make_ctx()
+        c.trace()
+        return c
+    return f
 
 
 if __name__ == "__main__":

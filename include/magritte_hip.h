@@ -237,6 +237,15 @@ int mgr_get_ray(mgr_ctx *ctx, int64_t origin, int64_t ray, int64_t capacity,
  * npoints[(r)*ncells + o] = n_ar + n_r + 1 for r < nrays/2.                   */
 int mgr_get_npoints_on_rays(mgr_ctx *ctx, int64_t *npoints);
 
+/* Launch shape of the radiation-field kernel chosen by its autotuner (no
+ * counterpart in the reference): waves of 64 frequency bins per workgroup and
+ * traced steps staged per chunk, 0 before the first radiation field of that
+ * precision; with capacity > 0 also the shapes tried by the last tuning and
+ * their times on the tuning slice (ms).                                       */
+int mgr_radiation_field_shape(mgr_ctx *ctx, int precision, int *waves_per_group, int *steps_per_chunk,
+                              int capacity, int *n_tried, int *tried_waves, int *tried_steps,
+                              float *tried_ms);
+
 /* Kernel time of the last mgr_compute_radiation_field, measured with HIP
  * events on the context's stream (ms).                                        */
 int mgr_last_radiation_field_ms(mgr_ctx *ctx, double *ms);

@@ -183,6 +183,12 @@ struct mgr_ctx {
     int32_t *err_flag = nullptr;    // device error word
     bool     J_set = false;
     float    last_rf_ms = 0.0f;
+    // radiation-field launch shape per precision (FP32, FP64): waves per
+    // workgroup and staged steps per chunk; 0 until chosen
+    int      rf_nwg[2] = {0, 0}, rf_ch[2] = {0, 0};
+    bool     rf_autotune = true;
+    struct TuneEntry { int nwg, ch; float ms; };
+    std::vector<TuneEntry> tune_log;
 
     // device: trace cache
     int64_t *seg_off = nullptr;     // [N*R + 1]
@@ -223,6 +229,7 @@ int launch_emissivity_opacity(mgr_ctx *ctx);
 int launch_trace(mgr_ctx *ctx);
 int launch_radiation_field(mgr_ctx *ctx, int precision);   // kernel + Lambda*
 int launch_rf_kernel(mgr_ctx *ctx, int precision);
+int tune_radiation_field(mgr_ctx *ctx, int precision);   // outside graph capture
 int launch_lambda(mgr_ctx *ctx);
 int launch_jeff(mgr_ctx *ctx);
 int launch_stateq(mgr_ctx *ctx);

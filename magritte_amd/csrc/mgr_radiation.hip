@@ -47,7 +47,9 @@
 // offset x = ((nu_f - nu_l) + delta nu_f) / width formed from the per-lane
 // difference nu_f - nu_l and the point's shift delta = shift - 1, which keeps
 // the profile argument accurate in FP32.
+#include <algorithm>
 #include <cfloat>
+#include <vector>
 
 #include "mgr_common.hpp"
 #include "mgr_ray.hpp"
@@ -88,6 +90,7 @@ template <typename real>
 struct RFArgs {
     int64_t N;
     int     F, L, nchunk, nwg, ngroups, grid_real, ch;
+    int32_t o_begin;                          // first origin of this launch
     size_t  row_offset;                       // byte offset of the row region in LDS
     const LineTables *tb;
     const double  *nu;                        // [N*F]
@@ -446,10 +449,10 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
     const int bid = blockIdx.x;
     const int logical = (bid & 7) * per + (bid >> 3);
     if (logical >= a.grid_real) return;   // whole workgroup: before any barrier
-    const int32_t o = (int32_t)(logical / a.ngroups);
+    const int32_t o = a.o_begin + (int32_t)(logical / a.ngroups);
     const int32_t mdl = a.cell_model[o];
     if (!a.m_active[mdl]) return;         // converged model of a batch: whole workgroup
-    const int group = logical - o * a.ngroups;
+    const int group = logical - (o - a.o_begin) * a.ngroups;
     const int wave = threadIdx.x / WAVE;
     const int chunk = group * a.nwg + wave;
     const int F = a.F;
@@ -539,8 +542,21 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
     }
 }
 
+// Largest chunk of staged steps that fits the LDS budget with the cells'
+// line rows, at most CH_MAX.
 template <typename real>
-int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows)
+int max_chunk(const mgr_ctx *ctx)
+{
+    const size_t per_entry = sizeof(Step<real>) + sizeof(typename Prec<real>::row_t) * ctx->L + 64;
+    const int ch = (int)(LDS_LIMIT / per_entry) - 1;
+    return ch < CH_MAX ? ch : CH_MAX;
+}
+
+// Launch over origins [o_begin, o_begin + o_count) with nwg waves (64 bins
+// each) per workgroup and chunks of ch staged steps.
+template <typename real>
+int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int ch, int64_t o_begin,
+              int64_t o_count)
 {
     RFArgs<real> a;
     a.N = ctx->N;
@@ -552,8 +568,9 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows)
     a.F = (int)ctx->F;
     a.L = (int)ctx->L;
     a.nchunk = (int)((ctx->F + WAVE - 1) / WAVE);
-    a.nwg = a.nchunk < MAX_WAVES ? a.nchunk : MAX_WAVES;
+    a.nwg = nwg;
     a.ngroups = (a.nchunk + a.nwg - 1) / a.nwg;
+    a.o_begin = (int32_t)o_begin;
     a.tb = ctx->tables;
     a.nu = ctx->nu;
     a.rows = rows;
@@ -578,15 +595,13 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows)
     a.J = ctx->J;
     a.Lam = ctx->Lam;
     a.err_flag = ctx->err_flag;
-    const int64_t groups = ctx->N * a.ngroups;
+    if (nwg < 1 || nwg > MAX_WAVES || o_begin < 0 || o_count < 1 || o_begin + o_count > ctx->N)
+        return ctx->fail(MGR_ERR_ARGUMENT, "radiation-field launch shape out of range");
+    const int64_t groups = o_count * a.ngroups;
     if (groups > INT32_MAX - 8) return ctx->fail(MGR_ERR_UNSUPPORTED, "too many workgroups");
     a.grid_real = (int)groups;
     const int grid = (int)((groups + 7) / 8) * 8;
-    // chunk length: as many traced steps as fit the LDS budget, at most CH_MAX
-    const size_t per_entry = sizeof(Step<real>) + sizeof(typename Prec<real>::row_t) * ctx->L + 64;
-    int ch = (int)(LDS_LIMIT / per_entry) - 1;
-    ch = ch < CH_MAX ? ch : CH_MAX;
-    if (ch < 2) return ctx->fail(MGR_ERR_UNSUPPORTED, "line rows do not fit the LDS budget");
+    if (ch < 2 || ch > max_chunk<real>(ctx)) return ctx->fail(MGR_ERR_UNSUPPORTED, "line rows do not fit the LDS budget");
     a.ch = ch;
     a.row_offset = ((sizeof(Step<real>) * (ch + 1) + 63) / 64) * 64;
     const size_t lds = a.row_offset + sizeof(typename Prec<real>::row_t) * (ch + 1) * ctx->L;
@@ -600,9 +615,82 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows)
 
 namespace mgr {
 
+template <typename real>
+static int launch_rf_shape(mgr_ctx *ctx, int nwg, int ch, int64_t o_begin, int64_t o_count)
+{
+    if constexpr (sizeof(real) == 8)
+        return launch_rf<double>(ctx, ctx->row64, nwg, ch, o_begin, o_count);
+    else
+        return launch_rf<float>(ctx, ctx->row32, nwg, ch, o_begin, o_count);
+}
+
+// Launch-shape autotuner: waves (64-bin chunks) per workgroup and steps per
+// staged chunk, timed on a slice of up to 16384 origins in the middle of the
+// model; the fastest shape is kept for the context and precision.  Runs
+// outside graph capture, before the first radiation field of a precision.
+template <typename real>
+static int tune(mgr_ctx *ctx, int pi)
+{
+    const int nchunk = (int)((ctx->F + WAVE - 1) / WAVE);
+    const int chmax = max_chunk<real>(ctx);
+    if (chmax < 2) return ctx->fail(MGR_ERR_UNSUPPORTED, "line rows do not fit the LDS budget");
+    std::vector<int> nwgs;
+    for (int w = 1; w <= MAX_WAVES; w++)
+        if (w == 1 || w == nchunk || (nchunk % w) == 0 || w == std::min(nchunk, MAX_WAVES)) nwgs.push_back(w);
+    std::vector<int> chs;
+    for (int c : {16, 32, 64})
+        if (c <= chmax) chs.push_back(c);
+    if (chs.empty() || chs.back() != chmax) chs.push_back(chmax);
+    const int64_t cnt = std::min<int64_t>(ctx->N, 16384);
+    const int64_t o0 = (ctx->N - cnt) / 2;
+    int best_w = std::min(nchunk, MAX_WAVES), best_c = chmax;
+    float best = 1e30f;
+    hipEvent_t e0, e1;
+    MGR_HIP(hipEventCreate(&e0));
+    MGR_HIP(hipEventCreate(&e1));
+    int rc = launch_rf_shape<real>(ctx, best_w, best_c, o0, cnt);   // warm-up
+    ctx->tune_log.clear();
+    for (int w : nwgs) {
+        for (int c : chs) {
+            if (rc) break;
+            float ms = 0.0f;
+            if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = MGR_ERR_DEVICE; break; }
+            rc = launch_rf_shape<real>(ctx, w, c, o0, cnt);
+            if (rc) break;
+            if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = MGR_ERR_DEVICE;
+                break;
+            }
+            ctx->tune_log.push_back({w, c, ms});
+            if (ms < best) { best = ms; best_w = w; best_c = c; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc == MGR_ERR_DEVICE ? ctx->fail(MGR_ERR_DEVICE, "radiation-field tuning failed") : rc;
+    ctx->rf_nwg[pi] = best_w;
+    ctx->rf_ch[pi] = best_c;
+    return MGR_OK;
+}
+
+int tune_radiation_field(mgr_ctx *ctx, int precision)
+{
+    const int pi = precision == MGR_FP64 ? 1 : 0;
+    if (ctx->rf_nwg[pi] > 0) return MGR_OK;
+    if (ctx->rf_autotune) return pi ? tune<double>(ctx, 1) : tune<float>(ctx, 0);
+    const int nchunk = (int)((ctx->F + WAVE - 1) / WAVE);
+    ctx->rf_nwg[pi] = std::min(nchunk, MAX_WAVES);
+    ctx->rf_ch[pi] = pi ? max_chunk<double>(ctx) : max_chunk<float>(ctx);
+    return MGR_OK;
+}
+
 int launch_rf_kernel(mgr_ctx *ctx, int precision)
 {
-    return (precision == MGR_FP64) ? launch_rf<double>(ctx, ctx->row64) : launch_rf<float>(ctx, ctx->row32);
+    const int pi = precision == MGR_FP64 ? 1 : 0;
+    if (ctx->rf_nwg[pi] <= 0) return ctx->fail(MGR_ERR_STATE, "radiation-field launch shape not chosen");
+    return pi ? launch_rf_shape<double>(ctx, ctx->rf_nwg[1], ctx->rf_ch[1], 0, ctx->N)
+              : launch_rf_shape<float>(ctx, ctx->rf_nwg[0], ctx->rf_ch[0], 0, ctx->N);
 }
 
 int launch_radiation_field(mgr_ctx *ctx, int precision)
@@ -627,6 +715,7 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
         if (rc) return rc;
     }
     int rc0 = set_active(ctx, true);
+    if (!rc0) rc0 = tune_radiation_field(ctx, precision);
     if (rc0) return rc0;
     hipEvent_t e0, e1;
     MGR_HIP(hipEventCreate(&e0));
@@ -650,5 +739,24 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
     MGR_HIP(hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (errv) return ctx->fail(MGR_ERR_TRACE, "a ray pair ended off the boundary or an interior origin has an empty ray");
     ctx->J_set = true;
+    return MGR_OK;
+}
+
+extern "C" int mgr_radiation_field_shape(mgr_ctx *ctx, int precision, int *waves_per_group, int *steps_per_chunk,
+                                         int capacity, int *n_tried, int *tried_waves, int *tried_steps,
+                                         float *tried_ms)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (precision != MGR_FP32 && precision != MGR_FP64) return ctx->fail(MGR_ERR_ARGUMENT, "bad precision");
+    const int pi = precision == MGR_FP64 ? 1 : 0;
+    if (waves_per_group) *waves_per_group = ctx->rf_nwg[pi];
+    if (steps_per_chunk) *steps_per_chunk = ctx->rf_ch[pi];
+    const int n = (int)ctx->tune_log.size();
+    if (n_tried) *n_tried = n;
+    for (int i = 0; i < n && i < capacity; i++) {
+        if (tried_waves) tried_waves[i] = ctx->tune_log[i].nwg;
+        if (tried_steps) tried_steps[i] = ctx->tune_log[i].ch;
+        if (tried_ms) tried_ms[i] = ctx->tune_log[i].ms;
+    }
     return MGR_OK;
 }

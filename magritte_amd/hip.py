@@ -54,6 +54,7 @@ SIGNATURES = {
     "mgr_get_npoints_on_rays": ([_vp, _vp], _i),
     "mgr_last_radiation_field_ms": ([_vp, _vp], _i),
     "mgr_trace_stats": ([_vp, _vp, _vp], _i),
+    "mgr_radiation_field_shape": ([_vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp], _i),
     "mgr_mlp_num_params": ([], _i64),
     "mgr_mlp_create": ([_i, C.c_uint64, _vp], _i),
     "mgr_mlp_destroy": ([_vp], None),
@@ -262,6 +263,17 @@ class Context:
         v = C.c_double(0.0)
         self._check(self.lib.mgr_last_radiation_field_ms(self.h, C.byref(v)), "last_radiation_field_ms")
         return v.value
+
+    def radiation_field_shape(self, precision="fp32"):
+        """(waves per workgroup, steps per chunk) chosen by the autotuner, and
+        the shapes it tried with their times (ms)."""
+        w, c, n = C.c_int(0), C.c_int(0), C.c_int(0)
+        cap = 32
+        tw, tc, tm = np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(cap, np.float32)
+        self._check(self.lib.mgr_radiation_field_shape(self.h, _prec(precision), C.byref(w), C.byref(c), cap,
+                                                       C.byref(n), _p(tw), _p(tc), _p(tm)), "radiation_field_shape")
+        k = min(n.value, cap)
+        return (w.value, c.value), [(int(tw[i]), int(tc[i]), float(tm[i])) for i in range(k)]
 
     def trace_stats(self):
         a, b = C.c_int64(0), C.c_int64(0)
