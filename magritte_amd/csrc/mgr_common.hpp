@@ -196,6 +196,13 @@ struct mgr_ctx {
     int32_t *npts = nullptr;        // [N*R]
     int32_t *seg_cell = nullptr;    // [total segments]
     double  *seg_shift = nullptr, *seg_dZ = nullptr;
+    // the FP32 kernel's view of every step, split once at trace time:
+    // (cell, n | half << 16, shift offset s0, shift increment) and dZ/2 (scaled)
+    int4    *seg_f32 = nullptr;
+    float   *seg_hdz32 = nullptr;
+    // last emitted point of every (origin, ray): its cell (-1: none) and FP64 shift
+    int32_t *far_cell = nullptr;
+    double  *far_shift = nullptr;
     int64_t  total_segments = 0;
     uint64_t *shift_bits = nullptr;  // [2] min and max shift over all steps (FP64 bits)
     double   shift_min = 1.0, shift_max = 1.0;

@@ -98,6 +98,10 @@ struct RFArgs {
     const int64_t *seg_off;
     const int32_t *nseg, *npts, *seg_cell;
     const double  *seg_shift, *seg_dZ;
+    const int4    *seg_f32;                   // FP32 split of every step (cell, n | half << 16, s0, ds)
+    const float   *seg_hdz32;                 // its dZ/2 (scaled)
+    const int32_t *far_cell;                  // last emitted point of each (origin, ray), -1: none
+    const double  *far_shift;
     const int32_t *cell2bdy, *antipod;        // antipod: global ray indices
     const int32_t *cell_model;                // [N] model of each cell
     const int64_t *cell_qbase;                // [N] first (origin, ray) entry of the cell
@@ -268,6 +272,15 @@ __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, in
         s.pad2 = 0;
         if (j < 0) {
             s.cell = o; s.n = 0; s.half = 0; s.s0 = 0; s.ds = 0; s.dz = 0;
+        } else if constexpr (sizeof(real) == 4) {
+            // split at trace time with the same FP64 arithmetic as below
+            const int4 v = a.seg_f32[s0 + j];
+            s.cell = v.x;
+            s.n = v.y & 0xFFFF;
+            s.half = (int)((unsigned)v.y >> 16);
+            s.s0 = __int_as_float(v.z);
+            s.ds = __int_as_float(v.w);
+            s.dz = a.seg_hdz32[s0 + j];
         } else {
             const int32_t c = a.seg_cell[s0 + j];
             const double sn = a.seg_shift[s0 + j];
@@ -295,27 +308,6 @@ __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, in
         rw[idx] = a.rows[(int64_t)st[e].cell * L + l];
     }
     __syncthreads();
-}
-
-// Far end of a ray (last emitted point): its cell and exact FP64 shift.
-__device__ __forceinline__ bool far_end(const int64_t *__restrict__ seg_off,
-                                        const int32_t *__restrict__ nseg,
-                                        const int32_t *__restrict__ seg_cell,
-                                        const double *__restrict__ seg_shift,
-                                        const double *__restrict__ seg_dZ, int64_t q, double dsm,
-                                        int32_t &cell, double &shift)
-{
-    const int64_t s0 = seg_off[q];
-    for (int j = nseg[q] - 1; j >= 0; --j) {
-        const double sn = seg_shift[s0 + j];
-        const double sc = (j > 0) ? seg_shift[s0 + j - 1] : 1.0;
-        const StepSplit sp = split_step(sc, sn, seg_dZ[s0 + j], dsm);
-        if (sp.n == 0) continue;
-        cell = seg_cell[s0 + j];   // sub-point m = n >= half belongs to the next cell
-        shift = sp.interp ? sub_shift(sc, sp.n, sp.dshift) : sn;
-        return true;
-    }
-    return false;
 }
 
 template <typename real>
@@ -385,10 +377,10 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
     using row_t = typename Prec<real>::row_t;
     Sweep<real> sw;
     // incoming intensity at the far end
-    int32_t cf = o;
-    double shf = 1.0;
+    const int32_t cf = a.far_cell[q];
+    const double shf = a.far_shift[q];
     real Ib = 0;
-    if (far_end(a.seg_off, a.nseg, a.seg_cell, a.seg_shift, a.seg_dZ, q, dsm, cf, shf)) {
+    if (cf >= 0) {
         const int32_t b = a.cell2bdy[cf];
         if (b < 0) {
             err |= ERR_NOT_BOUNDARY;
@@ -580,6 +572,10 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     a.seg_cell = ctx->seg_cell;
     a.seg_shift = ctx->seg_shift;
     a.seg_dZ = ctx->seg_dZ;
+    a.seg_f32 = ctx->seg_f32;
+    a.seg_hdz32 = ctx->seg_hdz32;
+    a.far_cell = ctx->far_cell;
+    a.far_shift = ctx->far_shift;
     a.cell2bdy = ctx->cell2bdy;
     a.antipod = ctx->d_antipod;
     a.I_bdy = ctx->I_bdy;

@@ -44,6 +44,8 @@ __global__ void k_trace(int64_t N, int Rmax, const int32_t *__restrict__ cell_mo
                         int32_t *__restrict__ nseg, int32_t *__restrict__ npts,
                         const int64_t *__restrict__ seg_off, int32_t *__restrict__ seg_cell,
                         double *__restrict__ seg_shift, double *__restrict__ seg_dZ,
+                        int4 *__restrict__ seg_f32, float *__restrict__ seg_hdz32,
+                        int32_t *__restrict__ far_cell, double *__restrict__ far_shift,
                         uint64_t *__restrict__ shift_bits, int32_t *__restrict__ err_flag)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -61,6 +63,8 @@ __global__ void k_trace(int64_t N, int Rmax, const int32_t *__restrict__ cell_mo
     double smin = 1.0, smax = 1.0;
 
     double Z = 0.0, dZ = 0.0;
+    int32_t last_cell = -1;
+    double last_shift = 1.0;
     int32_t nxt = get_next(pos, nb_off, nb, o, nx, ny, nz, o, Z, dZ);
     if (nxt != -1) {
         double shift_crt = doppler_shift(vel, o, o, nx, ny, nz);
@@ -74,6 +78,16 @@ __global__ void k_trace(int64_t N, int Rmax, const int32_t *__restrict__ cell_mo
                 seg_cell[w] = nxt;
                 seg_shift[w] = shift_nxt;
                 seg_dZ[w] = dZ;
+                // the FP32 kernel's split: sub-point m at shift offset s0 + m ds
+                const int half = sp.interp ? sp.half : 0;
+                const float s0 = (float)((sp.interp ? shift_crt : shift_nxt) - 1.0);
+                const float ds = (float)sp.dshift;
+                seg_f32[w] = make_int4(nxt, sp.n | (half << 16), __float_as_int(s0), __float_as_int(ds));
+                seg_hdz32[w] = (float)(sp.dZ * (0.5 / CHI_SCALE));
+                if (sp.n > 0) {   // sub-point m = n >= half belongs to the next cell
+                    last_cell = nxt;
+                    last_shift = sp.interp ? sub_shift(shift_crt, sp.n, sp.dshift) : shift_nxt;
+                }
                 w++;
             }
             steps++;
@@ -89,6 +103,9 @@ __global__ void k_trace(int64_t N, int Rmax, const int32_t *__restrict__ cell_mo
         nseg[q] = steps;
         npts[q] = points;
         atomic_minmax_pos_double(shift_bits, smin, smax);
+    } else {
+        far_cell[q] = last_cell;
+        far_shift[q] = last_shift;
     }
 }
 
@@ -151,8 +168,8 @@ int launch_trace(mgr_ctx *ctx)
     hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->d_cell_model,
                        ctx->d_cell_qbase, ctx->d_m_R, ctx->d_m_rbase, ctx->pos,
                        ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->dshift_max,
-                       ctx->nseg, ctx->npts, nullptr, nullptr, nullptr, nullptr, ctx->shift_bits,
-                       ctx->err_flag);
+                       ctx->nseg, ctx->npts, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, ctx->shift_bits, ctx->err_flag);
     MGR_LAUNCH_CHECK();
     std::vector<int32_t> ns(NR), np(NR);
     int32_t errv = 0;
@@ -194,18 +211,26 @@ int launch_trace(mgr_ctx *ctx)
 
     if (ctx->seg_cell) {
         (void)hipFree(ctx->seg_cell); (void)hipFree(ctx->seg_shift); (void)hipFree(ctx->seg_dZ);
+        (void)hipFree(ctx->seg_f32); (void)hipFree(ctx->seg_hdz32);
         ctx->seg_cell = nullptr; ctx->seg_shift = nullptr; ctx->seg_dZ = nullptr;
+        ctx->seg_f32 = nullptr; ctx->seg_hdz32 = nullptr;
     }
     const int64_t T = std::max<int64_t>(ctx->total_segments, 1);
     MGR_HIP(hipMalloc((void **)&ctx->seg_cell, T * sizeof(int32_t)));
     MGR_HIP(hipMalloc((void **)&ctx->seg_shift, T * sizeof(double)));
     MGR_HIP(hipMalloc((void **)&ctx->seg_dZ, T * sizeof(double)));
+    MGR_HIP(hipMalloc((void **)&ctx->seg_f32, T * sizeof(int4)));
+    MGR_HIP(hipMalloc((void **)&ctx->seg_hdz32, T * sizeof(float)));
+    if (!ctx->far_cell) {
+        MGR_HIP(hipMalloc((void **)&ctx->far_cell, std::max<int64_t>(NR, 1) * sizeof(int32_t)));
+        MGR_HIP(hipMalloc((void **)&ctx->far_shift, std::max<int64_t>(NR, 1) * sizeof(double)));
+    }
     MGR_HIP(hipMemcpy(ctx->seg_off, off.data(), (NR + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->d_cell_model,
                        ctx->d_cell_qbase, ctx->d_m_R, ctx->d_m_rbase, ctx->pos,
                        ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->dshift_max,
                        nullptr, nullptr, ctx->seg_off, ctx->seg_cell, ctx->seg_shift, ctx->seg_dZ,
-                       nullptr, ctx->err_flag);
+                       ctx->seg_f32, ctx->seg_hdz32, ctx->far_cell, ctx->far_shift, nullptr, ctx->err_flag);
     MGR_LAUNCH_CHECK();
     MGR_HIP(hipStreamSynchronize(ctx->stream));
     ctx->traced = true;
