@@ -265,19 +265,6 @@ __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, in
 {
     const int64_t s0 = a.seg_off[q];
     const int nthr = blockDim.x;
-    if constexpr (sizeof(real) == 4) {
-        // FP32: the steps themselves are read by the sweep through the scalar
-        // cache (steps_f32); only the cells' line rows go to LDS
-        const int L = a.L;
-        for (int idx = threadIdx.x; idx < cnt * L; idx += nthr) {
-            const int e = idx / L, l = idx - e * L;
-            const int j = j0 + e;
-            const int32_t c = j < 0 ? o : a.seg_f32[s0 + j].x;
-            rw[idx] = a.rows[(int64_t)c * L + l];
-        }
-        __syncthreads();
-        return;
-    }
     for (int e = threadIdx.x; e < cnt; e += nthr) {
         const int j = j0 + e;
         Step<real> s;
@@ -378,43 +365,6 @@ __device__ __forceinline__ int rfl_int(int x) { return __builtin_amdgcn_readfirs
 __device__ __forceinline__ float rfl_real(float x) { return rfl(x); }
 __device__ __forceinline__ double rfl_real(double x) { return x; }
 
-// The FP32 split of step j of entry q, loaded through the constant address
-// space: the index is wave-uniform and the arrays are read-only during the
-// kernel, so the loads go to the scalar unit and the fields land in SGPRs.
-struct StepF32 {
-    int n, half;
-    float s0, ds, hdz;
-};
-typedef const __attribute__((address_space(4))) int *cint_ptr;
-typedef const __attribute__((address_space(4))) float *cfloat_ptr;
-typedef const __attribute__((address_space(4))) int64_t *cint64_ptr;
-
-__device__ __forceinline__ StepF32 steps_f32(const RFArgs<float> &a, int64_t q, int j)
-{
-    const int64_t base = ((cint64_ptr)a.seg_off)[q];
-    const cint_ptr v = (cint_ptr)(a.seg_f32 + (base + j));   // (cell, n | half << 16, s0, ds)
-    const int nh = v[1];
-    StepF32 r;
-    r.n = nh & 0xFFFF;
-    r.half = (int)((unsigned)nh >> 16);
-    r.s0 = __int_as_float(v[2]);
-    r.ds = __int_as_float(v[3]);
-    r.hdz = ((cfloat_ptr)a.seg_hdz32)[base + j];
-    return r;
-}
-__device__ __forceinline__ float to_vgpr(float x)
-{
-    float v;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
-    return v;
-}
-
-template <typename real>
-__device__ __forceinline__ StepF32 steps_f32(const RFArgs<real> &, int64_t, int)
-{
-    return StepF32{};
-}
-
 // One ray of a pair, eliminated from its boundary end inwards to the origin.
 template <typename real>
 __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const LaneCtx<real> &lc,
@@ -454,25 +404,11 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
         __syncthreads();   // the previous chunk is consumed by every wave
         stage_chunk<real>(a, o, q, a_ - 1, cnt, dsm, st, rw);
         for (int e = cnt - 1; e >= 1; --e) {
-            int n, half;
-            real s0, ds, hdz;
-            if constexpr (sizeof(real) == 4) {
-                // wave-uniform step fields through the scalar cache; s0 is
-                // copied to a VGPR once per step so that the per-point FMA
-                // reads one scalar operand (ds)
-                const StepF32 sf = steps_f32(a, q, a_ - 1 + e);
-                n = sf.n;
-                half = sf.half;
-                s0 = to_vgpr(sf.s0);
-                ds = sf.ds;
-                hdz = sf.hdz;
-            } else {
-                n = rfl_int(st[e].n);
-                half = rfl_int(st[e].half);
-                s0 = st[e].s0;   // stays in a VGPR: one scalar operand per FMA
-                ds = rfl_real(st[e].ds);
-                hdz = rfl_real(st[e].dz);
-            }
+            const int n = rfl_int(st[e].n);
+            const int half = rfl_int(st[e].half);
+            const real s0 = st[e].s0;   // stays in a VGPR: one scalar operand per FMA
+            const real ds = rfl_real(st[e].ds);
+            const real hdz = rfl_real(st[e].dz);
             // sub-points m >= half lie in the step's cell (row e), m < half in
             // the previous one (row e-1); two loops keep each row's address
             // loop-invariant
