@@ -56,24 +56,35 @@ def targets(model, pops):
     return np.log10(np.maximum(pops / ntot[:, None], 1e-30)).astype(np.float32)
 
 
+TRAIN_MAX_ITER = 100
+
+
 def build_dataset(n_models, seed=0, log=print):
+    """Features and converged log10 fractions of `n_models` training models.
+    A model whose LTE-started iteration does not meet the criterion within
+    TRAIN_MAX_ITER iterations, or ends with a non-positive population, gives
+    no usable target and is left out (counted in the returned `skipped`)."""
     rng = np.random.default_rng(seed)
     Xs, Ys, its = [], [], []
+    skipped = 0
     t0 = time.perf_counter()
     for i in range(n_models):
         m = training_model(rng)
         ctx = first_pass(m)
         X = ctx.features()
-        it = ctx.compute_level_populations(True, MAX_ITER, CRITERION, "fp32")
-        Y = targets(m, ctx.populations())
+        it = ctx.compute_level_populations(True, TRAIN_MAX_ITER, CRITERION, "fp32")
+        P = ctx.populations()
         ctx.close()
+        if it >= TRAIN_MAX_ITER or not np.all(P > 0.0) or not np.all(np.isfinite(X)):
+            skipped += 1
+            continue
         Xs.append(X)
-        Ys.append(Y)
+        Ys.append(targets(m, P))
         its.append(it)
         if (i + 1) % max(1, n_models // 20) == 0:
             log(f"[warmstart] {i + 1}/{n_models} models solved ({time.perf_counter() - t0:.1f} s, "
-                f"{sum(x.shape[0] for x in Xs)} cells)")
-    return np.concatenate(Xs), np.concatenate(Ys), np.array(its)
+                f"{sum(x.shape[0] for x in Xs)} cells, {skipped} not converged)")
+    return np.concatenate(Xs), np.concatenate(Ys), np.array(its), skipped
 
 
 def fit(X, Y, epochs=30, batch=4096, lr=2e-3, seed=0, log=print):
@@ -107,7 +118,7 @@ def load(path, device=0):
     return mlp
 
 
-def converge(model, mlp=None, precision="fp32"):
+def converge(model, mlp=None, precision="fp32", max_iter=MAX_ITER):
     """Iterations and seconds to the max-change criterion from LTE (mlp None)
     or from the network's populations.  The clock starts after the one-off
     set-up and trace and includes, for the warm start, the first radiation
@@ -119,20 +130,24 @@ def converge(model, mlp=None, precision="fp32"):
         ctx.compute_radiation_field(precision)
         ctx.compute_Jeff()
         ctx.warm_start(mlp)
-    it = ctx.compute_level_populations(True, MAX_ITER, CRITERION, precision)
+    it = ctx.compute_level_populations(True, max_iter, CRITERION, precision)
     dt = time.perf_counter() - t0
     pops = ctx.populations()
     ctx.close()
     return int(it), dt, pops
 
 
-def evaluate_model(model, mlp, precision="fp32"):
-    it_l, t_l, p_l = converge(model, None, precision)
-    it_n, t_n, p_n = converge(model, mlp, precision)
+def evaluate_model(model, mlp, precision="fp32", max_iter=MAX_ITER):
+    """LTE start against the warm start on one model.  `converged` is False
+    when the LTE-started iteration (the reference for the comparison) does not
+    meet the criterion within max_iter iterations."""
+    it_l, t_l, p_l = converge(model, None, precision, max_iter)
+    it_n, t_n, p_n = converge(model, mlp, precision, max_iter)
     ntot = model.abundance[:, model.linedata.num]
     mask = p_l > 1e-10 * ntot[:, None]
-    rel = np.abs(p_n - p_l)[mask] / p_l[mask]
-    return {"lte_iterations": it_l, "lte_seconds": round(t_l, 4), "nn_iterations": it_n,
+    rel = np.abs(p_n - p_l)[mask] / np.abs(p_l[mask])
+    return {"converged": bool(it_l < max_iter and it_n < max_iter),
+            "lte_iterations": it_l, "lte_seconds": round(t_l, 4), "nn_iterations": it_n,
             "nn_seconds": round(t_n, 4), "speedup": round(t_l / t_n, 3),
             "pops_max_rel_diff": float(rel.max())}
 
@@ -153,15 +168,16 @@ def main(argv=None):
     ev.add_argument("--n", type=int, default=64)
     ev.add_argument("--seed", type=int, default=12345)
     ev.add_argument("--out", default=None)
+    ev.add_argument("--max-iter", type=int, default=100)
     a = ap.parse_args(argv)
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
     if a.cmd == "train":
-        X, Y, its = build_dataset(a.models, a.seed, log)
+        X, Y, its, skipped = build_dataset(a.models, a.seed, log)
         mlp, norm, losses = fit(X, Y, a.epochs, a.batch, a.lr, a.seed, log)
-        meta = {"models": a.models, "cells": int(X.shape[0]), "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
+        meta = {"models": a.models, "models_not_converged": int(skipped), "cells": int(X.shape[0]), "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
                 "seed": a.seed, "mean_iterations_lte": float(its.mean()), "final_loss": float(losses[-1]),
                 "features": ["log10 n_partner", "log10 T", "log10 n_species", "log10 max |dv/dx| (1/s)",
                              "log10 Jlin_0/B_0", "log10 Jlin_1/B_1", "log10 Jlin_2/B_2", "log10 Jlin_3/B_3"],
@@ -177,14 +193,16 @@ def main(argv=None):
         T0 = rng.uniform(500.0, 3000.0)
         seed = int(rng.integers(0, 2 ** 31))
         m = synthetic.envelope_model(n=a.n, n0=n0, T0=T0, seed=seed)
-        r = {"n0": n0, "T0": T0, "seed": seed, **evaluate_model(m, mlp)}
+        r = {"n0": n0, "T0": T0, "seed": seed, **evaluate_model(m, mlp, max_iter=a.max_iter)}
         log(f"[warmstart] held-out {i}: {json.dumps(r)}")
         rows.append(r)
-    summary = {"models": a.models, "ncells": a.n ** 3,
-               "mean_speedup": float(np.mean([r["speedup"] for r in rows])),
-               "mean_lte_iterations": float(np.mean([r["lte_iterations"] for r in rows])),
-               "mean_nn_iterations": float(np.mean([r["nn_iterations"] for r in rows])),
-               "max_pops_rel_diff": float(np.max([r["pops_max_rel_diff"] for r in rows])), "rows": rows}
+    ok = [r for r in rows if r["converged"]] or [{"speedup": float("nan"), "lte_iterations": float("nan"),
+                                                  "nn_iterations": float("nan"), "pops_max_rel_diff": float("nan")}]
+    summary = {"models": a.models, "ncells": a.n ** 3, "converged_models": sum(r["converged"] for r in rows),
+               "mean_speedup": float(np.mean([r["speedup"] for r in ok])),
+               "mean_lte_iterations": float(np.mean([r["lte_iterations"] for r in ok])),
+               "mean_nn_iterations": float(np.mean([r["nn_iterations"] for r in ok])),
+               "max_pops_rel_diff": float(np.max([r["pops_max_rel_diff"] for r in ok])), "rows": rows}
     if a.out:
         with open(a.out, "w") as fh:
             json.dump(summary, fh, indent=1)

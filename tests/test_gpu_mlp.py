@@ -127,7 +127,8 @@ def test_warm_start_training_on_solved_models():
     envelope from the network converges to the LTE-started populations."""
     from magritte_amd import warmstart
     from magritte_amd.setup import synthetic
-    X, Y, its = warmstart.build_dataset(12, seed=21, log=lambda s: print(s, flush=True))
+    X, Y, its, skipped = warmstart.build_dataset(12, seed=21, log=lambda s: print(s, flush=True))
+    assert X.shape[0] > 0
     assert np.all(np.isfinite(X)) and np.all(np.isfinite(Y))
     net, norm, losses = warmstart.fit(X, Y, epochs=40, batch=2048, lr=3e-3, seed=0,
                                       log=lambda s: print(s, flush=True))
