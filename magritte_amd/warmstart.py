@@ -165,7 +165,7 @@ def main(argv=None):
     ev = sub.add_parser("evaluate")
     ev.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ev.add_argument("--models", type=int, default=8)
-    ev.add_argument("--n", type=int, default=64)
+    ev.add_argument("--n", type=int, default=64, help="cells per side; 0: models of the training distribution")
     ev.add_argument("--seed", type=int, default=12345)
     ev.add_argument("--out", default=None)
     ev.add_argument("--max-iter", type=int, default=100)
@@ -189,19 +189,27 @@ def main(argv=None):
     rng = np.random.default_rng(a.seed)
     rows = []
     for i in range(a.models):
-        n0 = 10.0 ** rng.uniform(12.0, 16.0)
-        T0 = rng.uniform(500.0, 3000.0)
-        seed = int(rng.integers(0, 2 ** 31))
-        m = synthetic.envelope_model(n=a.n, n0=n0, T0=T0, seed=seed)
-        r = {"n0": n0, "T0": T0, "seed": seed, **evaluate_model(m, mlp, max_iter=a.max_iter)}
+        if a.n > 0:
+            n0 = 10.0 ** rng.uniform(12.0, 16.0)
+            T0 = rng.uniform(500.0, 3000.0)
+            seed = int(rng.integers(0, 2 ** 31))
+            m = synthetic.envelope_model(n=a.n, n0=n0, T0=T0, seed=seed)
+            r = {"n0": n0, "T0": T0, "seed": seed}
+        else:   # held-out draws from the training distribution
+            m = training_model(rng)
+            r = {"n0": m.meta["n0"], "T0": m.meta["T0"], "ncells": m.ncells, "nrays": m.nrays}
+        r.update(evaluate_model(m, mlp, max_iter=a.max_iter))
         log(f"[warmstart] held-out {i}: {json.dumps(r)}")
         rows.append(r)
     ok = [r for r in rows if r["converged"]] or [{"speedup": float("nan"), "lte_iterations": float("nan"),
                                                   "nn_iterations": float("nan"), "pops_max_rel_diff": float("nan")}]
-    summary = {"models": a.models, "ncells": a.n ** 3, "converged_models": sum(r["converged"] for r in rows),
+    summary = {"models": a.models, "ncells": a.n ** 3 if a.n > 0 else "training distribution",
+               "converged_models": sum(r["converged"] for r in rows),
                "mean_speedup": float(np.mean([r["speedup"] for r in ok])),
                "mean_lte_iterations": float(np.mean([r["lte_iterations"] for r in ok])),
                "mean_nn_iterations": float(np.mean([r["nn_iterations"] for r in ok])),
+               "models_fewer_iterations": sum(r["nn_iterations"] < r["lte_iterations"] for r in rows if r["converged"]),
+               "models_more_iterations": sum(r["nn_iterations"] > r["lte_iterations"] for r in rows if r["converged"]),
                "max_pops_rel_diff": float(np.max([r["pops_max_rel_diff"] for r in ok])), "rows": rows}
     if a.out:
         with open(a.out, "w") as fh:
