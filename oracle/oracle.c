@@ -257,7 +257,18 @@ void orc_collision_rates(const orc_state *s, int64_t c, double temperature,
 /* ---- Model set-up ------------------------------------------------------- */
 
 /* Simulation::compute_spectral_discretisation (src/Simulation/sim_spectrum.tpp:13-196) */
-int orc_compute_spectral_discretisation(orc_state *s)
+static int o_spectral_discretisation(orc_state *s, int image, double image_width);
+
+int orc_compute_spectral_discretisation(orc_state *s) { return o_spectral_discretisation(s, 0, 0.0); }
+
+/* Simulation::compute_spectral_discretisation_image (sim_spectrum.tpp:206-311):
+ * the same construction with one fixed relative width for every cell. */
+int orc_compute_spectral_discretisation_image(orc_state *s, double width)
+{
+    return o_spectral_discretisation(s, 1, width);
+}
+
+static int o_spectral_discretisation(orc_state *s, int image, double image_width)
 {
     const int64_t F = s->F, L = s->L, Q = s->Q;
     #pragma omp parallel
@@ -273,7 +284,10 @@ int orc_compute_spectral_discretisation(orc_state *s)
                 const double freqs_line = s->line[k];
                 const double width = freqs_line * pw;
                 for (int64_t z = 0; z < Q; z++) {
-                    fr[index1].v = freqs_line + width * s->m.roots[z];
+                    if (image)   /* freqs_line + freqs_line * width * root (sim_spectrum.tpp:227) */
+                        fr[index1].v = freqs_line + freqs_line * image_width * s->m.roots[z];
+                    else
+                        fr[index1].v = freqs_line + width * s->m.roots[z];
                     fr[index1].i = index1;
                     index1++;
                 }
@@ -450,17 +464,33 @@ static int64_t o_set_data(int64_t crt, int64_t nxt, double shift_crt, double shi
     return cnt;
 }
 
+static int64_t o_trace_ray(const orc_state *s, int64_t o, int64_t r, double dshift_max, int64_t cap,
+                           int64_t *crt_out, int64_t *cellNr, double *shift, double *dZs, int rest);
+
 /* Geometry::trace_ray<CoMoving> (geometry.tpp:21-95) */
 int64_t orc_trace_ray(const orc_state *s, int64_t o, int64_t r, double dshift_max,
                       int64_t cap, int64_t *crt_out, int64_t *cellNr, double *shift, double *dZs)
+{
+    return o_trace_ray(s, o, r, dshift_max, cap, crt_out, cellNr, shift, dZs, 0);
+}
+
+/* Geometry::trace_ray<Rest> (geometry.tpp:21-95 with the rest-frame shift) */
+int64_t orc_trace_ray_rest(const orc_state *s, int64_t o, int64_t r, double dshift_max,
+                           int64_t cap, int64_t *crt_out, int64_t *cellNr, double *shift, double *dZs)
+{
+    return o_trace_ray(s, o, r, dshift_max, cap, crt_out, cellNr, shift, dZs, 1);
+}
+
+static int64_t o_trace_ray(const orc_state *s, int64_t o, int64_t r, double dshift_max, int64_t cap,
+                           int64_t *crt_out, int64_t *cellNr, double *shift, double *dZs, int rest)
 {
     double Z = 0.0, dZ = 0.0;
     int64_t cnt = 0;
     int64_t nxt = o_get_next(s, o, r, o, &Z, &dZ);
     if (nxt != -1) {
         int64_t crt = o;
-        double shift_crt = orc_get_doppler_shift(s, o, r, crt, 0);
-        double shift_nxt = orc_get_doppler_shift(s, o, r, nxt, 0);
+        double shift_crt = orc_get_doppler_shift(s, o, r, crt, rest);
+        double shift_nxt = orc_get_doppler_shift(s, o, r, nxt, rest);
         int64_t step = 0;
         cnt = o_set_data(crt, nxt, shift_crt, shift_nxt, dZ, dshift_max, cnt, cap,
                          crt_out, cellNr, shift, dZs, step);
@@ -469,7 +499,7 @@ int64_t orc_trace_ray(const orc_state *s, int64_t o, int64_t r, double dshift_ma
             crt = nxt;
             nxt = o_get_next(s, o, r, nxt, &Z, &dZ);
             if (nxt < 0) return -1;
-            shift_nxt = orc_get_doppler_shift(s, o, r, nxt, 0);
+            shift_nxt = orc_get_doppler_shift(s, o, r, nxt, rest);
             step++;
             cnt = o_set_data(crt, nxt, shift_crt, shift_nxt, dZ, dshift_max, cnt, cap,
                              crt_out, cellNr, shift, dZs, step);
@@ -673,18 +703,16 @@ static void o_buf_free(o_pair_buf *b)
     free(b->frs); free(b->dtau); free(b->Su); free(b->L_diag); free(b->nrs);
 }
 
-/* Simulation::setup (src/Simulation/simulation.tpp:185-353) followed by
- * RayPair::solve (n_off_diag = 0).  Side "ar" is the longer ray.  Returns u. */
-static double o_setup_and_solve(const orc_state *s, int64_t R, int64_t origin, int64_t f,
-                                int64_t n_ar, const int64_t *c_ar, const double *s_ar,
-                                const double *z_ar, int64_t *ln_ar,
-                                int64_t n_r, const int64_t *c_r, const double *s_r,
-                                const double *z_r, int64_t *ln_r,
-                                int64_t *lnotch_at_origin, o_pair_buf *b, double *L_origin,
-                                double *chi_origin)
+/* Simulation::setup (src/Simulation/simulation.tpp:185-353): term1, chi and
+ * dtau of the pair in b, the solved range [first, last] and the boundary
+ * intensities at both ends. */
+static void o_setup(const orc_state *s, int64_t R, int64_t origin, int64_t f,
+                    int64_t n_ar, const int64_t *c_ar, const double *s_ar, const double *z_ar,
+                    int64_t *ln_ar, int64_t n_r, const int64_t *c_r, const double *s_r,
+                    const double *z_r, int64_t *ln_r, int64_t *lnotch_at_origin, o_pair_buf *b,
+                    int64_t *first_out, int64_t *last_out, double *I0_out, double *In_out)
 {
     const int64_t F = s->F;
-    const int64_t ndep = n_ar + n_r + 1;
     int64_t first = 0, last = n_ar;
     double eta, chi, I_bdy_0, I_bdy_n;
     double freq_scaled = s->nu[origin*F + f];
@@ -733,11 +761,136 @@ static double o_setup_and_solve(const orc_state *s, int64_t R, int64_t origin, i
         I_bdy_n = s->I_bdy[(R*s->NB + s->cell2boundary_nr[origin])*F + f];
     }
 
+    *first_out = first;
+    *last_out = last;
+    *I0_out = I_bdy_0;
+    *In_out = I_bdy_n;
+}
+
+/* Simulation::setup followed by RayPair::solve (n_off_diag = 0).  Side "ar"
+ * is the longer ray.  Returns u at the origin. */
+static double o_setup_and_solve(const orc_state *s, int64_t R, int64_t origin, int64_t f,
+                                int64_t n_ar, const int64_t *c_ar, const double *s_ar,
+                                const double *z_ar, int64_t *ln_ar,
+                                int64_t n_r, const int64_t *c_r, const double *s_r,
+                                const double *z_r, int64_t *ln_r,
+                                int64_t *lnotch_at_origin, o_pair_buf *b, double *L_origin,
+                                double *chi_origin)
+{
+    const int64_t ndep = n_ar + n_r + 1;
+    int64_t first, last;
+    double I_bdy_0, I_bdy_n;
+    o_setup(s, R, origin, f, n_ar, c_ar, s_ar, z_ar, ln_ar, n_r, c_r, s_r, z_r, ln_r, lnotch_at_origin, b,
+            &first, &last, &I_bdy_0, &I_bdy_n);
     orc_raypair_solve(ndep, n_ar, first, last, 0, b->term1, b->dtau, I_bdy_0, I_bdy_n,
                       b->Su, b->L_diag, NULL, NULL, NULL, NULL);
     *L_origin = b->L_diag[n_ar];
     *chi_origin = b->chi[n_ar];
     return b->Su[n_ar];
+}
+
+/* RayPair::solve_for_image (src/Simulation/Raypair/raypair_solver.tpp:523-579):
+ * the same elimination with back-substitution over the whole pair, so that
+ * Su holds u at every point from 0 to last. */
+void orc_solve_for_image(int64_t ndep, int64_t first, int64_t last, const double *term1,
+                         const double *dtau, double I_bdy_0, double I_bdy_n, double *Su)
+{
+    double *A = (double *)xcalloc(ndep, sizeof(double));
+    double *C = (double *)xcalloc(ndep, sizeof(double));
+    double *F = (double *)xcalloc(ndep, sizeof(double));
+    double *iA = (double *)xcalloc(ndep, sizeof(double));
+    double *iC = (double *)xcalloc(ndep, sizeof(double));
+    double *i1F = (double *)xcalloc(ndep, sizeof(double));
+    const double inverse_dtau0 = 1.0 / dtau[first];
+    const double inverse_dtaud = 1.0 / dtau[last-1];
+    C[first] = 2.0 * inverse_dtau0 * inverse_dtau0;
+    const double B0_min_C0 = 1.0 + 2.0 * inverse_dtau0;
+    const double B0 = B0_min_C0 + C[first];
+    const double inverse_B0 = 1.0 / B0;
+    Su[first] = term1[first] + 2.0 * I_bdy_0 * inverse_dtau0;
+    for (int64_t n = first+1; n < last; n++) {
+        iA[n] = 0.5 * (dtau[n-1] + dtau[n]) * dtau[n-1];
+        iC[n] = 0.5 * (dtau[n-1] + dtau[n]) * dtau[n];
+        A[n] = 1.0 / iA[n];
+        C[n] = 1.0 / iC[n];
+        Su[n] = term1[n];
+    }
+    A[last] = 2.0 * inverse_dtaud * inverse_dtaud;
+    const double Bd_min_Ad = 1.0 + 2.0 * inverse_dtaud;
+    const double Bd = Bd_min_Ad + A[last];
+    Su[last] = term1[last] + 2.0 * I_bdy_n * inverse_dtaud;
+    Su[first] = Su[first] * inverse_B0;
+    F[first] = 0.5 * B0_min_C0 * dtau[first] * dtau[first];
+    i1F[first] = 1.0 / (1.0 + F[first]);
+    for (int64_t n = first+1; n < last; n++) {
+        F[n] = (1.0 + A[n]*F[n-1]*i1F[n-1]) * iC[n];
+        i1F[n] = 1.0 / (1.0 + F[n]);
+        Su[n] = (Su[n] + A[n]*Su[n-1]) * i1F[n] * iC[n];
+    }
+    const double denominator = 1.0 / (Bd_min_Ad + Bd*F[last-1]);
+    Su[last] = (Su[last] + A[last]*Su[last-1]) * (1.0 + F[last-1]) * denominator;
+    for (int64_t n = last-1; n >= 0; n--) Su[n] = Su[n] + Su[n+1] * i1F[n];
+    free(A); free(C); free(F); free(iA); free(iC); free(i1F);
+}
+
+/* Simulation::compute_and_write_image (src/Simulation/sim_radiation.tpp:345-478)
+ * for ray r < nrays/2: rest-frame traces of r and its antipode (no longer-ray
+ * swap), u over the whole pair, and the intensities leaving it,
+ * I_m[o][f] = 2 u_0 - I_bdy_0 (RayPair::get_Im_at_front) and
+ * I_p[o][f] = 2 u_last - I_bdy_n (RayPair::get_Ip_at_end); a pair of one point
+ * takes I_bdy of the origin's boundary entry.  Returns -1 on a trace error. */
+int orc_compute_image(orc_state *s, int64_t r, double *I_m, double *I_p)
+{
+    const int64_t F = s->F;
+    if (r < 0 || r >= s->R / 2) return -1;
+    const int64_t R = r;
+    int err = 0;
+    #pragma omp parallel reduction(|:err)
+    {
+        o_pair_buf b;
+        int64_t cap = 256;
+        o_buf_alloc(&b, cap);
+        #pragma omp for schedule(static)
+        for (int64_t o = 0; o < s->N; o++) {
+            const int64_t ar = s->antipod[r];
+            const double dshift_max = orc_get_dshift_max(s, o);
+            int64_t n_r, n_ar;
+            for (;;) {
+                n_r = orc_trace_ray_rest(s, o, r, dshift_max, b.cap, NULL, b.cell_r, b.shift_r, b.dZ_r);
+                n_ar = orc_trace_ray_rest(s, o, ar, dshift_max, b.cap, NULL, b.cell_ar, b.shift_ar, b.dZ_ar);
+                if (n_r > b.cap || n_ar > b.cap) {
+                    cap = 2 * (n_r > n_ar ? n_r : n_ar);
+                    o_buf_free(&b); o_buf_alloc(&b, cap);
+                    continue;
+                }
+                break;
+            }
+            if (n_r < 0 || n_ar < 0) { err = 1; continue; }
+            for (int64_t i = 0; i < n_r; i++) b.lnotch_r[i] = 0;
+            for (int64_t i = 0; i < n_ar; i++) b.lnotch_ar[i] = 0;
+            const int64_t ndep = n_ar + n_r + 1;
+            if (ndep > 1) {
+                int64_t lnotch_at_origin = 0;
+                for (int64_t f = 0; f < F; f++) {
+                    int64_t first, last;
+                    double I0, In;
+                    o_setup(s, R, o, f, n_ar, b.cell_ar, b.shift_ar, b.dZ_ar, b.lnotch_ar, n_r, b.cell_r,
+                            b.shift_r, b.dZ_r, b.lnotch_r, &lnotch_at_origin, &b, &first, &last, &I0, &In);
+                    orc_solve_for_image(ndep, first, last, b.term1, b.dtau, I0, In, b.Su);
+                    I_m[o*F + f] = 2.0 * b.Su[0] - I0;
+                    I_p[o*F + f] = 2.0 * b.Su[ndep-1] - In;
+                }
+            } else {
+                const int64_t bb = s->cell2boundary_nr[o];
+                for (int64_t f = 0; f < F; f++) {
+                    I_m[o*F + f] = s->I_bdy[(R*s->NB + bb)*F + f];
+                    I_p[o*F + f] = s->I_bdy[(R*s->NB + bb)*F + f];
+                }
+            }
+        }
+        o_buf_free(&b);
+    }
+    return err ? -1 : 0;
 }
 
 /* Simulation::compute_radiation_field (src/Simulation/sim_radiation.tpp:56-338),

@@ -42,6 +42,16 @@ double orc_get_dshift_max(const orc_state *s, int64_t o);
  * returns the number of points, or -1 if a ray found no neighbour (geometry.tpp:68-75). */
 int64_t orc_trace_ray(const orc_state *s, int64_t o, int64_t r, double dshift_max,
                       int64_t cap, int64_t *crt, int64_t *cellNr, double *shift, double *dZ);
+/* Geometry::trace_ray<Rest> (geometry.tpp:21-95, rest-frame shifts) */
+int64_t orc_trace_ray_rest(const orc_state *s, int64_t o, int64_t r, double dshift_max,
+                           int64_t cap, int64_t *crt, int64_t *cellNr, double *shift, double *dZ);
+/* src/Simulation/sim_spectrum.tpp:206-311 */
+int  orc_compute_spectral_discretisation_image(orc_state *s, double width);
+/* RayPair::solve_for_image (raypair_solver.tpp:523-579): u over [0, last] */
+void orc_solve_for_image(int64_t ndep, int64_t first, int64_t last, const double *term1,
+                         const double *dtau, double I_bdy_0, double I_bdy_n, double *Su);
+/* src/Simulation/sim_radiation.tpp:345-478 for ray r < nrays/2: I_m, I_p [ncells*nfreqs] */
+int  orc_compute_image(orc_state *s, int64_t r, double *I_m, double *I_p);
 /* src/Model/Geometry/geometry.tpp:498-543 */
 int64_t orc_get_npoints_on_ray(const orc_state *s, int64_t o, int64_t r, double dshift_max);
 /* src/Model/Geometry/geometry.tpp:433-465 */

@@ -56,6 +56,13 @@ def lib():
         L.orc_get_dshift_max.restype = C.c_double
         L.orc_trace_ray.argtypes = [vp, C.c_int64, C.c_int64, C.c_double, C.c_int64, vp, vp, vp, vp]
         L.orc_trace_ray.restype = C.c_int64
+        L.orc_trace_ray_rest.argtypes = [vp, C.c_int64, C.c_int64, C.c_double, C.c_int64, vp, vp, vp, vp]
+        L.orc_trace_ray_rest.restype = C.c_int64
+        L.orc_compute_spectral_discretisation_image.argtypes = [vp, C.c_double]
+        L.orc_compute_spectral_discretisation_image.restype = C.c_int
+        L.orc_solve_for_image.argtypes = [C.c_int64, C.c_int64, C.c_int64, vp, vp, C.c_double, C.c_double, vp]
+        L.orc_compute_image.argtypes = [vp, C.c_int64, vp, vp]
+        L.orc_compute_image.restype = C.c_int
         L.orc_get_npoints_on_ray.argtypes = [vp, C.c_int64, C.c_int64, C.c_double]
         L.orc_get_npoints_on_ray.restype = C.c_int64
         L.orc_get_doppler_shift.argtypes = [vp, C.c_int64, C.c_int64, C.c_int64, C.c_int]
@@ -135,14 +142,25 @@ class Oracle:
     def dshift_max(self, o):
         return lib().orc_get_dshift_max(self._s, o)
 
-    def trace_ray(self, o, r, dshift_max=None):
+    def compute_spectral_discretisation_image(self, width):
+        return lib().orc_compute_spectral_discretisation_image(self._s, width)
+
+    def compute_image(self, r):
+        """(I_m, I_p) [N, F] of the image along ray r < nrays/2."""
+        Im = np.zeros(self.N * self.F); Ip = np.zeros(self.N * self.F)
+        if lib().orc_compute_image(self._s, r, _p(Im), _p(Ip)) != 0:
+            raise RuntimeError("oracle image failed (trace error or ray out of range)")
+        return Im.reshape(self.N, self.F), Ip.reshape(self.N, self.F)
+
+    def trace_ray(self, o, r, dshift_max=None, rest=False):
         if dshift_max is None:
             dshift_max = self.dshift_max(o)
+        fn = lib().orc_trace_ray_rest if rest else lib().orc_trace_ray
         cap = 64
         while True:
             crt = np.zeros(cap, np.int64); cell = np.zeros(cap, np.int64)
             sh = np.zeros(cap); dz = np.zeros(cap)
-            n = lib().orc_trace_ray(self._s, o, r, dshift_max, cap, _p(crt), _p(cell), _p(sh), _p(dz))
+            n = fn(self._s, o, r, dshift_max, cap, _p(crt), _p(cell), _p(sh), _p(dz))
             if n < 0:
                 raise RuntimeError("ray found no forward neighbour")
             if n <= cap:
@@ -287,6 +305,18 @@ def raypair_solve(term1, dtau, I0, In, n_ar, first=None, last=None, n_off_diag=0
                             _p(Su), _p(Ld), _p(Lu), _p(Ll), _p(A), _p(Cc))
     return dict(Su=Su, L_diag=Ld, L_upper=Lu.reshape(-1, ndep), L_lower=Ll.reshape(-1, ndep),
                 A=A, C=Cc)
+
+
+def solve_for_image(term1, dtau, I0, In, first=None, last=None):
+    """RayPair::solve_for_image on given inputs (raypair_solver.tpp:523-579): u at every point."""
+    term1 = np.ascontiguousarray(term1, dtype=np.float64)
+    dtau = np.ascontiguousarray(dtau, dtype=np.float64)
+    ndep = term1.shape[0]
+    first = 0 if first is None else first
+    last = ndep - 1 if last is None else last
+    Su = np.zeros(ndep)
+    lib().orc_solve_for_image(ndep, first, last, _p(term1), _p(dtau), I0, In, _p(Su))
+    return Su
 
 
 def planck(T, nu):
