@@ -158,6 +158,12 @@ int  mgr_device_count(int *count);
 /* Simulation::compute_spectral_discretisation (src/Simulation/sim_spectrum.tpp:13-196) */
 int mgr_compute_spectral_discretisation(mgr_ctx *ctx);
 
+/* Simulation::compute_spectral_discretisation_image (src/Simulation/sim_spectrum.tpp:206-311):
+ * the same sorted grid with nu = line + line*width*root in every cell (width
+ * a fraction of the line frequency).  Level populations then refuse to run
+ * until compute_spectral_discretisation is called again (sim_lines.tpp:67-71). */
+int mgr_compute_spectral_discretisation_image(mgr_ctx *ctx, double width);
+
 /* Simulation::compute_boundary_intensities (src/Simulation/sim_radiation.tpp:17-48).
  * temperatures: [nboundary] K, or NULL for the CMB (T_CMB = 2.72548 K).      */
 int mgr_compute_boundary_intensities(mgr_ctx *ctx, const double *temperatures);
@@ -236,6 +242,22 @@ int mgr_get_ray(mgr_ctx *ctx, int64_t origin, int64_t ray, int64_t capacity,
 /* Simulation::get_npoints_on_rays<CoMoving> (simulation.tpp:540-552):
  * npoints[(r)*ncells + o] = n_ar + n_r + 1 for r < nrays/2.                   */
 int mgr_get_npoints_on_rays(mgr_ctx *ctx, int64_t *npoints);
+
+/* Simulation::get_npoints_on_rays<Rest> (simulation.tpp:540-552) with
+ * Geometry::get_doppler_shift<Rest> (geometry.tpp:448-463): the same counts in
+ * the rest frame, for a single model.                                         */
+int mgr_get_npoints_on_rays_rest(mgr_ctx *ctx, int64_t *npoints);
+
+/* Simulation::compute_and_write_image (src/Simulation/sim_radiation.tpp:345-478)
+ * with RayPair::solve_for_image (raypair_solver.tpp:523-579), for
+ * 0 <= ray < nrays/2 of a single model, after
+ * mgr_compute_spectral_discretisation_image and mgr_compute_boundary_intensities:
+ * the intensities leaving each origin's rest-frame ray pair,
+ * I_m[o*nfreqs + f] at the antipode's end (RayPair::get_Im_at_front) and
+ * I_p[o*nfreqs + f] at the ray's end (RayPair::get_Ip_at_end).  Either
+ * pointer may be NULL.  The image-plane coordinates (Image::set_coordinates,
+ * image.cpp:93-132) are host arithmetic and stay with the caller.            */
+int mgr_compute_image(mgr_ctx *ctx, int64_t ray, int precision, double *I_m, double *I_p);
 
 /* Launch shape of the radiation-field kernel chosen by its autotuner (no
  * counterpart in the reference): waves of 64 frequency bins per workgroup and

@@ -76,7 +76,7 @@ def build(verbose=False):
 
 
 HOST = os.path.join(CSRC, "host")
-HOST_UNITS = ["io_text.cpp", "model.cpp", "simulation.cpp", "core.cpp"]
+HOST_UNITS = ["io_text.cpp", "model.cpp", "image.cpp", "simulation.cpp", "core.cpp"]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-fvisibility=hidden"]
 
 

@@ -252,6 +252,16 @@ PYBIND11_MODULE(core, module)
         .def(py::init())
         .def("write", &Frequencies::write);
 
+    // Image (pyBindings.cpp:497-507)
+    py::class_<Image>(module, "Image")
+        .def_readonly("ImX", &Image::ImX)
+        .def_readonly("ImY", &Image::ImY)
+        .def_readonly("I_p", &Image::I_p)
+        .def_readonly("I_m", &Image::I_m)
+        .def(py::init<const long, const Parameters &>())
+        .def("write", &Image::write)
+        .def("set_coordinates", &Image::set_coordinates);
+
     py::class_<Simulation, Model>(module, "Simulation")
         .def(py::init<>())
         .def_readonly("error_max", &Simulation::error_max)
@@ -270,6 +280,9 @@ PYBIND11_MODULE(core, module)
         .def("compute_Jeff", &Simulation::compute_Jeff)
         .def("compute_level_populations_from_stateq", &Simulation::compute_level_populations_from_stateq)
         .def("compute_spectral_discretisation", &Simulation::compute_spectral_discretisation)
+        .def("compute_spectral_discretisation_image", &Simulation::compute_spectral_discretisation_image)
+        .def("compute_and_write_image", &Simulation::compute_and_write_image)
+        .def_readwrite("image_precision", &Simulation::image_precision)
         .def("compute_boundary_intensities",
              (int (Simulation::*)(const Double1 &)) & Simulation::compute_boundary_intensities)
         .def("compute_boundary_intensities", (int (Simulation::*)()) & Simulation::compute_boundary_intensities)
@@ -279,6 +292,7 @@ PYBIND11_MODULE(core, module)
         .def("compute_level_populations",
              (long (Simulation::*)(const Io &, bool, long)) & Simulation::compute_level_populations)
         .def("get_npoints_on_rays_comoving", &Simulation::get_npoints_on_rays_comoving)
+        .def("get_npoints_on_rays_rest", &Simulation::get_npoints_on_rays_rest)
         .def("reset_device", &Simulation::reset_device);
 
     module.attr("FP32") = MGR_FP32;

@@ -145,6 +145,25 @@ int Simulation::compute_spectral_discretisation()
 {
     ensure_context();
     check(mgr_compute_spectral_discretisation(ctx_), "compute_spectral_discretisation");
+    pull_spectral();
+    specDiscSetting = LineSet;
+    return 0;
+}
+
+// Simulation::compute_spectral_discretisation_image (sim_spectrum.tpp:206-311)
+int Simulation::compute_spectral_discretisation_image(const double width)
+{
+    ensure_context();
+    check(mgr_compute_spectral_discretisation_image(ctx_, width), "compute_spectral_discretisation_image");
+    pull_spectral();
+    specDiscSetting = ImageSet;
+    return 0;
+}
+
+// Frequencies::nu, nr_line and the unsorted-index back-maps of either grid
+// (sim_spectrum.tpp:153-184, 270-302).
+void Simulation::pull_spectral()
+{
     LineProducingSpecies &ls = lines.lineProducingSpecies[0];
     const size_t N = geometry.cells.position.size();
     const size_t K = ls.linedata.nrad, Q = ls.quadrature.roots.size(), F = K * Q;
@@ -170,8 +189,6 @@ int Simulation::compute_spectral_discretisation()
             fr.corresponding_k_for_tran[k * Q + z] = (long)k;
             fr.corresponding_z_for_line[k * Q + z] = (long)z;
         }
-    specDiscSetting = LineSet;
-    return 0;
 }
 
 // Copies the device's boundary intensities into radiation.I_bdy; the
@@ -395,6 +412,50 @@ long Simulation::compute_level_populations(const Io &io, bool use_Ng_acceleratio
     }
     pull_radiation();
     return iteration;
+}
+
+// Simulation::compute_and_write_image (sim_radiation.tpp:345-478): the
+// intensities leaving every cell's rest-frame ray pair along ray r, from the
+// Model's current populations; a ray outside [0, nrays/2) leaves them zero
+// (the reference's "This rank is not computing an image.").
+int Simulation::compute_and_write_image(const Io &io, const long r)
+{
+    if (specDiscSetting != ImageSet) {
+        std::fprintf(stderr, "Error: Spectral discretisation was not set for Images!\n");
+        return -1;
+    }
+    ensure_context();
+    Image image(r, parameters);
+    const size_t N = geometry.cells.position.size();
+    const size_t F = radiation.frequencies.nu.empty() ? 0 : radiation.frequencies.nu[0].size();
+    const long Rr = (long)geometry.rays.rays.size() / 2;
+    if (r >= 0 && r < Rr) {
+        push_populations();
+        Double1 Im(N * F), Ip(N * F);
+        check(mgr_compute_image(ctx_, r, image_precision, Im.data(), Ip.data()), "compute_image");
+        for (size_t p = 0; p < N; p++)
+            for (size_t f = 0; f < F; f++) {
+                image.I_m[p][f] = Im[p * F + f];
+                image.I_p[p][f] = Ip[p * F + f];
+            }
+    } else {
+        std::fprintf(stderr, "This rank is not computing an image.\n");
+    }
+    image.set_coordinates(geometry);
+    image.write(io);
+    return 0;
+}
+
+Long2 Simulation::get_npoints_on_rays_rest()
+{
+    ensure_context();
+    const size_t N = geometry.cells.position.size(), Rr = geometry.rays.rays.size() / 2;
+    std::vector<int64_t> np(N * Rr);
+    check(mgr_get_npoints_on_rays_rest(ctx_, np.data()), "get_npoints_on_rays_rest");
+    Long2 out(Rr, Long1(N));
+    for (size_t r = 0; r < Rr; r++)
+        for (size_t o = 0; o < N; o++) out[r][o] = (long)np[r * N + o];
+    return out;
 }
 
 Long2 Simulation::get_npoints_on_rays_comoving()

@@ -10,6 +10,7 @@
 #include <string>
 
 #include "../../../include/magritte_hip.h"
+#include "image.hpp"
 #include "model.hpp"
 
 namespace magritte {
@@ -24,6 +25,7 @@ struct Simulation : public Model {
 
     int device = 0;                      // HIP device of the context
     int precision = MGR_FP32;            // of compute_radiation_field
+    int image_precision = MGR_FP64;      // of compute_and_write_image (the reference's FP64)
     int convergence_criterion = MGR_CRIT_FRACTION;
 
     Simulation() = default;
@@ -32,6 +34,7 @@ struct Simulation : public Model {
     ~Simulation() override;
 
     int compute_spectral_discretisation();
+    int compute_spectral_discretisation_image(const double width);
     int compute_boundary_intensities();
     int compute_boundary_intensities(const Double1 &temperatures);
     int compute_LTE_level_populations();
@@ -50,7 +53,10 @@ struct Simulation : public Model {
     long compute_level_populations(const Io &io);
     long compute_level_populations(const Io &io, bool use_Ng_acceleration, long max_niterations);
 
+    int compute_and_write_image(const Io &io, const long r);
+
     Long2 get_npoints_on_rays_comoving();
+    Long2 get_npoints_on_rays_rest();
 
     // drop the device context (after the geometry or thermodynamics changed)
     void reset_device();
@@ -64,6 +70,7 @@ struct Simulation : public Model {
     void pull_radiation();
     void pull_jeff();
     void pull_boundary_intensities();
+    void pull_spectral();
     double pop_prec() const { return parameters.pop_prec_.is_set ? parameters.pop_prec() : 1.0e-6; }
     int run_radiation_field(int prec);
 };

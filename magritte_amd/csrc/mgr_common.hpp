@@ -209,6 +209,26 @@ struct mgr_ctx {
     bool     traced = false;
     int64_t  sum_ndep = 0, n_pairs_solved = 0;
 
+    // device: rest-frame trace of one image ray pair, entry 2o the antipode
+    // side and 2o + 1 the ray side of origin o (same layout as the trace
+    // cache), and the image intensities
+    struct ImageTrace {
+        int64_t  ray = -1;              // traced ray, -1: none
+        int64_t *seg_off = nullptr;     // [2N + 1]
+        int32_t *nseg = nullptr, *npts = nullptr;   // [2N]
+        int32_t *seg_cell = nullptr;
+        double  *seg_shift = nullptr, *seg_dZ = nullptr;
+        int4    *seg_f32 = nullptr;
+        float   *seg_hdz32 = nullptr;
+        int32_t *far_cell = nullptr;    // [2N]
+        double  *far_shift = nullptr;   // [2N]
+        double  *origin_shift = nullptr;   // [2N] rest-frame shift of the origin on each side
+        double   shift_min = 1.0, shift_max = 1.0;
+        int64_t  total_segments = 0;
+        double  *I_m = nullptr, *I_p = nullptr;   // [N*F]
+    } img;
+    bool     spectral_image = false;    // frequencies set by the image discretisation
+
     // reductions
     double  *partials = nullptr;    // [nchunks * 8]
     double  *reduced = nullptr;     // [8]
@@ -244,4 +264,6 @@ int launch_convergence(mgr_ctx *ctx);
 int launch_ng(mgr_ctx *ctx);
 int read_convergence(mgr_ctx *ctx, mgr_conv *conv);
 int set_active(mgr_ctx *ctx, bool all);   // all: every model of the batch iterates
+int launch_trace_image(mgr_ctx *ctx, int64_t ray);   // rest-frame trace of ray and its antipode
+void free_image(mgr_ctx *ctx);
 }  // namespace mgr

@@ -83,18 +83,21 @@ __global__ void k_cell_tables(int64_t N, const LineTables *__restrict__ tb,
 
 // Simulation::compute_spectral_discretisation (sim_spectrum.tpp:13-196):
 // nu(p, i) = line[k] + line[k]*pw(p)*root_z for i = k*Q + z, sorted per cell.
+// With pw null, compute_spectral_discretisation_image (sim_spectrum.tpp:206-311):
+// the same grid with the one width `image_width` in every cell.
 // One thread per (p, i) ranks its value among the cell's F values (ties by
 // original index), which gives heapsort's order for distinct values.
 __global__ void k_spectral(int64_t N, const LineTables *__restrict__ tb,
-                           const double *__restrict__ pw, const double *__restrict__ roots,
-                           double *__restrict__ nu, int32_t *__restrict__ nr_line)
+                           const double *__restrict__ pw, double image_width,
+                           const double *__restrict__ roots, double *__restrict__ nu,
+                           int32_t *__restrict__ nr_line)
 {
     const int F = tb->F, Q = tb->Q, L = tb->L;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N * F) return;
     const int64_t p = t / F;
     const int i = (int)(t % F);
-    const double w = pw[p];
+    const double w = pw ? pw[p] : image_width;
     const int ki = i / Q, zi = i % Q;
     const double li = tb->line[ki];
     const double vi = li + (li * w) * roots[zi];
@@ -597,6 +600,7 @@ int free_model(mgr_ctx *ctx)
     dfree(ctx->d_m_active); dfree(ctx->d_chunk_cell); dfree(ctx->d_m_chunk_off); dfree(ctx->d_mconv);
     dfree(ctx->d_m_cell_off);
     dfree(ctx->d_mng);
+    free_image(ctx);
     if (ctx->h_reduced) (void)hipHostFree(ctx->h_reduced);
     if (ctx->h_m_active) (void)hipHostFree(ctx->h_m_active);
     if (ctx->h_mconv) (void)hipHostFree(ctx->h_mconv);
@@ -694,13 +698,12 @@ const char *mgr_last_error(const mgr_ctx *ctx)
     return ctx->err.c_str();
 }
 
-int mgr_compute_spectral_discretisation(mgr_ctx *ctx)
+static int spectral_discretisation(mgr_ctx *ctx, bool image, double width)
 {
-    if (!ctx) return MGR_ERR_ARGUMENT;
     MGR_HIP(hipSetDevice(ctx->device));
     const int64_t N = ctx->N, F = ctx->F, L = ctx->L, Q = ctx->Q;
     hipLaunchKernelGGL(k_spectral, dim3(grid_for(N * F, 256)), dim3(256), 0, ctx->stream, N,
-                       ctx->tables, ctx->pw, ctx->roots, ctx->nu, ctx->nr_line);
+                       ctx->tables, image ? nullptr : ctx->pw, width, ctx->roots, ctx->nu, ctx->nr_line);
     MGR_LAUNCH_CHECK();
     // appears_in_line_integral / corresponding_{k,z}, indexed by the unsorted
     // position (sim_spectrum.tpp:153-184), and per transition the sorted
@@ -727,7 +730,21 @@ int mgr_compute_spectral_discretisation(mgr_ctx *ctx)
     }
     MGR_HIP(hipStreamSynchronize(ctx->stream));
     ctx->spectral_set = true;
+    ctx->spectral_image = image;
     return MGR_OK;
+}
+
+int mgr_compute_spectral_discretisation(mgr_ctx *ctx)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    return spectral_discretisation(ctx, false, 0.0);
+}
+
+int mgr_compute_spectral_discretisation_image(mgr_ctx *ctx, double width)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (!(width > 0.0)) return ctx->fail(MGR_ERR_ARGUMENT, "image width must be positive");
+    return spectral_discretisation(ctx, true, width);
 }
 
 int mgr_compute_boundary_intensities(mgr_ctx *ctx, const double *temperatures)

@@ -604,7 +604,8 @@ static int ensure_graphs(mgr_ctx *ctx, int precision)
 int mgr_ali_iteration(mgr_ctx *ctx, int ng, int precision, mgr_conv *conv)
 {
     if (!ctx) return MGR_ERR_ARGUMENT;
-    if (!ctx->spectral_set) return ctx->fail(MGR_ERR_STATE, "Spectral discretisation was not set for Lines");
+    if (!ctx->spectral_set || ctx->spectral_image)
+        return ctx->fail(MGR_ERR_STATE, "Spectral discretisation was not set for Lines");
     if (!ctx->bdy_set) return ctx->fail(MGR_ERR_STATE, "compute_boundary_intensities first");
     if (!ctx->pops_set) return ctx->fail(MGR_ERR_STATE, "level populations not set");
     if (precision != MGR_FP32 && precision != MGR_FP64) return ctx->fail(MGR_ERR_ARGUMENT, "bad precision");
@@ -624,7 +625,8 @@ int mgr_compute_level_populations(mgr_ctx *ctx, int use_Ng_acceleration, int64_t
     if (!ctx) return MGR_ERR_ARGUMENT;
     if (iterations_out) *iterations_out = -1;
     // sim_lines.tpp:67-71: requires the line spectral discretisation
-    if (!ctx->spectral_set) return ctx->fail(MGR_ERR_STATE, "Spectral discretisation was not set for Lines");
+    if (!ctx->spectral_set || ctx->spectral_image)
+        return ctx->fail(MGR_ERR_STATE, "Spectral discretisation was not set for Lines");
     if (!ctx->bdy_set) return ctx->fail(MGR_ERR_STATE, "compute_boundary_intensities first");
     if (!ctx->pops_set) return ctx->fail(MGR_ERR_STATE, "level populations not set");
     if (precision != MGR_FP32 && precision != MGR_FP64) return ctx->fail(MGR_ERR_ARGUMENT, "bad precision");

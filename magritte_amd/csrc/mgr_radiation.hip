@@ -86,6 +86,12 @@ struct Step {
     real    s0, ds, dz, pad2;
 };
 
+// Steps staged for a kernel: the FP32 split of the radiation field, or the
+// FP64 split (FP64 kernels, and the FP32 image, whose shifts stay FP64).
+template <typename real, bool X64> struct StepOf { using type = Step<real>; };
+template <> struct StepOf<float, true> { using type = Step<double>; };
+template <typename real, bool X64> using step_t = typename StepOf<real, X64>::type;
+
 template <typename real>
 struct RFArgs {
     int64_t N;
@@ -102,6 +108,7 @@ struct RFArgs {
     const float   *seg_hdz32;                 // its dZ/2 (scaled)
     const int32_t *far_cell;                  // last emitted point of each (origin, ray), -1: none
     const double  *far_shift;
+    const double  *origin_shift;              // FP64 shift of the origin per entry (null: 1, co-moving)
     const int32_t *cell2bdy, *antipod;        // antipod: global ray indices
     const int32_t *cell_model;                // [N] model of each cell
     const int64_t *cell_qbase;                // [N] first (origin, ray) entry of the cell
@@ -159,6 +166,7 @@ struct Lane32 {
     int    la, nc;
     bool   has2, has3; // wave-uniform: some lane has 2 / more than 2 candidates
     double nu_f64;
+    double A0d, A1d;   // the same offsets in FP64 (FP32 image variant)
 };
 
 __device__ __forceinline__ float rfl(float x)
@@ -188,6 +196,43 @@ __device__ __forceinline__ void eta_chi32(const RFArgs<float> &a, const float4 *
                 const int l = ln.la + c;
                 const float4 v = row[l];
                 const float x = (t + (float)(ln.nu_f64 - a.tb->line[l])) * v.x;
+                const float g = nus * __builtin_amdgcn_exp2f(-(x * x));
+                e_acc = fmaf(v.y, g, e_acc);
+                c_acc = fmaf(v.z, g, c_acc);
+            }
+        }
+    }
+    eta = e_acc;
+    chi = c_acc;
+}
+
+// FP32 image variant: the shift s (absolute, FP64) and the offset from each
+// candidate line in FP64, the profile and the sums in FP32.  An image bin can
+// sit far out in a line's wing and meet the line only where the rest-frame
+// shift cancels its offset; both terms are ~1e-4 nu there, and their FP32
+// rounding (6e-12 nu) would be a visible fraction of a line width.
+__device__ __forceinline__ void eta_chi32x(const RFArgs<float> &a, const float4 *__restrict__ row,
+                                           const Lane32 &ln, double s, float &eta, float &chi)
+{
+    const char *base = reinterpret_cast<const char *>(row);
+    const double t = (s - 1.0) * ln.nu_f64;
+    const float nus = (float)(t + ln.nu_f64);
+    const float4 v0 = *reinterpret_cast<const float4 *>(base + ln.off0);
+    const float x0 = (float)(t + ln.A0d) * v0.x;
+    const float g0 = nus * __builtin_amdgcn_exp2f(-(x0 * x0));
+    float e_acc = v0.y * g0;
+    float c_acc = fmaf(v0.z, g0, (float)(CHI_FLOOR * CHI_SCALE));
+    if (ln.has2) {
+        const float4 v1 = *reinterpret_cast<const float4 *>(base + ln.off1);
+        const float x1 = (float)(t + ln.A1d) * v1.x;
+        const float g1 = nus * __builtin_amdgcn_exp2f(-(x1 * x1));
+        e_acc = fmaf(v1.y, g1, e_acc);
+        c_acc = fmaf(v1.z, g1, c_acc);
+        if (ln.has3) {
+            for (int c = 2; c < ln.nc; c++) {
+                const int l = ln.la + c;
+                const float4 v = row[l];
+                const float x = (float)(t + (ln.nu_f64 - a.tb->line[l])) * v.x;
                 const float g = nus * __builtin_amdgcn_exp2f(-(x * x));
                 e_acc = fmaf(v.y, g, e_acc);
                 c_acc = fmaf(v.z, g, c_acc);
@@ -258,21 +303,21 @@ struct Sweep {
 
 // Stage the entries e = 0..cnt-1 of chunk [a, b): entry e holds step j = a-1+e
 // (j = -1 is the origin) and the line rows of its cell.
-template <typename real>
+template <typename real, bool X64 = false>
 __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, int64_t q, int j0,
-                                            int cnt, double dsm, Step<real> *__restrict__ st,
+                                            int cnt, double dsm, step_t<real, X64> *__restrict__ st,
                                             typename Prec<real>::row_t *__restrict__ rw)
 {
     const int64_t s0 = a.seg_off[q];
     const int nthr = blockDim.x;
     for (int e = threadIdx.x; e < cnt; e += nthr) {
         const int j = j0 + e;
-        Step<real> s;
+        step_t<real, X64> s;
         s.pad = 0;
         s.pad2 = 0;
         if (j < 0) {
             s.cell = o; s.n = 0; s.half = 0; s.s0 = 0; s.ds = 0; s.dz = 0;
-        } else if constexpr (sizeof(real) == 4) {
+        } else if constexpr (sizeof(real) == 4 && !X64) {
             // split at trace time with the same FP64 arithmetic as below
             const int4 v = a.seg_f32[s0 + j];
             s.cell = v.x;
@@ -285,19 +330,14 @@ __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, in
             const int32_t c = a.seg_cell[s0 + j];
             const double sn = a.seg_shift[s0 + j];
             const double z = a.seg_dZ[s0 + j];
-            const double sc = (j > 0) ? a.seg_shift[s0 + j - 1] : 1.0;
+            const double sc = (j > 0) ? a.seg_shift[s0 + j - 1] : (a.origin_shift ? a.origin_shift[q] : 1.0);
             const StepSplit sp = split_step(sc, sn, z, dsm);
             s.cell = c;
             s.n = sp.n;
             s.half = sp.interp ? sp.half : 0;
-            if (sizeof(real) == 8) {
-                s.s0 = (real)(sp.interp ? sc : sn);
-                s.ds = (real)sp.dshift;
-            } else {
-                s.s0 = (real)((sp.interp ? sc : sn) - 1.0);
-                s.ds = (real)sp.dshift;
-            }
-            s.dz = (real)(sp.dZ * (0.5 / CHI_SCALE));   // dz/2
+            s.s0 = sp.interp ? sc : sn;               // absolute FP64 shift
+            s.ds = sp.dshift;
+            s.dz = sp.dZ * (0.5 / CHI_SCALE);         // dz/2
         }
         st[e] = s;
     }
@@ -349,6 +389,17 @@ __device__ __forceinline__ void eval_point<double>(const RFArgs<double> &a, cons
     eta_chi64(a, row, fs, lc.lstart, eta, chi);
 }
 
+// The point evaluation of a kernel: eval_point, or the FP32 image variant
+// with FP64 shifts.
+template <typename real, bool X64, typename SR>
+__device__ __forceinline__ void eval_at(const RFArgs<real> &a, const LaneCtx<real> &lc,
+                                        const typename Prec<real>::row_t *__restrict__ row, SR s,
+                                        real &eta, real &chi)
+{
+    if constexpr (X64) eta_chi32x(a, row, lc.ln, s, eta, chi);
+    else eval_point<real>(a, lc, row, s, eta, chi);
+}
+
 // shift of sub-point m: FP64 exactly s0 + m ds as Geometry::set_data;
 // FP32 the offset shift - 1 in one fused multiply-add.
 __device__ __forceinline__ double sub_point_shift(double s0, int m, double ds)
@@ -365,35 +416,41 @@ __device__ __forceinline__ int rfl_int(int x) { return __builtin_amdgcn_readfirs
 __device__ __forceinline__ float rfl_real(float x) { return rfl(x); }
 __device__ __forceinline__ double rfl_real(double x) { return x; }
 
-// One ray of a pair, eliminated from its boundary end inwards to the origin.
+// Incoming intensity at the far end of entry q (FP64, unscaled): the
+// boundary intensity of the cell of the last emitted point, rescaled to that
+// point's frequency shift * nu_f (Simulation::setup, simulation.tpp:266-272,
+// 327-333); 0 for a ray without points.
 template <typename real>
+__device__ __forceinline__ double far_intensity(const RFArgs<real> &a, int64_t q, double nu_f, int F,
+                                                int32_t &err)
+{
+    const int32_t cf = a.far_cell[q];
+    if (cf < 0) return 0.0;
+    const int32_t b = a.cell2bdy[cf];
+    if (b < 0) {
+        err |= ERR_NOT_BOUNDARY;
+        return 0.0;
+    }
+    double fs;
+    {
+#pragma clang fp contract(off)
+        fs = a.far_shift[q] * nu_f;
+    }
+    return rescale_I_bdy(a.nu + (int64_t)cf * F, a.I_bdy + (int64_t)b * F, F, fs);
+}
+
+// One ray of a pair, eliminated from its boundary end (incoming intensity
+// I_far, scaled) inwards to the origin.
+template <typename real, bool X64 = false>
 __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const LaneCtx<real> &lc,
-                                                  int32_t o, int64_t q, double nu_f, real S_o,
-                                                  real chi_o, double dsm, int F,
-                                                  Step<real> *__restrict__ st,
-                                                  typename Prec<real>::row_t *__restrict__ rw,
-                                                  int32_t &err)
+                                                  int32_t o, int64_t q, real I_far, real S_o,
+                                                  real chi_o, double dsm,
+                                                  step_t<real, X64> *__restrict__ st,
+                                                  typename Prec<real>::row_t *__restrict__ rw)
 {
     using row_t = typename Prec<real>::row_t;
     Sweep<real> sw;
-    // incoming intensity at the far end
-    const int32_t cf = a.far_cell[q];
-    const double shf = a.far_shift[q];
-    real Ib = 0;
-    if (cf >= 0) {
-        const int32_t b = a.cell2bdy[cf];
-        if (b < 0) {
-            err |= ERR_NOT_BOUNDARY;
-        } else {
-            double fs;
-            {
-#pragma clang fp contract(off)
-                fs = shf * nu_f;
-            }
-            Ib = (real)(rescale_I_bdy(a.nu + (int64_t)cf * F, a.I_bdy + (int64_t)b * F, F, fs) * I_SCALE);
-        }
-    }
-    sw.init(Ib);
+    sw.init(I_far);
     bool have_q = false;   // wave-uniform
     const int ns = a.nseg[q];
     const int L = a.L;
@@ -402,13 +459,13 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
         const int a_ = b_ > CH ? b_ - CH : 0;
         const int cnt = b_ - a_ + 1;
         __syncthreads();   // the previous chunk is consumed by every wave
-        stage_chunk<real>(a, o, q, a_ - 1, cnt, dsm, st, rw);
+        stage_chunk<real, X64>(a, o, q, a_ - 1, cnt, dsm, st, rw);
         for (int e = cnt - 1; e >= 1; --e) {
             const int n = rfl_int(st[e].n);
             const int half = rfl_int(st[e].half);
-            const real s0 = st[e].s0;   // stays in a VGPR: one scalar operand per FMA
-            const real ds = rfl_real(st[e].ds);
-            const real hdz = rfl_real(st[e].dz);
+            const auto s0 = st[e].s0;   // stays in a VGPR: one scalar operand per FMA
+            const auto ds = rfl_real(st[e].ds);
+            const real hdz = (real)rfl_real(st[e].dz);
             // sub-points m >= half lie in the step's cell (row e), m < half in
             // the previous one (row e-1); two loops keep each row's address
             // loop-invariant
@@ -417,7 +474,7 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
             const int mlo = half > 1 ? half : 1;
             for (int m = n; m >= mlo; --m) {
                 real eta, chi;
-                eval_point<real>(a, lc, rn, sub_point_shift(s0, m, ds), eta, chi);
+                eval_at<real, X64>(a, lc, rn, sub_point_shift(s0, m, ds), eta, chi);
                 const real S = eta * Prec<real>::rcp(chi);
                 if (have_q) sw.step(S, chi);
                 have_q = true;
@@ -425,7 +482,7 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
             }
             for (int m = (half - 1 < n ? half - 1 : n); m >= 1; --m) {
                 real eta, chi;
-                eval_point<real>(a, lc, rc, sub_point_shift(s0, m, ds), eta, chi);
+                eval_at<real, X64>(a, lc, rc, sub_point_shift(s0, m, ds), eta, chi);
                 const real S = eta * Prec<real>::rcp(chi);
                 if (have_q) sw.step(S, chi);
                 have_q = true;
@@ -435,6 +492,108 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
     }
     sw.step(S_o, chi_o);   // the origin closes the innermost row
     return sw;
+}
+
+// Continues an elimination outwards along entry q, from the origin (the
+// pending point of sw) to the far end of the ray.  Each point closes the
+// pending row with its own dZ (the outer point's, as Simulation::setup pairs
+// them, simulation.tpp:306) and becomes the pending point; on return the
+// far-end point is pending.
+template <typename real, bool X64 = false>
+__device__ __forceinline__ void sweep_outward(const RFArgs<real> &a, const LaneCtx<real> &lc, int32_t o,
+                                              int64_t q, double dsm, Sweep<real> &sw,
+                                              step_t<real, X64> *__restrict__ st,
+                                              typename Prec<real>::row_t *__restrict__ rw)
+{
+    using row_t = typename Prec<real>::row_t;
+    const int ns = a.nseg[q];
+    const int L = a.L;
+    const int CH = a.ch;
+    for (int a_ = 0; a_ < ns; a_ += CH) {
+        const int b_ = a_ + CH < ns ? a_ + CH : ns;
+        const int cnt = b_ - a_ + 1;
+        __syncthreads();   // the previous chunk is consumed by every wave
+        stage_chunk<real, X64>(a, o, q, a_ - 1, cnt, dsm, st, rw);
+        for (int e = 1; e < cnt; ++e) {
+            const int n = rfl_int(st[e].n);
+            const int half = rfl_int(st[e].half);
+            const auto s0 = st[e].s0;
+            const auto ds = rfl_real(st[e].ds);
+            const real hdz = (real)rfl_real(st[e].dz);
+            const row_t *rn = rw + e * L;
+            const row_t *rc = rw + (e - 1) * L;
+            const int mhi = half - 1 < n ? half - 1 : n;
+            for (int m = 1; m <= mhi; ++m) {
+                real eta, chi;
+                eval_at<real, X64>(a, lc, rc, sub_point_shift(s0, m, ds), eta, chi);
+                const real S = eta * Prec<real>::rcp(chi);
+                sw.hdz_q = hdz;
+                sw.step(S, chi);
+                sw.pend(S, chi, hdz);
+            }
+            for (int m = half > 1 ? half : 1; m <= n; ++m) {
+                real eta, chi;
+                eval_at<real, X64>(a, lc, rn, sub_point_shift(s0, m, ds), eta, chi);
+                const real S = eta * Prec<real>::rcp(chi);
+                sw.hdz_q = hdz;
+                sw.step(S, chi);
+                sw.pend(S, chi, hdz);
+            }
+        }
+    }
+}
+
+// u at the pending end point of a full sweep, closed by the incoming
+// intensity I (scaled) there: the boundary row of RayPair::solve's last point
+// in the elimination variables.
+template <typename real>
+__device__ __forceinline__ real end_u(const Sweep<real> &sw, real I)
+{
+    const real h = sw.bh;
+    const real den = Prec<real>::rcp(h + (real)1 + sw.fq());
+    return (sw.S_q * h + I + sw.zeta) * den;
+}
+
+// Per-lane constants of origin o's bin and its own emissivity and opacity at
+// the unshifted frequency nu_f (Simulation::setup takes the origin at
+// nu(o, f), simulation.tpp:209-222).
+template <typename real, bool X64 = false>
+__device__ __forceinline__ void origin_point(const RFArgs<real> &a, double nu_f,
+                                             const typename Prec<real>::row_t *__restrict__ orow,
+                                             LaneCtx<real> &lc, real &S_o, real &chi_o)
+{
+    real eta_o;
+    if constexpr (sizeof(real) == 4) {
+        // candidate lines: nu_l (1 - w) <= s_max nu_f and nu_l (1 + w) >= s_min nu_f
+        int la = 0;
+        while (la < a.L && a.tb->line[la] * a.cand_lo < nu_f) la++;
+        int lb = la;
+        while (lb < a.L && a.tb->line[lb] * a.cand_hi <= nu_f) lb++;
+        const int nc = lb - la;
+        const int l0 = la < a.L ? la : a.L - 1;
+        const int l1 = la + 1 < a.L ? la + 1 : a.L - 1;
+        lc.ln.la = l0;
+        lc.ln.nc = nc;
+        lc.ln.nuf = (float)nu_f;
+        lc.ln.nu_f64 = nu_f;
+        lc.ln.A0 = (nc > 0) ? (float)(nu_f - a.tb->line[l0]) : 1.0e30f;
+        lc.ln.A1 = (nc > 1) ? (float)(nu_f - a.tb->line[l1]) : 1.0e30f;
+        lc.ln.A0d = (nc > 0) ? nu_f - a.tb->line[l0] : 1.0e30;
+        lc.ln.A1d = (nc > 1) ? nu_f - a.tb->line[l1] : 1.0e30;
+        lc.ln.off0 = l0 * (int)sizeof(float4);
+        lc.ln.off1 = l1 * (int)sizeof(float4);
+        lc.ln.has2 = __any(nc > 1);
+        lc.ln.has3 = __any(nc > 2);
+        if constexpr (X64) eta_chi32x(a, orow, lc.ln, 1.0, eta_o, chi_o);
+        else eval_point<real>(a, lc, orow, (real)0, eta_o, chi_o);
+    } else {
+        int ls = 0;
+        while (ls < a.L - 1 && a.tb->line[ls] * a.lstart_threshold < nu_f) ls++;
+        lc.nu_f = nu_f;
+        lc.lstart = ls;
+        eval_point<real>(a, lc, orow, (real)1, eta_o, chi_o);
+    }
+    S_o = eta_o * Prec<real>::rcp(chi_o);
 }
 
 template <typename real>
@@ -468,35 +627,8 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
     const row_t *orow = a.rows + (int64_t)o * a.L;
 
     LaneCtx<real> lc;
-    real eta_o, chi_o;
-    if constexpr (sizeof(real) == 4) {
-        // candidate lines: nu_l (1 - w) <= s_max nu_f and nu_l (1 + w) >= s_min nu_f
-        int la = 0;
-        while (la < a.L && a.tb->line[la] * a.cand_lo < nu_f) la++;
-        int lb = la;
-        while (lb < a.L && a.tb->line[lb] * a.cand_hi <= nu_f) lb++;
-        const int nc = lb - la;
-        const int l0 = la < a.L ? la : a.L - 1;
-        const int l1 = la + 1 < a.L ? la + 1 : a.L - 1;
-        lc.ln.la = l0;
-        lc.ln.nc = nc;
-        lc.ln.nuf = (float)nu_f;
-        lc.ln.nu_f64 = nu_f;
-        lc.ln.A0 = (nc > 0) ? (float)(nu_f - a.tb->line[l0]) : 1.0e30f;
-        lc.ln.A1 = (nc > 1) ? (float)(nu_f - a.tb->line[l1]) : 1.0e30f;
-        lc.ln.off0 = l0 * (int)sizeof(float4);
-        lc.ln.off1 = l1 * (int)sizeof(float4);
-        lc.ln.has2 = __any(nc > 1);
-        lc.ln.has3 = __any(nc > 2);
-        eval_point<real>(a, lc, orow, (real)0, eta_o, chi_o);
-    } else {
-        int ls = 0;
-        while (ls < a.L - 1 && a.tb->line[ls] * a.lstart_threshold < nu_f) ls++;
-        lc.nu_f = nu_f;
-        lc.lstart = ls;
-        eval_point<real>(a, lc, orow, (real)1, eta_o, chi_o);
-    }
-    const real S_o = eta_o * Prec<real>::rcp(chi_o);
+    real S_o, chi_o;
+    origin_point<real>(a, nu_f, orow, lc, S_o, chi_o);
 
     int32_t err = 0;
     double J_acc = 0.0, Lam_acc = 0.0;
@@ -518,15 +650,18 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
         }
         real u, lam;
         if (n_r > 0 && n_a > 0) {
-            const Sweep<real> A = sweep_side<real>(a, lc, o, qa, nu_f, S_o, chi_o, dsm, F, st, rw, err);
-            const Sweep<real> B = sweep_side<real>(a, lc, o, qr, nu_f, S_o, chi_o, dsm, F, st, rw, err);
+            const real Ia = (real)(far_intensity(a, qa, nu_f, F, err) * I_SCALE);
+            const Sweep<real> A = sweep_side<real>(a, lc, o, qa, Ia, S_o, chi_o, dsm, st, rw);
+            const real Ir = (real)(far_intensity(a, qr, nu_f, F, err) * I_SCALE);
+            const Sweep<real> B = sweep_side<real>(a, lc, o, qr, Ir, S_o, chi_o, dsm, st, rw);
             const real h = A.bh + B.bh;
             const real den = Prec<real>::rcp(h + A.fq() + B.fq());
             u = (S_o * h + A.zeta + B.zeta) * den;
             lam = h * den;
         } else {
             const int64_t qs = (n_a > 0) ? qa : qr;
-            const Sweep<real> A = sweep_side<real>(a, lc, o, qs, nu_f, S_o, chi_o, dsm, F, st, rw, err);
+            const real Is = (real)(far_intensity(a, qs, nu_f, F, err) * I_SCALE);
+            const Sweep<real> A = sweep_side<real>(a, lc, o, qs, Is, S_o, chi_o, dsm, st, rw);
             real I_o = 0;
             if (bo < 0) err |= ERR_EMPTY_SIDE;
             else I_o = (real)(a.I_bdy[(int64_t)bo * F + fa] * I_SCALE);
@@ -545,23 +680,113 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
     }
 }
 
+// Image of ray r (Simulation::compute_and_write_image, sim_radiation.tpp:345-478;
+// RayPair::solve_for_image, raypair_solver.tpp:523-579).  Entries 2o (the
+// antipode, index 0 end of the pair) and 2o + 1 (ray r, index ndep-1 end) of
+// the rest-frame trace.  The intensities leaving the pair are
+// I_p = 2 u(ndep-1) - I_bdy_n and I_m = 2 u(0) - I_bdy_0 (RayPair::get_Ip_at_end,
+// get_Im_at_front, raypair.tpp:212-229).  Instead of a stored forward
+// elimination and back-substitution, each end's u comes from one full sweep
+// that starts at the other end, passes the origin and closes at the end
+// itself with its boundary row: two sweeps, no per-point storage.
+template <typename real, bool X64>
+__global__ __launch_bounds__(WAVE * MAX_WAVES) void k_image(RFArgs<real> a, double *__restrict__ I_m,
+                                                            double *__restrict__ I_p)
+{
+    using row_t = typename Prec<real>::row_t;
+    extern __shared__ __align__(16) unsigned char smem[];
+    step_t<real, X64> *st = reinterpret_cast<step_t<real, X64> *>(smem);
+    row_t *rw = reinterpret_cast<row_t *>(smem + a.row_offset);
+
+    const int G = gridDim.x;
+    const int per = G >> 3;
+    const int bid = blockIdx.x;
+    const int logical = (bid & 7) * per + (bid >> 3);
+    if (logical >= a.grid_real) return;   // whole workgroup: before any barrier
+    const int32_t o = a.o_begin + (int32_t)(logical / a.ngroups);
+    const int group = logical - (o - a.o_begin) * a.ngroups;
+    const int wave = threadIdx.x / WAVE;
+    const int chunk = group * a.nwg + wave;
+    const int F = a.F;
+    const int f = chunk * WAVE + (int)(threadIdx.x & (WAVE - 1));
+    const bool active = f < F;
+    const int fa = active ? f : F - 1;
+    const double nu_f = a.nu[(int64_t)o * F + fa];
+    const double dsm = a.dshift_max[o];
+    const int32_t bo = a.cell2bdy[o];
+    const row_t *orow = a.rows + (int64_t)o * a.L;
+
+    LaneCtx<real> lc;
+    real S_o, chi_o;
+    origin_point<real, X64>(a, nu_f, orow, lc, S_o, chi_o);
+
+    int32_t err = 0;
+    const int64_t qa = 2 * (int64_t)o, qr = qa + 1;
+    const int n_a = a.npts[qa];
+    const int n_r = a.npts[qr];
+    double Im = 0.0, Ip = 0.0;
+    if (n_a == 0 && n_r == 0) {
+        // too short a ray: both intensities are I_bdy of the origin's boundary entry
+        if (bo < 0) err |= ERR_EMPTY_SIDE;
+        else Im = Ip = a.I_bdy[(int64_t)bo * F + fa];
+    } else {
+        double I_o = 0.0;   // an empty side takes I_bdy of the origin's boundary entry
+        if (n_a == 0 || n_r == 0) {
+            if (bo < 0) err |= ERR_EMPTY_SIDE;
+            else I_o = a.I_bdy[(int64_t)bo * F + fa];
+        }
+        const double I0 = n_a > 0 ? far_intensity(a, qa, nu_f, F, err) : I_o;
+        const double In = n_r > 0 ? far_intensity(a, qr, nu_f, F, err) : I_o;
+        const real I0s = (real)(I0 * I_SCALE);
+        const real Ins = (real)(In * I_SCALE);
+        real u_n, u_0;
+        {
+            // from the antipode's far end through the origin to ray r's end
+            Sweep<real> A;
+            if (n_a > 0) A = sweep_side<real, X64>(a, lc, o, qa, I0s, S_o, chi_o, dsm, st, rw);
+            else A.init(I0s);
+            A.pend(S_o, chi_o, (real)0);
+            sweep_outward<real, X64>(a, lc, o, qr, dsm, A, st, rw);
+            u_n = end_u(A, Ins);
+        }
+        {
+            // and back
+            Sweep<real> B;
+            if (n_r > 0) B = sweep_side<real, X64>(a, lc, o, qr, Ins, S_o, chi_o, dsm, st, rw);
+            else B.init(Ins);
+            B.pend(S_o, chi_o, (real)0);
+            sweep_outward<real, X64>(a, lc, o, qa, dsm, B, st, rw);
+            u_0 = end_u(B, I0s);
+        }
+        Ip = 2.0 * ((double)u_n * (1.0 / I_SCALE)) - In;
+        Im = 2.0 * ((double)u_0 * (1.0 / I_SCALE)) - I0;
+    }
+    if (err) atomicOr(a.err_flag, err);
+    if (active) {
+        I_m[(int64_t)o * F + f] = Im;
+        I_p[(int64_t)o * F + f] = Ip;
+    }
+}
+
 // Largest chunk of staged steps that fits the LDS budget with the cells'
 // line rows, at most CH_MAX.
-template <typename real>
+template <typename real, bool X64 = false>
 int max_chunk(const mgr_ctx *ctx)
 {
-    const size_t per_entry = sizeof(Step<real>) + sizeof(typename Prec<real>::row_t) * ctx->L + 64;
+    const size_t per_entry = sizeof(step_t<real, X64>) + sizeof(typename Prec<real>::row_t) * ctx->L + 64;
     const int ch = (int)(LDS_LIMIT / per_entry) - 1;
     return ch < CH_MAX ? ch : CH_MAX;
 }
 
-// Launch over origins [o_begin, o_begin + o_count) with nwg waves (64 bins
-// each) per workgroup and chunks of ch staged steps.
-template <typename real>
-int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int ch, int64_t o_begin,
-              int64_t o_count)
+// Arguments shared by the radiation-field and image kernels: model tables,
+// rows, candidate-line bounds for shifts in [shift_min, shift_max], launch
+// shape (nwg waves of 64 bins per workgroup, chunks of ch staged steps) over
+// origins [o_begin, o_begin + o_count).  The trace pointers are the caller's.
+template <typename real, bool X64 = false>
+int make_args(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int ch, int64_t o_begin,
+              int64_t o_count, double shift_min, double shift_max, RFArgs<real> &a)
 {
-    RFArgs<real> a;
+    a = RFArgs<real>();
     a.N = ctx->N;
     a.cell_model = ctx->d_cell_model;
     a.cell_qbase = ctx->d_cell_qbase;
@@ -577,6 +802,53 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     a.tb = ctx->tables;
     a.nu = ctx->nu;
     a.rows = rows;
+    a.cell2bdy = ctx->cell2bdy;
+    a.antipod = ctx->d_antipod;
+    a.I_bdy = ctx->I_bdy;
+    a.ray_w = ctx->ray_w;
+    a.dshift_max = ctx->dshift_max;
+    // A line can be inside the window of bin nu_f at some point only if
+    // nu_l (1 - 11 pw_max) <= s_max nu_f and nu_l (1 + 11 pw_max) >= s_min nu_f;
+    // the margins cover rounding of the shifts and widths.
+    const double w = LINE_WINDOW * ctx->pw_max;
+    a.cand_lo = (1.0 + w) * (1.0 + 1.0e-9) / shift_min;   // line * cand_lo < nu_f: below
+    a.cand_hi = (1.0 - w) * (1.0 - 1.0e-9) / shift_max;   // line * cand_hi <= nu_f: candidate
+    a.lstart_threshold = a.cand_lo;
+    a.J = ctx->J;
+    a.Lam = ctx->Lam;
+    a.err_flag = ctx->err_flag;
+    if (nwg < 1 || nwg > MAX_WAVES || o_begin < 0 || o_count < 1 || o_begin + o_count > ctx->N)
+        return ctx->fail(MGR_ERR_ARGUMENT, "radiation-field launch shape out of range");
+    const int64_t groups = o_count * a.ngroups;
+    if (groups > INT32_MAX - 8) return ctx->fail(MGR_ERR_UNSUPPORTED, "too many workgroups");
+    a.grid_real = (int)groups;
+    if (ch < 2 || ch > max_chunk<real, X64>(ctx)) return ctx->fail(MGR_ERR_UNSUPPORTED, "line rows do not fit the LDS budget");
+    a.ch = ch;
+    a.row_offset = ((sizeof(step_t<real, X64>) * (ch + 1) + 63) / 64) * 64;
+    return MGR_OK;
+}
+
+template <typename real>
+size_t lds_bytes(const mgr_ctx *ctx, const RFArgs<real> &a)
+{
+    return a.row_offset + sizeof(typename Prec<real>::row_t) * (a.ch + 1) * ctx->L;
+}
+
+template <typename real>
+unsigned grid_of(const RFArgs<real> &a)
+{
+    return (unsigned)(((int64_t)a.grid_real + 7) / 8) * 8;
+}
+
+// Launch over origins [o_begin, o_begin + o_count) with nwg waves (64 bins
+// each) per workgroup and chunks of ch staged steps.
+template <typename real>
+int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int ch, int64_t o_begin,
+              int64_t o_count)
+{
+    RFArgs<real> a;
+    int rc = make_args<real>(ctx, rows, nwg, ch, o_begin, o_count, ctx->shift_min, ctx->shift_max, a);
+    if (rc) return rc;
     a.seg_off = ctx->seg_off;
     a.nseg = ctx->nseg;
     a.npts = ctx->npts;
@@ -587,33 +859,36 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     a.seg_hdz32 = ctx->seg_hdz32;
     a.far_cell = ctx->far_cell;
     a.far_shift = ctx->far_shift;
-    a.cell2bdy = ctx->cell2bdy;
-    a.antipod = ctx->d_antipod;
-    a.I_bdy = ctx->I_bdy;
-    a.ray_w = ctx->ray_w;
-    a.dshift_max = ctx->dshift_max;
-    // A line can be inside the window of bin nu_f at some point only if
-    // nu_l (1 - 11 pw_max) <= s_max nu_f and nu_l (1 + 11 pw_max) >= s_min nu_f;
-    // the margins cover rounding of the shifts and widths.
-    const double w = LINE_WINDOW * ctx->pw_max;
-    a.cand_lo = (1.0 + w) * (1.0 + 1.0e-9) / ctx->shift_min;   // line * cand_lo < nu_f: below
-    a.cand_hi = (1.0 - w) * (1.0 - 1.0e-9) / ctx->shift_max;   // line * cand_hi <= nu_f: candidate
-    a.lstart_threshold = a.cand_lo;
-    a.J = ctx->J;
-    a.Lam = ctx->Lam;
-    a.err_flag = ctx->err_flag;
-    if (nwg < 1 || nwg > MAX_WAVES || o_begin < 0 || o_count < 1 || o_begin + o_count > ctx->N)
-        return ctx->fail(MGR_ERR_ARGUMENT, "radiation-field launch shape out of range");
-    const int64_t groups = o_count * a.ngroups;
-    if (groups > INT32_MAX - 8) return ctx->fail(MGR_ERR_UNSUPPORTED, "too many workgroups");
-    a.grid_real = (int)groups;
-    const int grid = (int)((groups + 7) / 8) * 8;
-    if (ch < 2 || ch > max_chunk<real>(ctx)) return ctx->fail(MGR_ERR_UNSUPPORTED, "line rows do not fit the LDS budget");
-    a.ch = ch;
-    a.row_offset = ((sizeof(Step<real>) * (ch + 1) + 63) / 64) * 64;
-    const size_t lds = a.row_offset + sizeof(typename Prec<real>::row_t) * (ch + 1) * ctx->L;
-    hipLaunchKernelGGL(k_radiation_field<real>, dim3((unsigned)grid), dim3(WAVE * a.nwg), lds,
+    a.origin_shift = nullptr;
+    hipLaunchKernelGGL(k_radiation_field<real>, dim3(grid_of(a)), dim3(WAVE * a.nwg), lds_bytes(ctx, a),
                        ctx->stream, a);
+    MGR_LAUNCH_CHECK();
+    return MGR_OK;
+}
+
+// Image kernel over every origin, on the rest-frame trace of ctx->img.
+template <typename real, bool X64>
+int launch_image(mgr_ctx *ctx, const typename Prec<real>::row_t *rows)
+{
+    const int nchunk = (int)((ctx->F + WAVE - 1) / WAVE);
+    const mgr_ctx::ImageTrace &g = ctx->img;
+    RFArgs<real> a;
+    int rc = make_args<real, X64>(ctx, rows, std::min(nchunk, MAX_WAVES), max_chunk<real, X64>(ctx), 0,
+                                  ctx->N, g.shift_min, g.shift_max, a);
+    if (rc) return rc;
+    a.seg_off = g.seg_off;
+    a.nseg = g.nseg;
+    a.npts = g.npts;
+    a.seg_cell = g.seg_cell;
+    a.seg_shift = g.seg_shift;
+    a.seg_dZ = g.seg_dZ;
+    a.seg_f32 = g.seg_f32;
+    a.seg_hdz32 = g.seg_hdz32;
+    a.far_cell = g.far_cell;
+    a.far_shift = g.far_shift;
+    a.origin_shift = g.origin_shift;
+    hipLaunchKernelGGL((k_image<real, X64>), dim3(grid_of(a)), dim3(WAVE * a.nwg), lds_bytes(ctx, a), ctx->stream, a,
+                       g.I_m, g.I_p);
     MGR_LAUNCH_CHECK();
     return MGR_OK;
 }
@@ -765,5 +1040,39 @@ extern "C" int mgr_radiation_field_shape(mgr_ctx *ctx, int precision, int *waves
         if (tried_steps) tried_steps[i] = ctx->tune_log[i].ch;
         if (tried_ms) tried_ms[i] = ctx->tune_log[i].ms;
     }
+    return MGR_OK;
+}
+
+extern "C" int mgr_compute_image(mgr_ctx *ctx, int64_t ray, int precision, double *I_m, double *I_p)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (precision != MGR_FP32 && precision != MGR_FP64) return ctx->fail(MGR_ERR_ARGUMENT, "precision must be MGR_FP32 or MGR_FP64");
+    // sim_radiation.tpp:353-358
+    if (!ctx->spectral_set || !ctx->spectral_image)
+        return ctx->fail(MGR_ERR_STATE, "Spectral discretisation was not set for Images");
+    if (!ctx->bdy_set) return ctx->fail(MGR_ERR_STATE, "compute_boundary_intensities first");
+    if (!ctx->pops_set) return ctx->fail(MGR_ERR_STATE, "level populations not set");
+    if (ctx->M != 1) return ctx->fail(MGR_ERR_UNSUPPORTED, "images are per model (batch of one)");
+    if (ray < 0 || ray >= ctx->R / 2) return ctx->fail(MGR_ERR_ARGUMENT, "image ray must be in [0, nrays/2)");
+    MGR_HIP(hipSetDevice(ctx->device));
+    int rc = launch_trace_image(ctx, ray);
+    if (rc) return rc;
+    mgr_ctx::ImageTrace &g = ctx->img;
+    const int64_t NF = ctx->N * ctx->F;
+    if (!g.I_m) {
+        MGR_HIP(hipMalloc((void **)&g.I_m, NF * sizeof(double)));
+        MGR_HIP(hipMalloc((void **)&g.I_p, NF * sizeof(double)));
+    }
+    MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
+    // FP32: the sweep and profiles in FP32, shifts and profile arguments in FP64
+    rc = precision == MGR_FP64 ? launch_image<double, false>(ctx, ctx->row64)
+                               : launch_image<float, true>(ctx, ctx->row32);
+    if (rc) return rc;
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    int32_t errv = 0;
+    MGR_HIP(hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (errv) return ctx->fail(MGR_ERR_TRACE, "an image ray ended off the boundary or an interior origin has an empty ray");
+    if (I_m) MGR_HIP(hipMemcpy(I_m, g.I_m, NF * sizeof(double), hipMemcpyDeviceToHost));
+    if (I_p) MGR_HIP(hipMemcpy(I_p, g.I_p, NF * sizeof(double), hipMemcpyDeviceToHost));
     return MGR_OK;
 }

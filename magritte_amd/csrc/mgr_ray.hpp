@@ -64,6 +64,17 @@ __device__ __forceinline__ double doppler_shift(const double *__restrict__ vel, 
     return 1.0 - dot3(dx, dy, dz, nx, ny, nz);
 }
 
+// Geometry::get_doppler_shift<Rest> (geometry.tpp:448-463): 1 - v(current).n
+// with n the direction of the ray, or of its antipode for the second half of
+// the rays ("up the ray").
+__device__ __forceinline__ double rest_shift(const double *__restrict__ vel, int32_t current,
+                                             double nx, double ny, double nz)
+{
+#pragma clang fp contract(off)
+    return 1.0 - dot3(vel[3 * (int64_t)current + 0], vel[3 * (int64_t)current + 1],
+                      vel[3 * (int64_t)current + 2], nx, ny, nz);
+}
+
 // Interpolation points of one traced step (Geometry::set_data,
 // geometry.tpp:100-207).  Returns the number of projected points the step
 // emits: 1 for a resolved velocity step, n_interpl for an interpolated one,

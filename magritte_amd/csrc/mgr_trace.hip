@@ -31,6 +31,71 @@ __device__ __forceinline__ void atomic_minmax_pos_double(uint64_t *addr, double 
     if (hi > 0.0) atomicMax((unsigned long long *)addr + 1, (unsigned long long)__double_as_longlong(hi));
 }
 
+// The walk of one (origin, ray): Geometry::trace_ray (geometry.tpp:21-95)
+// with set_data's interpolation split of every step (geometry.tpp:100-207).
+// The ray advances along (nx, ny, nz); shifts are co-moving,
+// 1 - (v_c - v_o).n, or for REST the rest-frame 1 - v_c.s with s the
+// direction of get_doppler_shift<Rest>.  FILL writes the steps from w on.
+struct Walk {
+    int32_t steps, points, last_cell;   // last_cell: cell of the last emitted point, -1: none
+    double  smin, smax, last_shift;     // shift range of the walk (origin included)
+};
+
+template <bool FILL, bool REST>
+__device__ __forceinline__ Walk trace_walk(int64_t N, const double *__restrict__ pos,
+                                           const double *__restrict__ vel,
+                                           const int64_t *__restrict__ nb_off,
+                                           const int32_t *__restrict__ nb,
+                                           const uint8_t *__restrict__ is_boundary, int32_t o,
+                                           double nx, double ny, double nz, double sx, double sy,
+                                           double sz, double dsm, int64_t w,
+                                           int32_t *__restrict__ seg_cell,
+                                           double *__restrict__ seg_shift, double *__restrict__ seg_dZ,
+                                           int4 *__restrict__ seg_f32, float *__restrict__ seg_hdz32,
+                                           int32_t *__restrict__ err_flag)
+{
+    Walk k;
+    k.steps = 0; k.points = 0; k.last_cell = -1;
+    k.smin = 1.0; k.smax = 1.0; k.last_shift = 1.0;
+    double Z = 0.0, dZ = 0.0;
+    int32_t nxt = get_next(pos, nb_off, nb, o, nx, ny, nz, o, Z, dZ);
+    if (nxt == -1) return k;
+    double shift_crt = REST ? rest_shift(vel, o, sx, sy, sz) : doppler_shift(vel, o, o, nx, ny, nz);
+    double shift_nxt = REST ? rest_shift(vel, nxt, sx, sy, sz) : doppler_shift(vel, o, nxt, nx, ny, nz);
+    k.smin = fmin(k.smin, shift_crt);
+    k.smax = fmax(k.smax, shift_crt);
+    for (;;) {
+        const StepSplit sp = split_step(shift_crt, shift_nxt, dZ, dsm);
+        k.points += sp.n;
+        k.smin = fmin(k.smin, shift_nxt);
+        k.smax = fmax(k.smax, shift_nxt);
+        if (FILL) {
+            seg_cell[w] = nxt;
+            seg_shift[w] = shift_nxt;
+            seg_dZ[w] = dZ;
+            // the FP32 kernel's split: sub-point m at shift offset s0 + m ds
+            const int half = sp.interp ? sp.half : 0;
+            const float s0 = (float)((sp.interp ? shift_crt : shift_nxt) - 1.0);
+            const float ds = (float)sp.dshift;
+            seg_f32[w] = make_int4(nxt, sp.n | (half << 16), __float_as_int(s0), __float_as_int(ds));
+            seg_hdz32[w] = (float)(sp.dZ * (0.5 / CHI_SCALE));
+            if (sp.n > 0) {   // sub-point m = n >= half belongs to the next cell
+                k.last_cell = nxt;
+                k.last_shift = sp.interp ? sub_shift(shift_crt, sp.n, sp.dshift) : shift_nxt;
+            }
+            w++;
+        }
+        k.steps++;
+        if (is_boundary[nxt]) break;
+        if (k.steps >= N) { atomicOr(err_flag, ERR_TOO_LONG); break; }
+        shift_crt = shift_nxt;
+        nxt = get_next(pos, nb_off, nb, o, nx, ny, nz, nxt, Z, dZ);
+        if (nxt < 0) { atomicOr(err_flag, ERR_NO_NEIGHBOUR); break; }
+        shift_nxt = REST ? rest_shift(vel, nxt, sx, sy, sz) : doppler_shift(vel, o, nxt, nx, ny, nz);
+    }
+    return k;
+}
+
 // One thread per (origin o, ray r < Rmax); r beyond the origin's own ray
 // count exits.  Entry q = cell_qbase[o] + r of the (origin, ray) arrays; the
 // ray direction is the global ray m_rbase[model] + r.
@@ -57,55 +122,57 @@ __global__ void k_trace(int64_t N, int Rmax, const int32_t *__restrict__ cell_mo
     const int64_t q = cell_qbase[o] + r;
     const int64_t rg = (int64_t)m_rbase[mdl] + r;
     const double nx = rays[3 * rg + 0], ny = rays[3 * rg + 1], nz = rays[3 * rg + 2];
-    const double dsm = dshift_max[o];
-    int64_t w = FILL ? seg_off[q] : 0;
-    int32_t steps = 0, points = 0;
-    double smin = 1.0, smax = 1.0;
-
-    double Z = 0.0, dZ = 0.0;
-    int32_t last_cell = -1;
-    double last_shift = 1.0;
-    int32_t nxt = get_next(pos, nb_off, nb, o, nx, ny, nz, o, Z, dZ);
-    if (nxt != -1) {
-        double shift_crt = doppler_shift(vel, o, o, nx, ny, nz);
-        double shift_nxt = doppler_shift(vel, o, nxt, nx, ny, nz);
-        for (;;) {
-            const StepSplit sp = split_step(shift_crt, shift_nxt, dZ, dsm);
-            points += sp.n;
-            smin = fmin(smin, shift_nxt);
-            smax = fmax(smax, shift_nxt);
-            if (FILL) {
-                seg_cell[w] = nxt;
-                seg_shift[w] = shift_nxt;
-                seg_dZ[w] = dZ;
-                // the FP32 kernel's split: sub-point m at shift offset s0 + m ds
-                const int half = sp.interp ? sp.half : 0;
-                const float s0 = (float)((sp.interp ? shift_crt : shift_nxt) - 1.0);
-                const float ds = (float)sp.dshift;
-                seg_f32[w] = make_int4(nxt, sp.n | (half << 16), __float_as_int(s0), __float_as_int(ds));
-                seg_hdz32[w] = (float)(sp.dZ * (0.5 / CHI_SCALE));
-                if (sp.n > 0) {   // sub-point m = n >= half belongs to the next cell
-                    last_cell = nxt;
-                    last_shift = sp.interp ? sub_shift(shift_crt, sp.n, sp.dshift) : shift_nxt;
-                }
-                w++;
-            }
-            steps++;
-            if (is_boundary[nxt]) break;
-            if (steps >= N) { atomicOr(err_flag, ERR_TOO_LONG); break; }
-            shift_crt = shift_nxt;
-            nxt = get_next(pos, nb_off, nb, o, nx, ny, nz, nxt, Z, dZ);
-            if (nxt < 0) { atomicOr(err_flag, ERR_NO_NEIGHBOUR); break; }
-            shift_nxt = doppler_shift(vel, o, nxt, nx, ny, nz);
-        }
-    }
+    const Walk k = trace_walk<FILL, false>(N, pos, vel, nb_off, nb, is_boundary, o, nx, ny, nz, nx, ny, nz,
+                                           dshift_max[o], FILL ? seg_off[q] : 0, seg_cell, seg_shift,
+                                           seg_dZ, seg_f32, seg_hdz32, err_flag);
     if (!FILL) {
-        nseg[q] = steps;
-        npts[q] = points;
-        atomic_minmax_pos_double(shift_bits, smin, smax);
+        nseg[q] = k.steps;
+        npts[q] = k.points;
+        atomic_minmax_pos_double(shift_bits, k.smin, k.smax);
     } else {
-        far_cell[q] = last_cell;
-        far_shift[q] = last_shift;
+        far_cell[q] = k.last_cell;
+        far_shift[q] = k.last_shift;
+    }
+}
+
+// Rest-frame walks of one model (Geometry::trace_ray<Rest>).  With r_img >= 0
+// the pair of an image: entry 2o the antipode of r_img, 2o + 1 the ray r_img
+// itself; with r_img < 0 every ray, entry o R + r (for npoints_on_rays<Rest>).
+// The rest-frame direction of ray r is r for r < R/2 and its antipode
+// otherwise (geometry.tpp:452-460).
+template <bool FILL>
+__global__ void k_trace_rest(int64_t N, int R, int r_img, const double *__restrict__ pos,
+                             const double *__restrict__ vel, const int64_t *__restrict__ nb_off,
+                             const int32_t *__restrict__ nb, const uint8_t *__restrict__ is_boundary,
+                             const double *__restrict__ rays, const int32_t *__restrict__ antipod,
+                             const double *__restrict__ dshift_max, int32_t *__restrict__ nseg,
+                             int32_t *__restrict__ npts, const int64_t *__restrict__ seg_off,
+                             int32_t *__restrict__ seg_cell, double *__restrict__ seg_shift,
+                             double *__restrict__ seg_dZ, int4 *__restrict__ seg_f32,
+                             float *__restrict__ seg_hdz32, int32_t *__restrict__ far_cell,
+                             double *__restrict__ far_shift, double *__restrict__ origin_shift,
+                             uint64_t *__restrict__ shift_bits, int32_t *__restrict__ err_flag)
+{
+    const int nlist = r_img >= 0 ? 2 : R;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * nlist) return;
+    const int32_t o = (int32_t)(t / nlist);
+    const int i = (int)(t % nlist);
+    const int ray = r_img >= 0 ? (i ? r_img : antipod[r_img]) : i;
+    const int rs = ray >= R / 2 ? antipod[ray] : ray;
+    const double nx = rays[3 * ray + 0], ny = rays[3 * ray + 1], nz = rays[3 * ray + 2];
+    const double sx = rays[3 * rs + 0], sy = rays[3 * rs + 1], sz = rays[3 * rs + 2];
+    const Walk k = trace_walk<FILL, true>(N, pos, vel, nb_off, nb, is_boundary, o, nx, ny, nz, sx, sy, sz,
+                                          dshift_max[o], FILL ? seg_off[t] : 0, seg_cell, seg_shift,
+                                          seg_dZ, seg_f32, seg_hdz32, err_flag);
+    if (!FILL) {
+        nseg[t] = k.steps;
+        npts[t] = k.points;
+        if (shift_bits) atomic_minmax_pos_double(shift_bits, k.smin, k.smax);
+    } else {
+        far_cell[t] = k.last_cell;
+        far_shift[t] = k.last_shift;
+        origin_shift[t] = rest_shift(vel, o, sx, sy, sz);
     }
 }
 
@@ -237,6 +304,88 @@ int launch_trace(mgr_ctx *ctx)
     return MGR_OK;
 }
 
+void free_image(mgr_ctx *ctx)
+{
+    mgr_ctx::ImageTrace &g = ctx->img;
+    for (void *p : {(void *)g.seg_off, (void *)g.nseg, (void *)g.npts, (void *)g.seg_cell, (void *)g.seg_shift,
+                    (void *)g.seg_dZ, (void *)g.seg_f32, (void *)g.seg_hdz32, (void *)g.far_cell,
+                    (void *)g.far_shift, (void *)g.origin_shift, (void *)g.I_m, (void *)g.I_p})
+        if (p) (void)hipFree(p);
+    g = mgr_ctx::ImageTrace();
+}
+
+static int rest_trace_error(mgr_ctx *ctx, int32_t errv)
+{
+    return ctx->fail(MGR_ERR_TRACE, errv & ERR_NO_NEIGHBOUR ? "a ray found no forward neighbour inside the mesh"
+                                                            : "a ray visited more steps than there are cells");
+}
+
+// Rest-frame trace of ray `ray` and its antipode for an image: counting
+// pass, host scan, filling pass into ctx->img (kept until another ray is
+// imaged; the geometry is static).
+int launch_trace_image(mgr_ctx *ctx, int64_t ray)
+{
+    if (ctx->img.ray == ray) return MGR_OK;
+    const int64_t N = ctx->N, R = ctx->R, NQ = 2 * N;
+    mgr_ctx::ImageTrace &g = ctx->img;
+    g.ray = -1;
+    if (!g.nseg) {
+        MGR_HIP(hipMalloc((void **)&g.nseg, NQ * sizeof(int32_t)));
+        MGR_HIP(hipMalloc((void **)&g.npts, NQ * sizeof(int32_t)));
+        MGR_HIP(hipMalloc((void **)&g.seg_off, (NQ + 1) * sizeof(int64_t)));
+        MGR_HIP(hipMalloc((void **)&g.far_cell, NQ * sizeof(int32_t)));
+        MGR_HIP(hipMalloc((void **)&g.far_shift, NQ * sizeof(double)));
+        MGR_HIP(hipMalloc((void **)&g.origin_shift, NQ * sizeof(double)));
+    }
+    const unsigned grid = (unsigned)((NQ + 127) / 128);
+    static const uint64_t one_bits[2] = {0x3FF0000000000000ull, 0x3FF0000000000000ull};   // 1.0, 1.0
+    MGR_HIP(hipMemcpyAsync(ctx->shift_bits, one_bits, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+    MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
+    hipLaunchKernelGGL(k_trace_rest<false>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, (int)ray, ctx->pos,
+                       ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->d_antipod,
+                       ctx->dshift_max, g.nseg, g.npts, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, ctx->shift_bits, ctx->err_flag);
+    MGR_LAUNCH_CHECK();
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    int32_t errv = 0;
+    MGR_HIP(hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (errv) return rest_trace_error(ctx, errv);
+    std::vector<int32_t> ns(NQ);
+    uint64_t sbits[2] = {0, 0};
+    MGR_HIP(hipMemcpy(ns.data(), g.nseg, NQ * sizeof(int32_t), hipMemcpyDeviceToHost));
+    MGR_HIP(hipMemcpy(sbits, ctx->shift_bits, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    std::vector<int64_t> off(NQ + 1, 0);
+    for (int64_t q = 0; q < NQ; q++) off[q + 1] = off[q] + ns[q];
+    double smin, smax;
+    std::memcpy(&smin, &sbits[0], sizeof(double));
+    std::memcpy(&smax, &sbits[1], sizeof(double));
+    g.shift_min = std::min(1.0, smin);
+    g.shift_max = std::max(1.0, smax);
+    if (off[NQ] > g.total_segments || !g.seg_cell) {
+        for (void *p : {(void *)g.seg_cell, (void *)g.seg_shift, (void *)g.seg_dZ, (void *)g.seg_f32,
+                        (void *)g.seg_hdz32})
+            if (p) (void)hipFree(p);
+        g.seg_cell = nullptr; g.seg_shift = nullptr; g.seg_dZ = nullptr; g.seg_f32 = nullptr; g.seg_hdz32 = nullptr;
+        const int64_t T = std::max<int64_t>(off[NQ], 1);
+        MGR_HIP(hipMalloc((void **)&g.seg_cell, T * sizeof(int32_t)));
+        MGR_HIP(hipMalloc((void **)&g.seg_shift, T * sizeof(double)));
+        MGR_HIP(hipMalloc((void **)&g.seg_dZ, T * sizeof(double)));
+        MGR_HIP(hipMalloc((void **)&g.seg_f32, T * sizeof(int4)));
+        MGR_HIP(hipMalloc((void **)&g.seg_hdz32, T * sizeof(float)));
+        g.total_segments = T;
+    }
+    MGR_HIP(hipMemcpy(g.seg_off, off.data(), (NQ + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_trace_rest<true>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, (int)ray, ctx->pos,
+                       ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->d_antipod,
+                       ctx->dshift_max, nullptr, nullptr, g.seg_off, g.seg_cell, g.seg_shift, g.seg_dZ,
+                       g.seg_f32, g.seg_hdz32, g.far_cell, g.far_shift, g.origin_shift, nullptr,
+                       ctx->err_flag);
+    MGR_LAUNCH_CHECK();
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    g.ray = ray;
+    return MGR_OK;
+}
+
 }  // namespace mgr
 
 extern "C" {
@@ -286,6 +435,42 @@ int mgr_get_ray(mgr_ctx *ctx, int64_t origin, int64_t ray, int64_t capacity, int
     }
     (void)hipFree(dc); (void)hipFree(ds); (void)hipFree(dz); (void)hipFree(dn);
     return rc;
+}
+
+int mgr_get_npoints_on_rays_rest(mgr_ctx *ctx, int64_t *npoints)
+{
+    if (!ctx || !npoints) return MGR_ERR_ARGUMENT;
+    if (ctx->M != 1) return ctx->fail(MGR_ERR_UNSUPPORTED, "rest-frame traces are per model (batch of one)");
+    MGR_HIP(hipSetDevice(ctx->device));
+    const int64_t N = ctx->N, R = ctx->R, NQ = N * R;
+    int32_t *d_ns = nullptr, *d_np = nullptr;
+    MGR_HIP(hipMalloc((void **)&d_ns, std::max<int64_t>(NQ, 1) * sizeof(int32_t)));
+    if (hipMalloc((void **)&d_np, std::max<int64_t>(NQ, 1) * sizeof(int32_t)) != hipSuccess) {
+        (void)hipFree(d_ns);
+        return ctx->fail(MGR_ERR_DEVICE, "out of device memory");
+    }
+    int rc = MGR_OK;
+    std::vector<int32_t> np(NQ);
+    int32_t errv = 0;
+    if (hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream) != hipSuccess) rc = MGR_ERR_DEVICE;
+    if (!rc) {
+        hipLaunchKernelGGL(k_trace_rest<false>, dim3((unsigned)((NQ + 127) / 128)), dim3(128), 0, ctx->stream, N,
+                           (int)R, -1, ctx->pos, ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays,
+                           ctx->d_antipod, ctx->dshift_max, d_ns, d_np, nullptr, nullptr, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->err_flag);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess ||
+            hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(np.data(), d_np, NQ * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = MGR_ERR_DEVICE;
+    }
+    (void)hipFree(d_ns);
+    (void)hipFree(d_np);
+    if (rc) return ctx->fail(rc, "rest-frame ray count failed");
+    if (errv) return rest_trace_error(ctx, errv);
+    for (int64_t o = 0; o < N; o++)
+        for (int64_t r = 0; r < R / 2; r++)
+            npoints[r * N + o] = (int64_t)np[o * R + r] + np[o * R + ctx->antipod[r]] + 1;
+    return MGR_OK;
 }
 
 int mgr_get_npoints_on_rays(mgr_ctx *ctx, int64_t *npoints)

@@ -52,6 +52,9 @@ SIGNATURES = {
     "mgr_get_emissivity_opacity": ([_vp, _vp, _vp], _i),
     "mgr_get_ray": ([_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp], _i),
     "mgr_get_npoints_on_rays": ([_vp, _vp], _i),
+    "mgr_get_npoints_on_rays_rest": ([_vp, _vp], _i),
+    "mgr_compute_spectral_discretisation_image": ([_vp, _d], _i),
+    "mgr_compute_image": ([_vp, _i64, _i, _vp, _vp], _i),
     "mgr_last_radiation_field_ms": ([_vp, _vp], _i),
     "mgr_trace_stats": ([_vp, _vp, _vp], _i),
     "mgr_radiation_field_shape": ([_vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp], _i),
@@ -139,6 +142,10 @@ class Context:
     # ---- set-up -------------------------------------------------------------
     def compute_spectral_discretisation(self):
         self._check(self.lib.mgr_compute_spectral_discretisation(self.h), "compute_spectral_discretisation")
+
+    def compute_spectral_discretisation_image(self, width):
+        self._check(self.lib.mgr_compute_spectral_discretisation_image(self.h, float(width)),
+                    "compute_spectral_discretisation_image")
 
     def compute_boundary_intensities(self, temperatures=None):
         t = None
@@ -258,6 +265,19 @@ class Context:
         a = np.zeros((self.R // 2) * self.N, np.int64)
         self._check(self.lib.mgr_get_npoints_on_rays(self.h, _p(a)), "get_npoints_on_rays")
         return a.reshape(self.R // 2, self.N)
+
+    def npoints_on_rays_rest(self):
+        a = np.zeros((self.R // 2) * self.N, np.int64)
+        self._check(self.lib.mgr_get_npoints_on_rays_rest(self.h, _p(a)), "get_npoints_on_rays_rest")
+        return a.reshape(self.R // 2, self.N)
+
+    def compute_image(self, ray, precision="fp64"):
+        """(I_m, I_p), each [ncells, nfreqs], of the image along ray < nrays/2."""
+        I_m = np.zeros((self.N, self.F))
+        I_p = np.zeros((self.N, self.F))
+        self._check(self.lib.mgr_compute_image(self.h, int(ray), _prec(precision), _p(I_m), _p(I_p)),
+                    "compute_image")
+        return I_m, I_p
 
     def last_radiation_field_ms(self):
         v = C.c_double(0.0)

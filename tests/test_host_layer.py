@@ -124,3 +124,66 @@ def test_simulation_matches_oracle(tmp_path):
     # per-iteration population files, tag "_rank_0_iteration_<k>" (sim_lines.tpp:149-151)
     written = os.listdir(str(tmp_path) + "/Lines/LineProducingSpecies_0")
     assert f"population_rank_0_iteration_{it}.txt" in written
+
+
+def _read_text_array(path):
+    return np.loadtxt(path, ndmin=2)
+
+
+def test_image_coordinates_and_write(tmp_path):
+    """Image::set_coordinates (image.cpp:93-132) puts each cell on the plane
+    normal to the ray: with the ray's unit vector n, ImX^2 + ImY^2 + (x.n)^2
+    = |x|^2, and ImX is orthogonal to z; Image::write (image.cpp:48-86) uses
+    Simulation/Image/{ImX,ImY,I_m,I_p}_<ray>."""
+    m = _small()
+    sim = to_simulation(m)
+    r = 2
+    img = core.Image(r, sim.parameters)
+    img.set_coordinates(sim.geometry)
+    x, y = np.array(img.ImX), np.array(img.ImY)
+    n = np.asarray(m.rays[r])
+    pos = np.asarray(m.position)
+    np.testing.assert_allclose(x ** 2 + y ** 2 + (pos @ n) ** 2, np.sum(pos ** 2, axis=1), rtol=1e-12)
+    d = np.hypot(n[0], n[1])
+    np.testing.assert_allclose(x, (n[1] * pos[:, 0] - n[0] * pos[:, 1]) / d, rtol=1e-13)
+    img.write(core.IoText(str(tmp_path) + "/"))
+    base = str(tmp_path) + "/Simulation/Image/"
+    np.testing.assert_array_equal(np.loadtxt(base + f"ImX_{r}.txt"), x)
+    Im = _read_text_array(base + f"I_m_{r}.txt")
+    assert Im.shape == (m.ncells, m.nfreqs) and not Im.any()
+
+
+def test_image_without_image_discretisation_returns_minus_one(tmp_path):
+    sim = to_simulation(_small())
+    assert sim.compute_and_write_image(core.IoText(str(tmp_path) + "/"), 0) == -1
+
+
+@pytest.mark.gpu
+def test_compute_and_write_image_matches_oracle(tmp_path):
+    from oracle.py_oracle import Oracle
+    m = synthetic.envelope_model(n=8, seed=7)
+    print("\n[host] envelope8 seed 7: compute_and_write_image vs oracle", flush=True)
+    width = 3.0e-5
+    orc = Oracle(m)
+    orc.compute_spectral_discretisation_image(width)
+    orc.compute_boundary_intensities()
+    orc.compute_LTE_level_populations()
+    sim = to_simulation(m)
+    assert sim.compute_spectral_discretisation_image(width) == 0
+    assert sim.compute_boundary_intensities() == 0
+    assert sim.compute_LTE_level_populations() == 0
+    io = core.IoText(str(tmp_path) + "/")
+    assert sim.compute_level_populations(io, True, 5) == -1   # needs the line grid
+    r = 3
+    assert sim.compute_and_write_image(io, r) == 0
+    base = str(tmp_path) + "/Simulation/Image/"
+    Im_o, Ip_o = orc.compute_image(r)
+    for name, ref in (("I_m", Im_o), ("I_p", Ip_o)):
+        got = _read_text_array(base + f"{name}_{r}.txt")
+        assert got.shape == ref.shape
+        assert float(np.max(np.abs(got - ref) / np.abs(ref))) < 1e-10, name
+    assert os.path.exists(base + f"ImY_{r}.txt")
+    np.testing.assert_array_equal(np.array(sim.get_npoints_on_rays_rest())[r, :5],
+                                  [len(orc.trace_ray(o, r, rest=True)[1]) +
+                                   len(orc.trace_ray(o, int(orc.antipod()[r]), rest=True)[1]) + 1
+                                   for o in range(5)])
