@@ -43,6 +43,10 @@ constexpr double CHI_SCALE = 73786976294838206464.0;     // 2^66 ~ 7.4e19
 constexpr double I_SCALE   = 1125899906842624.0;         // 2^50 ~ 1.1e15
 
 constexpr double SQRT_LOG2E = 1.2011224087864498;   // sqrt(log2(e))
+constexpr double SQRT2      = 1.4142135623730951;
+// Path-length factor of the solve's scaled interval c = dtau / sqrt(2):
+// c = (chi_a + chi_b) * dZ * HDZ_SCALE with chi scaled by CHI_SCALE.
+constexpr double HDZ_SCALE  = 0.35355339059327373 / CHI_SCALE;   // 1 / (2 sqrt 2) / CHI_SCALE
 
 constexpr int WAVE = 64;                 // CDNA wavefront width
 constexpr int MAX_INTERPOLATION = 10000; // geometry.tpp:138
@@ -197,7 +201,7 @@ struct mgr_ctx {
     int32_t *seg_cell = nullptr;    // [total segments]
     double  *seg_shift = nullptr, *seg_dZ = nullptr;
     // the FP32 kernel's view of every step, split once at trace time:
-    // (cell, n | half << 16, shift offset s0, shift increment) and dZ/2 (scaled)
+    // (cell, n | half << 16, shift offset s0, shift increment) and dZ * HDZ_SCALE
     int4    *seg_f32 = nullptr;
     float   *seg_hdz32 = nullptr;
     // last emitted point of every (origin, ray): its cell (-1: none) and FP64 shift

@@ -38,6 +38,14 @@
 // boundary cell) enters as the virtual start: phi q = 1, zeta = I, b = 0.
 // phi stays below 1 + tau and q in (0, 1], so the FP32 variant neither
 // overflows nor loses the optically thin limit.
+// The kernels carry these in scaled variables, c = b/sqrt2, x = sqrt2 phi,
+// z = sqrt2 zeta, y = x q + c and p = S c_prev + z_prev, in which
+//   x_n = c_n + y_{n-1},  q_n = 1/(1 + c_n x_n),  y_n = x_n q_n + c_n,
+//   z_n = (S_n c_n + p_n) q_n,
+// so that no constant factor and no value from two points back is needed
+// (no register copies around the loop); the virtual start is y = sqrt2,
+// z = p = sqrt2 I, c = 0, and at the origin
+//   u = (p_A + p_B) / (y_A + y_B),  Lambda = (c_A + c_B) / (y_A + y_B).
 //
 // Line profiles.  FP64 variant: the window of get_eta_and_chi is decided in
 // FP64 exactly as the reference.  FP32 variant: each lane evaluates the lines
@@ -78,11 +86,11 @@ template <> struct Prec<double> {
 
 // One traced step in LDS.  Sub-point m = 1..n of the step lies at
 // shift = s0 + m ds (FP64: s0 = shift of the current point, FP32: s0 and ds
-// are shift - 1 and its increment) with half path length dz (scaled) and
+// are shift - 1 and its increment) with path-length factor dz = dZ HDZ_SCALE and
 // belongs to the current cell for m < half, to `cell` otherwise.
 template <typename real>
 struct Step {
-    int32_t cell, n, half, pad;
+    int32_t cell, nh, pad0, pad1;   // nh = n | half << 16
     real    s0, ds, dz, pad2;
 };
 
@@ -105,7 +113,7 @@ struct RFArgs {
     const int32_t *nseg, *npts, *seg_cell;
     const double  *seg_shift, *seg_dZ;
     const int4    *seg_f32;                   // FP32 split of every step (cell, n | half << 16, s0, ds)
-    const float   *seg_hdz32;                 // its dZ/2 (scaled)
+    const float   *seg_hdz32;                 // its dZ * HDZ_SCALE
     const int32_t *far_cell;                  // last emitted point of each (origin, ray), -1: none
     const double  *far_shift;
     const double  *origin_shift;              // FP64 shift of the origin per entry (null: 1, co-moving)
@@ -266,37 +274,51 @@ __device__ __forceinline__ double rescale_I_bdy(const double *__restrict__ nup,
 
 template <typename real>
 struct Sweep {
-    real S_q, chi_q, hdz_q;   // pending (outer) point: source function, opacity, dz/2
-    real phi, q, zeta, bh;    // elimination state; bh = half the last closed dtau
+    real S_q, chi_q, hdz_q, cq;   // pending (outer) point: S, chi, its dz' and chi dz'
+    real c, y, z, p;              // last closed interval and the scaled state
 
+    // Virtual start.  With hdz_q = cq = 0 a step closes nothing: c = 0,
+    // x = y, q = 1 exactly, so the first point may be stepped like any other.
     __device__ __forceinline__ void init(real I_far)
     {
-        S_q = chi_q = hdz_q = 0;
-        phi = 1;
-        q = 1;
-        zeta = I_far;
-        bh = 0;
+        S_q = chi_q = hdz_q = cq = 0;
+        c = 0;
+        y = (real)SQRT2;
+        z = (real)SQRT2 * I_far;
+        p = z;
     }
 
     __device__ __forceinline__ void pend(real S, real chi, real hdz)
     {
+        p = S * c + z;
         S_q = S;
         chi_q = chi;
         hdz_q = hdz;
+        cq = chi * hdz;
     }
 
-    // close the pending row with the next point (S, chi) inwards
-    __device__ __forceinline__ void step(real S, real chi)
+    // close the pending row with the next point (S, chi) inwards; the
+    // interval takes the pending point's dz'
+    __device__ __forceinline__ void step(real chi)
     {
-        const real b = (chi_q + chi) * hdz_q;
-        const real h = b * (real)0.5 + bh;
-        bh = b * (real)0.5;
-        phi = phi * q + h;
-        q = Prec<real>::rcp(b * phi + (real)1);
-        zeta = (S_q * h + zeta) * q;
+        c = chi * hdz_q + cq;
+        close_row();
     }
 
-    __device__ __forceinline__ real fq() const { return phi * q; }
+    // outward sweeps: the interval takes the new (outer) point's dz'
+    __device__ __forceinline__ void step_out(real chi, real hdz)
+    {
+        c = (chi_q + chi) * hdz;
+        close_row();
+    }
+
+    __device__ __forceinline__ void close_row()
+    {
+        const real x = c + y;
+        const real q = Prec<real>::rcp(c * x + (real)1);
+        y = x * q + c;
+        z = (S_q * c + p) * q;
+    }
 };
 
 // ---- staging ----------------------------------------------------------------
@@ -313,16 +335,16 @@ __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, in
     for (int e = threadIdx.x; e < cnt; e += nthr) {
         const int j = j0 + e;
         step_t<real, X64> s;
-        s.pad = 0;
+        s.pad0 = 0;
+        s.pad1 = 0;
         s.pad2 = 0;
         if (j < 0) {
-            s.cell = o; s.n = 0; s.half = 0; s.s0 = 0; s.ds = 0; s.dz = 0;
+            s.cell = o; s.nh = 0; s.s0 = 0; s.ds = 0; s.dz = 0;
         } else if constexpr (sizeof(real) == 4 && !X64) {
             // split at trace time with the same FP64 arithmetic as below
             const int4 v = a.seg_f32[s0 + j];
             s.cell = v.x;
-            s.n = v.y & 0xFFFF;
-            s.half = (int)((unsigned)v.y >> 16);
+            s.nh = v.y;
             s.s0 = __int_as_float(v.z);
             s.ds = __int_as_float(v.w);
             s.dz = a.seg_hdz32[s0 + j];
@@ -333,11 +355,10 @@ __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, in
             const double sc = (j > 0) ? a.seg_shift[s0 + j - 1] : (a.origin_shift ? a.origin_shift[q] : 1.0);
             const StepSplit sp = split_step(sc, sn, z, dsm);
             s.cell = c;
-            s.n = sp.n;
-            s.half = sp.interp ? sp.half : 0;
+            s.nh = sp.n | ((sp.interp ? sp.half : 0) << 16);
             s.s0 = sp.interp ? sc : sn;               // absolute FP64 shift
             s.ds = sp.dshift;
-            s.dz = sp.dZ * (0.5 / CHI_SCALE);         // dz/2
+            s.dz = sp.dZ * HDZ_SCALE;                 // dz'
         }
         st[e] = s;
     }
@@ -440,7 +461,7 @@ __device__ __forceinline__ double far_intensity(const RFArgs<real> &a, int64_t q
 }
 
 // One ray of a pair, eliminated from its boundary end (incoming intensity
-// I_far, scaled) inwards to the origin.
+// I_far, scaled) inwards to the origin, which it leaves pending.
 template <typename real, bool X64 = false>
 __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const LaneCtx<real> &lc,
                                                   int32_t o, int64_t q, real I_far, real S_o,
@@ -451,7 +472,6 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
     using row_t = typename Prec<real>::row_t;
     Sweep<real> sw;
     sw.init(I_far);
-    bool have_q = false;   // wave-uniform
     const int ns = a.nseg[q];
     const int L = a.L;
     const int CH = a.ch;
@@ -461,10 +481,11 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
         __syncthreads();   // the previous chunk is consumed by every wave
         stage_chunk<real, X64>(a, o, q, a_ - 1, cnt, dsm, st, rw);
         for (int e = cnt - 1; e >= 1; --e) {
-            const int n = rfl_int(st[e].n);
-            const int half = rfl_int(st[e].half);
+            const int nh = rfl_int(st[e].nh);   // one readfirstlane: n and half are wave-uniform
+            const int n = nh & 0xFFFF;
+            const int half = (int)((unsigned)nh >> 16);
             const auto s0 = st[e].s0;   // stays in a VGPR: one scalar operand per FMA
-            const auto ds = rfl_real(st[e].ds);
+            const auto ds = st[e].ds;
             const real hdz = (real)rfl_real(st[e].dz);
             // sub-points m >= half lie in the step's cell (row e), m < half in
             // the previous one (row e-1); two loops keep each row's address
@@ -476,21 +497,20 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
                 real eta, chi;
                 eval_at<real, X64>(a, lc, rn, sub_point_shift(s0, m, ds), eta, chi);
                 const real S = eta * Prec<real>::rcp(chi);
-                if (have_q) sw.step(S, chi);
-                have_q = true;
+                sw.step(chi);   // on the first point an exact no-op (c = 0, q = 1)
                 sw.pend(S, chi, hdz);
             }
             for (int m = (half - 1 < n ? half - 1 : n); m >= 1; --m) {
                 real eta, chi;
                 eval_at<real, X64>(a, lc, rc, sub_point_shift(s0, m, ds), eta, chi);
                 const real S = eta * Prec<real>::rcp(chi);
-                if (have_q) sw.step(S, chi);
-                have_q = true;
+                sw.step(chi);
                 sw.pend(S, chi, hdz);
             }
         }
     }
-    sw.step(S_o, chi_o);   // the origin closes the innermost row
+    sw.step(chi_o);   // the origin closes the innermost row and is left pending
+    sw.pend(S_o, chi_o, (real)0);
     return sw;
 }
 
@@ -515,10 +535,11 @@ __device__ __forceinline__ void sweep_outward(const RFArgs<real> &a, const LaneC
         __syncthreads();   // the previous chunk is consumed by every wave
         stage_chunk<real, X64>(a, o, q, a_ - 1, cnt, dsm, st, rw);
         for (int e = 1; e < cnt; ++e) {
-            const int n = rfl_int(st[e].n);
-            const int half = rfl_int(st[e].half);
+            const int nh = rfl_int(st[e].nh);   // one readfirstlane: n and half are wave-uniform
+            const int n = nh & 0xFFFF;
+            const int half = (int)((unsigned)nh >> 16);
             const auto s0 = st[e].s0;
-            const auto ds = rfl_real(st[e].ds);
+            const auto ds = st[e].ds;
             const real hdz = (real)rfl_real(st[e].dz);
             const row_t *rn = rw + e * L;
             const row_t *rc = rw + (e - 1) * L;
@@ -527,16 +548,14 @@ __device__ __forceinline__ void sweep_outward(const RFArgs<real> &a, const LaneC
                 real eta, chi;
                 eval_at<real, X64>(a, lc, rc, sub_point_shift(s0, m, ds), eta, chi);
                 const real S = eta * Prec<real>::rcp(chi);
-                sw.hdz_q = hdz;
-                sw.step(S, chi);
+                sw.step_out(chi, hdz);
                 sw.pend(S, chi, hdz);
             }
             for (int m = half > 1 ? half : 1; m <= n; ++m) {
                 real eta, chi;
                 eval_at<real, X64>(a, lc, rn, sub_point_shift(s0, m, ds), eta, chi);
                 const real S = eta * Prec<real>::rcp(chi);
-                sw.hdz_q = hdz;
-                sw.step(S, chi);
+                sw.step_out(chi, hdz);
                 sw.pend(S, chi, hdz);
             }
         }
@@ -549,9 +568,7 @@ __device__ __forceinline__ void sweep_outward(const RFArgs<real> &a, const LaneC
 template <typename real>
 __device__ __forceinline__ real end_u(const Sweep<real> &sw, real I)
 {
-    const real h = sw.bh;
-    const real den = Prec<real>::rcp(h + (real)1 + sw.fq());
-    return (sw.S_q * h + I + sw.zeta) * den;
+    return (sw.p + (real)SQRT2 * I) * Prec<real>::rcp(sw.y + (real)SQRT2);
 }
 
 // Per-lane constants of origin o's bin and its own emissivity and opacity at
@@ -654,10 +671,9 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
             const Sweep<real> A = sweep_side<real>(a, lc, o, qa, Ia, S_o, chi_o, dsm, st, rw);
             const real Ir = (real)(far_intensity(a, qr, nu_f, F, err) * I_SCALE);
             const Sweep<real> B = sweep_side<real>(a, lc, o, qr, Ir, S_o, chi_o, dsm, st, rw);
-            const real h = A.bh + B.bh;
-            const real den = Prec<real>::rcp(h + A.fq() + B.fq());
-            u = (S_o * h + A.zeta + B.zeta) * den;
-            lam = h * den;
+            const real den = Prec<real>::rcp(A.y + B.y);
+            u = (A.p + B.p) * den;
+            lam = (A.c + B.c) * den;
         } else {
             const int64_t qs = (n_a > 0) ? qa : qr;
             const real Is = (real)(far_intensity(a, qs, nu_f, F, err) * I_SCALE);
@@ -665,10 +681,9 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
             real I_o = 0;
             if (bo < 0) err |= ERR_EMPTY_SIDE;
             else I_o = (real)(a.I_bdy[(int64_t)bo * F + fa] * I_SCALE);
-            const real h = A.bh;
-            const real den = Prec<real>::rcp(h + (real)1 + A.fq());
-            u = (S_o * h + I_o + A.zeta) * den;
-            lam = h * den;
+            const real den = Prec<real>::rcp(A.y + (real)SQRT2);
+            u = (A.p + (real)SQRT2 * I_o) * den;
+            lam = A.c * den;
         }
         J_acc += w2 * ((double)u * (1.0 / I_SCALE));
         Lam_acc += w2 * (double)lam;
@@ -743,18 +758,24 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_image(RFArgs<real> a, doub
         {
             // from the antipode's far end through the origin to ray r's end
             Sweep<real> A;
-            if (n_a > 0) A = sweep_side<real, X64>(a, lc, o, qa, I0s, S_o, chi_o, dsm, st, rw);
-            else A.init(I0s);
-            A.pend(S_o, chi_o, (real)0);
+            if (n_a > 0) {
+                A = sweep_side<real, X64>(a, lc, o, qa, I0s, S_o, chi_o, dsm, st, rw);
+            } else {
+                A.init(I0s);
+                A.pend(S_o, chi_o, (real)0);
+            }
             sweep_outward<real, X64>(a, lc, o, qr, dsm, A, st, rw);
             u_n = end_u(A, Ins);
         }
         {
             // and back
             Sweep<real> B;
-            if (n_r > 0) B = sweep_side<real, X64>(a, lc, o, qr, Ins, S_o, chi_o, dsm, st, rw);
-            else B.init(Ins);
-            B.pend(S_o, chi_o, (real)0);
+            if (n_r > 0) {
+                B = sweep_side<real, X64>(a, lc, o, qr, Ins, S_o, chi_o, dsm, st, rw);
+            } else {
+                B.init(Ins);
+                B.pend(S_o, chi_o, (real)0);
+            }
             sweep_outward<real, X64>(a, lc, o, qa, dsm, B, st, rw);
             u_0 = end_u(B, I0s);
         }

@@ -78,7 +78,7 @@ __device__ __forceinline__ Walk trace_walk(int64_t N, const double *__restrict__
             const float s0 = (float)((sp.interp ? shift_crt : shift_nxt) - 1.0);
             const float ds = (float)sp.dshift;
             seg_f32[w] = make_int4(nxt, sp.n | (half << 16), __float_as_int(s0), __float_as_int(ds));
-            seg_hdz32[w] = (float)(sp.dZ * (0.5 / CHI_SCALE));
+            seg_hdz32[w] = (float)(sp.dZ * HDZ_SCALE);
             if (sp.n > 0) {   // sub-point m = n >= half belongs to the next cell
                 k.last_cell = nxt;
                 k.last_shift = sp.interp ? sub_shift(shift_crt, sp.n, sp.dshift) : shift_nxt;
