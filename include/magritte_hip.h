@@ -184,6 +184,13 @@ int mgr_trace(mgr_ctx *ctx);
  * J(p,f) = sum_r 2 w_r u_r(p,f) and the Lambda* diagonal, n_off_diag = 0.     */
 int mgr_compute_radiation_field(mgr_ctx *ctx, int precision);
 
+/* Parameters::use_scattering: Radiation::u (src/Model/Radiation/radiation.cpp:117-126).
+ * With scattering on, every radiation field also keeps each ray pair's u at
+ * the origin (sim_radiation.tpp:226-237, 258-262), [nrays/2][ncells*nfreqs]
+ * doubles at (r*ncells + p)*nfreqs + f, read with mgr_get_u.                 */
+int mgr_set_scattering(mgr_ctx *ctx, int use_scattering);
+int mgr_get_u(mgr_ctx *ctx, double *u);
+
 /* Simulation::compute_Jeff (src/Simulation/sim_lines.tpp:176-220)             */
 int mgr_compute_Jeff(mgr_ctx *ctx);
 

@@ -457,6 +457,11 @@ void Radiation::read(const Io &io, Parameters &parameters)
     frequencies.corresponding_z_for_line.assign(nfreqs, (long)nfreqs);
     J.assign(ncells * nfreqs, 0.0);
     I_bdy.assign(nrays_red, Double2(parameters.nboundary(), Double1(nfreqs, 0.0)));
+    // u of every reduced ray with scattering (radiation.cpp:117-126)
+    if (parameters.use_scattering_.is_set && parameters.use_scattering())
+        u.assign(nrays_red, Double1(ncells * nfreqs, 0.0));
+    else
+        u.clear();
     (void)io;
 }
 

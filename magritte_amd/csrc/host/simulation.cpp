@@ -1,6 +1,7 @@
 // simulation.cpp -- bodies of the drop-in Simulation methods over the C-ABI.
 #include "simulation.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <stdexcept>
 
@@ -138,6 +139,12 @@ void Simulation::ensure_context()
         ctx_ = nullptr;
         throw std::runtime_error(std::string("mgr_create failed: ") + mgr_last_error(nullptr));
     }
+    if (use_scattering()) check(mgr_set_scattering(ctx_, 1), "set_scattering");
+}
+
+bool Simulation::use_scattering() const
+{
+    return parameters.use_scattering_.is_set && parameters.use_scattering();
 }
 
 // Simulation::compute_spectral_discretisation (sim_spectrum.tpp:13-196)
@@ -291,6 +298,15 @@ void Simulation::pull_radiation()
                 ls.lambda.nr[p][k] = {(long)p};
                 ls.lambda.size[k + K * p] = 1;
             }
+    // Radiation::u with scattering (radiation.cpp:117-126), u[R][f + p*F]
+    if (use_scattering()) {
+        const size_t Rr = geometry.rays.rays.size() / 2;
+        Double1 u(Rr * N * F);
+        check(mgr_get_u(ctx_, u.data()), "get_u");
+        radiation.u.assign(Rr, Double1(N * F));
+        for (size_t r = 0; r < Rr; r++)
+            std::copy(u.begin() + r * N * F, u.begin() + (r + 1) * N * F, radiation.u[r].begin());
+    }
 }
 
 void Simulation::pull_jeff()

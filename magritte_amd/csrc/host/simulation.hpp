@@ -71,6 +71,7 @@ struct Simulation : public Model {
     void pull_jeff();
     void pull_boundary_intensities();
     void pull_spectral();
+    bool use_scattering() const;
     double pop_prec() const { return parameters.pop_prec_.is_set ? parameters.pop_prec() : 1.0e-6; }
     int run_radiation_field(int prec);
 };

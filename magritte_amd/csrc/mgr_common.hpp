@@ -184,6 +184,9 @@ struct mgr_ctx {
     double  *Lam = nullptr;         // [N*F]  sum_r 2 w_r Lambda_ar / chi
     double  *Ls = nullptr;          // [N*L]
     double  *Jlin = nullptr, *Jeff = nullptr, *Jdif = nullptr;  // [N*L]
+    // Radiation::u with scattering: each ray pair's u at the origin,
+    // [R/2][N*F] at (r N + p) F + f; null without scattering
+    double  *u_store = nullptr;
     int32_t *err_flag = nullptr;    // device error word
     bool     J_set = false;
     float    last_rf_ms = 0.0f;

@@ -600,6 +600,7 @@ int free_model(mgr_ctx *ctx)
     dfree(ctx->d_m_active); dfree(ctx->d_chunk_cell); dfree(ctx->d_m_chunk_off); dfree(ctx->d_mconv);
     dfree(ctx->d_m_cell_off);
     dfree(ctx->d_mng);
+    dfree(ctx->u_store);
     free_image(ctx);
     if (ctx->h_reduced) (void)hipHostFree(ctx->h_reduced);
     if (ctx->h_m_active) (void)hipHostFree(ctx->h_m_active);

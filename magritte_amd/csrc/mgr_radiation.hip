@@ -126,6 +126,7 @@ struct RFArgs {
     double         cand_lo, cand_hi;          // candidate-line bounds (FP32 variant)
     double         lstart_threshold;          // first line that can be in a window (FP64 variant)
     double        *J, *Lam;                   // [N*F]
+    double        *u_out;                     // [R/2][N*F] per-pair u (scattering), or null
     int32_t       *err_flag;
 };
 
@@ -662,7 +663,9 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
         if (n_r == 0 && n_a == 0) {
             // ndep == 1: u = I_bdy of the origin's boundary entry (sim_radiation.tpp:243-264)
             if (bo < 0) { err |= ERR_EMPTY_SIDE; continue; }
-            J_acc += w2 * a.I_bdy[(int64_t)bo * F + fa];
+            const double Ib = a.I_bdy[(int64_t)bo * F + fa];
+            J_acc += w2 * Ib;
+            if (a.u_out && active) a.u_out[((int64_t)r * a.N + o) * F + f] = Ib;
             continue;
         }
         real u, lam;
@@ -685,8 +688,10 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
             u = (A.p + (real)SQRT2 * I_o) * den;
             lam = A.c * den;
         }
-        J_acc += w2 * ((double)u * (1.0 / I_SCALE));
+        const double u64 = (double)u * (1.0 / I_SCALE);
+        J_acc += w2 * u64;
         Lam_acc += w2 * (double)lam;
+        if (a.u_out && active) a.u_out[((int64_t)r * a.N + o) * F + f] = u64;   // radiation.u[R][ind]
     }
     if (err) atomicOr(a.err_flag, err);
     if (active) {
@@ -881,6 +886,7 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     a.far_cell = ctx->far_cell;
     a.far_shift = ctx->far_shift;
     a.origin_shift = nullptr;
+    a.u_out = ctx->u_store;
     hipLaunchKernelGGL(k_radiation_field<real>, dim3(grid_of(a)), dim3(WAVE * a.nwg), lds_bytes(ctx, a),
                        ctx->stream, a);
     MGR_LAUNCH_CHECK();
@@ -1095,5 +1101,37 @@ extern "C" int mgr_compute_image(mgr_ctx *ctx, int64_t ray, int precision, doubl
     if (errv) return ctx->fail(MGR_ERR_TRACE, "an image ray ended off the boundary or an interior origin has an empty ray");
     if (I_m) MGR_HIP(hipMemcpy(I_m, g.I_m, NF * sizeof(double), hipMemcpyDeviceToHost));
     if (I_p) MGR_HIP(hipMemcpy(I_p, g.I_p, NF * sizeof(double), hipMemcpyDeviceToHost));
+    return MGR_OK;
+}
+
+// Radiation::u (radiation.cpp:117-126): with scattering every radiation
+// field keeps each ray pair's u at the origin (sim_radiation.tpp:226-237,
+// 258-262).  Allocates [R/2][N*F] doubles on the device; the ALI graphs are
+// rebuilt so that they write it too.
+extern "C" int mgr_set_scattering(mgr_ctx *ctx, int use_scattering)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    MGR_HIP(hipSetDevice(ctx->device));
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    if (use_scattering && !ctx->u_store) {
+        const size_t n = (size_t)(ctx->R / 2) * ctx->N * ctx->F;
+        MGR_HIP(hipMalloc((void **)&ctx->u_store, std::max<size_t>(n, 1) * sizeof(double)));
+        MGR_HIP(hipMemset(ctx->u_store, 0, std::max<size_t>(n, 1) * sizeof(double)));
+    } else if (!use_scattering && ctx->u_store) {
+        (void)hipFree(ctx->u_store);
+        ctx->u_store = nullptr;
+    }
+    ctx->graph_precision = -1;   // captured launches hold the old pointer
+    return MGR_OK;
+}
+
+extern "C" int mgr_get_u(mgr_ctx *ctx, double *u)
+{
+    if (!ctx || !u) return MGR_ERR_ARGUMENT;
+    if (!ctx->u_store) return ctx->fail(MGR_ERR_STATE, "scattering is off (mgr_set_scattering)");
+    MGR_HIP(hipSetDevice(ctx->device));
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    MGR_HIP(hipMemcpy(u, ctx->u_store, (size_t)(ctx->R / 2) * ctx->N * ctx->F * sizeof(double),
+                      hipMemcpyDeviceToHost));
     return MGR_OK;
 }

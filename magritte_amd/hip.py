@@ -53,6 +53,8 @@ SIGNATURES = {
     "mgr_get_ray": ([_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp], _i),
     "mgr_get_npoints_on_rays": ([_vp, _vp], _i),
     "mgr_get_npoints_on_rays_rest": ([_vp, _vp], _i),
+    "mgr_set_scattering": ([_vp, _i], _i),
+    "mgr_get_u": ([_vp, _vp], _i),
     "mgr_compute_spectral_discretisation_image": ([_vp, _d], _i),
     "mgr_compute_image": ([_vp, _i64, _i, _vp, _vp], _i),
     "mgr_last_radiation_field_ms": ([_vp, _vp], _i),
@@ -265,6 +267,15 @@ class Context:
         a = np.zeros((self.R // 2) * self.N, np.int64)
         self._check(self.lib.mgr_get_npoints_on_rays(self.h, _p(a)), "get_npoints_on_rays")
         return a.reshape(self.R // 2, self.N)
+
+    def set_scattering(self, use_scattering=True):
+        self._check(self.lib.mgr_set_scattering(self.h, 1 if use_scattering else 0), "set_scattering")
+
+    def u(self):
+        """Radiation::u [nrays/2, N, F] of the last radiation field (scattering on)."""
+        a = np.zeros((self.R // 2) * self.N * self.F)
+        self._check(self.lib.mgr_get_u(self.h, _p(a)), "get_u")
+        return a.reshape(self.R // 2, self.N, self.F)
 
     def npoints_on_rays_rest(self):
         a = np.zeros((self.R // 2) * self.N, np.int64)

@@ -19,7 +19,7 @@ def core():
     return _core
 
 
-def to_simulation(m: Model):
+def to_simulation(m: Model, use_scattering: bool = False):
     c = core()
     sim = c.Simulation()
     N, R, NB = m.ncells, m.nrays, m.nboundary
@@ -38,6 +38,7 @@ def to_simulation(m: Model):
     p.set_pop_prec(float(m.pop_prec))
     p.max_width_fraction = float(m.max_width_fraction)
     p.n_off_diag = int(m.n_off_diag)
+    p.set_use_scattering(bool(use_scattering))
 
     cells = sim.geometry.cells
     cells.position = np.asarray(m.position, dtype=np.float64).tolist()
@@ -121,4 +122,6 @@ def to_simulation(m: Model):
     sim.lines.lineProducingSpecies.append(lsp)
     sim.lines.line = sorted(ld.frequency.tolist())
     sim.lines.line_index = [int(i) for i in np.argsort(ld.frequency, kind="stable")]
+    if use_scattering:   # Radiation::read sizes u (radiation.cpp:117-126)
+        sim.radiation.u = [[0.0] * (N * ld.nrad * m.nquads) for _ in range(R // 2)]
     return sim

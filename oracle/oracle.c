@@ -65,6 +65,7 @@ struct orc_state {
     double  *J;                                           /* [N*F] */
     double  *Ls;                                          /* [N*L] */
     double  *Jlin, *Jeff, *Jdif;                          /* [N*L] */
+    double  *u;                                           /* [R/2][N*F] with scattering, else NULL */
     double   fnc, rcm, rcmax;
 };
 
@@ -168,6 +169,7 @@ void orc_destroy(orc_state *s)
     free(s->corr_l); free(s->corr_k); free(s->corr_z); free(s->I_bdy); free(s->pop);
     free(s->prev1); free(s->prev2); free(s->prev3); free(s->pop_tot); free(s->emissivity);
     free(s->opacity); free(s->J); free(s->Ls); free(s->Jlin); free(s->Jeff); free(s->Jdif);
+    free(s->u);
     free(s);
 }
 
@@ -899,6 +901,20 @@ int orc_compute_image(orc_state *s, int64_t r, double *I_m, double *I_p)
 static int o_radiation_field(orc_state *s, int nthreads, const int64_t *list, int64_t o_begin,
                              int64_t o_end);
 
+/* Radiation::u (radiation.cpp:117-126, sim_radiation.tpp:226-237, 258-262):
+ * with scattering every compute_radiation_field keeps each ray pair's u at
+ * the origin, u[R][f + p*F]. */
+void orc_set_scattering(orc_state *s, int use_scattering)
+{
+    free(s->u);
+    s->u = use_scattering ? (double *)xcalloc((size_t)(s->R / 2) * s->N * s->F, sizeof(double)) : NULL;
+}
+
+void orc_get_u(const orc_state *s, double *u)
+{
+    if (s->u) memcpy(u, s->u, (size_t)(s->R / 2) * s->N * s->F * sizeof(double));
+}
+
 int orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, int64_t o_end)
 {
     if (o_begin < 0) o_begin = 0;
@@ -980,6 +996,7 @@ static int o_radiation_field(orc_state *s, int nthreads, const int64_t *list, in
                             s->Ls[o*L + k] += Lel;
                         }
                         s->J[o*F + f] += 2.0 * weight_ang * u;
+                        if (s->u) s->u[(R*s->N + o)*F + f] = u;   /* radiation.u[R][ind] */
                     }
                 } else {
                     const int64_t bb = s->cell2boundary_nr[o];
@@ -987,6 +1004,7 @@ static int o_radiation_field(orc_state *s, int nthreads, const int64_t *list, in
                         const double I = s->I_bdy[(R*s->NB + bb)*F + f];
                         const double u = 0.5 * (I + I);
                         s->J[o*F + f] += 2.0 * weight_ang * u;
+                        if (s->u) s->u[(R*s->N + o)*F + f] = u;
                     }
                 }
             }

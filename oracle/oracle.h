@@ -69,6 +69,9 @@ void orc_raypair_solve(int64_t ndep, int64_t n_ar, int64_t first, int64_t last,
 /* src/Simulation/sim_radiation.tpp:56-338 for origins [o_begin, o_end) of
  * every reduced ray; nthreads <= 0 uses all OpenMP threads.  J and Lambda of
  * the covered origins are reset first. */
+/* Radiation::u with scattering: u[(R*N + p)*F + f] of every ray pair R < nrays/2 */
+void orc_set_scattering(orc_state *s, int use_scattering);
+void orc_get_u(const orc_state *s, double *u);
 int  orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, int64_t o_end);
 /* the same for an explicit list of origins (bounded CPU-baseline samples) */
 int  orc_compute_radiation_field_list(orc_state *s, int nthreads, const int64_t *origins, int64_t n);

@@ -84,6 +84,8 @@ def lib():
             getattr(L, name).argtypes = [vp, vp]
         L.orc_get_frequencies.argtypes = [vp, vp, vp]
         L.orc_get_Jeff.argtypes = [vp, vp, vp, vp]
+        L.orc_set_scattering.argtypes = [vp, C.c_int]
+        L.orc_get_u.argtypes = [vp, vp]
         L.orc_get_population_history.argtypes = [vp, vp, vp, vp]
         L.orc_set_population_history.argtypes = [vp, vp, vp, vp]
         L.orc_get_emissivity_opacity.argtypes = [vp, vp, vp]
@@ -220,6 +222,15 @@ class Oracle:
         a = np.zeros(self.NB * self.F)
         lib().orc_get_boundary_intensities(self._s, _p(a))
         return a.reshape(self.NB, self.F)
+
+    def set_scattering(self, use_scattering=True):
+        lib().orc_set_scattering(self._s, 1 if use_scattering else 0)
+
+    def u(self):
+        """Radiation::u [nrays/2, N, F] of the last radiation field (scattering on)."""
+        a = np.zeros((self.R // 2) * self.N * self.F)
+        lib().orc_get_u(self._s, _p(a))
+        return a.reshape(self.R // 2, self.N, self.F)
 
     def J(self):
         a = np.zeros(self.N * self.F)

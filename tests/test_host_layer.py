@@ -187,3 +187,26 @@ def test_compute_and_write_image_matches_oracle(tmp_path):
                                   [len(orc.trace_ray(o, r, rest=True)[1]) +
                                    len(orc.trace_ray(o, int(orc.antipod()[r]), rest=True)[1]) + 1
                                    for o in range(5)])
+
+
+@pytest.mark.gpu
+def test_scattering_fills_radiation_u(tmp_path):
+    """Parameters.use_scattering: Radiation::read sizes u [nrays/2][ncells*nfreqs]
+    (radiation.cpp:117-126) and compute_radiation_field fills it."""
+    from oracle.py_oracle import Oracle
+    m = synthetic.envelope_model(n=6, seed=3)
+    print("\n[host] envelope6 seed 3: scattering u", flush=True)
+    sim = to_simulation(m, use_scattering=True)
+    assert sim.compute_spectral_discretisation() == 0
+    assert sim.compute_boundary_intensities() == 0
+    assert sim.compute_LTE_level_populations() == 0
+    sim.precision = core.FP64
+    assert sim.compute_radiation_field() == 0
+    u = np.array(sim.radiation.u)
+    orc = Oracle(m).setup()
+    orc.set_scattering(True)
+    orc.compute_radiation_field()
+    u_o = orc.u().reshape(u.shape)
+    assert float(np.max(np.abs(u - u_o) / np.abs(u_o))) < 1e-10
+    sim.write(core.IoText(str(tmp_path) + "/"))
+    assert os.path.exists(str(tmp_path) + "/Radiation/u.txt")

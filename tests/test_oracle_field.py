@@ -89,3 +89,23 @@ def test_check_for_convergence_statistics():
     assert fnc == pytest.approx(np.count_nonzero(rc[mask] > 1e-6) / (N * V), rel=1e-12)
     assert mean == pytest.approx(rc[mask].sum() / (N * V), rel=1e-12)
     assert mx == pytest.approx(rc[mask].max(), rel=1e-12)
+
+
+def test_scattering_u_per_ray_sums_to_J():
+    """Radiation::u with scattering (radiation.cpp:117-126, sim_radiation.tpp:226-262):
+    the oracle keeps every ray pair's u at the origin; J is the ray-weighted
+    sum 2 w_r u_r in ray order, and a pair of one point holds I_bdy."""
+    m = synthetic.regular_grid_model(n=6)
+    o = Oracle(m).setup()
+    o.set_scattering(True)
+    o.compute_radiation_field()
+    u = o.u()
+    w = np.asarray(m.weights)
+    J = np.zeros_like(u[0])
+    for r in range(m.nrays // 2):
+        J = J + 2.0 * w[r] * u[r]
+    np.testing.assert_array_equal(J, o.J())
+    assert np.all(u > 0)
+    o.set_scattering(False)
+    o.compute_radiation_field()
+    np.testing.assert_array_equal(o.J(), J)
