@@ -59,7 +59,7 @@ typedef struct mgr_model_view {
     int64_t nquads;
     double  pop_prec;            /* required precision for populations           */
     double  max_width_fraction;  /* Parameters::max_width_fraction, default 0.5  */
-    int64_t n_off_diag;          /* ALO bandwidth; only 0 is supported here      */
+    int64_t n_off_diag;          /* ALO bandwidth: 0, or 1 (the same diagonal operator) */
 
     /* Geometry/Cells (src/Model/Geometry/Cells/cells.cpp:20-93) */
     const double  *position;     /* [ncells][3] m                                 */

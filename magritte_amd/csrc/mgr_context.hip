@@ -397,8 +397,12 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
     const int64_t R = Rtot, NB = ctx->NB, L = ctx->L, V = ctx->V, Q = ctx->Q;
 
     if (L > MAX_LINES) return ctx->fail(MGR_ERR_UNSUPPORTED, "more than 64 line transitions");
-    if (m->n_off_diag != 0)
-        return ctx->fail(MGR_ERR_UNSUPPORTED, "n_off_diag > 0 (banded ALO) is not supported");
+    // n_off_diag = 1 is the diagonal operator: RayPair::update_Lambda adds
+    // bands m = 1 .. n_off_diag-1 only (raypair_lambda.tpp:129-146), and the
+    // back-substitution branch's L_diag (raypair_solver.tpp:358-370) is the
+    // same diagonal of the inverse.  Wider bands are not built.
+    if (m->n_off_diag < 0 || m->n_off_diag > 1)
+        return ctx->fail(MGR_ERR_UNSUPPORTED, "n_off_diag > 1 (banded ALO) is not supported");
     if (m->num < 0 || m->num >= m->nspecs) return ctx->fail(MGR_ERR_ARGUMENT, "line species index out of range");
 
     // Lines::setup (src/Model/Lines/lines.cpp:43-87): sorted line list.

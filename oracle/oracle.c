@@ -681,7 +681,7 @@ typedef struct {
     int64_t cap;
     int64_t *cell_r, *cell_ar; double *shift_r, *shift_ar, *dZ_r, *dZ_ar;
     int64_t *lnotch_r, *lnotch_ar;
-    double *term1, *chi, *frs, *dtau, *Su, *L_diag;
+    double *term1, *chi, *frs, *dtau, *Su, *L_diag, *L_upper, *L_lower;
     int64_t *nrs;
 } o_pair_buf;
 
@@ -696,6 +696,7 @@ static void o_buf_alloc(o_pair_buf *b, int64_t cap)
     b->term1 = (double *)xcalloc(d, 8); b->chi = (double *)xcalloc(d, 8);
     b->frs = (double *)xcalloc(d, 8);   b->dtau = (double *)xcalloc(d, 8);
     b->Su = (double *)xcalloc(d, 8);    b->L_diag = (double *)xcalloc(d, 8);
+    b->L_upper = (double *)xcalloc(d, 8); b->L_lower = (double *)xcalloc(d, 8);
     b->nrs = (int64_t *)xcalloc(d, 8);
 }
 static void o_buf_free(o_pair_buf *b)
@@ -703,6 +704,7 @@ static void o_buf_free(o_pair_buf *b)
     free(b->cell_r); free(b->cell_ar); free(b->shift_r); free(b->shift_ar); free(b->dZ_r);
     free(b->dZ_ar); free(b->lnotch_r); free(b->lnotch_ar); free(b->term1); free(b->chi);
     free(b->frs); free(b->dtau); free(b->Su); free(b->L_diag); free(b->nrs);
+    free(b->L_upper); free(b->L_lower);
 }
 
 /* Simulation::setup (src/Simulation/simulation.tpp:185-353): term1, chi and
@@ -769,8 +771,10 @@ static void o_setup(const orc_state *s, int64_t R, int64_t origin, int64_t f,
     *In_out = I_bdy_n;
 }
 
-/* Simulation::setup followed by RayPair::solve (n_off_diag = 0).  Side "ar"
- * is the longer ray.  Returns u at the origin. */
+/* Simulation::setup followed by RayPair::solve with the model's n_off_diag
+ * (0, or 1: the back-substitution branch, raypair_solver.tpp:336-388, whose
+ * first band update_Lambda never reads).  Side "ar" is the longer ray.
+ * Returns u at the origin. */
 static double o_setup_and_solve(const orc_state *s, int64_t R, int64_t origin, int64_t f,
                                 int64_t n_ar, const int64_t *c_ar, const double *s_ar,
                                 const double *z_ar, int64_t *ln_ar,
@@ -784,8 +788,8 @@ static double o_setup_and_solve(const orc_state *s, int64_t R, int64_t origin, i
     double I_bdy_0, I_bdy_n;
     o_setup(s, R, origin, f, n_ar, c_ar, s_ar, z_ar, ln_ar, n_r, c_r, s_r, z_r, ln_r, lnotch_at_origin, b,
             &first, &last, &I_bdy_0, &I_bdy_n);
-    orc_raypair_solve(ndep, n_ar, first, last, 0, b->term1, b->dtau, I_bdy_0, I_bdy_n,
-                      b->Su, b->L_diag, NULL, NULL, NULL, NULL);
+    orc_raypair_solve(ndep, n_ar, first, last, s->m.n_off_diag, b->term1, b->dtau, I_bdy_0, I_bdy_n,
+                      b->Su, b->L_diag, b->L_upper, b->L_lower, NULL, NULL);
     *L_origin = b->L_diag[n_ar];
     *chi_origin = b->chi[n_ar];
     return b->Su[n_ar];
@@ -917,6 +921,9 @@ void orc_get_u(const orc_state *s, double *u)
 
 int orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, int64_t o_end)
 {
+    /* n_off_diag >= 2 adds off-diagonal elements (raypair_lambda.tpp:129-146)
+     * that are not restated here */
+    if (s->m.n_off_diag > 1) return -1;
     if (o_begin < 0) o_begin = 0;
     if (o_end > s->N || o_end < 0) o_end = s->N;
     return o_radiation_field(s, nthreads, NULL, o_begin, o_end);
@@ -924,6 +931,7 @@ int orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, int
 
 int orc_compute_radiation_field_list(orc_state *s, int nthreads, const int64_t *origins, int64_t n)
 {
+    if (s->m.n_off_diag > 1) return -1;
     return o_radiation_field(s, nthreads, origins, 0, n);
 }
 

@@ -173,3 +173,26 @@ def test_converged_populations(pair):
           flush=True)
     assert err < 1e-5
     assert abs(it_g - it_o) <= 2
+
+
+def test_n_off_diag_one_matches_oracle():
+    """n_off_diag = 1 (the diagonal operator reached through the reference's
+    back-substitution branch) against the oracle; n_off_diag = 2 is refused."""
+    from magritte_amd.hip import Context, HipError
+    m = synthetic.envelope_model(n=8, seed=7)
+    m.n_off_diag = 1
+    print("\n[parity] envelope8 n_off_diag 1", flush=True)
+    orc = Oracle(m).setup()
+    ctx = Context(m).setup()
+    orc.compute_radiation_field()
+    ctx.compute_radiation_field("fp64")
+    assert rel_err(ctx.J(), orc.J()) < J_TOL["fp64"]
+    ctx.compute_Jeff()
+    orc.compute_Jeff()
+    L_o, L_g = orc.lambda_diag(), ctx.lambda_diag()
+    mask = np.abs(L_o) > 0
+    assert rel_err(L_g[mask], L_o[mask]) < J_TOL["fp64"]
+    m2 = synthetic.envelope_model(n=4, seed=7)
+    m2.n_off_diag = 2
+    with pytest.raises(HipError, match="n_off_diag"):
+        Context(m2)

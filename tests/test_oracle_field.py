@@ -109,3 +109,23 @@ def test_scattering_u_per_ray_sums_to_J():
     o.set_scattering(False)
     o.compute_radiation_field()
     np.testing.assert_array_equal(o.J(), J)
+
+
+def test_n_off_diag_one_is_the_diagonal_operator():
+    """n_off_diag = 1 solves each pair with the back-substitution branch
+    (raypair_solver.tpp:336-388) but update_Lambda adds bands from m = 1 on
+    (raypair_lambda.tpp:129-146): J and Lambda* equal the n_off_diag = 0 ones to
+    rounding; n_off_diag = 2 is refused."""
+    m0 = synthetic.envelope_model(n=6, seed=5)
+    o0 = Oracle(m0).setup()
+    o0.compute_radiation_field()
+    m1 = synthetic.envelope_model(n=6, seed=5)
+    m1.n_off_diag = 1
+    o1 = Oracle(m1).setup()
+    o1.compute_radiation_field()
+    np.testing.assert_allclose(o1.J(), o0.J(), rtol=1e-12, atol=0.0)
+    np.testing.assert_allclose(o1.lambda_diag(), o0.lambda_diag(), rtol=1e-12, atol=0.0)
+    m2 = synthetic.envelope_model(n=6, seed=5)
+    m2.n_off_diag = 2
+    with pytest.raises(RuntimeError):
+        Oracle(m2).setup().compute_radiation_field()
