@@ -23,6 +23,7 @@ namespace {
 
 constexpr int32_t ERR_NO_NEIGHBOUR = 1;
 constexpr int32_t ERR_TOO_LONG     = 8;
+constexpr int32_t ERR_TOO_MANY     = 16;   // a step needs more than 10000 points
 
 // positive doubles order like their bit patterns
 __device__ __forceinline__ void atomic_minmax_pos_double(uint64_t *addr, double lo, double hi)
@@ -66,6 +67,9 @@ __device__ __forceinline__ Walk trace_walk(int64_t N, const double *__restrict__
     k.smax = fmax(k.smax, shift_crt);
     for (;;) {
         const StepSplit sp = split_step(shift_crt, shift_nxt, dZ, dsm);
+        // get_required_npoints throws for more than 10000 points
+        // (geometry.tpp:484), before any radiation field is computed
+        if (!FILL && sp.interp && sp.n == 0) atomicOr(err_flag, ERR_TOO_MANY);
         k.points += sp.n;
         k.smin = fmin(k.smin, shift_nxt);
         k.smax = fmax(k.smax, shift_nxt);
@@ -220,6 +224,13 @@ __global__ void k_expand_ray(int32_t o, int64_t q, const int64_t *__restrict__ s
 
 namespace mgr {
 
+static int trace_error(mgr_ctx *ctx, int32_t errv)
+{
+    if (errv & ERR_TOO_MANY) return ctx->fail(MGR_ERR_TRACE, "Too many (> 10 000) interpolations needed!");
+    return ctx->fail(MGR_ERR_TRACE, errv & ERR_NO_NEIGHBOUR ? "a ray found no forward neighbour inside the mesh"
+                                                            : "a ray visited more steps than there are cells");
+}
+
 int launch_trace(mgr_ctx *ctx)
 {
     const int64_t N = ctx->N, R = ctx->R, NR = ctx->NQ;
@@ -243,11 +254,7 @@ int launch_trace(mgr_ctx *ctx)
     uint64_t sbits[2] = {0, 0};
     MGR_HIP(hipStreamSynchronize(ctx->stream));
     MGR_HIP(hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (errv) {
-        return ctx->fail(MGR_ERR_TRACE, errv & ERR_NO_NEIGHBOUR
-                                            ? "a ray found no forward neighbour inside the mesh"
-                                            : "a ray visited more steps than there are cells");
-    }
+    if (errv) return trace_error(ctx, errv);
     MGR_HIP(hipMemcpy(ns.data(), ctx->nseg, NR * sizeof(int32_t), hipMemcpyDeviceToHost));
     MGR_HIP(hipMemcpy(np.data(), ctx->npts, NR * sizeof(int32_t), hipMemcpyDeviceToHost));
     MGR_HIP(hipMemcpy(sbits, ctx->shift_bits, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost));
@@ -314,11 +321,6 @@ void free_image(mgr_ctx *ctx)
     g = mgr_ctx::ImageTrace();
 }
 
-static int rest_trace_error(mgr_ctx *ctx, int32_t errv)
-{
-    return ctx->fail(MGR_ERR_TRACE, errv & ERR_NO_NEIGHBOUR ? "a ray found no forward neighbour inside the mesh"
-                                                            : "a ray visited more steps than there are cells");
-}
 
 // Rest-frame trace of ray `ray` and its antipode for an image: counting
 // pass, host scan, filling pass into ctx->img (kept until another ray is
@@ -349,7 +351,7 @@ int launch_trace_image(mgr_ctx *ctx, int64_t ray)
     MGR_HIP(hipStreamSynchronize(ctx->stream));
     int32_t errv = 0;
     MGR_HIP(hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (errv) return rest_trace_error(ctx, errv);
+    if (errv) return trace_error(ctx, errv);
     std::vector<int32_t> ns(NQ);
     uint64_t sbits[2] = {0, 0};
     MGR_HIP(hipMemcpy(ns.data(), g.nseg, NQ * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -466,7 +468,7 @@ int mgr_get_npoints_on_rays_rest(mgr_ctx *ctx, int64_t *npoints)
     (void)hipFree(d_ns);
     (void)hipFree(d_np);
     if (rc) return ctx->fail(rc, "rest-frame ray count failed");
-    if (errv) return rest_trace_error(ctx, errv);
+    if (errv) return trace_error(ctx, errv);
     for (int64_t o = 0; o < N; o++)
         for (int64_t r = 0; r < R / 2; r++)
             npoints[r * N + o] = (int64_t)np[o * R + r] + np[o * R + ctx->antipod[r]] + 1;

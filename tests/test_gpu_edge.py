@@ -1,0 +1,94 @@
+"""Edge cases of the radiation-field path on the GPU, through the C-ABI,
+against the FP64 oracle.
+
+* Heavy interpolation: a velocity jump of 8.4e-4 c across a regular grid
+  needs ~3000 interpolation points per step (Geometry::set_data,
+  geometry.tpp:100-207); traces stay bit-exact and J within the usual bars.
+* The interpolation cap: a jump of 1e-2 c needs more than 10 000 points, and
+  the reference's first radiation field throws "Too many (> 10 000)
+  interpolations needed!" from get_max_npoints_on_rays (geometry.tpp:484);
+  the trace returns that error.
+* Every cell on the boundary (a 2^3 grid) or all but one (3^3): pairs with
+  an empty side take I_bdy of the origin's boundary entry
+  (simulation.tpp:283-288, 342-347).
+"""
+import numpy as np
+import pytest
+
+from magritte_amd.setup import synthetic
+from oracle.py_oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+J_TOL = {"fp32": 1e-5, "fp64": 1e-10}
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def jump_model(n, jump):
+    """A regular grid whose upper half in x moves at `jump` (units of c) along x."""
+    m = synthetic.regular_grid_model(n=n)
+    v = np.zeros((m.ncells, 3))
+    v[m.position[:, 0] > 0.5 * m.position[:, 0].max(), 0] = jump
+    m.velocity = v
+    return m
+
+
+def _ctx(m):
+    from magritte_amd.hip import Context
+    return Context(m).setup()
+
+
+def test_heavy_interpolation_bit_exact_and_parity():
+    m = jump_model(6, 8.4e-4)
+    print("\n[edge] regular6 jump 8.4e-4 c", flush=True)
+    orc = Oracle(m).setup()
+    ctx = _ctx(m)
+    ctx.trace()
+    longest = 0
+    for o in range(m.ncells):
+        for r in range(m.nrays):
+            _, c_o, s_o, z_o = orc.trace_ray(o, r)
+            c_g, s_g, z_g = ctx.ray(o, r)
+            assert np.array_equal(c_o, c_g) and np.array_equal(s_o, s_g) and np.array_equal(z_o, z_g), (o, r)
+            longest = max(longest, len(c_o))
+    assert longest > 2000   # the jump is resolved by thousands of points
+    orc.compute_radiation_field()
+    for prec in ("fp64", "fp32"):
+        ctx.compute_radiation_field(prec)
+        err = rel_err(ctx.J(), orc.J())
+        print(f"[edge] heavy interpolation {prec} J max rel err {err:.2e}", flush=True)
+        assert err < J_TOL[prec]
+
+
+def test_interpolation_cap_is_an_error():
+    from magritte_amd.hip import HipError
+    m = jump_model(4, 1.0e-2)
+    print("\n[edge] regular4 jump 1e-2 c", flush=True)
+    ctx = _ctx(m)
+    with pytest.raises(HipError, match="Too many"):
+        ctx.trace()
+    with pytest.raises(HipError, match="Too many"):
+        ctx.compute_radiation_field("fp32")
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_all_or_most_cells_on_the_boundary(n):
+    m = synthetic.regular_grid_model(n=n)
+    print(f"\n[edge] regular{n}", flush=True)
+    orc = Oracle(m).setup()
+    ctx = _ctx(m)
+    orc.compute_radiation_field()
+    for prec in ("fp64", "fp32"):
+        ctx.compute_radiation_field(prec)
+        assert rel_err(ctx.J(), orc.J()) < J_TOL[prec]
+    ctx.compute_Jeff()
+    orc.compute_Jeff()
+    L_o, L_g = orc.lambda_diag(), ctx.lambda_diag()
+    mask = np.abs(L_o) > 0
+    assert rel_err(L_g[mask], L_o[mask]) < J_TOL["fp32"]
+    assert np.all(L_g[~mask] == 0.0)
