@@ -51,6 +51,13 @@ constexpr double HDZ_SCALE  = 0.35355339059327373 / CHI_SCALE;   // 1 / (2 sqrt 
 constexpr int WAVE = 64;                 // CDNA wavefront width
 constexpr int MAX_INTERPOLATION = 10000; // geometry.tpp:138
 
+// One neighbour entry of the tracer: the neighbour's index and position, so
+// that a step reads its candidates contiguously (one gather less per step).
+struct alignas(32) NbRec {
+    double  x, y, z;
+    int64_t c;
+};
+
 // Small per-model line tables, read by every kernel.
 struct LineTables {
     int    L;              // nrad (one species)
@@ -132,6 +139,7 @@ struct mgr_ctx {
     double  *pos = nullptr, *vel = nullptr;        // [N*3]
     int64_t *nb_off = nullptr;                     // [N+1]
     int32_t *nb = nullptr;                         // [sum n_neighbors]
+    mgr::NbRec *nb_rec = nullptr;                  // [sum n_neighbors] neighbour index and position
     uint8_t *is_boundary = nullptr;                // [N]
     int32_t *cell2bdy = nullptr;                   // [N], -1 for interior cells
     int32_t *bdy2cell = nullptr;                   // [NB]

@@ -478,6 +478,19 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
     if ((rc = upload(ctx, &ctx->vel, m->velocity, N * 3))) return rc;
     if ((rc = upload(ctx, &ctx->nb_off, off.data(), N + 1))) return rc;
     if ((rc = upload_cast<int32_t>(ctx, &ctx->nb, m->neighbors, M))) return rc;
+    {
+        // the tracer's neighbour records: index and position of every entry
+        std::vector<NbRec> rec((size_t)std::max<int64_t>(M, 1));
+        for (int64_t q = 0; q < M; q++) {
+            const int64_t c = m->neighbors[q];
+            rec[q].x = m->position[3 * c + 0];
+            rec[q].y = m->position[3 * c + 1];
+            rec[q].z = m->position[3 * c + 2];
+            rec[q].c = c;
+        }
+        MGR_HIP(hipMalloc((void **)&ctx->nb_rec, rec.size() * sizeof(NbRec)));
+        MGR_HIP(hipMemcpy(ctx->nb_rec, rec.data(), rec.size() * sizeof(NbRec), hipMemcpyHostToDevice));
+    }
     if ((rc = upload(ctx, &ctx->is_boundary, isb.data(), N))) return rc;
     if ((rc = upload(ctx, &ctx->cell2bdy, c2b.data(), N))) return rc;
     if ((rc = upload_cast<int32_t>(ctx, &ctx->bdy2cell, m->boundary2cell_nr, NB))) return rc;
@@ -583,6 +596,7 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
 int free_model(mgr_ctx *ctx)
 {
     dfree(ctx->pos); dfree(ctx->vel); dfree(ctx->nb_off); dfree(ctx->nb); dfree(ctx->is_boundary);
+    dfree(ctx->nb_rec);
     dfree(ctx->cell2bdy); dfree(ctx->bdy2cell); dfree(ctx->rays); dfree(ctx->ray_w);
     dfree(ctx->d_antipod); dfree(ctx->temperature); dfree(ctx->vturb2); dfree(ctx->abundance);
     dfree(ctx->tables); dfree(ctx->A); dfree(ctx->Bs); dfree(ctx->Ba); dfree(ctx->frequency);

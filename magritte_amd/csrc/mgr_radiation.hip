@@ -4,8 +4,11 @@
 // Replaces Simulation::compute_radiation_field (src/Simulation/sim_radiation.tpp:56-338)
 // with its inner calls Simulation::setup / get_eta_and_chi (simulation.tpp:185-481),
 // Radiation::rescale_I_bdy (radiation.tpp:341-373), RayPair::solve
-// (raypair_solver.tpp:200-334, n_off_diag = 0) and RayPair::update_Lambda
-// (raypair_lambda.tpp:94-149).
+// (raypair_solver.tpp:200-388, n_off_diag 0 or 1: the same diagonal operator)
+// and RayPair::update_Lambda (raypair_lambda.tpp:94-149); with scattering it
+// also keeps each pair's u (Radiation::u, sim_radiation.tpp:226-262).  The
+// image kernel (Simulation::compute_and_write_image) reuses the same staging
+// and elimination on a rest-frame trace.
 //
 // Work mapping (CDNA4, 64-wide wavefronts): one workgroup per origin cell,
 // one wavefront per chunk of 64 frequency bins (lane = bin).  The workgroup
@@ -14,7 +17,8 @@
 // their cells (11 width, 1/width and the emissivity/opacity prefactors per
 // line), so that the sweep reads uniform LDS broadcasts instead of dependent
 // global gathers.  The interpolation split of every step (set_data,
-// geometry.tpp:100-207) is computed once while staging.  J and the Lambda*
+// geometry.tpp:100-207) is precomputed at trace time (FP32) or computed once
+// while staging (FP64).  J and the Lambda*
 // diagonal accumulate in FP64 registers over rays and are written once per
 // (cell, bin): no atomics.  Workgroups are mapped to XCDs in contiguous origin
 // ranges so that neighbouring origins share an L2.
@@ -76,12 +80,10 @@ template <typename real> struct Prec;
 template <> struct Prec<float> {
     using row_t = float4;   // (1/width, E, K, 0)
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-    static __device__ __forceinline__ float expneg(float x2) { return __expf(-x2); }
 };
 template <> struct Prec<double> {
     using row_t = double4;  // (11 width, 1/width, E, K)
     static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-    static __device__ __forceinline__ double expneg(double x2) { return exp(-x2); }
 };
 
 // One traced step in LDS.  Sub-point m = 1..n of the step lies at

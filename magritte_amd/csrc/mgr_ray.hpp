@@ -21,30 +21,28 @@ __device__ __forceinline__ double dot3(double ax, double ay, double az, double b
 
 // Geometry::get_next_general (src/Model/Geometry/geometry.tpp:328-421):
 // the neighbour with Z_new > Z closest to the ray (strict <, first wins).
-__device__ __forceinline__ int32_t get_next(const double *__restrict__ pos,
-                                            const int64_t *__restrict__ nb_off,
-                                            const int32_t *__restrict__ nb, int32_t origin,
-                                            double nx, double ny, double nz, int32_t current,
+// (ox, oy, oz) is the origin's position; nbr holds each neighbour's index and
+// position in the order of the neighbour list.
+__device__ __forceinline__ int32_t get_next(const NbRec *__restrict__ nbr,
+                                            const int64_t *__restrict__ nb_off, double ox, double oy,
+                                            double oz, double nx, double ny, double nz, int32_t current,
                                             double &Z, double &dZ)
 {
 #pragma clang fp contract(off)
     double dmin = DBL_MAX;
     int32_t next = -1;
-    const double ox = pos[3 * (int64_t)origin + 0];
-    const double oy = pos[3 * (int64_t)origin + 1];
-    const double oz = pos[3 * (int64_t)origin + 2];
     const int64_t q1 = nb_off[current + 1];
     for (int64_t q = nb_off[current]; q < q1; q++) {
-        const int32_t c = nb[q];
-        const double Rx = pos[3 * (int64_t)c + 0] - ox;
-        const double Ry = pos[3 * (int64_t)c + 1] - oy;
-        const double Rz = pos[3 * (int64_t)c + 2] - oz;
+        const NbRec r = nbr[q];
+        const double Rx = r.x - ox;
+        const double Ry = r.y - oy;
+        const double Rz = r.z - oz;
         const double Z_new = dot3(Rx, Ry, Rz, nx, ny, nz);
         if (Z_new > Z) {
             const double distance_from_ray2 = dot3(Rx, Ry, Rz, Rx, Ry, Rz) - Z_new * Z_new;
             if (distance_from_ray2 < dmin) {
                 dmin = distance_from_ray2;
-                next = c;
+                next = (int32_t)r.c;
                 dZ = Z_new - Z;
             }
         }

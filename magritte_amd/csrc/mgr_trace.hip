@@ -46,7 +46,7 @@ template <bool FILL, bool REST>
 __device__ __forceinline__ Walk trace_walk(int64_t N, const double *__restrict__ pos,
                                            const double *__restrict__ vel,
                                            const int64_t *__restrict__ nb_off,
-                                           const int32_t *__restrict__ nb,
+                                           const NbRec *__restrict__ nb,
                                            const uint8_t *__restrict__ is_boundary, int32_t o,
                                            double nx, double ny, double nz, double sx, double sy,
                                            double sz, double dsm, int64_t w,
@@ -59,7 +59,8 @@ __device__ __forceinline__ Walk trace_walk(int64_t N, const double *__restrict__
     k.steps = 0; k.points = 0; k.last_cell = -1;
     k.smin = 1.0; k.smax = 1.0; k.last_shift = 1.0;
     double Z = 0.0, dZ = 0.0;
-    int32_t nxt = get_next(pos, nb_off, nb, o, nx, ny, nz, o, Z, dZ);
+    const double ox = pos[3 * (int64_t)o + 0], oy = pos[3 * (int64_t)o + 1], oz = pos[3 * (int64_t)o + 2];
+    int32_t nxt = get_next(nb, nb_off, ox, oy, oz, nx, ny, nz, o, Z, dZ);
     if (nxt == -1) return k;
     double shift_crt = REST ? rest_shift(vel, o, sx, sy, sz) : doppler_shift(vel, o, o, nx, ny, nz);
     double shift_nxt = REST ? rest_shift(vel, nxt, sx, sy, sz) : doppler_shift(vel, o, nxt, nx, ny, nz);
@@ -93,7 +94,7 @@ __device__ __forceinline__ Walk trace_walk(int64_t N, const double *__restrict__
         if (is_boundary[nxt]) break;
         if (k.steps >= N) { atomicOr(err_flag, ERR_TOO_LONG); break; }
         shift_crt = shift_nxt;
-        nxt = get_next(pos, nb_off, nb, o, nx, ny, nz, nxt, Z, dZ);
+        nxt = get_next(nb, nb_off, ox, oy, oz, nx, ny, nz, nxt, Z, dZ);
         if (nxt < 0) { atomicOr(err_flag, ERR_NO_NEIGHBOUR); break; }
         shift_nxt = REST ? rest_shift(vel, nxt, sx, sy, sz) : doppler_shift(vel, o, nxt, nx, ny, nz);
     }
@@ -108,7 +109,7 @@ __global__ void k_trace(int64_t N, int Rmax, const int32_t *__restrict__ cell_mo
                         const int64_t *__restrict__ cell_qbase, const int32_t *__restrict__ m_R,
                         const int32_t *__restrict__ m_rbase, const double *__restrict__ pos,
                         const double *__restrict__ vel, const int64_t *__restrict__ nb_off,
-                        const int32_t *__restrict__ nb, const uint8_t *__restrict__ is_boundary,
+                        const NbRec *__restrict__ nb, const uint8_t *__restrict__ is_boundary,
                         const double *__restrict__ rays, const double *__restrict__ dshift_max,
                         int32_t *__restrict__ nseg, int32_t *__restrict__ npts,
                         const int64_t *__restrict__ seg_off, int32_t *__restrict__ seg_cell,
@@ -147,7 +148,7 @@ __global__ void k_trace(int64_t N, int Rmax, const int32_t *__restrict__ cell_mo
 template <bool FILL>
 __global__ void k_trace_rest(int64_t N, int R, int r_img, const double *__restrict__ pos,
                              const double *__restrict__ vel, const int64_t *__restrict__ nb_off,
-                             const int32_t *__restrict__ nb, const uint8_t *__restrict__ is_boundary,
+                             const NbRec *__restrict__ nb, const uint8_t *__restrict__ is_boundary,
                              const double *__restrict__ rays, const int32_t *__restrict__ antipod,
                              const double *__restrict__ dshift_max, int32_t *__restrict__ nseg,
                              int32_t *__restrict__ npts, const int64_t *__restrict__ seg_off,
@@ -245,7 +246,7 @@ int launch_trace(mgr_ctx *ctx)
     MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
     hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->d_cell_model,
                        ctx->d_cell_qbase, ctx->d_m_R, ctx->d_m_rbase, ctx->pos,
-                       ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->dshift_max,
+                       ctx->vel, ctx->nb_off, ctx->nb_rec, ctx->is_boundary, ctx->rays, ctx->dshift_max,
                        ctx->nseg, ctx->npts, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                        nullptr, ctx->shift_bits, ctx->err_flag);
     MGR_LAUNCH_CHECK();
@@ -302,7 +303,7 @@ int launch_trace(mgr_ctx *ctx)
     MGR_HIP(hipMemcpy(ctx->seg_off, off.data(), (NR + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->d_cell_model,
                        ctx->d_cell_qbase, ctx->d_m_R, ctx->d_m_rbase, ctx->pos,
-                       ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->dshift_max,
+                       ctx->vel, ctx->nb_off, ctx->nb_rec, ctx->is_boundary, ctx->rays, ctx->dshift_max,
                        nullptr, nullptr, ctx->seg_off, ctx->seg_cell, ctx->seg_shift, ctx->seg_dZ,
                        ctx->seg_f32, ctx->seg_hdz32, ctx->far_cell, ctx->far_shift, nullptr, ctx->err_flag);
     MGR_LAUNCH_CHECK();
@@ -344,7 +345,7 @@ int launch_trace_image(mgr_ctx *ctx, int64_t ray)
     MGR_HIP(hipMemcpyAsync(ctx->shift_bits, one_bits, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
     MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
     hipLaunchKernelGGL(k_trace_rest<false>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, (int)ray, ctx->pos,
-                       ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->d_antipod,
+                       ctx->vel, ctx->nb_off, ctx->nb_rec, ctx->is_boundary, ctx->rays, ctx->d_antipod,
                        ctx->dshift_max, g.nseg, g.npts, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                        nullptr, nullptr, nullptr, ctx->shift_bits, ctx->err_flag);
     MGR_LAUNCH_CHECK();
@@ -378,7 +379,7 @@ int launch_trace_image(mgr_ctx *ctx, int64_t ray)
     }
     MGR_HIP(hipMemcpy(g.seg_off, off.data(), (NQ + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_trace_rest<true>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, (int)ray, ctx->pos,
-                       ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays, ctx->d_antipod,
+                       ctx->vel, ctx->nb_off, ctx->nb_rec, ctx->is_boundary, ctx->rays, ctx->d_antipod,
                        ctx->dshift_max, nullptr, nullptr, g.seg_off, g.seg_cell, g.seg_shift, g.seg_dZ,
                        g.seg_f32, g.seg_hdz32, g.far_cell, g.far_shift, g.origin_shift, nullptr,
                        ctx->err_flag);
@@ -457,7 +458,7 @@ int mgr_get_npoints_on_rays_rest(mgr_ctx *ctx, int64_t *npoints)
     if (hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream) != hipSuccess) rc = MGR_ERR_DEVICE;
     if (!rc) {
         hipLaunchKernelGGL(k_trace_rest<false>, dim3((unsigned)((NQ + 127) / 128)), dim3(128), 0, ctx->stream, N,
-                           (int)R, -1, ctx->pos, ctx->vel, ctx->nb_off, ctx->nb, ctx->is_boundary, ctx->rays,
+                           (int)R, -1, ctx->pos, ctx->vel, ctx->nb_off, ctx->nb_rec, ctx->is_boundary, ctx->rays,
                            ctx->d_antipod, ctx->dshift_max, d_ns, d_np, nullptr, nullptr, nullptr, nullptr,
                            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->err_flag);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess ||
