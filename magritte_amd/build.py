@@ -80,6 +80,17 @@ HOST_UNITS = ["io_text.cpp", "model.cpp", "image.cpp", "simulation.cpp", "core.c
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-fvisibility=hidden"]
 
 
+def hdf5_prefix():
+    """Install prefix of the HDF5 C library, or None: the IoHDF5 back end is
+    built only where the library is."""
+    for prefix in (os.environ.get("HDF5_DIR", ""), "/opt/conda", "/usr", "/usr/local"):
+        if prefix and os.path.exists(os.path.join(prefix, "include", "hdf5.h")):
+            for libdir in ("lib", "lib64", "lib/x86_64-linux-gnu"):
+                if os.path.exists(os.path.join(prefix, libdir, "libhdf5.so")):
+                    return prefix, os.path.join(prefix, libdir)
+    return None
+
+
 def core_path():
     return os.path.join(HERE, "core" + sysconfig.get_config_var("EXT_SUFFIX"))
 
@@ -88,18 +99,26 @@ def build_host(verbose=False):
     """g++ build of the host layer into the extension module `core`."""
     import pybind11
     inc = ["-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"]]
+    units, link_extra = list(HOST_UNITS), []
+    hdf5 = hdf5_prefix()
+    if hdf5:
+        prefix, libdir = hdf5
+        units.append("io_hdf5.cpp")
+        inc += ["-DMGR_WITH_HDF5"]
+        link_extra = ["-L" + libdir, "-lhdf5", "-Wl,-rpath," + libdir]
 
     def comp(unit):
         obj = os.path.join(BUILD, "host_" + unit.replace(".cpp", ".o"))
-        cmd = ["g++", *HOST_FLAGS, *inc, "-c", os.path.join(HOST, unit), "-o", obj]
+        extra = ["-isystem", os.path.join(hdf5[0], "include")] if (hdf5 and unit == "io_hdf5.cpp") else []
+        cmd = ["g++", *HOST_FLAGS, *inc, *extra, "-c", os.path.join(HOST, unit), "-o", obj]
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
             raise RuntimeError(f"compile of {unit} failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
         return obj, res.stderr
 
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=len(HOST_UNITS)) as ex:
-        results = list(ex.map(comp, HOST_UNITS))
+    with ThreadPoolExecutor(max_workers=len(units)) as ex:
+        results = list(ex.map(comp, units))
     if verbose:
         for _, err in results:
             if err.strip():
@@ -107,7 +126,7 @@ def build_host(verbose=False):
     out = core_path()
     tmp = out + ".tmp"
     cmd = ["g++", "-shared", "-o", tmp, *[r[0] for r in results], "-L" + HERE, "-lmagritte_hip",
-           "-Wl,-rpath,$ORIGIN"]
+           "-Wl,-rpath,$ORIGIN", *link_extra]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link of core failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

@@ -9,6 +9,9 @@
 #include <pybind11/stl_bind.h>
 
 #include "simulation.hpp"
+#ifdef MGR_WITH_HDF5
+#include "io_hdf5.hpp"
+#endif
 
 namespace py = pybind11;
 using namespace magritte;
@@ -27,6 +30,21 @@ PYBIND11_MODULE(core, module)
     py::class_<IoText, Io>(module, "IoText")
         .def_readonly("io_file", &IoText::io_file)
         .def(py::init<const std::string &>());
+#ifdef MGR_WITH_HDF5
+    // IoPython("hdf5", file) (pyBindings.cpp:46-56), native HDF5
+    py::class_<IoHDF5, Io>(module, "IoHDF5")
+        .def_readonly("io_file", &IoHDF5::io_file)
+        .def(py::init<const std::string &>());
+    py::class_<IoPython, IoHDF5>(module, "IoPython")
+        .def_readonly("implementation", &IoPython::implementation)
+        .def_readonly("io_file", &IoPython::io_file)
+        .def(py::init<const std::string &, const std::string &>())
+        .def("read_number", [](const IoPython &io, const std::string &name) {
+            size_t v = 0;
+            io.read_number(name, v);
+            return v;
+        });
+#endif
 
     py::class_<Model>(module, "Model")
         .def_readwrite("parameters", &Model::parameters)

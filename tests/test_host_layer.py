@@ -39,12 +39,22 @@ def test_set_once_semantics():
     assert p.ncells() == 10
 
 
-def test_io_text_round_trip(tmp_path):
+def _io(kind, tmp_path):
+    if kind == "text":
+        return core.IoText(str(tmp_path) + "/model/")
+    if not hasattr(core, "IoPython"):
+        pytest.skip("HDF5 C library not found at build time")
+    return core.IoPython("hdf5", str(tmp_path) + "/model.hdf5")
+
+
+@pytest.mark.parametrize("kind", ["text", "hdf5"])
+def test_io_round_trip(tmp_path, kind):
+    """Model::write then Model::read through IoText and through the HDF5
+    back end (IoPython("hdf5"), io_python_hdf5.py) reproduce every member."""
     m = _small()
     sim = to_simulation(m)
-    io = core.IoText(str(tmp_path) + "/model/")
+    io = _io(kind, tmp_path)
     sim.write(io)
-    assert os.path.exists(str(tmp_path) + "/model/parameters.txt") or os.listdir(str(tmp_path) + "/model")
     back = core.Model()
     back.read(io)
     assert back.parameters.ncells() == m.ncells
@@ -79,6 +89,37 @@ def test_io_text_round_trip(tmp_path):
     rays = np.array(back.geometry.rays.rays)
     for r, a in enumerate(back.geometry.rays.antipod):
         np.testing.assert_allclose(rays[a], -rays[r], atol=1e-12)
+
+
+def test_hdf5_layout(tmp_path):
+    """The HDF5 file has the reference's layout: "<group>/.<name>" are
+    attributes (root attributes for the parameters), lists and arrays are
+    datasets at their paths (io_python_hdf5.py), booleans are words."""
+    if not hasattr(core, "IoPython"):
+        pytest.skip("HDF5 C library not found at build time")
+    import shutil
+    import subprocess
+    h5ls = shutil.which("h5ls") or "/opt/conda/bin/h5ls"
+    if not os.path.exists(h5ls):
+        pytest.skip("h5ls not available")
+    m = _small()
+    sim = to_simulation(m)
+    path = str(tmp_path) + "/model.hdf5"
+    sim.write(core.IoPython("hdf5", path))
+    listing = subprocess.run([h5ls, "-r", path], capture_output=True, text=True, check=True).stdout
+    assert f"/Geometry/Cells/position Dataset {{{m.ncells}, 3}}" in listing
+    assert f"/Geometry/Cells/neighbors Dataset {{{len(m.neighbors)}}}" in listing
+    assert "/Lines/LineProducingSpecies_0/Linedata Group" in listing
+    h5dump = os.path.join(os.path.dirname(h5ls), "h5dump")
+    attrs = subprocess.run([h5dump, "-A", path], capture_output=True, text=True, check=True).stdout
+    assert 'ATTRIBUTE "ncells"' in attrs and 'ATTRIBUTE "use_scattering"' in attrs
+    assert 'GROUP "Linedata"' in attrs and 'ATTRIBUTE "num"' in attrs
+    assert '"false"' in attrs   # booleans are words
+    io = core.IoPython("hdf5", path)
+    assert io.read_number(".ncells") == m.ncells
+    assert io.implementation == "io_python_hdf5"
+    with pytest.raises(Exception):
+        core.IoPython("text", path)
 
 
 def test_level_populations_without_spectral_discretisation_returns_minus_one(tmp_path):
@@ -210,3 +251,27 @@ def test_scattering_fills_radiation_u(tmp_path):
     assert float(np.max(np.abs(u - u_o) / np.abs(u_o))) < 1e-10
     sim.write(core.IoText(str(tmp_path) + "/"))
     assert os.path.exists(str(tmp_path) + "/Radiation/u.txt")
+
+
+@pytest.mark.gpu
+def test_simulation_read_from_hdf5_matches_oracle(tmp_path):
+    """A model written to HDF5 and read back into a Simulation (Model::read over
+    IoPython("hdf5")) gives the oracle's radiation field."""
+    if not hasattr(core, "IoPython"):
+        pytest.skip("HDF5 C library not found at build time")
+    from oracle.py_oracle import Oracle
+    m = synthetic.envelope_model(n=6, seed=3)
+    print("\n[host] envelope6 seed 3 through HDF5", flush=True)
+    path = str(tmp_path) + "/model.hdf5"
+    to_simulation(m).write(core.IoPython("hdf5", path))
+    sim = core.Simulation()
+    sim.read(core.IoPython("hdf5", path))
+    assert sim.compute_spectral_discretisation() == 0
+    assert sim.compute_boundary_intensities() == 0
+    assert sim.compute_LTE_level_populations() == 0
+    sim.precision = core.FP64
+    assert sim.compute_radiation_field() == 0
+    orc = Oracle(m).setup()
+    orc.compute_radiation_field()
+    J_o = orc.J().ravel()
+    assert float(np.max(np.abs(np.array(sim.radiation.J) - J_o) / np.abs(J_o))) < 1e-10
