@@ -275,6 +275,16 @@ int mgr_radiation_field_shape(mgr_ctx *ctx, int precision, int *waves_per_group,
                               int capacity, int *n_tried, int *tried_waves, int *tried_steps,
                               float *tried_ms);
 
+/* Out-of-window skip of the FP32 radiation-field kernel (no counterpart in
+ * the reference; on by default): steps at which no bin of a 64-bin wave comes
+ * within 11.5 line widths of any line are eliminated with eta = 0 and chi at
+ * its floor without evaluating the profiles, which are exactly 0 there in
+ * FP32; results are bit-identical either way.  Off evaluates every point.    */
+int mgr_set_window_skip(mgr_ctx *ctx, int enable);
+/* Interpolated points x 64-bin chunks of the cached trace, and how many of
+ * them the skip leaves out (chunks beyond the fifth are never skipped).      */
+int mgr_window_skip_stats(mgr_ctx *ctx, int64_t *point_chunks, int64_t *skipped);
+
 /* Kernel time of the last mgr_compute_radiation_field, measured with HIP
  * events on the context's stream (ms).                                        */
 int mgr_last_radiation_field_ms(mgr_ctx *ctx, double *ms);

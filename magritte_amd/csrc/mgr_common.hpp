@@ -49,6 +49,15 @@ constexpr double SQRT2      = 1.4142135623730951;
 constexpr double HDZ_SCALE  = 0.35355339059327373 / CHI_SCALE;   // 1 / (2 sqrt 2) / CHI_SCALE
 
 constexpr int WAVE = 64;                 // CDNA wavefront width
+// Packing of a step's counts (the FP32 trace record and the staged step):
+// sub-points n (bits 0-13, at most 10000), the first sub-point of the step's
+// own cell `half` (bits 14-26), and one bit per 64-bin chunk
+// k < STEP_MASK_CHUNKS (bit 27 + k) set when every bin of the chunk is more
+// than MASK_WINDOW line widths from every line at every sub-point of the step:
+// the FP32 profile exp2(-x^2 log2 e) is then exactly zero (below 2^-149).
+constexpr int STEP_N_MASK = 0x3FFF, STEP_HALF_SHIFT = 14, STEP_HALF_MASK = 0x1FFF, STEP_OUT_SHIFT = 27;
+constexpr int STEP_MASK_CHUNKS = 5;
+constexpr double MASK_WINDOW = 11.5;
 constexpr int MAX_INTERPOLATION = 10000; // geometry.tpp:138
 
 // One neighbour entry of the tracer: the neighbour's index and position, so
@@ -222,6 +231,11 @@ struct mgr_ctx {
     uint64_t *shift_bits = nullptr;  // [2] min and max shift over all steps (FP64 bits)
     double   shift_min = 1.0, shift_max = 1.0;
     bool     traced = false;
+    // per step, bit k set when no bin of 64-bin chunk k meets a line window at
+    // any of the step's sub-points; valid for the current trace and frequencies
+    uint8_t *step_mask = nullptr;   // [total segments]
+    bool     masks_valid = false;
+    bool     rf_window_skip = true;
     int64_t  sum_ndep = 0, n_pairs_solved = 0;
 
     // device: rest-frame trace of one image ray pair, entry 2o the antipode
@@ -280,5 +294,6 @@ int launch_ng(mgr_ctx *ctx);
 int read_convergence(mgr_ctx *ctx, mgr_conv *conv);
 int set_active(mgr_ctx *ctx, bool all);   // all: every model of the batch iterates
 int launch_trace_image(mgr_ctx *ctx, int64_t ray);   // rest-frame trace of ray and its antipode
+int launch_step_masks(mgr_ctx *ctx);   // out-of-window bits of the cached steps (when stale)
 void free_image(mgr_ctx *ctx);
 }  // namespace mgr

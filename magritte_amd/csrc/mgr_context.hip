@@ -613,6 +613,7 @@ int free_model(mgr_ctx *ctx)
     dfree(ctx->Jeff); dfree(ctx->Jdif); dfree(ctx->err_flag); dfree(ctx->seg_off); dfree(ctx->nseg);
     dfree(ctx->npts); dfree(ctx->seg_cell); dfree(ctx->seg_shift); dfree(ctx->seg_dZ);
     dfree(ctx->seg_f32); dfree(ctx->seg_hdz32); dfree(ctx->far_cell); dfree(ctx->far_shift);
+    dfree(ctx->step_mask);
     dfree(ctx->shift_bits); dfree(ctx->partials); dfree(ctx->reduced);
     dfree(ctx->d_cell_model); dfree(ctx->d_cell_qbase); dfree(ctx->d_m_R); dfree(ctx->d_m_rbase);
     dfree(ctx->d_m_active); dfree(ctx->d_chunk_cell); dfree(ctx->d_m_chunk_off); dfree(ctx->d_mconv);
@@ -750,6 +751,7 @@ static int spectral_discretisation(mgr_ctx *ctx, bool image, double width)
     MGR_HIP(hipStreamSynchronize(ctx->stream));
     ctx->spectral_set = true;
     ctx->spectral_image = image;
+    ctx->masks_valid = false;   // the out-of-window bits depend on the frequencies
     return MGR_OK;
 }
 

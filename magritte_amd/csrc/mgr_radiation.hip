@@ -92,7 +92,7 @@ template <> struct Prec<double> {
 // belongs to the current cell for m < half, to `cell` otherwise.
 template <typename real>
 struct Step {
-    int32_t cell, nh, pad0, pad1;   // nh = n | half << 16
+    int32_t cell, nh, pad0, pad1;   // nh = n | half << 14 | out-of-window bits << 27 (STEP_*)
     real    s0, ds, dz, pad2;
 };
 
@@ -119,6 +119,7 @@ struct RFArgs {
     const int32_t *far_cell;                  // last emitted point of each (origin, ray), -1: none
     const double  *far_shift;
     const double  *origin_shift;              // FP64 shift of the origin per entry (null: 1, co-moving)
+    const uint8_t *step_mask;                 // per step, out-of-window bit per 64-bin chunk (or null)
     const int32_t *cell2bdy, *antipod;        // antipod: global ray indices
     const int32_t *cell_model;                // [N] model of each cell
     const int64_t *cell_qbase;                // [N] first (origin, ray) entry of the cell
@@ -347,7 +348,7 @@ __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, in
             // split at trace time with the same FP64 arithmetic as below
             const int4 v = a.seg_f32[s0 + j];
             s.cell = v.x;
-            s.nh = v.y;
+            s.nh = v.y | (a.step_mask ? (int)a.step_mask[s0 + j] << STEP_OUT_SHIFT : 0);
             s.s0 = __int_as_float(v.z);
             s.ds = __int_as_float(v.w);
             s.dz = a.seg_hdz32[s0 + j];
@@ -358,7 +359,7 @@ __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, in
             const double sc = (j > 0) ? a.seg_shift[s0 + j - 1] : (a.origin_shift ? a.origin_shift[q] : 1.0);
             const StepSplit sp = split_step(sc, sn, z, dsm);
             s.cell = c;
-            s.nh = sp.n | ((sp.interp ? sp.half : 0) << 16);
+            s.nh = sp.n | ((sp.interp ? sp.half : 0) << STEP_HALF_SHIFT);
             s.s0 = sp.interp ? sc : sn;               // absolute FP64 shift
             s.ds = sp.dshift;
             s.dz = sp.dZ * HDZ_SCALE;                 // dz'
@@ -470,11 +471,13 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
                                                   int32_t o, int64_t q, real I_far, real S_o,
                                                   real chi_o, double dsm,
                                                   step_t<real, X64> *__restrict__ st,
-                                                  typename Prec<real>::row_t *__restrict__ rw)
+                                                  typename Prec<real>::row_t *__restrict__ rw,
+                                                  int chunk = STEP_MASK_CHUNKS)
 {
     using row_t = typename Prec<real>::row_t;
     Sweep<real> sw;
     sw.init(I_far);
+    const real chi_floor = (real)(CHI_FLOOR * CHI_SCALE);
     const int ns = a.nseg[q];
     const int L = a.L;
     const int CH = a.ch;
@@ -484,12 +487,21 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
         __syncthreads();   // the previous chunk is consumed by every wave
         stage_chunk<real, X64>(a, o, q, a_ - 1, cnt, dsm, st, rw);
         for (int e = cnt - 1; e >= 1; --e) {
-            const int nh = rfl_int(st[e].nh);   // one readfirstlane: n and half are wave-uniform
-            const int n = nh & 0xFFFF;
-            const int half = (int)((unsigned)nh >> 16);
+            const int nh = rfl_int(st[e].nh);   // one readfirstlane: n, half and the chunk bits
+            const int n = nh & STEP_N_MASK;
+            const int half = (nh >> STEP_HALF_SHIFT) & STEP_HALF_MASK;
+            const real hdz = (real)rfl_real(st[e].dz);
+            if (chunk < STEP_MASK_CHUNKS && (((unsigned)nh >> (STEP_OUT_SHIFT + chunk)) & 1u)) {
+                // no bin of this wave meets a line window in this step: every
+                // sub-point has eta = 0 and chi at the floor, as evaluated
+                for (int m = n; m >= 1; --m) {
+                    sw.step(chi_floor);
+                    sw.pend((real)0, chi_floor, hdz);
+                }
+                continue;
+            }
             const auto s0 = st[e].s0;   // stays in a VGPR: one scalar operand per FMA
             const auto ds = st[e].ds;
-            const real hdz = (real)rfl_real(st[e].dz);
             // sub-points m >= half lie in the step's cell (row e), m < half in
             // the previous one (row e-1); two loops keep each row's address
             // loop-invariant
@@ -539,8 +551,8 @@ __device__ __forceinline__ void sweep_outward(const RFArgs<real> &a, const LaneC
         stage_chunk<real, X64>(a, o, q, a_ - 1, cnt, dsm, st, rw);
         for (int e = 1; e < cnt; ++e) {
             const int nh = rfl_int(st[e].nh);   // one readfirstlane: n and half are wave-uniform
-            const int n = nh & 0xFFFF;
-            const int half = (int)((unsigned)nh >> 16);
+            const int n = nh & STEP_N_MASK;
+            const int half = (nh >> STEP_HALF_SHIFT) & STEP_HALF_MASK;
             const auto s0 = st[e].s0;
             const auto ds = st[e].ds;
             const real hdz = (real)rfl_real(st[e].dz);
@@ -616,8 +628,11 @@ __device__ __forceinline__ void origin_point(const RFArgs<real> &a, double nu_f,
     S_o = eta_o * Prec<real>::rcp(chi_o);
 }
 
+// FP32: at least 8 waves per SIMD (at most 64 VGPRs), which hide the LDS
+// latency of the line rows; FP64 is left to the compiler.
 template <typename real>
-__global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<real> a)
+__global__ __launch_bounds__(WAVE * MAX_WAVES) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 ? 8 : 1)))
+void k_radiation_field(RFArgs<real> a)
 {
     using row_t = typename Prec<real>::row_t;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -636,7 +651,7 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
     if (!a.m_active[mdl]) return;         // converged model of a batch: whole workgroup
     const int group = logical - (o - a.o_begin) * a.ngroups;
     const int wave = threadIdx.x / WAVE;
-    const int chunk = group * a.nwg + wave;
+    const int chunk = rfl_int(group * a.nwg + wave);   // wave-uniform: scalar skip tests
     const int F = a.F;
     const int f = chunk * WAVE + (int)(threadIdx.x & (WAVE - 1));
     const bool active = f < F;
@@ -673,16 +688,16 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_radiation_field(RFArgs<rea
         real u, lam;
         if (n_r > 0 && n_a > 0) {
             const real Ia = (real)(far_intensity(a, qa, nu_f, F, err) * I_SCALE);
-            const Sweep<real> A = sweep_side<real>(a, lc, o, qa, Ia, S_o, chi_o, dsm, st, rw);
+            const Sweep<real> A = sweep_side<real>(a, lc, o, qa, Ia, S_o, chi_o, dsm, st, rw, chunk);
             const real Ir = (real)(far_intensity(a, qr, nu_f, F, err) * I_SCALE);
-            const Sweep<real> B = sweep_side<real>(a, lc, o, qr, Ir, S_o, chi_o, dsm, st, rw);
+            const Sweep<real> B = sweep_side<real>(a, lc, o, qr, Ir, S_o, chi_o, dsm, st, rw, chunk);
             const real den = Prec<real>::rcp(A.y + B.y);
             u = (A.p + B.p) * den;
             lam = (A.c + B.c) * den;
         } else {
             const int64_t qs = (n_a > 0) ? qa : qr;
             const real Is = (real)(far_intensity(a, qs, nu_f, F, err) * I_SCALE);
-            const Sweep<real> A = sweep_side<real>(a, lc, o, qs, Is, S_o, chi_o, dsm, st, rw);
+            const Sweep<real> A = sweep_side<real>(a, lc, o, qs, Is, S_o, chi_o, dsm, st, rw, chunk);
             real I_o = 0;
             if (bo < 0) err |= ERR_EMPTY_SIDE;
             else I_o = (real)(a.I_bdy[(int64_t)bo * F + fa] * I_SCALE);
@@ -728,7 +743,7 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_image(RFArgs<real> a, doub
     const int32_t o = a.o_begin + (int32_t)(logical / a.ngroups);
     const int group = logical - (o - a.o_begin) * a.ngroups;
     const int wave = threadIdx.x / WAVE;
-    const int chunk = group * a.nwg + wave;
+    const int chunk = rfl_int(group * a.nwg + wave);   // wave-uniform: scalar skip tests
     const int F = a.F;
     const int f = chunk * WAVE + (int)(threadIdx.x & (WAVE - 1));
     const bool active = f < F;
@@ -889,6 +904,8 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     a.far_shift = ctx->far_shift;
     a.origin_shift = nullptr;
     a.u_out = ctx->u_store;
+    // the out-of-window bits are exact for the FP32 profile only
+    a.step_mask = (sizeof(real) == 4 && ctx->masks_valid && ctx->rf_window_skip) ? ctx->step_mask : nullptr;
     hipLaunchKernelGGL(k_radiation_field<real>, dim3(grid_of(a)), dim3(WAVE * a.nwg), lds_bytes(ctx, a),
                        ctx->stream, a);
     MGR_LAUNCH_CHECK();
@@ -1026,6 +1043,7 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
         if (rc) return rc;
     }
     int rc0 = set_active(ctx, true);
+    if (!rc0) rc0 = launch_step_masks(ctx);
     if (!rc0) rc0 = tune_radiation_field(ctx, precision);
     if (rc0) return rc0;
     hipEvent_t e0, e1;
@@ -1124,6 +1142,14 @@ extern "C" int mgr_set_scattering(mgr_ctx *ctx, int use_scattering)
         ctx->u_store = nullptr;
     }
     ctx->graph_precision = -1;   // captured launches hold the old pointer
+    return MGR_OK;
+}
+
+extern "C" int mgr_set_window_skip(mgr_ctx *ctx, int enable)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    ctx->rf_window_skip = enable != 0;
+    ctx->graph_precision = -1;   // captured launches hold the old choice
     return MGR_OK;
 }
 

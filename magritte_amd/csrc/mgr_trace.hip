@@ -82,7 +82,7 @@ __device__ __forceinline__ Walk trace_walk(int64_t N, const double *__restrict__
             const int half = sp.interp ? sp.half : 0;
             const float s0 = (float)((sp.interp ? shift_crt : shift_nxt) - 1.0);
             const float ds = (float)sp.dshift;
-            seg_f32[w] = make_int4(nxt, sp.n | (half << 16), __float_as_int(s0), __float_as_int(ds));
+            seg_f32[w] = make_int4(nxt, sp.n | (half << STEP_HALF_SHIFT), __float_as_int(s0), __float_as_int(ds));
             seg_hdz32[w] = (float)(sp.dZ * HDZ_SCALE);
             if (sp.n > 0) {   // sub-point m = n >= half belongs to the next cell
                 k.last_cell = nxt;
@@ -221,6 +221,124 @@ __global__ void k_expand_ray(int32_t o, int64_t q, const int64_t *__restrict__ s
     *n_out = n;
 }
 
+
+// Shift intervals in which some bin of a 64-bin chunk can meet a line window.
+// Bin f meets line l at shift s when |s nu_f - nu_l| <= MASK_WINDOW nu_l pw, i.e.
+// s in [(1 - MASK_WINDOW pw) c, (1 + MASK_WINDOW pw) c] with c = nu_l / nu_f.
+// For each line, the bins whose c can reach the trace's shift range
+// [smin, smax] at the widest width pw_max form one contiguous range of f
+// (c falls with f); interval i of chunk k is (a, b) = (min c, max c) over that
+// range, to be widened by the step's own width.  One thread per (origin,
+// chunk); cnt = -1 when a chunk needs more than MAX_WIN intervals (never
+// skipped).
+constexpr int MAX_WIN = 4;
+
+__global__ void k_chunk_windows(int64_t N, int F, int nchunk, const LineTables *__restrict__ tb,
+                                const double *__restrict__ nu, double smin, double smax, double pw_max,
+                                double *__restrict__ win, int8_t *__restrict__ cnt)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * nchunk) return;
+    const int64_t o = t / nchunk;
+    const int k = (int)(t % nchunk);
+    const int f0 = k * WAVE, f1 = min(F, f0 + WAVE);
+    const double reach = MASK_WINDOW * pw_max * 1.01;
+    int n = 0;
+    for (int l = 0; l < tb->L && n >= 0; l++) {
+        const double nl = tb->line[l];
+        double a = 0.0, b = 0.0;
+        bool any = false;
+        for (int f = f0; f < f1; f++) {
+            const double c = nl / nu[o * F + f];
+            if (c * (1.0 - reach) <= smax * (1.0 + 1e-9) && c * (1.0 + reach) >= smin * (1.0 - 1e-9)) {
+                if (!any) { a = c; b = c; any = true; }
+                a = fmin(a, c);
+                b = fmax(b, c);
+            }
+        }
+        if (!any) continue;
+        if (n == MAX_WIN) { n = -1; break; }
+        win[((o * STEP_MASK_CHUNKS + k) * MAX_WIN + n) * 2 + 0] = a;
+        win[((o * STEP_MASK_CHUNKS + k) * MAX_WIN + n) * 2 + 1] = b;
+        n++;
+    }
+    cnt[o * STEP_MASK_CHUNKS + k] = (int8_t)n;
+}
+
+// Out-of-window bits of every cached step: the step's sub-points lie between
+// the shifts of its two ends and in its two cells, so bit k is set when the
+// shift range [lo, hi] misses every interval of chunk k widened by the wider
+// of the two cells' widths.  One thread per (origin, ray).
+__global__ void k_step_masks(int64_t N, int Rmax, int nchunk, const int32_t *__restrict__ cell_model,
+                             const int64_t *__restrict__ cell_qbase, const int32_t *__restrict__ m_R,
+                             const double *__restrict__ pw, const int64_t *__restrict__ seg_off,
+                             const int32_t *__restrict__ nseg, const int32_t *__restrict__ seg_cell,
+                             const double *__restrict__ seg_shift, const double *__restrict__ win,
+                             const int8_t *__restrict__ cnt, uint8_t *__restrict__ mask)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * Rmax) return;
+    const int32_t o = (int32_t)(t / Rmax);
+    const int r = (int)(t % Rmax);
+    if (r >= m_R[cell_model[o]]) return;
+    const int64_t q = cell_qbase[o] + r;
+    const int64_t s0 = seg_off[q];
+    const int ns = nseg[q];
+    int32_t c_prev = o;
+    double s_prev = 1.0;   // co-moving shift of the origin
+    for (int j = 0; j < ns; j++) {
+        const int32_t c = seg_cell[s0 + j];
+        const double s = seg_shift[s0 + j];
+        const double lo = fmin(s_prev, s), hi = fmax(s_prev, s);
+        const double w = MASK_WINDOW * fmax(pw[c_prev], pw[c]);
+        unsigned bits = 0;
+        for (int k = 0; k < nchunk; k++) {
+            const int n = cnt[(int64_t)o * STEP_MASK_CHUNKS + k];
+            if (n < 0) continue;
+            bool out = true;
+            for (int i = 0; i < n; i++) {
+                const double a = win[(((int64_t)o * STEP_MASK_CHUNKS + k) * MAX_WIN + i) * 2 + 0];
+                const double b = win[(((int64_t)o * STEP_MASK_CHUNKS + k) * MAX_WIN + i) * 2 + 1];
+                if (hi >= (1.0 - w) * a && lo <= (1.0 + w) * b) out = false;
+            }
+            if (out) bits |= 1u << k;
+        }
+        mask[s0 + j] = (uint8_t)bits;
+        c_prev = c;
+        s_prev = s;
+    }
+}
+
+// Sums over the cached steps of (points x chunks) and of (points x chunks
+// whose out-of-window bit is set): one thread per step, a block sum, one
+// atomic add per block.
+__global__ void k_mask_stats(int64_t T, int nchunk, const int4 *__restrict__ seg_f32,
+                             const uint8_t *__restrict__ mask, unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long s_all[256], s_skip[256];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long all = 0, skip = 0;
+    if (t < T) {
+        const unsigned n = (unsigned)(seg_f32[t].y & STEP_N_MASK);
+        all = (unsigned long long)n * nchunk;
+        skip = (unsigned long long)n * __popc((unsigned)mask[t] & ((1u << nchunk) - 1u));
+    }
+    s_all[threadIdx.x] = all;
+    s_skip[threadIdx.x] = skip;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s_all[threadIdx.x] += s_all[threadIdx.x + w];
+            s_skip[threadIdx.x] += s_skip[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&out[0], s_all[0]);
+        atomicAdd(&out[1], s_skip[0]);
+    }
+}
+
 }  // namespace
 
 namespace mgr {
@@ -284,13 +402,17 @@ int launch_trace(mgr_ctx *ctx)
     ctx->sum_ndep = sum_ndep;
     ctx->n_pairs_solved = solved;
 
+    // the ALI graphs captured pointers into the old cache
+    ctx->graph_precision = -1;
+    ctx->masks_valid = false;
     if (ctx->seg_cell) {
         (void)hipFree(ctx->seg_cell); (void)hipFree(ctx->seg_shift); (void)hipFree(ctx->seg_dZ);
-        (void)hipFree(ctx->seg_f32); (void)hipFree(ctx->seg_hdz32);
+        (void)hipFree(ctx->seg_f32); (void)hipFree(ctx->seg_hdz32); (void)hipFree(ctx->step_mask);
         ctx->seg_cell = nullptr; ctx->seg_shift = nullptr; ctx->seg_dZ = nullptr;
-        ctx->seg_f32 = nullptr; ctx->seg_hdz32 = nullptr;
+        ctx->seg_f32 = nullptr; ctx->seg_hdz32 = nullptr; ctx->step_mask = nullptr;
     }
     const int64_t T = std::max<int64_t>(ctx->total_segments, 1);
+    MGR_HIP(hipMalloc((void **)&ctx->step_mask, T * sizeof(uint8_t)));
     MGR_HIP(hipMalloc((void **)&ctx->seg_cell, T * sizeof(int32_t)));
     MGR_HIP(hipMalloc((void **)&ctx->seg_shift, T * sizeof(double)));
     MGR_HIP(hipMalloc((void **)&ctx->seg_dZ, T * sizeof(double)));
@@ -311,6 +433,69 @@ int launch_trace(mgr_ctx *ctx)
     ctx->traced = true;
     return MGR_OK;
 }
+
+// Computes the out-of-window bits of the cached steps for the current
+// frequencies (once per trace and spectral discretisation).  Models with more
+// than STEP_MASK_CHUNKS chunks of 64 bins get bits for the first ones only.
+int launch_step_masks(mgr_ctx *ctx)
+{
+    if (ctx->masks_valid || !ctx->traced || !ctx->spectral_set) return MGR_OK;
+    const int64_t N = ctx->N;
+    const int nchunk = (int)std::min<int64_t>((ctx->F + WAVE - 1) / WAVE, STEP_MASK_CHUNKS);
+    double *win = nullptr;
+    int8_t *cnt = nullptr;
+    MGR_HIP(hipMalloc((void **)&win, (size_t)N * STEP_MASK_CHUNKS * MAX_WIN * 2 * sizeof(double)));
+    if (hipMalloc((void **)&cnt, (size_t)N * STEP_MASK_CHUNKS) != hipSuccess) {
+        (void)hipFree(win);
+        return ctx->fail(MGR_ERR_DEVICE, "out of device memory");
+    }
+    hipLaunchKernelGGL(k_chunk_windows, dim3((unsigned)((N * nchunk + 127) / 128)), dim3(128), 0, ctx->stream, N,
+                       (int)ctx->F, nchunk, ctx->tables, ctx->nu, ctx->shift_min, ctx->shift_max, ctx->pw_max,
+                       win, cnt);
+    hipLaunchKernelGGL(k_step_masks, dim3((unsigned)((N * ctx->R + 127) / 128)), dim3(128), 0, ctx->stream, N,
+                       (int)ctx->R, nchunk, ctx->d_cell_model, ctx->d_cell_qbase, ctx->d_m_R, ctx->pw, ctx->seg_off,
+                       ctx->nseg, ctx->seg_cell, ctx->seg_shift, win, cnt, ctx->step_mask);
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(win);
+    (void)hipFree(cnt);
+    if (le != hipSuccess || se != hipSuccess) return ctx->fail(MGR_ERR_DEVICE, "step-mask kernels failed");
+    ctx->masks_valid = true;
+    return MGR_OK;
+}
+
+}  // namespace mgr
+
+// Point-chunks of the cached trace (interpolation sub-points x 64-bin chunks
+// with a bit, up to STEP_MASK_CHUNKS) and those the FP32 kernel skips.
+extern "C" int mgr_window_skip_stats(mgr_ctx *ctx, int64_t *point_chunks, int64_t *skipped)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (!ctx->traced) return ctx->fail(MGR_ERR_STATE, "mgr_trace first");
+    if (!ctx->spectral_set) return ctx->fail(MGR_ERR_STATE, "compute_spectral_discretisation first");
+    MGR_HIP(hipSetDevice(ctx->device));
+    int rc = mgr::launch_step_masks(ctx);
+    if (rc) return rc;
+    const int64_t T = ctx->total_segments;
+    const int nchunk = (int)std::min<int64_t>((ctx->F + mgr::WAVE - 1) / mgr::WAVE, mgr::STEP_MASK_CHUNKS);
+    unsigned long long *d = nullptr, h[2] = {0, 0};
+    MGR_HIP(hipMalloc((void **)&d, sizeof(h)));
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(h), ctx->stream);
+    if (e == hipSuccess && T > 0) {
+        hipLaunchKernelGGL(k_mask_stats, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, ctx->stream, T, nchunk,
+                           ctx->seg_f32, ctx->step_mask, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return ctx->fail(MGR_ERR_DEVICE, std::string("window-skip stats: ") + hipGetErrorString(e));
+    if (point_chunks) *point_chunks = (int64_t)h[0];
+    if (skipped) *skipped = (int64_t)h[1];
+    return MGR_OK;
+}
+
+namespace mgr {
 
 void free_image(mgr_ctx *ctx)
 {

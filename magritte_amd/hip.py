@@ -54,6 +54,8 @@ SIGNATURES = {
     "mgr_get_npoints_on_rays": ([_vp, _vp], _i),
     "mgr_get_npoints_on_rays_rest": ([_vp, _vp], _i),
     "mgr_set_scattering": ([_vp, _i], _i),
+    "mgr_set_window_skip": ([_vp, _i], _i),
+    "mgr_window_skip_stats": ([_vp, _vp, _vp], _i),
     "mgr_get_u": ([_vp, _vp], _i),
     "mgr_compute_spectral_discretisation_image": ([_vp, _d], _i),
     "mgr_compute_image": ([_vp, _i64, _i, _vp, _vp], _i),
@@ -270,6 +272,17 @@ class Context:
 
     def set_scattering(self, use_scattering=True):
         self._check(self.lib.mgr_set_scattering(self.h, 1 if use_scattering else 0), "set_scattering")
+
+    def set_window_skip(self, enable=True):
+        """FP32 kernel: eliminate steps out of every line window without
+        evaluating their profiles (bit-identical; on by default)."""
+        self._check(self.lib.mgr_set_window_skip(self.h, 1 if enable else 0), "set_window_skip")
+
+    def window_skip_stats(self):
+        """(point-chunks of the trace, point-chunks the FP32 kernel skips)."""
+        a, b = C.c_int64(0), C.c_int64(0)
+        self._check(self.lib.mgr_window_skip_stats(self.h, C.byref(a), C.byref(b)), "window_skip_stats")
+        return a.value, b.value
 
     def u(self):
         """Radiation::u [nrays/2, N, F] of the last radiation field (scattering on)."""

@@ -1,0 +1,89 @@
+"""The out-of-window skip of the FP32 radiation-field kernel against the full
+evaluation, through the C-ABI.
+
+A step whose sub-points keep every bin of a 64-bin wave more than 11.5 line
+widths from every line has FP32 profiles of exactly 0 (exp2 of less than
+-149), so eta = 0 and chi sits at its floor; the skip eliminates such a step
+without evaluating the profiles.  Bar: J, the Lambda diagonal, Radiation::u
+and the populations after ALI iterations are bit-identical with the skip on
+and off, and a model with velocity fields does skip a share of its points.
+The masks follow the frequencies: after an image discretisation and back the
+line-grid result is unchanged.
+"""
+import numpy as np
+import pytest
+
+from magritte_amd.setup import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def _jump_model():
+    m = synthetic.regular_grid_model(n=6)
+    v = np.zeros((m.ncells, 3))
+    v[m.position[:, 0] > 0.5 * m.position[:, 0].max(), 0] = 8.4e-4
+    m.velocity = v
+    return m
+
+
+MODELS = {
+    "envelope12": (lambda: synthetic.envelope_model(n=12, seed=5), True),
+    "stress8": (lambda: synthetic.stress_model(n=8, seed=11), True),
+    "regular8": (lambda: synthetic.regular_grid_model(n=8), False),
+    "jump6": (_jump_model, False),
+}
+
+
+def _field(ctx):
+    ctx.compute_radiation_field("fp32")
+    return ctx.J(), ctx.lambda_diag(), ctx.u()
+
+
+@pytest.mark.parametrize("name", list(MODELS))
+def test_skip_is_bit_identical(name):
+    from magritte_amd.hip import Context
+    make, must_skip = MODELS[name]
+    print(f"\n[skip] model {name}", flush=True)
+    ctx = Context(make()).setup()
+    ctx.set_scattering(True)
+    J1, L1, u1 = _field(ctx)
+    total, skipped = ctx.window_skip_stats()
+    print(f"[skip] {name}: {skipped} of {total} point-chunks skipped ({skipped / max(total, 1):.1%})", flush=True)
+    assert 0 <= skipped <= total
+    if must_skip:
+        assert skipped > 0
+    ctx.set_window_skip(False)
+    J0, L0, u0 = _field(ctx)
+    assert np.array_equal(J1, J0)
+    assert np.array_equal(L1, L0)
+    assert np.array_equal(u1, u0)
+
+
+def test_skip_in_ali_iterations():
+    from magritte_amd.hip import Context
+    print("\n[skip] envelope10 ALI", flush=True)
+    ctx = Context(synthetic.envelope_model(n=10, seed=9)).setup()
+    pops = {}
+    for on in (True, False):
+        ctx.set_window_skip(on)
+        ctx.compute_LTE_level_populations()
+        for _ in range(3):
+            ctx.ali_iteration(ng=False, precision="fp32")
+        pops[on] = ctx.populations()
+    assert np.array_equal(pops[True], pops[False])
+
+
+def test_masks_follow_the_frequencies():
+    from magritte_amd.hip import Context
+    print("\n[skip] envelope10 line -> image -> line grid", flush=True)
+    ctx = Context(synthetic.envelope_model(n=10, seed=9)).setup()
+    ctx.compute_radiation_field("fp32")
+    J1 = ctx.J()
+    ctx.compute_spectral_discretisation_image(3.0e-5)
+    ctx.compute_boundary_intensities()
+    Im, Ip = ctx.compute_image(0, "fp32")
+    assert np.all(np.isfinite(Im)) and np.all(np.isfinite(Ip))
+    ctx.compute_spectral_discretisation()
+    ctx.compute_boundary_intensities()
+    ctx.compute_radiation_field("fp32")
+    assert np.array_equal(ctx.J(), J1)
