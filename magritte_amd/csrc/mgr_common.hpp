@@ -79,6 +79,15 @@ struct LineTables {
 };
 constexpr int MAX_LINES = 64;
 
+// Bin held by lane slot i (0 <= i < F = L Q) of the radiation-field kernel:
+// the bin itself, or in root-major order root i / L of sorted line i % L (bin
+// l Q + z), so that each 64-bin wave holds the same few roots of every line
+// and leaves all line windows at the same shifts.
+__host__ __device__ inline int lane_bin(int i, int L, int F, bool root_major)
+{
+    return root_major ? (i % L) * (F / L) + i / L : i;
+}
+
 #define MGR_HIP(call)                                                              \
     do {                                                                           \
         hipError_t e_ = (call);                                                    \
@@ -236,6 +245,10 @@ struct mgr_ctx {
     uint8_t *step_mask = nullptr;   // [total segments]
     bool     masks_valid = false;
     bool     rf_window_skip = true;
+    // lane order of the radiation-field kernel (lane_bin), chosen with the
+    // masks: root-major when no bin can meet a second line's window
+    bool     root_major = false;
+    double   root_max = 0.0;            // max |root| of the line quadrature
     int64_t  sum_ndep = 0, n_pairs_solved = 0;
 
     // device: rest-frame trace of one image ray pair, entry 2o the antipode

@@ -528,6 +528,8 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
     if ((rc = upload(ctx, &ctx->energy, m->energy, V))) return rc;
     if ((rc = upload(ctx, &ctx->lweight, m->weight, V))) return rc;
     if ((rc = upload(ctx, &ctx->roots, m->roots, Q))) return rc;
+    ctx->root_max = 0.0;
+    for (int64_t z = 0; z < Q; z++) ctx->root_max = std::max(ctx->root_max, std::fabs(m->roots[z]));
     if ((rc = upload(ctx, &ctx->qweights, m->qweights, Q))) return rc;
     if ((rc = upload(ctx, &ctx->cp_num, cpn.data(), ctx->ncolpar))) return rc;
     ctx->h_cp_num.assign(m->colpar_num, m->colpar_num + ctx->ncolpar);

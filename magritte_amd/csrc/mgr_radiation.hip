@@ -120,6 +120,7 @@ struct RFArgs {
     const double  *far_shift;
     const double  *origin_shift;              // FP64 shift of the origin per entry (null: 1, co-moving)
     const uint8_t *step_mask;                 // per step, out-of-window bit per 64-bin chunk (or null)
+    bool           root_major;                // lane order (lane_bin)
     const int32_t *cell2bdy, *antipod;        // antipod: global ray indices
     const int32_t *cell_model;                // [N] model of each cell
     const int64_t *cell_qbase;                // [N] first (origin, ray) entry of the cell
@@ -653,10 +654,10 @@ void k_radiation_field(RFArgs<real> a)
     const int wave = threadIdx.x / WAVE;
     const int chunk = rfl_int(group * a.nwg + wave);   // wave-uniform: scalar skip tests
     const int F = a.F;
-    const int f = chunk * WAVE + (int)(threadIdx.x & (WAVE - 1));
-    const bool active = f < F;
-    const int fa = active ? f : F - 1;
-    const double nu_f = a.nu[(int64_t)o * F + fa];
+    const int slot = chunk * WAVE + (int)(threadIdx.x & (WAVE - 1));
+    const bool active = slot < F;
+    const int f = lane_bin(active ? slot : F - 1, a.L, F, a.root_major);
+    const double nu_f = a.nu[(int64_t)o * F + f];
     const double dsm = a.dshift_max[o];
     const int32_t bo = a.cell2bdy[o];
     const row_t *orow = a.rows + (int64_t)o * a.L;
@@ -680,7 +681,7 @@ void k_radiation_field(RFArgs<real> a)
         if (n_r == 0 && n_a == 0) {
             // ndep == 1: u = I_bdy of the origin's boundary entry (sim_radiation.tpp:243-264)
             if (bo < 0) { err |= ERR_EMPTY_SIDE; continue; }
-            const double Ib = a.I_bdy[(int64_t)bo * F + fa];
+            const double Ib = a.I_bdy[(int64_t)bo * F + f];
             J_acc += w2 * Ib;
             if (a.u_out && active) a.u_out[((int64_t)r * a.N + o) * F + f] = Ib;
             continue;
@@ -700,7 +701,7 @@ void k_radiation_field(RFArgs<real> a)
             const Sweep<real> A = sweep_side<real>(a, lc, o, qs, Is, S_o, chi_o, dsm, st, rw, chunk);
             real I_o = 0;
             if (bo < 0) err |= ERR_EMPTY_SIDE;
-            else I_o = (real)(a.I_bdy[(int64_t)bo * F + fa] * I_SCALE);
+            else I_o = (real)(a.I_bdy[(int64_t)bo * F + f] * I_SCALE);
             const real den = Prec<real>::rcp(A.y + (real)SQRT2);
             u = (A.p + (real)SQRT2 * I_o) * den;
             lam = A.c * den;
@@ -906,6 +907,7 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     a.u_out = ctx->u_store;
     // the out-of-window bits are exact for the FP32 profile only
     a.step_mask = (sizeof(real) == 4 && ctx->masks_valid && ctx->rf_window_skip) ? ctx->step_mask : nullptr;
+    a.root_major = ctx->root_major;
     hipLaunchKernelGGL(k_radiation_field<real>, dim3(grid_of(a)), dim3(WAVE * a.nwg), lds_bytes(ctx, a),
                        ctx->stream, a);
     MGR_LAUNCH_CHECK();

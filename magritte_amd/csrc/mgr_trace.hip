@@ -222,18 +222,19 @@ __global__ void k_expand_ray(int32_t o, int64_t q, const int64_t *__restrict__ s
 }
 
 
-// Shift intervals in which some bin of a 64-bin chunk can meet a line window.
-// Bin f meets line l at shift s when |s nu_f - nu_l| <= MASK_WINDOW nu_l pw, i.e.
-// s in [(1 - MASK_WINDOW pw) c, (1 + MASK_WINDOW pw) c] with c = nu_l / nu_f.
-// For each line, the bins whose c can reach the trace's shift range
-// [smin, smax] at the widest width pw_max form one contiguous range of f
-// (c falls with f); interval i of chunk k is (a, b) = (min c, max c) over that
-// range, to be widened by the step's own width.  One thread per (origin,
+// Shift intervals in which some bin of a 64-bin chunk (the bins of lane slots
+// 64k .. 64k + 63, lane_bin) can meet a line window.  Bin f meets line l at
+// shift s when |s nu_f - nu_l| <= MASK_WINDOW nu_l pw, i.e. s in
+// [(1 - MASK_WINDOW pw) c, (1 + MASK_WINDOW pw) c] with c = nu_l / nu_f.  Per
+// line, the hull (a, b) = (min c, max c) over the chunk's bins whose c can
+// reach the trace's shift range [smin, smax] at the widest width pw_max covers
+// all of them; it is widened by the step's own width when tested.  Hulls that
+// overlap (the same roots of different lines) merge.  One thread per (origin,
 // chunk); cnt = -1 when a chunk needs more than MAX_WIN intervals (never
 // skipped).
 constexpr int MAX_WIN = 4;
 
-__global__ void k_chunk_windows(int64_t N, int F, int nchunk, const LineTables *__restrict__ tb,
+__global__ void k_chunk_windows(int64_t N, int F, int nchunk, bool root_major, const LineTables *__restrict__ tb,
                                 const double *__restrict__ nu, double smin, double smax, double pw_max,
                                 double *__restrict__ win, int8_t *__restrict__ cnt)
 {
@@ -243,13 +244,14 @@ __global__ void k_chunk_windows(int64_t N, int F, int nchunk, const LineTables *
     const int k = (int)(t % nchunk);
     const int f0 = k * WAVE, f1 = min(F, f0 + WAVE);
     const double reach = MASK_WINDOW * pw_max * 1.01;
+    double wa[MAX_WIN], wb[MAX_WIN];
     int n = 0;
     for (int l = 0; l < tb->L && n >= 0; l++) {
         const double nl = tb->line[l];
         double a = 0.0, b = 0.0;
         bool any = false;
-        for (int f = f0; f < f1; f++) {
-            const double c = nl / nu[o * F + f];
+        for (int i = f0; i < f1; i++) {
+            const double c = nl / nu[o * F + lane_bin(i, tb->L, F, root_major)];
             if (c * (1.0 - reach) <= smax * (1.0 + 1e-9) && c * (1.0 + reach) >= smin * (1.0 - 1e-9)) {
                 if (!any) { a = c; b = c; any = true; }
                 a = fmin(a, c);
@@ -257,23 +259,39 @@ __global__ void k_chunk_windows(int64_t N, int F, int nchunk, const LineTables *
             }
         }
         if (!any) continue;
+        // overlapping intervals (the same roots of different lines) merge
+        bool merged = false;
+        for (int i = 0; i < n && !merged; i++) {
+            if (a <= wb[i] && b >= wa[i]) {
+                wa[i] = fmin(wa[i], a);
+                wb[i] = fmax(wb[i], b);
+                merged = true;
+            }
+        }
+        if (merged) continue;
         if (n == MAX_WIN) { n = -1; break; }
-        win[((o * STEP_MASK_CHUNKS + k) * MAX_WIN + n) * 2 + 0] = a;
-        win[((o * STEP_MASK_CHUNKS + k) * MAX_WIN + n) * 2 + 1] = b;
+        wa[n] = a;
+        wb[n] = b;
         n++;
+    }
+    for (int i = 0; i < n; i++) {
+        win[((o * STEP_MASK_CHUNKS + k) * MAX_WIN + i) * 2 + 0] = wa[i];
+        win[((o * STEP_MASK_CHUNKS + k) * MAX_WIN + i) * 2 + 1] = wb[i];
     }
     cnt[o * STEP_MASK_CHUNKS + k] = (int8_t)n;
 }
 
-// Out-of-window bits of every cached step: the step's sub-points lie between
-// the shifts of its two ends and in its two cells, so bit k is set when the
-// shift range [lo, hi] misses every interval of chunk k widened by the wider
-// of the two cells' widths.  One thread per (origin, ray).
+// Out-of-window bits of every cached step.  A resolved step has one point, at
+// its own shift in its own cell; the sub-points of an interpolated step lie
+// between the shifts of its two ends and in its two cells.  Bit k is set when
+// that shift range [lo, hi] misses every interval of chunk k widened by the
+// (wider) cell width.  One thread per (origin, ray).
 __global__ void k_step_masks(int64_t N, int Rmax, int nchunk, const int32_t *__restrict__ cell_model,
                              const int64_t *__restrict__ cell_qbase, const int32_t *__restrict__ m_R,
-                             const double *__restrict__ pw, const int64_t *__restrict__ seg_off,
-                             const int32_t *__restrict__ nseg, const int32_t *__restrict__ seg_cell,
-                             const double *__restrict__ seg_shift, const double *__restrict__ win,
+                             const double *__restrict__ pw, const double *__restrict__ dshift_max,
+                             const int64_t *__restrict__ seg_off, const int32_t *__restrict__ nseg,
+                             const int32_t *__restrict__ seg_cell, const double *__restrict__ seg_shift,
+                             const double *__restrict__ win,
                              const int8_t *__restrict__ cnt, uint8_t *__restrict__ mask)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -284,13 +302,15 @@ __global__ void k_step_masks(int64_t N, int Rmax, int nchunk, const int32_t *__r
     const int64_t q = cell_qbase[o] + r;
     const int64_t s0 = seg_off[q];
     const int ns = nseg[q];
+    const double dsm = dshift_max[o];
     int32_t c_prev = o;
     double s_prev = 1.0;   // co-moving shift of the origin
     for (int j = 0; j < ns; j++) {
         const int32_t c = seg_cell[s0 + j];
         const double s = seg_shift[s0 + j];
-        const double lo = fmin(s_prev, s), hi = fmax(s_prev, s);
-        const double w = MASK_WINDOW * fmax(pw[c_prev], pw[c]);
+        const bool interp = fabs(s - s_prev) > dsm;   // as split_step
+        const double lo = interp ? fmin(s_prev, s) : s, hi = interp ? fmax(s_prev, s) : s;
+        const double w = MASK_WINDOW * (interp ? fmax(pw[c_prev], pw[c]) : pw[c]);
         unsigned bits = 0;
         for (int k = 0; k < nchunk; k++) {
             const int n = cnt[(int64_t)o * STEP_MASK_CHUNKS + k];
@@ -442,6 +462,19 @@ int launch_step_masks(mgr_ctx *ctx)
     if (ctx->masks_valid || !ctx->traced || !ctx->spectral_set) return MGR_OK;
     const int64_t N = ctx->N;
     const int nchunk = (int)std::min<int64_t>((ctx->F + WAVE - 1) / WAVE, STEP_MASK_CHUNKS);
+    // Root-major lanes when the lines are further apart than the trace's
+    // shift range plus both bins' spans and windows: then no bin meets a
+    // second line and each wave's bins leave their windows together.
+    bool rm = false;
+    const mgr::LineTables &tb = ctx->h_tables;
+    if (!ctx->spectral_image && tb.L > 1 && ctx->F == (int64_t)tb.L * tb.Q) {
+        const double need = (ctx->shift_max - ctx->shift_min) + 2.0 * ctx->pw_max * (ctx->root_max + MASK_WINDOW);
+        rm = true;
+        for (int l = 0; l + 1 < tb.L; l++)
+            if (!(tb.line[l + 1] > tb.line[l] * (1.0 + 2.0 * need))) rm = false;
+    }
+    if (rm != ctx->root_major) ctx->graph_precision = -1;   // captured launches hold the old order
+    ctx->root_major = rm;
     double *win = nullptr;
     int8_t *cnt = nullptr;
     MGR_HIP(hipMalloc((void **)&win, (size_t)N * STEP_MASK_CHUNKS * MAX_WIN * 2 * sizeof(double)));
@@ -450,10 +483,11 @@ int launch_step_masks(mgr_ctx *ctx)
         return ctx->fail(MGR_ERR_DEVICE, "out of device memory");
     }
     hipLaunchKernelGGL(k_chunk_windows, dim3((unsigned)((N * nchunk + 127) / 128)), dim3(128), 0, ctx->stream, N,
-                       (int)ctx->F, nchunk, ctx->tables, ctx->nu, ctx->shift_min, ctx->shift_max, ctx->pw_max,
+                       (int)ctx->F, nchunk, rm, ctx->tables, ctx->nu, ctx->shift_min, ctx->shift_max, ctx->pw_max,
                        win, cnt);
     hipLaunchKernelGGL(k_step_masks, dim3((unsigned)((N * ctx->R + 127) / 128)), dim3(128), 0, ctx->stream, N,
-                       (int)ctx->R, nchunk, ctx->d_cell_model, ctx->d_cell_qbase, ctx->d_m_R, ctx->pw, ctx->seg_off,
+                       (int)ctx->R, nchunk, ctx->d_cell_model, ctx->d_cell_qbase, ctx->d_m_R, ctx->pw,
+                       ctx->dshift_max, ctx->seg_off,
                        ctx->nseg, ctx->seg_cell, ctx->seg_shift, win, cnt, ctx->step_mask);
     const hipError_t le = hipGetLastError();
     const hipError_t se = hipStreamSynchronize(ctx->stream);
