@@ -91,9 +91,10 @@ template <> struct Prec<double> {
 // are shift - 1 and its increment) with path-length factor dz = dZ HDZ_SCALE and
 // belongs to the current cell for m < half, to `cell` otherwise.
 template <typename real>
-struct Step {
-    int32_t cell, nh, pad0, pad1;   // nh = n | half << 14 | out-of-window bits << 27 (STEP_*)
-    real    s0, ds, dz, pad2;
+struct alignas(32) Step {
+    real    s0, ds, dz;   // first sub-point shift (offset), shift increment, dz'
+    int32_t nh;           // n | half << 14 | out-of-window bits << 27 (STEP_*): one 16-byte read with the above
+    int32_t cell;
 };
 
 // Steps staged for a kernel: the FP32 split of the radiation field, or the
@@ -101,6 +102,23 @@ struct Step {
 template <typename real, bool X64> struct StepOf { using type = Step<real>; };
 template <> struct StepOf<float, true> { using type = Step<double>; };
 template <typename real, bool X64> using step_t = typename StepOf<real, X64>::type;
+
+// The sweep's view of staged step e: one 16-byte LDS read for an FP32 record.
+__device__ __forceinline__ void read_step(const Step<float> *st, int e, float &s0, float &ds, float &dz, int &nh)
+{
+    const float4 v = reinterpret_cast<const float4 *>(st + e)[0];
+    s0 = v.x;
+    ds = v.y;
+    dz = v.z;
+    nh = __float_as_int(v.w);
+}
+__device__ __forceinline__ void read_step(const Step<double> *st, int e, double &s0, double &ds, double &dz, int &nh)
+{
+    s0 = st[e].s0;
+    ds = st[e].ds;
+    dz = st[e].dz;
+    nh = st[e].nh;
+}
 
 template <typename real>
 struct RFArgs {
@@ -340,9 +358,6 @@ __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, in
     for (int e = threadIdx.x; e < cnt; e += nthr) {
         const int j = j0 + e;
         step_t<real, X64> s;
-        s.pad0 = 0;
-        s.pad1 = 0;
-        s.pad2 = 0;
         if (j < 0) {
             s.cell = o; s.nh = 0; s.s0 = 0; s.ds = 0; s.dz = 0;
         } else if constexpr (sizeof(real) == 4 && !X64) {
@@ -488,10 +503,14 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
         __syncthreads();   // the previous chunk is consumed by every wave
         stage_chunk<real, X64>(a, o, q, a_ - 1, cnt, dsm, st, rw);
         for (int e = cnt - 1; e >= 1; --e) {
-            const int nh = rfl_int(st[e].nh);   // one readfirstlane: n, half and the chunk bits
+            decltype(st->s0) s0, ds, dzs;
+            int nhs;
+            read_step(st, e, s0, ds, dzs, nhs);   // s0 and ds stay in VGPRs: one scalar operand per FMA
+            const int nh = rfl_int(nhs);   // one readfirstlane: n, half and the chunk bits
             const int n = nh & STEP_N_MASK;
+            __builtin_assume(n >= 1);   // a traced step emits 1 .. 10000 points (0 fails the trace)
             const int half = (nh >> STEP_HALF_SHIFT) & STEP_HALF_MASK;
-            const real hdz = (real)rfl_real(st[e].dz);
+            const real hdz = (real)rfl_real(dzs);
             if (chunk < STEP_MASK_CHUNKS && (((unsigned)nh >> (STEP_OUT_SHIFT + chunk)) & 1u)) {
                 // no bin of this wave meets a line window in this step: every
                 // sub-point has eta = 0 and chi at the floor, as evaluated
@@ -501,8 +520,6 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
                 }
                 continue;
             }
-            const auto s0 = st[e].s0;   // stays in a VGPR: one scalar operand per FMA
-            const auto ds = st[e].ds;
             // sub-points m >= half lie in the step's cell (row e), m < half in
             // the previous one (row e-1); two loops keep each row's address
             // loop-invariant
