@@ -135,7 +135,8 @@ def roofline_entry(sum_ndep, nfreqs, solves, k_ms, precision, counters_workload)
     same = prof is not None and pw == counters_workload and precision == "fp32"
     return {
         "bound": "hbm",
-        "kernel": "k_radiation_field<float>" if precision == "fp32" else "k_radiation_field<double>",
+        # the envelope's rotor lines are far apart: the one-candidate-line FP32 instantiation runs
+        "kernel": "k_radiation_field<float, true>" if precision == "fp32" else "k_radiation_field<double, false>",
         "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
         "frac": round(achieved / PEAK_HBM_GBS, 4),
         "traffic": prof.get("hbm_bytes_per_launch") if same else None,

@@ -606,8 +606,9 @@ __device__ __forceinline__ real end_u(const Sweep<real> &sw, real I)
 
 // Per-lane constants of origin o's bin and its own emissivity and opacity at
 // the unshifted frequency nu_f (Simulation::setup takes the origin at
-// nu(o, f), simulation.tpp:209-222).
-template <typename real, bool X64 = false>
+// nu(o, f), simulation.tpp:209-222).  ONE: no bin has more than one candidate
+// line (known on the host), so the second-line code is compiled out.
+template <typename real, bool X64 = false, bool ONE = false>
 __device__ __forceinline__ void origin_point(const RFArgs<real> &a, double nu_f,
                                              const typename Prec<real>::row_t *__restrict__ orow,
                                              LaneCtx<real> &lc, real &S_o, real &chi_o)
@@ -632,8 +633,8 @@ __device__ __forceinline__ void origin_point(const RFArgs<real> &a, double nu_f,
         lc.ln.A1d = (nc > 1) ? nu_f - a.tb->line[l1] : 1.0e30;
         lc.ln.off0 = l0 * (int)sizeof(float4);
         lc.ln.off1 = l1 * (int)sizeof(float4);
-        lc.ln.has2 = __any(nc > 1);
-        lc.ln.has3 = __any(nc > 2);
+        lc.ln.has2 = ONE ? false : __any(nc > 1);
+        lc.ln.has3 = ONE ? false : __any(nc > 2);
         if constexpr (X64) eta_chi32x(a, orow, lc.ln, 1.0, eta_o, chi_o);
         else eval_point<real>(a, lc, orow, (real)0, eta_o, chi_o);
     } else {
@@ -648,7 +649,7 @@ __device__ __forceinline__ void origin_point(const RFArgs<real> &a, double nu_f,
 
 // FP32: at least 8 waves per SIMD (at most 64 VGPRs), which hide the LDS
 // latency of the line rows; FP64 is left to the compiler.
-template <typename real>
+template <typename real, bool ONE = false>
 __global__ __launch_bounds__(WAVE * MAX_WAVES) __attribute__((amdgpu_waves_per_eu(sizeof(real) == 4 ? 8 : 1)))
 void k_radiation_field(RFArgs<real> a)
 {
@@ -681,7 +682,7 @@ void k_radiation_field(RFArgs<real> a)
 
     LaneCtx<real> lc;
     real S_o, chi_o;
-    origin_point<real>(a, nu_f, orow, lc, S_o, chi_o);
+    origin_point<real, false, ONE>(a, nu_f, orow, lc, S_o, chi_o);
 
     int32_t err = 0;
     double J_acc = 0.0, Lam_acc = 0.0;
@@ -925,8 +926,19 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     // the out-of-window bits are exact for the FP32 profile only
     a.step_mask = (sizeof(real) == 4 && ctx->masks_valid && ctx->rf_window_skip) ? ctx->step_mask : nullptr;
     a.root_major = ctx->root_major;
-    hipLaunchKernelGGL(k_radiation_field<real>, dim3(grid_of(a)), dim3(WAVE * a.nwg), lds_bytes(ctx, a),
-                       ctx->stream, a);
+    // FP32: one candidate line per bin when the lines' candidate ranges are disjoint
+    bool one = false;
+    if constexpr (sizeof(real) == 4) {
+        one = true;
+        for (int l = 0; l + 1 < ctx->h_tables.L; l++)
+            if (!(ctx->h_tables.line[l] * a.cand_lo <= ctx->h_tables.line[l + 1] * a.cand_hi)) one = false;
+        if (one)
+            hipLaunchKernelGGL((k_radiation_field<real, true>), dim3(grid_of(a)), dim3(WAVE * a.nwg),
+                               lds_bytes(ctx, a), ctx->stream, a);
+    }
+    if (!one)
+        hipLaunchKernelGGL((k_radiation_field<real, false>), dim3(grid_of(a)), dim3(WAVE * a.nwg),
+                           lds_bytes(ctx, a), ctx->stream, a);
     MGR_LAUNCH_CHECK();
     return MGR_OK;
 }

@@ -161,3 +161,34 @@ def linear_rotor(nlev=7, B=57.6e9, A1=7.2e-8, mass=28.0, num=1, partner_num=2,
     cp = CollisionPartner(partner_num, "n", icol, jcol, tmp, Cd, Ce)
     return Linedata(num=num, sym="CO", inverse_mass=1.0 / mass, energy=energy, weight=weight,
                     irad=irad, jrad=jrad, A=A, colpar=[cp])
+
+
+def split_rotor(nlev_J=4, B=57.6e9, eps=2.9e5, A1=7.2e-8, mass=28.0, num=1, partner_num=2,
+                T_table=None, C0_cgs=1e-10) -> Linedata:
+    """Linear rotor whose levels J >= 1 are split in two, E = h (B J (J+1) -/+ eps J),
+    g = 2J+1 each: the lines J -> J-1 form blends of 2 (J = 1) or 4 (J >= 2)
+    lines a few line widths apart at 2BJ (eps = 290 kHz: about one thermal width
+    of CO at 100 K at 230 GHz), A = A1 J^3 / 2 each; collisional
+    de-excitation for every pair i > j as linear_rotor."""
+    levels = [(0, 0)] + [(J, s) for J in range(1, nlev_J) for s in (-1, 1)]
+    energy = np.array([HH * (B * J * (J + 1.0) + s * eps * J) for J, s in levels])
+    weight = np.array([2.0 * J + 1.0 for J, _ in levels])
+    rad = [(i, j) for i, (Ji, _) in enumerate(levels) for j, (Jj, _) in enumerate(levels) if Jj == Ji - 1]
+    irad = np.array([r[0] for r in rad])
+    jrad = np.array([r[1] for r in rad])
+    A = np.array([A1 * levels[i][0] ** 3 / 2.0 for i in irad])
+    if T_table is None:
+        T_table = np.logspace(1.0, np.log10(3000.0), 12)
+    tmp = np.asarray(T_table, dtype=np.float64)
+    nl = len(levels)
+    pairs = [(i, j) for i in range(1, nl) for j in range(i)]
+    icol = np.array([q[0] for q in pairs])
+    jcol = np.array([q[1] for q in pairs])
+    Cd = np.zeros((len(tmp), len(pairs)))
+    for t, T in enumerate(tmp):
+        for k, (i, j) in enumerate(pairs):
+            Cd[t, k] = C0_cgs * 1.0E-6 * np.exp(-(energy[i] - energy[j]) / (KB * T))
+    Ce = detailed_balance_Ce(Cd, icol, jcol, weight, energy, tmp)
+    cp = CollisionPartner(partner_num, "n", icol, jcol, tmp, Cd, Ce)
+    return Linedata(num=num, sym="CO", inverse_mass=1.0 / mass, energy=energy, weight=weight,
+                    irad=irad, jrad=jrad, A=A, colpar=[cp])

@@ -16,7 +16,7 @@ import numpy as np
 
 from ..model import Model
 from . import healpix
-from .linedata import CC, linear_rotor, two_level
+from .linedata import CC, linear_rotor, split_rotor, two_level
 from .quadrature import gaussian_linspace
 
 # Neighbour offsets of the 6-neighbour lists, in list order (ties in the ray
@@ -75,7 +75,7 @@ def regular_grid_model(n=16, side=1.0e14, nside=1, nquads=39, n_H2=1.0e12,
 def envelope_model(n=64, side=1.0e15, nside=2, nquads=31, n0=1.0e14, T0=2500.0,
                    v_inf=15.0e3, abundance=1.0e-4, vturb=150.0, jitter=0.3, seed=1234,
                    r0_cells=2.0, jump_fraction=0.0, jump_speed=30.0e3, nlev=7,
-                   pop_prec=1.0e-6):
+                   pop_prec=1.0e-6, linedata=None):
     rng = np.random.default_rng(seed)
     h = side / n
     i, j, k = grid_index(n)
@@ -105,11 +105,18 @@ def envelope_model(n=64, side=1.0e15, nside=2, nquads=31, n0=1.0e14, T0=2500.0,
     return Model(position=position, velocity=velocity.astype(np.float64), n_neighbors=n_nb,
                  neighbors=nbs, rays=rays, weights=weights, boundary2cell_nr=grid_boundary(n),
                  temperature=T.astype(np.float64), vturb2=np.full(N, (vturb / CC) ** 2),
-                 abundance=ab, species=["dummy", "CO", "H2"], linedata=linear_rotor(nlev=nlev),
+                 abundance=ab, species=["dummy", "CO", "H2"], linedata=linear_rotor(nlev=nlev) if linedata is None else linedata,
                  roots=roots, qweights=qw, pop_prec=pop_prec,
                  meta=dict(kind="envelope", n=n, side=side, r0=r0, R_star=r0, n0=n0, T0=T0,
                            seed=seed, jitter=jitter, jump_fraction=jump_fraction,
                            neighbours="26 grid neighbours, dz outer, dy, dx inner"))
+
+
+def blended_model(n=8, seed=21, **kw):
+    """An envelope whose species has blended lines (split_rotor: 10 lines in
+    groups of 2 and 4, about one width apart), so that a bin sees up to four
+    lines in its window."""
+    return envelope_model(n=n, seed=seed, linedata=split_rotor(), **kw)
 
 
 def stress_model(n=64, seed=4321, **kw):

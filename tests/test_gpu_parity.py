@@ -35,6 +35,8 @@ MODELS = {
     "regular16": lambda: synthetic.regular_grid_model(n=16),
     "envelope8": lambda: synthetic.envelope_model(n=8, seed=7),
     "stress8": lambda: synthetic.stress_model(n=8, seed=11),
+    # lines in blends of 2 and 4 about a width apart: up to four candidate lines per bin
+    "blended8": lambda: synthetic.blended_model(n=8, seed=21),
 }
 
 

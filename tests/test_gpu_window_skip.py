@@ -31,6 +31,7 @@ MODELS = {
     "stress8": (lambda: synthetic.stress_model(n=8, seed=11), True),
     "regular8": (lambda: synthetic.regular_grid_model(n=8), False),
     "jump6": (_jump_model, False),
+    "blended8": (lambda: synthetic.blended_model(n=8, seed=21), False),
 }
 
 

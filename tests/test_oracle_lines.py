@@ -62,9 +62,11 @@ def brute_eta_chi(m, emis, opac, freq, p):
     return eta, chi
 
 
-@pytest.fixture(scope="module")
-def env():
-    m = synthetic.envelope_model(n=6, seed=2)
+@pytest.fixture(scope="module", params=["envelope", "blended"])
+def env(request):
+    # blended: groups of 2 and 4 lines about one width apart (several lines in a window)
+    m = (synthetic.envelope_model(n=6, seed=2) if request.param == "envelope"
+         else synthetic.blended_model(n=5, seed=2))
     orc = Oracle(m).setup()
     return m, orc
 
