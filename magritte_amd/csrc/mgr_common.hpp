@@ -249,6 +249,12 @@ struct mgr_ctx {
     // masks: root-major when no bin can meet a second line's window
     bool     root_major = false;
     double   root_max = 0.0;            // max |root| of the line quadrature
+    // FP32 incoming intensity at the far end of every (origin, ray) entry and
+    // bin, (float)(rescaled I_bdy * I_SCALE) at q F + f; valid for the current
+    // trace, frequencies and boundary intensities.  Allocated when it fits a
+    // quarter of the free device memory; null otherwise (computed in the kernel).
+    float   *far_I32 = nullptr;
+    bool     far_valid = false, far_bad = false, far_cache = true;
     int64_t  sum_ndep = 0, n_pairs_solved = 0;
 
     // device: rest-frame trace of one image ray pair, entry 2o the antipode
@@ -297,6 +303,7 @@ int launch_cell_tables(mgr_ctx *ctx);
 int launch_emissivity_opacity(mgr_ctx *ctx);
 int launch_trace(mgr_ctx *ctx);
 int launch_radiation_field(mgr_ctx *ctx, int precision);   // kernel + Lambda*
+int launch_far_cache(mgr_ctx *ctx);   // outside graph capture
 int launch_rf_kernel(mgr_ctx *ctx, int precision);
 int tune_radiation_field(mgr_ctx *ctx, int precision);   // outside graph capture
 int launch_lambda(mgr_ctx *ctx);

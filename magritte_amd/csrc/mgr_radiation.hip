@@ -136,6 +136,7 @@ struct RFArgs {
     const float   *seg_hdz32;                 // its dZ * HDZ_SCALE
     const int32_t *far_cell;                  // last emitted point of each (origin, ray), -1: none
     const double  *far_shift;
+    const float   *far32;                     // cached FP32 far-end intensities [q F + f] (or null)
     const double  *origin_shift;              // FP64 shift of the origin per entry (null: 1, co-moving)
     const uint8_t *step_mask;                 // per step, out-of-window bit per 64-bin chunk (or null)
     bool           root_major;                // lane order (lane_bin)
@@ -480,6 +481,36 @@ __device__ __forceinline__ double far_intensity(const RFArgs<real> &a, int64_t q
     return rescale_I_bdy(a.nu + (int64_t)cf * F, a.I_bdy + (int64_t)b * F, F, fs);
 }
 
+// Scaled incoming intensity of entry q in the sweep's precision: the FP32
+// cache when it is built (the same value, computed once per trace, frequency
+// grid and boundary intensities), else far_intensity.
+template <typename real>
+__device__ __forceinline__ real far_in(const RFArgs<real> &a, int64_t q, double nu_f, int F, int f, int32_t &err)
+{
+    if constexpr (sizeof(real) == 4)
+        if (a.far32) return a.far32[q * F + f];
+    return (real)(far_intensity(a, q, nu_f, F, err) * I_SCALE);
+}
+
+// The far-end intensity cache: one thread per (origin, ray, bin), the value
+// far_in computes in the kernel.
+__global__ void k_far_cache(RFArgs<float> a, int Rmax, float *__restrict__ out, int32_t *__restrict__ err_flag)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int F = a.F;
+    const int64_t per = (int64_t)Rmax * F;
+    if (t >= a.N * per) return;
+    const int32_t o = (int32_t)(t / per);
+    const int64_t rem = t - (int64_t)o * per;
+    const int r = (int)(rem / F);
+    const int f = (int)(rem - (int64_t)r * F);
+    if (r >= a.m_R[a.cell_model[o]]) return;
+    const int64_t q = a.cell_qbase[o] + r;
+    int32_t err = 0;
+    out[q * F + f] = (float)(far_intensity(a, q, a.nu[(int64_t)o * F + f], F, err) * I_SCALE);
+    if (err) atomicOr(err_flag, err);
+}
+
 // One ray of a pair, eliminated from its boundary end (incoming intensity
 // I_far, scaled) inwards to the origin, which it leaves pending.
 template <typename real, bool X64 = false>
@@ -706,16 +737,16 @@ void k_radiation_field(RFArgs<real> a)
         }
         real u, lam;
         if (n_r > 0 && n_a > 0) {
-            const real Ia = (real)(far_intensity(a, qa, nu_f, F, err) * I_SCALE);
+            const real Ia = far_in<real>(a, qa, nu_f, F, f, err);
             const Sweep<real> A = sweep_side<real>(a, lc, o, qa, Ia, S_o, chi_o, dsm, st, rw, chunk);
-            const real Ir = (real)(far_intensity(a, qr, nu_f, F, err) * I_SCALE);
+            const real Ir = far_in<real>(a, qr, nu_f, F, f, err);
             const Sweep<real> B = sweep_side<real>(a, lc, o, qr, Ir, S_o, chi_o, dsm, st, rw, chunk);
             const real den = Prec<real>::rcp(A.y + B.y);
             u = (A.p + B.p) * den;
             lam = (A.c + B.c) * den;
         } else {
             const int64_t qs = (n_a > 0) ? qa : qr;
-            const real Is = (real)(far_intensity(a, qs, nu_f, F, err) * I_SCALE);
+            const real Is = far_in<real>(a, qs, nu_f, F, f, err);
             const Sweep<real> A = sweep_side<real>(a, lc, o, qs, Is, S_o, chi_o, dsm, st, rw, chunk);
             real I_o = 0;
             if (bo < 0) err |= ERR_EMPTY_SIDE;
@@ -926,6 +957,7 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     // the out-of-window bits are exact for the FP32 profile only
     a.step_mask = (sizeof(real) == 4 && ctx->masks_valid && ctx->rf_window_skip) ? ctx->step_mask : nullptr;
     a.root_major = ctx->root_major;
+    a.far32 = (sizeof(real) == 4 && ctx->far_cache && ctx->far_valid) ? ctx->far_I32 : nullptr;
     // FP32: one candidate line per bin when the lines' candidate ranges are disjoint
     bool one = false;
     if constexpr (sizeof(real) == 4) {
@@ -1075,6 +1107,7 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
     }
     int rc0 = set_active(ctx, true);
     if (!rc0) rc0 = launch_step_masks(ctx);
+    if (!rc0) rc0 = launch_far_cache(ctx);
     if (!rc0) rc0 = tune_radiation_field(ctx, precision);
     if (rc0) return rc0;
     hipEvent_t e0, e1;
@@ -1173,6 +1206,70 @@ extern "C" int mgr_set_scattering(mgr_ctx *ctx, int use_scattering)
         ctx->u_store = nullptr;
     }
     ctx->graph_precision = -1;   // captured launches hold the old pointer
+    return MGR_OK;
+}
+
+namespace mgr {
+
+// Builds the FP32 far-end intensity cache when it is missing or stale (once
+// per trace, frequency grid and boundary intensities).  It is skipped, and
+// the kernel computes the intensities itself, when the cache would take more
+// than a quarter of the free device memory or an entry has no boundary cell
+// at its far end (the kernel then reports that error).
+int launch_far_cache(mgr_ctx *ctx)
+{
+    if (!ctx->far_cache || ctx->far_valid || ctx->far_bad || !ctx->traced || !ctx->spectral_set || !ctx->bdy_set)
+        return MGR_OK;
+    const size_t bytes = (size_t)ctx->NQ * (size_t)ctx->F * sizeof(float);
+    if (!ctx->far_I32) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess || bytes > fr / 4 ||
+            hipMalloc((void **)&ctx->far_I32, std::max<size_t>(bytes, 4)) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->far_I32 = nullptr;
+            ctx->far_bad = true;
+            return MGR_OK;
+        }
+    }
+    ctx->graph_precision = -1;   // captured launches may have read no cache, or a stale one
+    RFArgs<float> a = RFArgs<float>();
+    a.N = ctx->N;
+    a.F = (int)ctx->F;
+    a.cell_model = ctx->d_cell_model;
+    a.cell_qbase = ctx->d_cell_qbase;
+    a.m_R = ctx->d_m_R;
+    a.nu = ctx->nu;
+    a.cell2bdy = ctx->cell2bdy;
+    a.I_bdy = ctx->I_bdy;
+    a.far_cell = ctx->far_cell;
+    a.far_shift = ctx->far_shift;
+    const int64_t threads = ctx->N * ctx->R * ctx->F;
+    MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
+    hipLaunchKernelGGL(k_far_cache, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, ctx->stream, a,
+                       (int)ctx->R, ctx->far_I32, ctx->err_flag);
+    MGR_LAUNCH_CHECK();
+    int32_t errv = 0;
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    MGR_HIP(hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (errv) ctx->far_bad = true;
+    else ctx->far_valid = true;
+    return MGR_OK;
+}
+
+}  // namespace mgr
+
+extern "C" int mgr_set_far_cache(mgr_ctx *ctx, int enable)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    MGR_HIP(hipSetDevice(ctx->device));
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->far_cache = enable != 0;
+    if (!ctx->far_cache && ctx->far_I32) {
+        (void)hipFree(ctx->far_I32);
+        ctx->far_I32 = nullptr;
+    }
+    ctx->far_valid = ctx->far_bad = false;
+    ctx->graph_precision = -1;   // captured launches hold the old choice
     return MGR_OK;
 }
 

@@ -55,6 +55,7 @@ SIGNATURES = {
     "mgr_get_npoints_on_rays_rest": ([_vp, _vp], _i),
     "mgr_set_scattering": ([_vp, _i], _i),
     "mgr_set_window_skip": ([_vp, _i], _i),
+    "mgr_set_far_cache": ([_vp, _i], _i),
     "mgr_window_skip_stats": ([_vp, _vp, _vp], _i),
     "mgr_get_u": ([_vp, _vp], _i),
     "mgr_compute_spectral_discretisation_image": ([_vp, _d], _i),
@@ -277,6 +278,11 @@ class Context:
         """FP32 kernel: eliminate steps out of every line window without
         evaluating their profiles (bit-identical; on by default)."""
         self._check(self.lib.mgr_set_window_skip(self.h, 1 if enable else 0), "set_window_skip")
+
+    def set_far_cache(self, enable=True):
+        """FP32 kernel: keep the far-end intensities of every ray and bin
+        instead of searching for them in each radiation field (bit-identical)."""
+        self._check(self.lib.mgr_set_far_cache(self.h, 1 if enable else 0), "set_far_cache")
 
     def window_skip_stats(self):
         """(point-chunks of the trace, point-chunks the FP32 kernel skips)."""

@@ -1,5 +1,5 @@
-"""The out-of-window skip of the FP32 radiation-field kernel against the full
-evaluation, through the C-ABI.
+"""The out-of-window skip and the far-end intensity cache of the FP32
+radiation-field kernel against the full evaluation, through the C-ABI.
 
 A step whose sub-points keep every bin of a 64-bin wave more than 11.5 line
 widths from every line has FP32 profiles of exactly 0 (exp2 of less than
@@ -8,7 +8,9 @@ without evaluating the profiles.  Bar: J, the Lambda diagonal, Radiation::u
 and the populations after ALI iterations are bit-identical with the skip on
 and off, and a model with velocity fields does skip a share of its points.
 The masks follow the frequencies: after an image discretisation and back the
-line-grid result is unchanged.
+line-grid result is unchanged.  The far-end intensity cache (the rescaled
+boundary intensity of every ray and bin, kept in FP32) gives the same bits as
+the kernel's own search, and is rebuilt when the boundary intensities change.
 """
 import numpy as np
 import pytest
@@ -67,11 +69,32 @@ def test_skip_in_ali_iterations():
     pops = {}
     for on in (True, False):
         ctx.set_window_skip(on)
+        ctx.set_far_cache(on)
         ctx.compute_LTE_level_populations()
         for _ in range(3):
             ctx.ali_iteration(ng=False, precision="fp32")
         pops[on] = ctx.populations()
     assert np.array_equal(pops[True], pops[False])
+
+
+def test_far_cache_is_bit_identical_and_follows_the_boundary():
+    from magritte_amd.hip import Context
+    print("\n[skip] stress8 far-end intensity cache", flush=True)
+    m = synthetic.stress_model(n=8, seed=11)
+    ctx = Context(m).setup()
+    ctx.set_scattering(True)
+    J1, L1, u1 = _field(ctx)
+    ctx.set_far_cache(False)
+    J0, L0, u0 = _field(ctx)
+    assert np.array_equal(J1, J0) and np.array_equal(L1, L0) and np.array_equal(u1, u0)
+    # new boundary temperatures: the cache is rebuilt from them
+    ctx.set_far_cache(True)
+    ctx.compute_boundary_intensities(np.full(m.nboundary, 40.0))
+    J2, _, _ = _field(ctx)
+    ctx.set_far_cache(False)
+    J3, _, _ = _field(ctx)
+    assert np.array_equal(J2, J3)
+    assert not np.array_equal(J2, J1)
 
 
 def test_masks_follow_the_frequencies():
