@@ -59,7 +59,7 @@ def build_workload(n):
 def profile_counters(path):
     """Counter summary of the radiation-field kernel committed under profiles/
     (HBM bytes per launch from FETCH_SIZE + WRITE_SIZE, copy-kernel scaled;
-    VALU issue fraction from SQ_INSTS_VALU)."""
+    VALU issue fraction from SQ_INSTS_VALU net of transcendentals)."""
     try:
         with open(path) as fh:
             d = json.load(fh)
