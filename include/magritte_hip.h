@@ -285,7 +285,7 @@ int mgr_set_window_skip(mgr_ctx *ctx, int enable);
  * default): the rescaled boundary intensity entering every (origin, ray) and
  * bin, Radiation::rescale_I_bdy at the last point's shift, kept in FP32 once
  * per trace, frequency grid and boundary intensities (4 B per ray and bin, when
- * that fits half of the free device memory) instead of searched for in
+ * that fits three quarters of the free device memory) instead of searched for in
  * every radiation field.  Results are bit-identical either way; off frees it. */
 int mgr_set_far_cache(mgr_ctx *ctx, int enable);
 /* Interpolated points x 64-bin chunks of the cached trace, and how many of

@@ -251,8 +251,8 @@ struct mgr_ctx {
     double   root_max = 0.0;            // max |root| of the line quadrature
     // FP32 incoming intensity at the far end of every (origin, ray) entry and
     // bin, (float)(rescaled I_bdy * I_SCALE) at q F + f; valid for the current
-    // trace, frequencies and boundary intensities.  Allocated when it fits half
-    // of the free device memory; null otherwise (computed in the kernel).
+    // trace, frequencies and boundary intensities.  Allocated when it fits three
+    // quarters of the free device memory; null otherwise (computed in the kernel).
     float   *far_I32 = nullptr;
     bool     far_valid = false, far_bad = false, far_cache = true;
     int64_t  sum_ndep = 0, n_pairs_solved = 0;

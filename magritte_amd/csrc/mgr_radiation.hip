@@ -1214,7 +1214,7 @@ namespace mgr {
 // Builds the FP32 far-end intensity cache when it is missing or stale (once
 // per trace, frequency grid and boundary intensities).  It is skipped, and
 // the kernel computes the intensities itself, when the cache would take more
-// than half of the free device memory or an entry has no boundary cell
+// than three quarters of the free device memory or an entry has no boundary cell
 // at its far end (the kernel then reports that error).
 int launch_far_cache(mgr_ctx *ctx)
 {
@@ -1223,7 +1223,7 @@ int launch_far_cache(mgr_ctx *ctx)
     const size_t bytes = (size_t)ctx->NQ * (size_t)ctx->F * sizeof(float);
     if (!ctx->far_I32) {
         size_t fr = 0, tot = 0;
-        if (hipMemGetInfo(&fr, &tot) != hipSuccess || bytes > fr / 2 ||
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess || bytes > fr / 4 * 3 ||
             hipMalloc((void **)&ctx->far_I32, std::max<size_t>(bytes, 4)) != hipSuccess) {
             (void)hipGetLastError();
             ctx->far_I32 = nullptr;
