@@ -1106,8 +1106,8 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
         if (rc) return rc;
     }
     int rc0 = set_active(ctx, true);
-    if (!rc0) rc0 = launch_step_masks(ctx);
-    if (!rc0) rc0 = launch_far_cache(ctx);
+    if (!rc0 && precision == MGR_FP32) rc0 = launch_step_masks(ctx);   // both serve the FP32 kernel only
+    if (!rc0 && precision == MGR_FP32) rc0 = launch_far_cache(ctx);
     if (!rc0) rc0 = tune_radiation_field(ctx, precision);
     if (rc0) return rc0;
     hipEvent_t e0, e1;
