@@ -230,7 +230,7 @@ struct mgr_ctx {
     int32_t *seg_cell = nullptr;    // [total segments]
     double  *seg_shift = nullptr, *seg_dZ = nullptr;
     // the FP32 kernel's view of every step, split once at trace time:
-    // (cell, n | half << 16, shift offset s0, shift increment) and dZ * HDZ_SCALE
+    // (cell, n | half << STEP_HALF_SHIFT, shift offset s0, shift increment) and dZ * HDZ_SCALE
     int4    *seg_f32 = nullptr;
     float   *seg_hdz32 = nullptr;
     // last emitted point of every (origin, ray): its cell (-1: none) and FP64 shift

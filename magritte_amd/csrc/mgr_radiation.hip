@@ -132,7 +132,7 @@ struct RFArgs {
     const int64_t *seg_off;
     const int32_t *nseg, *npts, *seg_cell;
     const double  *seg_shift, *seg_dZ;
-    const int4    *seg_f32;                   // FP32 split of every step (cell, n | half << 16, s0, ds)
+    const int4    *seg_f32;                   // FP32 split of every step (cell, n | half << 14, s0, ds)
     const float   *seg_hdz32;                 // its dZ * HDZ_SCALE
     const int32_t *far_cell;                  // last emitted point of each (origin, ray), -1: none
     const double  *far_shift;
