@@ -137,6 +137,7 @@ struct RFArgs {
     const int32_t *far_cell;                  // last emitted point of each (origin, ray), -1: none
     const double  *far_shift;
     const float   *far32;                     // cached FP32 far-end intensities [q F + f] (or null)
+    uint64_t       L_magic;                   // ceil(2^32 / L): idx / L = (idx * L_magic) >> 32
     const double  *origin_shift;              // FP64 shift of the origin per entry (null: 1, co-moving)
     const uint8_t *step_mask;                 // per step, out-of-window bit per 64-bin chunk (or null)
     bool           root_major;                // lane order (lane_bin)
@@ -386,7 +387,8 @@ __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, in
     __syncthreads();
     const int L = a.L;
     for (int idx = threadIdx.x; idx < cnt * L; idx += nthr) {
-        const int e = idx / L, l = idx - e * L;
+        // idx / L by a multiply with ceil(2^32 / L): exact for idx < 2^32 / L
+        const int e = (int)(((uint64_t)(uint32_t)idx * a.L_magic) >> 32), l = idx - e * L;
         rw[idx] = a.rows[(int64_t)st[e].cell * L + l];
     }
     __syncthreads();
@@ -888,6 +890,7 @@ int make_args(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     a.m_active = ctx->d_m_active;
     a.F = (int)ctx->F;
     a.L = (int)ctx->L;
+    a.L_magic = 0xFFFFFFFFull / (uint64_t)ctx->L + 1;
     a.nchunk = (int)((ctx->F + WAVE - 1) / WAVE);
     a.nwg = nwg;
     a.ngroups = (a.nchunk + a.nwg - 1) / a.nwg;
