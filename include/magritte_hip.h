@@ -277,7 +277,7 @@ int mgr_radiation_field_shape(mgr_ctx *ctx, int precision, int *waves_per_group,
 
 /* Out-of-window skip of the FP32 radiation-field kernel (no counterpart in
  * the reference; on by default): steps at which no bin of a 64-bin wave comes
- * within 11.5 line widths of any line are eliminated with eta = 0 and chi at
+ * within 10.5 line widths of any line are eliminated with eta = 0 and chi at
  * its floor without evaluating the profiles, which are exactly 0 there in
  * FP32; results are bit-identical either way.  Off evaluates every point.    */
 int mgr_set_window_skip(mgr_ctx *ctx, int enable);

@@ -54,10 +54,12 @@ constexpr int WAVE = 64;                 // CDNA wavefront width
 // own cell `half` (bits 14-26), and one bit per 64-bin chunk
 // k < STEP_MASK_CHUNKS (bit 27 + k) set when every bin of the chunk is more
 // than MASK_WINDOW line widths from every line at every sub-point of the step:
-// the FP32 profile exp2(-x^2 log2 e) is then exactly zero (below 2^-149).
+// the FP32 profile exp2(-x^2 log2 e) is then exactly zero, since x >= 10.5
+// gives exp2 of less than -159, below the smallest denormal 2^-149 (zero from
+// x > 10.2; the FP32 rounding of x is ~1e-6 widths).
 constexpr int STEP_N_MASK = 0x3FFF, STEP_HALF_SHIFT = 14, STEP_HALF_MASK = 0x1FFF, STEP_OUT_SHIFT = 27;
 constexpr int STEP_MASK_CHUNKS = 5;
-constexpr double MASK_WINDOW = 11.5;
+constexpr double MASK_WINDOW = 10.5;
 constexpr int MAX_INTERPOLATION = 10000; // geometry.tpp:138
 
 // One neighbour entry of the tracer: the neighbour's index and position, so

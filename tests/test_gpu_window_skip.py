@@ -1,7 +1,7 @@
 """The out-of-window skip and the far-end intensity cache of the FP32
 radiation-field kernel against the full evaluation, through the C-ABI.
 
-A step whose sub-points keep every bin of a 64-bin wave more than 11.5 line
+A step whose sub-points keep every bin of a 64-bin wave more than 10.5 line
 widths from every line has FP32 profiles of exactly 0 (exp2 of less than
 -149), so eta = 0 and chi sits at its floor; the skip eliminates such a step
 without evaluating the profiles.  Bar: J, the Lambda diagonal, Radiation::u
