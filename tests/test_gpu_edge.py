@@ -11,6 +11,8 @@ against the FP64 oracle.
 * Every cell on the boundary (a 2^3 grid) or all but one (3^3): pairs with
   an empty side take I_bdy of the origin's boundary entry
   (simulation.tpp:283-288, 342-347).
+* More frequency bins (341) than the five 64-bin chunks with out-of-window
+  bits: the sixth wave is always evaluated.
 """
 import numpy as np
 import pytest
@@ -92,3 +94,30 @@ def test_all_or_most_cells_on_the_boundary(n):
     mask = np.abs(L_o) > 0
     assert rel_err(L_g[mask], L_o[mask]) < J_TOL["fp32"]
     assert np.all(L_g[~mask] == 0.0)
+
+
+def test_more_bins_than_mask_chunks():
+    """11 lines x 31 bins = 341 bins: six 64-bin waves, of which the sixth has
+    no out-of-window bit (it is always evaluated), root-major lanes over 11
+    lines; J and the Lambda diagonal against the oracle, and the skip on and
+    off bit-identical."""
+    m = synthetic.envelope_model(n=7, seed=3, nlev=12)
+    assert m.nfreqs == 341
+    print("\n[edge] envelope7 with 11 lines (341 bins)", flush=True)
+    orc = Oracle(m).setup()
+    orc.compute_radiation_field()
+    ctx = _ctx(m)
+    for prec in ("fp64", "fp32"):
+        ctx.compute_radiation_field(prec)
+        err = rel_err(ctx.J(), orc.J())
+        L_o, L_g = orc.lambda_diag(), ctx.lambda_diag()
+        mask = np.abs(L_o) > 0
+        errL = rel_err(L_g[mask], L_o[mask])
+        print(f"[edge] 341 bins {prec} J max rel err {err:.2e}, Lambda {errL:.2e}", flush=True)
+        assert err < J_TOL[prec] and errL < J_TOL[prec]
+    J1 = ctx.J()
+    total, skipped = ctx.window_skip_stats()
+    print(f"[edge] 341 bins: {skipped} of {total} point-chunks skipped", flush=True)
+    ctx.set_window_skip(False)
+    ctx.compute_radiation_field("fp32")
+    assert np.array_equal(ctx.J(), J1)
