@@ -527,6 +527,8 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
     Sweep<real> sw;
     sw.init(I_far);
     const real chi_floor = (real)(CHI_FLOOR * CHI_SCALE);
+    // this wave's out-of-window bit in the staged nh word (none past the masked chunks)
+    const unsigned cbit = chunk < STEP_MASK_CHUNKS ? 1u << (STEP_OUT_SHIFT + chunk) : 0u;
     const int ns = a.nseg[q];
     const int L = a.L;
     const int CH = a.ch;
@@ -544,7 +546,7 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
             __builtin_assume(n >= 1);   // a traced step emits 1 .. 10000 points (0 fails the trace)
             const int half = (nh >> STEP_HALF_SHIFT) & STEP_HALF_MASK;
             const real hdz = (real)rfl_real(dzs);
-            if (chunk < STEP_MASK_CHUNKS && (((unsigned)nh >> (STEP_OUT_SHIFT + chunk)) & 1u)) {
+            if ((unsigned)nh & cbit) {
                 // no bin of this wave meets a line window in this step: every
                 // sub-point has eta = 0 and chi at the floor, as evaluated
                 for (int m = n; m >= 1; --m) {
