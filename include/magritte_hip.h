@@ -281,12 +281,13 @@ int mgr_radiation_field_shape(mgr_ctx *ctx, int precision, int *waves_per_group,
  * its floor without evaluating the profiles, which are exactly 0 there in
  * FP32; results are bit-identical either way.  Off evaluates every point.    */
 int mgr_set_window_skip(mgr_ctx *ctx, int enable);
-/* FP32 far-end intensity cache (no counterpart in the reference; on by
- * default): the rescaled boundary intensity entering every (origin, ray) and
- * bin, Radiation::rescale_I_bdy at the last point's shift, kept in FP32 once
- * per trace, frequency grid and boundary intensities (4 B per ray and bin, when
- * that fits three quarters of the free device memory) instead of searched for in
- * every radiation field.  Results are bit-identical either way; off frees it. */
+/* Far-end intensity cache (no counterpart in the reference; on by default):
+ * the rescaled boundary intensity entering every (origin, ray) and bin,
+ * Radiation::rescale_I_bdy at the last point's shift, kept in the kernel's
+ * precision once per trace, frequency grid and boundary intensities (4 B per
+ * ray and bin for FP32, 8 B for FP64, when that fits three quarters of the
+ * free device memory) instead of searched for in every radiation field.
+ * Results are bit-identical either way; off frees it.                        */
 int mgr_set_far_cache(mgr_ctx *ctx, int enable);
 /* Interpolated points x 64-bin chunks of the cached trace, and how many of
  * them the skip leaves out (chunks beyond the fifth are never skipped).      */

@@ -251,12 +251,15 @@ struct mgr_ctx {
     // masks: root-major when no bin can meet a second line's window
     bool     root_major = false;
     double   root_max = 0.0;            // max |root| of the line quadrature
-    // FP32 incoming intensity at the far end of every (origin, ray) entry and
-    // bin, (float)(rescaled I_bdy * I_SCALE) at q F + f; valid for the current
+    // Incoming intensity at the far end of every (origin, ray) entry and bin,
+    // (real)(rescaled I_bdy * I_SCALE) at q F + f, in FP32 ([0], float) and FP64
+    // ([1], double) for the kernels of each precision; valid for the current
     // trace, frequencies and boundary intensities.  Allocated when it fits three
     // quarters of the free device memory; null otherwise (computed in the kernel).
     float   *far_I32 = nullptr;
-    bool     far_valid = false, far_bad = false, far_cache = true;
+    double  *far_I64 = nullptr;
+    bool     far_valid[2] = {false, false}, far_bad[2] = {false, false}, far_cache = true;
+    void far_invalidate() { far_valid[0] = far_valid[1] = far_bad[0] = far_bad[1] = false; }
     int64_t  sum_ndep = 0, n_pairs_solved = 0;
 
     // device: rest-frame trace of one image ray pair, entry 2o the antipode
@@ -305,7 +308,7 @@ int launch_cell_tables(mgr_ctx *ctx);
 int launch_emissivity_opacity(mgr_ctx *ctx);
 int launch_trace(mgr_ctx *ctx);
 int launch_radiation_field(mgr_ctx *ctx, int precision);   // kernel + Lambda*
-int launch_far_cache(mgr_ctx *ctx);   // outside graph capture
+int launch_far_cache(mgr_ctx *ctx, int precision);   // outside graph capture
 int launch_rf_kernel(mgr_ctx *ctx, int precision);
 int tune_radiation_field(mgr_ctx *ctx, int precision);   // outside graph capture
 int launch_lambda(mgr_ctx *ctx);

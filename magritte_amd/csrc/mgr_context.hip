@@ -617,6 +617,7 @@ int free_model(mgr_ctx *ctx)
     dfree(ctx->seg_f32); dfree(ctx->seg_hdz32); dfree(ctx->far_cell); dfree(ctx->far_shift);
     dfree(ctx->step_mask);
     dfree(ctx->far_I32);
+    dfree(ctx->far_I64);
     dfree(ctx->shift_bits); dfree(ctx->partials); dfree(ctx->reduced);
     dfree(ctx->d_cell_model); dfree(ctx->d_cell_qbase); dfree(ctx->d_m_R); dfree(ctx->d_m_rbase);
     dfree(ctx->d_m_active); dfree(ctx->d_chunk_cell); dfree(ctx->d_m_chunk_off); dfree(ctx->d_mconv);
@@ -755,7 +756,7 @@ static int spectral_discretisation(mgr_ctx *ctx, bool image, double width)
     ctx->spectral_set = true;
     ctx->spectral_image = image;
     ctx->masks_valid = false;   // the out-of-window bits depend on the frequencies
-    ctx->far_valid = ctx->far_bad = false;
+    ctx->far_invalidate();
     return MGR_OK;
 }
 
@@ -790,7 +791,7 @@ int mgr_compute_boundary_intensities(mgr_ctx *ctx, const double *temperatures)
     if (le != hipSuccess || se != hipSuccess)
         return ctx->fail(MGR_ERR_DEVICE, "boundary intensities kernel failed");
     ctx->bdy_set = true;
-    ctx->far_valid = ctx->far_bad = false;
+    ctx->far_invalidate();
     return MGR_OK;
 }
 

@@ -136,7 +136,8 @@ struct RFArgs {
     const float   *seg_hdz32;                 // its dZ * HDZ_SCALE
     const int32_t *far_cell;                  // last emitted point of each (origin, ray), -1: none
     const double  *far_shift;
-    const float   *far32;                     // cached FP32 far-end intensities [q F + f] (or null)
+    const float   *far32;                     // cached far-end intensities [q F + f] of the FP32 kernel (or null)
+    const double  *far64;                     // the same for the FP64 kernel (or null)
     uint64_t       L_magic;                   // ceil(2^32 / L): idx / L = (idx * L_magic) >> 32
     const double  *origin_shift;              // FP64 shift of the origin per entry (null: 1, co-moving)
     const uint8_t *step_mask;                 // per step, out-of-window bit per 64-bin chunk (or null)
@@ -483,20 +484,24 @@ __device__ __forceinline__ double far_intensity(const RFArgs<real> &a, int64_t q
     return rescale_I_bdy(a.nu + (int64_t)cf * F, a.I_bdy + (int64_t)b * F, F, fs);
 }
 
-// Scaled incoming intensity of entry q in the sweep's precision: the FP32
-// cache when it is built (the same value, computed once per trace, frequency
-// grid and boundary intensities), else far_intensity.
+// Scaled incoming intensity of entry q in the sweep's precision: the cache of
+// that precision when it is built (the same value, computed once per trace,
+// frequency grid and boundary intensities), else far_intensity.
 template <typename real>
 __device__ __forceinline__ real far_in(const RFArgs<real> &a, int64_t q, double nu_f, int F, int f, int32_t &err)
 {
-    if constexpr (sizeof(real) == 4)
+    if constexpr (sizeof(real) == 4) {
         if (a.far32) return a.far32[q * F + f];
+    } else {
+        if (a.far64) return a.far64[q * F + f];
+    }
     return (real)(far_intensity(a, q, nu_f, F, err) * I_SCALE);
 }
 
 // The far-end intensity cache: one thread per (origin, ray, bin), the value
-// far_in computes in the kernel.
-__global__ void k_far_cache(RFArgs<float> a, int Rmax, float *__restrict__ out, int32_t *__restrict__ err_flag)
+// far_in computes in the kernel of precision T.
+template <typename T>
+__global__ void k_far_cache(RFArgs<float> a, int Rmax, T *__restrict__ out, int32_t *__restrict__ err_flag)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int F = a.F;
@@ -509,7 +514,7 @@ __global__ void k_far_cache(RFArgs<float> a, int Rmax, float *__restrict__ out, 
     if (r >= a.m_R[a.cell_model[o]]) return;
     const int64_t q = a.cell_qbase[o] + r;
     int32_t err = 0;
-    out[q * F + f] = (float)(far_intensity(a, q, a.nu[(int64_t)o * F + f], F, err) * I_SCALE);
+    out[q * F + f] = (T)(far_intensity(a, q, a.nu[(int64_t)o * F + f], F, err) * I_SCALE);
     if (err) atomicOr(err_flag, err);
 }
 
@@ -962,7 +967,8 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     // the out-of-window bits are exact for the FP32 profile only
     a.step_mask = (sizeof(real) == 4 && ctx->masks_valid && ctx->rf_window_skip) ? ctx->step_mask : nullptr;
     a.root_major = ctx->root_major;
-    a.far32 = (sizeof(real) == 4 && ctx->far_cache && ctx->far_valid) ? ctx->far_I32 : nullptr;
+    a.far32 = (sizeof(real) == 4 && ctx->far_cache && ctx->far_valid[0]) ? ctx->far_I32 : nullptr;
+    a.far64 = (sizeof(real) == 8 && ctx->far_cache && ctx->far_valid[1]) ? ctx->far_I64 : nullptr;
     // FP32: one candidate line per bin when the lines' candidate ranges are disjoint
     bool one = false;
     if constexpr (sizeof(real) == 4) {
@@ -1122,7 +1128,7 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
     }
     int rc0 = set_active(ctx, true);
     if (!rc0 && precision == MGR_FP32) rc0 = launch_step_masks(ctx);   // both serve the FP32 kernel only
-    if (!rc0 && precision == MGR_FP32) rc0 = launch_far_cache(ctx);
+    if (!rc0) rc0 = launch_far_cache(ctx, precision);
     if (!rc0) rc0 = tune_radiation_field(ctx, precision);
     if (rc0) return rc0;
     hipEvent_t e0, e1;
@@ -1226,23 +1232,26 @@ extern "C" int mgr_set_scattering(mgr_ctx *ctx, int use_scattering)
 
 namespace mgr {
 
-// Builds the FP32 far-end intensity cache when it is missing or stale (once
-// per trace, frequency grid and boundary intensities).  It is skipped, and
-// the kernel computes the intensities itself, when the cache would take more
-// than three quarters of the free device memory or an entry has no boundary cell
-// at its far end (the kernel then reports that error).
-int launch_far_cache(mgr_ctx *ctx)
+// Builds the far-end intensity cache of a precision when it is missing or
+// stale (once per trace, frequency grid and boundary intensities).  It is
+// skipped, and the kernel computes the intensities itself, when the cache
+// would take more than three quarters of the free device memory or an entry
+// has no boundary cell at its far end (the kernel then reports that error).
+int launch_far_cache(mgr_ctx *ctx, int precision)
 {
-    if (!ctx->far_cache || ctx->far_valid || ctx->far_bad || !ctx->traced || !ctx->spectral_set || !ctx->bdy_set)
+    const int pi = precision == MGR_FP64 ? 1 : 0;
+    if (!ctx->far_cache || ctx->far_valid[pi] || ctx->far_bad[pi] || !ctx->traced || !ctx->spectral_set ||
+        !ctx->bdy_set)
         return MGR_OK;
-    const size_t bytes = (size_t)ctx->NQ * (size_t)ctx->F * sizeof(float);
-    if (!ctx->far_I32) {
+    void **buf = pi ? (void **)&ctx->far_I64 : (void **)&ctx->far_I32;
+    const size_t bytes = (size_t)ctx->NQ * (size_t)ctx->F * (pi ? sizeof(double) : sizeof(float));
+    if (!*buf) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess || bytes > fr / 4 * 3 ||
-            hipMalloc((void **)&ctx->far_I32, std::max<size_t>(bytes, 4)) != hipSuccess) {
+            hipMalloc(buf, std::max<size_t>(bytes, 8)) != hipSuccess) {
             (void)hipGetLastError();
-            ctx->far_I32 = nullptr;
-            ctx->far_bad = true;
+            *buf = nullptr;
+            ctx->far_bad[pi] = true;
             return MGR_OK;
         }
     }
@@ -1259,15 +1268,20 @@ int launch_far_cache(mgr_ctx *ctx)
     a.far_cell = ctx->far_cell;
     a.far_shift = ctx->far_shift;
     const int64_t threads = ctx->N * ctx->R * ctx->F;
+    const dim3 grid((unsigned)((threads + 255) / 256));
     MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
-    hipLaunchKernelGGL(k_far_cache, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, ctx->stream, a,
-                       (int)ctx->R, ctx->far_I32, ctx->err_flag);
+    if (pi)
+        hipLaunchKernelGGL(k_far_cache<double>, grid, dim3(256), 0, ctx->stream, a, (int)ctx->R, ctx->far_I64,
+                           ctx->err_flag);
+    else
+        hipLaunchKernelGGL(k_far_cache<float>, grid, dim3(256), 0, ctx->stream, a, (int)ctx->R, ctx->far_I32,
+                           ctx->err_flag);
     MGR_LAUNCH_CHECK();
     int32_t errv = 0;
     MGR_HIP(hipStreamSynchronize(ctx->stream));
     MGR_HIP(hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (errv) ctx->far_bad = true;
-    else ctx->far_valid = true;
+    if (errv) ctx->far_bad[pi] = true;
+    else ctx->far_valid[pi] = true;
     return MGR_OK;
 }
 
@@ -1279,11 +1293,13 @@ extern "C" int mgr_set_far_cache(mgr_ctx *ctx, int enable)
     MGR_HIP(hipSetDevice(ctx->device));
     MGR_HIP(hipStreamSynchronize(ctx->stream));
     ctx->far_cache = enable != 0;
-    if (!ctx->far_cache && ctx->far_I32) {
-        (void)hipFree(ctx->far_I32);
+    if (!ctx->far_cache) {
+        if (ctx->far_I32) (void)hipFree(ctx->far_I32);
+        if (ctx->far_I64) (void)hipFree(ctx->far_I64);
         ctx->far_I32 = nullptr;
+        ctx->far_I64 = nullptr;
     }
-    ctx->far_valid = ctx->far_bad = false;
+    ctx->far_invalidate();
     ctx->graph_precision = -1;   // captured launches hold the old choice
     return MGR_OK;
 }

@@ -425,7 +425,7 @@ int launch_trace(mgr_ctx *ctx)
     // the ALI graphs captured pointers into the old cache
     ctx->graph_precision = -1;
     ctx->masks_valid = false;
-    ctx->far_valid = ctx->far_bad = false;
+    ctx->far_invalidate();
     if (ctx->seg_cell) {
         (void)hipFree(ctx->seg_cell); (void)hipFree(ctx->seg_shift); (void)hipFree(ctx->seg_dZ);
         (void)hipFree(ctx->seg_f32); (void)hipFree(ctx->seg_hdz32); (void)hipFree(ctx->step_mask);

@@ -111,3 +111,14 @@ def test_masks_follow_the_frequencies():
     ctx.compute_boundary_intensities()
     ctx.compute_radiation_field("fp32")
     assert np.array_equal(ctx.J(), J1)
+
+
+def test_far_cache_fp64_is_bit_identical():
+    from magritte_amd.hip import Context
+    print("\n[skip] stress8 far-end intensity cache, FP64 kernel", flush=True)
+    ctx = Context(synthetic.stress_model(n=8, seed=11)).setup()
+    ctx.compute_radiation_field("fp64")
+    J1, L1 = ctx.J(), ctx.lambda_diag()
+    ctx.set_far_cache(False)
+    ctx.compute_radiation_field("fp64")
+    assert np.array_equal(ctx.J(), J1) and np.array_equal(ctx.lambda_diag(), L1)
