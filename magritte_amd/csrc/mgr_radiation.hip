@@ -1027,8 +1027,9 @@ static int launch_rf_shape(mgr_ctx *ctx, int nwg, int ch, int64_t o_begin, int64
 }
 
 // Launch-shape autotuner: waves (64-bin chunks) per workgroup and steps per
-// staged chunk, timed on a slice of up to 16384 origins in the middle of the
-// model; the fastest shape is kept for the context and precision.  Runs
+// staged chunk, timed on up to 16384 origins of the model; the fastest shape
+// (or the one with the fewest waves among near-ties) is kept for the context
+// and precision.  Runs
 // outside graph capture, before the first radiation field of a precision.
 template <typename real>
 static int tune(mgr_ctx *ctx, int pi)
@@ -1043,21 +1044,33 @@ static int tune(mgr_ctx *ctx, int pi)
     for (int c : {16, 32, 64})
         if (c <= chmax) chs.push_back(c);
     if (chs.empty() || chs.back() != chmax) chs.push_back(chmax);
+    // up to 16384 origins in four slices spread over the model (at 1/8, 3/8,
+    // 5/8 and 7/8 of its cells), so that the timing sees inner and outer cells
     const int64_t cnt = std::min<int64_t>(ctx->N, 16384);
-    const int64_t o0 = (ctx->N - cnt) / 2;
+    const int nslice = cnt >= 4096 ? 4 : 1;
+    const int64_t per = cnt / nslice;
+    int64_t os[4];
+    for (int k = 0; k < nslice; k++)
+        os[k] = nslice == 1 ? (ctx->N - cnt) / 2
+                            : std::min<int64_t>(std::max<int64_t>((2 * k + 1) * ctx->N / 8 - per / 2, 0), ctx->N - per);
+    auto launch_slices = [&](int w, int c) {
+        int r = MGR_OK;
+        for (int k = 0; k < nslice && !r; k++) r = launch_rf_shape<real>(ctx, w, c, os[k], per);
+        return r;
+    };
     int best_w = std::min(nchunk, MAX_WAVES), best_c = chmax;
     float best = 1e30f;
     hipEvent_t e0, e1;
     MGR_HIP(hipEventCreate(&e0));
     MGR_HIP(hipEventCreate(&e1));
-    int rc = launch_rf_shape<real>(ctx, best_w, best_c, o0, cnt);   // warm-up
+    int rc = launch_slices(best_w, best_c);   // warm-up
     ctx->tune_log.clear();
     for (int w : nwgs) {
         for (int c : chs) {
             if (rc) break;
             float ms = 0.0f;
             if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = MGR_ERR_DEVICE; break; }
-            rc = launch_rf_shape<real>(ctx, w, c, o0, cnt);
+            rc = launch_slices(w, c);
             if (rc) break;
             if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
