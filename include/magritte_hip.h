@@ -274,6 +274,12 @@ int mgr_compute_image(mgr_ctx *ctx, int64_t ray, int precision, double *I_m, dou
 int mgr_radiation_field_shape(mgr_ctx *ctx, int precision, int *waves_per_group, int *steps_per_chunk,
                               int capacity, int *n_tried, int *tried_waves, int *tried_steps,
                               float *tried_ms);
+/* Forces that launch shape for later radiation fields of that precision
+ * instead of tuning (no counterpart in the reference): 1..4 waves per
+ * workgroup, 2..(the model's LDS limit, at most 64) steps per chunk; both 0
+ * hands the choice back to the autotuner.  J and Lambda* do not depend on
+ * the shape.                                                                  */
+int mgr_set_radiation_field_shape(mgr_ctx *ctx, int precision, int waves_per_group, int steps_per_chunk);
 
 /* Out-of-window skip of the FP32 radiation-field kernel (no counterpart in
  * the reference; on by default): steps at which no bin of a 64-bin wave comes

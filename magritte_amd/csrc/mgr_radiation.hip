@@ -1188,6 +1188,34 @@ extern "C" int mgr_radiation_field_shape(mgr_ctx *ctx, int precision, int *waves
     return MGR_OK;
 }
 
+extern "C" int mgr_set_radiation_field_shape(mgr_ctx *ctx, int precision, int waves_per_group, int steps_per_chunk)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (precision != MGR_FP32 && precision != MGR_FP64) return ctx->fail(MGR_ERR_ARGUMENT, "bad precision");
+    const int pi = precision == MGR_FP64 ? 1 : 0;
+    if (waves_per_group == 0 && steps_per_chunk == 0) {
+        ctx->rf_nwg[pi] = ctx->rf_ch[pi] = 0;   // the next radiation field tunes again
+    } else {
+        const int chmax = pi ? max_chunk<double>(ctx) : max_chunk<float>(ctx);
+        if (waves_per_group < 1 || waves_per_group > MAX_WAVES)
+            return ctx->fail(MGR_ERR_ARGUMENT, "waves_per_group must be 1.." + std::to_string(MAX_WAVES));
+        if (steps_per_chunk < 2 || steps_per_chunk > chmax)
+            return ctx->fail(MGR_ERR_ARGUMENT, "steps_per_chunk must be 2.." + std::to_string(chmax) +
+                                                   " for this model's line rows");
+        ctx->rf_nwg[pi] = waves_per_group;
+        ctx->rf_ch[pi] = steps_per_chunk;
+    }
+    // the ALI graphs were captured with the previous shape
+    if (ctx->graph_precision == precision) {
+        MGR_HIP(hipSetDevice(ctx->device));
+        if (ctx->graph_normal) (void)hipGraphExecDestroy(ctx->graph_normal);
+        if (ctx->graph_ng) (void)hipGraphExecDestroy(ctx->graph_ng);
+        ctx->graph_normal = ctx->graph_ng = nullptr;
+        ctx->graph_precision = -1;
+    }
+    return MGR_OK;
+}
+
 extern "C" int mgr_compute_image(mgr_ctx *ctx, int64_t ray, int precision, double *I_m, double *I_p)
 {
     if (!ctx) return MGR_ERR_ARGUMENT;

@@ -63,6 +63,7 @@ SIGNATURES = {
     "mgr_last_radiation_field_ms": ([_vp, _vp], _i),
     "mgr_trace_stats": ([_vp, _vp, _vp], _i),
     "mgr_radiation_field_shape": ([_vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp], _i),
+    "mgr_set_radiation_field_shape": ([_vp, _i, _i, _i], _i),
     "mgr_mlp_num_params": ([], _i64),
     "mgr_mlp_create": ([_i, C.c_uint64, _vp], _i),
     "mgr_mlp_destroy": ([_vp], None),
@@ -324,6 +325,11 @@ class Context:
                                                        C.byref(n), _p(tw), _p(tc), _p(tm)), "radiation_field_shape")
         k = min(n.value, cap)
         return (w.value, c.value), [(int(tw[i]), int(tc[i]), float(tm[i])) for i in range(k)]
+
+    def set_radiation_field_shape(self, precision="fp32", waves_per_group=0, steps_per_chunk=0):
+        """Force the launch shape (0, 0: back to the autotuner)."""
+        self._check(self.lib.mgr_set_radiation_field_shape(self.h, _prec(precision), int(waves_per_group),
+                                                           int(steps_per_chunk)), "set_radiation_field_shape")
 
     def trace_stats(self):
         a, b = C.c_int64(0), C.c_int64(0)

@@ -583,6 +583,13 @@ static double o_rescale_I_bdy(const orc_state *s, double freq_scaled, int64_t R,
     return (I2 - I1) / (nu2 - nu1) * (freq_scaled - nu1) + I1;
 }
 
+double orc_rescale_I_bdy(const orc_state *s, double freq_scaled, int64_t R, int64_t p, int64_t b)
+{
+    return o_rescale_I_bdy(s, freq_scaled, R, p, b);
+}
+
+int64_t orc_cell2boundary_nr(const orc_state *s, int64_t p) { return s->cell2boundary_nr[p]; }
+
 /* ---- Feautrier solve ---------------------------------------------------- */
 
 /* RayPair::solve (src/Simulation/Raypair/raypair_solver.tpp:200-420) */

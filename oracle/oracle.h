@@ -52,6 +52,12 @@ void orc_solve_for_image(int64_t ndep, int64_t first, int64_t last, const double
                          const double *dtau, double I_bdy_0, double I_bdy_n, double *Su);
 /* src/Simulation/sim_radiation.tpp:345-478 for ray r < nrays/2: I_m, I_p [ncells*nfreqs] */
 int  orc_compute_image(orc_state *s, int64_t r, double *I_m, double *I_p);
+/* Radiation::rescale_I_bdy (src/Model/Radiation/radiation.tpp:341-373): the
+ * boundary intensity of boundary cell b (cell p) for reduced ray R at
+ * freq_scaled, searched in cell p's frequency grid from notch 0. */
+double orc_rescale_I_bdy(const orc_state *s, double freq_scaled, int64_t R, int64_t p, int64_t b);
+/* Boundary::cell2boundary_nr (src/Model/Geometry/Boundary/boundary.cpp) */
+int64_t orc_cell2boundary_nr(const orc_state *s, int64_t p);
 /* src/Model/Geometry/geometry.tpp:498-543 */
 int64_t orc_get_npoints_on_ray(const orc_state *s, int64_t o, int64_t r, double dshift_max);
 /* src/Model/Geometry/geometry.tpp:433-465 */

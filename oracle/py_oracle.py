@@ -52,6 +52,10 @@ def lib():
         L.orc_compute_boundary_intensities.argtypes = [vp, vp]
         L.orc_compute_boundary_intensities.restype = C.c_int
         L.orc_set_emissivity_and_opacity.argtypes = [vp]
+        L.orc_rescale_I_bdy.argtypes = [vp, C.c_double, C.c_int64, C.c_int64, C.c_int64]
+        L.orc_rescale_I_bdy.restype = C.c_double
+        L.orc_cell2boundary_nr.argtypes = [vp, C.c_int64]
+        L.orc_cell2boundary_nr.restype = C.c_int64
         L.orc_get_dshift_max.argtypes = [vp, C.c_int64]
         L.orc_get_dshift_max.restype = C.c_double
         L.orc_trace_ray.argtypes = [vp, C.c_int64, C.c_int64, C.c_double, C.c_int64, vp, vp, vp, vp]
@@ -139,6 +143,17 @@ class Oracle:
         self.compute_boundary_intensities()
         self.compute_LTE_level_populations()
         return self
+
+    def set_emissivity_and_opacity(self):
+        lib().orc_set_emissivity_and_opacity(self._s)
+
+    def rescale_I_bdy(self, freq_scaled, R, p):
+        """Boundary intensity of boundary cell p for reduced ray R at freq_scaled."""
+        return lib().orc_rescale_I_bdy(self._s, float(freq_scaled), int(R), int(p),
+                                       lib().orc_cell2boundary_nr(self._s, int(p)))
+
+    def cell2boundary_nr(self, p):
+        return lib().orc_cell2boundary_nr(self._s, int(p))
 
     # geometry
     def dshift_max(self, o):

@@ -1,0 +1,179 @@
+"""GPU parity at the workloads' full sizes, against the FP64 oracle.
+
+* 64^3 envelope (262,144 cells, 48 rays, 7-level rotor, 6 lines x 31 bins):
+  J and the Lambda* diagonal of the FP32 and FP64 kernels on 256 random
+  origins (every ray pair and bin of each) within 1e-5 / 1e-10 of the oracle;
+  the 64^3 trace cache, multi-chunk LDS staging of its long rays, the
+  out-of-window skip and the far-end intensity cache all take part.
+* Launch shapes: J and Lambda* bit-identical for every (waves per workgroup,
+  steps per chunk) the autotuner may pick, forced one by one, on the 64^3
+  envelope (rays up to ~100 steps, so 16-step chunks stage several chunks).
+* Batch of 16 envelopes with 1,000-32,768 cells, 12 and 48 rays, n0 1e12-1e16,
+  T0 500-3000 K in one context: J per model against the oracle on sampled
+  origins; and three small models whose ALI the oracle runs to convergence:
+  the batch's per-model iteration counts equal the oracle's and its converged
+  populations agree within 1e-5.
+* Learned warm start on four held-out 64^3 envelopes: the warm-started ALI
+  converges to the LTE-started populations within 1e-5 and needs at most one
+  iteration more than the LTE start on every model.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from magritte_amd.setup import synthetic
+from oracle.py_oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+@pytest.fixture(scope="module")
+def envelope64():
+    from magritte_amd.hip import Context
+    print("\n[workload] envelope 64^3: oracle on 256 origins", flush=True)
+    m = synthetic.envelope_model(n=64)
+    rng = np.random.default_rng(64)
+    sample = np.sort(rng.choice(m.ncells, size=256, replace=False))
+    orc = Oracle(m).setup()
+    orc.compute_radiation_field_list(sample)
+    J_o = orc.J()[sample]
+    orc.compute_Jeff()
+    L_o = orc.lambda_diag()[sample]
+    del orc
+    ctx = Context(m).setup()
+    ctx.trace()
+    yield m, sample, J_o, L_o, ctx
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("fp64", 1e-10)])
+def test_envelope64_against_oracle(envelope64, precision, tol):
+    m, sample, J_o, L_o, ctx = envelope64
+    ctx.compute_radiation_field(precision)
+    J = ctx.J()[sample]
+    L = ctx.lambda_diag()[sample]
+    errJ = rel_err(J, J_o)
+    mask = L_o > 0
+    errL = rel_err(L[mask], L_o[mask])
+    shape, _ = ctx.radiation_field_shape(precision)
+    print(f"[workload] envelope64 {precision} shape {shape}: J max rel err {errJ:.3e}, "
+          f"Lambda* {errL:.3e}", flush=True)
+    assert errJ < tol
+    assert errL < tol
+    assert np.all(L[~mask] == 0.0)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_envelope64_launch_shapes_bit_identical(envelope64, precision):
+    _, sample, J_o, _, ctx = envelope64
+    ctx.set_radiation_field_shape(precision)           # back to the autotuner
+    ctx.compute_radiation_field(precision)
+    tuned, tried = ctx.radiation_field_shape(precision)
+    J0, L0 = ctx.J(), ctx.lambda_diag()
+    shapes = sorted({(w, c) for w, c, _ in tried})
+    assert (1, 16) in shapes and tuned in shapes
+    for w, c in shapes:
+        ctx.set_radiation_field_shape(precision, w, c)
+        ctx.compute_radiation_field(precision)
+        assert ctx.radiation_field_shape(precision)[0] == (w, c)
+        same = np.array_equal(ctx.J(), J0) and np.array_equal(ctx.lambda_diag(), L0)
+        print(f"[workload] envelope64 {precision} shape {(w, c)}: bit-identical {same}", flush=True)
+        assert same, (w, c)
+    # the ALI graphs follow a forced shape
+    ctx.set_radiation_field_shape(precision, 1, 16)
+    ctx.ali_iteration(False, precision)
+    assert rel_err(ctx.J()[sample], J_o) < (1e-5 if precision == "fp32" else 1e-10)
+    ctx.set_radiation_field_shape(precision)
+
+
+def _batch_models():
+    """16 envelopes spanning 1,000-32,768 cells (10^3 .. 32^3), 12 and 48 rays."""
+    rng = np.random.default_rng(3003)
+    sides = [10, 11, 12, 13, 14, 15, 16, 18, 20, 22, 24, 26, 28, 30, 31, 32]
+    ms = []
+    for k, n in enumerate(sides):
+        ms.append(synthetic.envelope_model(n=n, n0=10.0 ** rng.uniform(12.0, 16.0), T0=rng.uniform(500.0, 3000.0),
+                                           nside=1 + (k % 2), seed=int(rng.integers(0, 2 ** 31))))
+    return ms
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("fp64", 1e-10)])
+def test_batch16_against_oracle(precision, tol):
+    from magritte_amd.hip import BatchContext
+    ms = _batch_models()
+    cells = [m.ncells for m in ms]
+    assert min(cells) == 1000 and max(cells) == 32768
+    assert {m.nrays for m in ms} == {12, 48}
+    print(f"\n[workload] batch of 16 envelopes, {sum(cells)} cells, {precision}", flush=True)
+    b = BatchContext(ms).setup()
+    b.trace()
+    b.compute_radiation_field(precision)
+    Jb = b.split(b.J())
+    Lb = b.split(b.lambda_diag())
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for k, m in enumerate(ms):
+        sample = np.sort(rng.choice(m.ncells, size=24, replace=False))
+        orc = Oracle(m).setup()
+        orc.compute_radiation_field_list(sample)
+        orc.compute_Jeff()
+        errJ = rel_err(Jb[k][sample], orc.J()[sample])
+        L_o = orc.lambda_diag()[sample]
+        mask = L_o > 0
+        errL = rel_err(Lb[k][sample][mask], L_o[mask])
+        worst = max(worst, errJ, errL)
+        assert errJ < tol, (k, errJ)
+        assert errL < tol, (k, errL)
+    print(f"[workload] batch16 {precision}: worst J / Lambda* rel err {worst:.3e}", flush=True)
+    b.close()
+
+
+def test_batch_ali_against_oracle():
+    from magritte_amd.hip import BatchContext
+    ms = [synthetic.envelope_model(n=10, n0=3e12, T0=800.0, nside=1, seed=3),
+          synthetic.envelope_model(n=10, n0=1e13, T0=1500.0, nside=1, seed=4),
+          synthetic.envelope_model(n=11, n0=1e14, T0=1500.0, nside=1, seed=5)]
+    its_o, P_o = [], []
+    for m in ms:
+        orc = Oracle(m).setup()
+        its_o.append(orc.compute_level_populations(True, 60, 1))
+        P_o.append(orc.populations())
+    print(f"\n[workload] batch ALI: oracle iterations {its_o}", flush=True)
+    for precision, tol in (("fp32", 1e-5), ("fp64", 1e-9)):
+        b = BatchContext(ms).setup()
+        b.trace()
+        b.compute_level_populations(True, 60, "max", precision)
+        its, conv = b.convergence()
+        P = b.split(b.populations())
+        errs = [rel_err(P[k], P_o[k]) for k in range(len(ms))]
+        print(f"[workload] batch ALI {precision}: iterations {list(its)}, pops max rel err "
+              f"{max(errs):.3e}", flush=True)
+        assert list(its) == its_o
+        assert max(errs) < tol
+        b.close()
+
+
+def test_warm_start_held_out_envelope64():
+    from magritte_amd import warmstart
+    if not os.path.exists(warmstart.DEFAULT_WEIGHTS):
+        pytest.skip("no trained warm-start weights in magritte_amd/data")
+    net = warmstart.load(warmstart.DEFAULT_WEIGHTS)
+    rng = np.random.default_rng(2026)
+    for k in range(4):
+        n0 = 10.0 ** rng.uniform(13.0, 16.0)
+        T0 = rng.uniform(500.0, 3000.0)
+        seed = int(rng.integers(0, 2 ** 31))
+        m = synthetic.envelope_model(n=64, n0=n0, T0=T0, seed=seed)
+        r = warmstart.evaluate_model(m, net, max_iter=100)
+        print(f"\n[workload] warm start held-out 64^3 {k} (n0 {n0:.3g}, T0 {T0:.0f}): {r}", flush=True)
+        assert r["converged"]
+        assert r["pops_max_rel_diff"] < 1e-5
+        assert r["nn_iterations"] <= r["lte_iterations"] + 1
+    net.close()

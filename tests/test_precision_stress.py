@@ -3,11 +3,15 @@ with +-30 km/s radial velocity jumps): errors of the FP32 radiation-field
 kernel against the FP64 kernel and the FP64 oracle, for J and for the
 converged level populations, per level.
 
-The size is MGR_STRESS_N cells per side (default 16; the full 64^3 case is
-run as MGR_STRESS_N=64).  The oracle's J is computed on a random sample of
-origins (all ray pairs and bins of each), its converged populations only when
-N <= 16^3, where its ALI finishes in about half a minute.  With MGR_REPORT set the tables
-are written there as JSON.
+Cases: the full 64^3 model (262,144 cells), the same physics at 16^3, where
+the oracle's own ALI also runs (about half a minute), and a 16^3 variant with
+the same velocity jumps at n0 = 1e13 m^-3, where the populations are far from
+LTE and the ALI needs several iterations and an Ng step, so that the FP32
+radiation field's error reaches populations that actually move (at n0 = 1e17
+the collision rates exceed A by ~1e6 and LTE is already converged).
+MGR_STRESS_N selects one size of the n0 = 1e17 model instead.  The oracle's J
+is computed on a random sample of origins (all ray pairs and bins of each).
+With MGR_REPORT set the tables are written there as JSON.
 
 Bars (the precision targets of the FP32 path): J within 1e-5 relative of the
 FP64 solutions; converged populations within 1e-5 relative.
@@ -23,8 +27,11 @@ from magritte_amd.setup import synthetic
 
 pytestmark = pytest.mark.gpu
 
-N_SIDE = int(os.environ.get("MGR_STRESS_N", "16"))
 SAMPLE = int(os.environ.get("MGR_STRESS_SAMPLE", "256"))
+if os.environ.get("MGR_STRESS_N"):
+    CASES = [(int(os.environ["MGR_STRESS_N"]), 1.0e17)]
+else:
+    CASES = [(64, 1.0e17), (16, 1.0e17), (16, 1.0e13)]
 
 
 def _stats(a, b, mask=None):
@@ -36,13 +43,14 @@ def _stats(a, b, mask=None):
     return {"max": float(rel.max()), "rms": float(np.sqrt(np.mean(rel ** 2)))}
 
 
-def test_precision_stress_report():
+@pytest.mark.parametrize("N_SIDE,n0", CASES)
+def test_precision_stress_report(N_SIDE, n0):
     from magritte_amd.hip import Context
     from oracle.py_oracle import Oracle
 
-    print(f"\n[stress] n={N_SIDE}^3, seed 4321, sample {SAMPLE} origins", flush=True)
-    m = synthetic.stress_model(n=N_SIDE)
-    report = {"model": f"stress envelope {N_SIDE}^3 (n0 1e17, 20% +-30 km/s jumps, seed 4321)",
+    print(f"\n[stress] n={N_SIDE}^3, n0 {n0:.0e}, seed 4321, sample {SAMPLE} origins", flush=True)
+    m = synthetic.stress_model(n=N_SIDE, n0=n0)
+    report = {"model": f"stress envelope {N_SIDE}^3 (n0 {n0:.0e}, 20% +-30 km/s jumps, seed 4321)",
               "ncells": m.ncells}
 
     c32 = Context(m).setup()
@@ -94,7 +102,7 @@ def test_precision_stress_report():
     print("[stress] populations", json.dumps(report["populations"]), flush=True)
     out = os.environ.get("MGR_REPORT")
     if out:
-        with open(out, "w") as fh:
+        with open(out.replace(".json", f"_{N_SIDE}_{n0:.0e}.json"), "w") as fh:
             json.dump(report, fh, indent=1)
     c32.close()
     c64.close()
