@@ -89,7 +89,9 @@ def test_one_ng_step_matches_oracle():
     for a, b in zip(ctx.population_history(), orc.population_history()):
         assert np.array_equal(a, b)
     assert np.array_equal(ctx.population_history()[0], pop)
-    assert conv_g[0] == conv_o[0]
+    # the reference sums the weight 1/(ncells nlev) per unconverged level under an
+    # OpenMP reduction, so the fraction's last bits depend on the summation order
+    assert abs(conv_g[0] - conv_o[0]) <= 1e-12 * max(conv_o[0], 1.0 / P_o.size)
     assert abs(conv_g[1] - conv_o[1]) <= 1e-12 * conv_o[1]
     assert abs(conv_g[2] - conv_o[2]) <= 1e-12 * conv_o[2]
     e_o, k_o = orc.emissivity_opacity()

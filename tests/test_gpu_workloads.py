@@ -86,11 +86,15 @@ def test_envelope64_launch_shapes_bit_identical(envelope64, precision):
         same = np.array_equal(ctx.J(), J0) and np.array_equal(ctx.lambda_diag(), L0)
         print(f"[workload] envelope64 {precision} shape {(w, c)}: bit-identical {same}", flush=True)
         assert same, (w, c)
-    # the ALI graphs follow a forced shape
+    # the ALI graphs follow a forced shape; the iteration's radiation field
+    # sees LTE populations, as the oracle's J did, and its stat-eq step moves
+    # them, so LTE is restored for the tests that follow
     ctx.set_radiation_field_shape(precision, 1, 16)
+    ctx.compute_LTE_level_populations()
     ctx.ali_iteration(False, precision)
     assert rel_err(ctx.J()[sample], J_o) < (1e-5 if precision == "fp32" else 1e-10)
     ctx.set_radiation_field_shape(precision)
+    ctx.compute_LTE_level_populations()
 
 
 def _batch_models():
