@@ -9,14 +9,24 @@ A step is one compute_radiation_field over the whole model (every ray pair x
 every frequency bin), inputs resident in HBM.  value = ray-pair.freq solves
 per second over the timed steps.  The JSON line also carries:
   roofline      -- the radiation-field kernel, timed with HIP events on its
-                   own stream: algorithmic bytes per launch (8 B per point and
-                   bin for dtau and S, 8 B per solve for J and Lambda*) over the
-                   kernel time, against the HBM peak.  The kernel fuses the
-                   line profiles and the Feautrier solve, so those bytes never
-                   reach HBM ("traffic" holds the measured HBM bytes per
-                   launch); its limiter is VALU issue, reported beside it;
+                   own stream.  It fuses the line profiles and the Feautrier
+                   solve and is bound by vector-ALU issue, so "bound" is
+                   "valu": achieved = the solve's algorithmic FP32 flops
+                   (28 per point and bin) per second against MI355X's FP32
+                   peak (157.3 TFLOP/s published; 78.6 without packed FP32,
+                   which this build does not use, in "frac_non_packed").
+                   "traffic" and "hbm" give the measured HBM bytes per launch
+                   and their rate against the 8 TB/s peak; "equivalent_bytes"
+                   are the dtau/S/u/Lambda* bytes an HBM-staged solver would
+                   move, which this kernel never moves.  The counter figures
+                   come from profiles/ and are reported only when the profiled
+                   kernel sources hash to the benched ones;
   cpu_baseline  -- the FP64 OpenMP oracle (restatement of the reference's
-                   RayPair path) timed on a bounded random sample of origins;
+                   RayPair path) timed on a bounded random sample of origins,
+                   with the host CPU model and thread count, and the largest
+                   relative difference of the GPU's J from the oracle's over
+                   that sample;
+  fp64_kernel   -- the FP64 radiation-field kernel on the same workload;
   time_to_converged -- the ALI loop to max relative change < 1e-6, from LTE
                    and from the learned warm start (trained weights in
                    magritte_amd/data), with the converged populations' match.
@@ -25,6 +35,7 @@ per second over the timed steps.  The JSON line also carries:
 and ray counts solved together in one context (one launch per kernel).
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -45,6 +56,15 @@ FLOPS_PER_POINT_BIN = 28
 BYTES_PER_POINT_BIN = 8
 BYTES_PER_SOLVE = 8
 PEAK_HBM_GBS = 8000.0
+# MI355X published dense peaks (TFLOP/s): FP32 with packed FP32 instructions,
+# and the vector rate without them (one FMA per lane per cycle); FP64 vector.
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_NON_PACKED_TFLOPS = 78.6
+PEAK_FP64_TFLOPS = 78.6
+# Kernel sources whose hash the counter summary under profiles/ records: the
+# counters describe the benched kernel only when the hashes agree.
+KERNEL_SOURCES = ("magritte_amd/csrc/mgr_radiation.hip", "magritte_amd/csrc/mgr_ray.hpp",
+                  "magritte_amd/csrc/mgr_common.hpp")
 
 
 def log(msg):
@@ -56,19 +76,40 @@ def build_workload(n):
     return synthetic.envelope_model(n=n)
 
 
+def kernel_source_sha1():
+    h = hashlib.sha1()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
 def profile_counters(path):
     """Counter summary of the radiation-field kernel committed under profiles/
     (HBM bytes per launch from FETCH_SIZE + WRITE_SIZE, copy-kernel scaled;
     VALU issue fraction from SQ_INSTS_VALU net of transcendentals)."""
     try:
         with open(path) as fh:
-            d = json.load(fh)
-        return d, d.get("workload")
+            return json.load(fh)
     except (OSError, ValueError):
-        return None, None
+        return None
 
 
-def cpu_baseline(model, target_s, seed=0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(model, target_s, J_gpu=None, seed=0):
+    """The FP64 oracle on a random sample of origins (all ray pairs and bins),
+    in solves per second; with the GPU's J of the same model, also the largest
+    relative difference of J over the sample."""
     from oracle.py_oracle import Oracle
     orc = Oracle(model).setup()
     rng = np.random.default_rng(seed)
@@ -95,10 +136,15 @@ def cpu_baseline(model, target_s, seed=0):
         n = min(model.ncells, int(n * max(2.0, min(8.0, target_s / max(dt, 1e-3)))))
     from oracle import py_oracle
     cores = py_oracle.lib().orc_nthreads_max()
-    return dict(value=solves(sample) / dt, unit="ray-pair·freq solves/s", cores=int(cores),
-                kind="port",
-                sample=f"{len(sample)} random origins x all {model.nrays // 2} ray pairs x {F} bins, "
-                       f"FP64 oracle, {dt:.1f} s")
+    out = dict(value=solves(sample) / dt, unit="ray-pair·freq solves/s", cores=int(cores),
+               cpu_model=cpu_model(), kind="port",
+               sample=f"{len(sample)} random origins x all {model.nrays // 2} ray pairs x {F} bins, "
+                      f"FP64 oracle, {dt:.1f} s")
+    if J_gpu is not None:
+        J_o = orc.J()[sample]
+        Jg = J_gpu[sample]
+        out["J_max_rel_err_gpu_vs_oracle"] = float(np.max(np.abs(Jg - J_o) / np.abs(J_o)))
+    return out
 
 
 def copy_probe(gib):
@@ -126,26 +172,40 @@ def time_radiation_field(ctx, steps, warmup, precision):
     return time.perf_counter() - t0, float(np.mean(kernel_ms))
 
 
-def roofline_entry(sum_ndep, nfreqs, solves, k_ms, precision, counters_workload):
+def roofline_entry(sum_ndep, nfreqs, solves, k_ms, precision, kernel, counters_workload):
     points = sum_ndep * nfreqs                 # sum over solves of their points
-    alg_bytes = BYTES_PER_POINT_BIN * points + BYTES_PER_SOLVE * solves
+    equiv_bytes = BYTES_PER_POINT_BIN * points + BYTES_PER_SOLVE * solves
     flops = FLOPS_PER_POINT_BIN * points
-    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
-    prof, pw = profile_counters(os.path.join(ROOT, "profiles", "radiation_field_envelope64_counters.json"))
-    same = prof is not None and pw == counters_workload and precision == "fp32"
-    return {
-        "bound": "hbm",
-        # the envelope's rotor lines are far apart: the one-candidate-line FP32 instantiation runs
-        "kernel": "k_radiation_field<float, true>" if precision == "fp32" else "k_radiation_field<double, false>",
-        "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-        "frac": round(achieved / PEAK_HBM_GBS, 4),
-        "traffic": prof.get("hbm_bytes_per_launch") if same else None,
+    tflops = flops / (k_ms * 1e-3) / 1e12
+    prof = profile_counters(os.path.join(ROOT, "profiles", "radiation_field_envelope64_counters.json"))
+    sha = kernel_source_sha1()
+    same = (prof is not None and prof.get("workload") == counters_workload and prof.get("kernel") == kernel
+            and prof.get("source_sha1") == sha)
+    traffic = prof.get("hbm_bytes_per_launch") if same else None
+    fp32 = precision == "fp32"
+    peak = PEAK_FP32_TFLOPS if fp32 else PEAK_FP64_TFLOPS
+    out = {
+        "bound": "valu",
+        "kernel": kernel,
+        "achieved": round(tflops, 2), "peak": peak, "unit": "TFLOP/s",
+        "frac": round(tflops / peak, 4),
+        "traffic": traffic,
         "kernel_ms": round(k_ms, 3),
-        "algorithmic_bytes_per_launch": int(alg_bytes),
-        "limiter": "valu",
+        "flops_per_point_bin": FLOPS_PER_POINT_BIN,
+        "hbm": ({"achieved": round(traffic / (k_ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                 "frac": round(traffic / (k_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)} if traffic else None),
         "valu_issue_frac": prof.get("valu_issue_frac") if same else None,
-        "fp32_tflops": round(flops / (k_ms * 1e-3) / 1e12, 3),
+        "counters_source_sha1": prof.get("source_sha1") if prof else None,
+        "kernel_source_sha1": sha,
+        "equivalent_bytes": {"per_launch": int(equiv_bytes),
+                             "rate_GBs": round(equiv_bytes / (k_ms * 1e-3) / 1e9, 1),
+                             "note": "dtau and S per point and bin, u and Lambda* per solve, as an HBM-staged "
+                                     "solver would move them; this fused kernel never moves these bytes"},
     }
+    if fp32:
+        out["peak_non_packed"] = PEAK_FP32_NON_PACKED_TFLOPS
+        out["frac_non_packed"] = round(tflops / PEAK_FP32_NON_PACKED_TFLOPS, 4)
+    return out
 
 
 def time_to_converged(make_ctx, precision, weights):
@@ -192,6 +252,7 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=2, help="timed FP64 radiation fields beside the FP32 ones")
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--weights", default=os.path.join(ROOT, "magritte_amd", "data", "warmstart_mlp.npz"))
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
@@ -232,10 +293,20 @@ def main():
     log(f"[bench] set-up + trace {time.perf_counter() - t0:.1f} s; {pairs} ray pairs solved, sum ndep {sum_ndep}")
     solves_per_step = pairs * model.nfreqs
     wall, k_ms = time_radiation_field(ctx, args.steps, args.warmup, args.precision)
+    kernel = ctx.last_radiation_field_kernel()
     shape, tried = ctx.radiation_field_shape(args.precision)
-    log(f"[bench] launch shape (waves per workgroup, steps per chunk) {shape}; tried {tried}")
+    log(f"[bench] {kernel}: launch shape (waves per workgroup, steps per chunk) {shape}; tried {tried}")
     ms_per_step = 1e3 * wall / args.steps
     value = solves_per_step * args.steps / wall
+    J_gpu = ctx.J() if args.workload == "envelope" and not args.no_cpu else None
+    other = None
+    if args.precision == "fp32" and args.fp64_steps > 0:
+        # the FP64 kernel on the same workload, for comparison
+        w64, k64 = time_radiation_field(ctx, args.fp64_steps, 1, "fp64")
+        other = {"precision": "fp64", "kernel": ctx.last_radiation_field_kernel(), "kernel_ms": round(k64, 3),
+                 "ms_per_step": round(1e3 * w64 / args.fp64_steps, 3),
+                 "solves_per_s": solves_per_step * args.fp64_steps / w64}
+        log(f"[bench] FP64 kernel {k64:.1f} ms")
 
     out = {
         "metric": METRIC, "value": value, "unit": "ray-pair·freq solves/s", "n_gpus": 1,
@@ -246,8 +317,10 @@ def main():
                    "nrays": max(m.nrays for m in models), "nfreqs": model.nfreqs,
                    "solves_per_step": int(solves_per_step), "sum_ndep": int(sum_ndep),
                    "launch_shape": {"waves_per_workgroup": shape[0], "steps_per_chunk": shape[1]}},
-        "roofline": roofline_entry(sum_ndep, model.nfreqs, solves_per_step, k_ms, args.precision, tag),
+        "roofline": roofline_entry(sum_ndep, model.nfreqs, solves_per_step, k_ms, args.precision, kernel, tag),
     }
+    if other:
+        out["fp64_kernel"] = other
     ctx.close()
 
     if not args.no_converge:
@@ -255,7 +328,9 @@ def main():
 
     if not args.no_cpu and args.workload == "envelope":
         log("[bench] CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(model, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(model, args.cpu_seconds, J_gpu)
+        log(f"[bench] GPU J against the oracle on the sample: max rel err "
+            f"{out['cpu_baseline']['J_max_rel_err_gpu_vs_oracle']:.3e}")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

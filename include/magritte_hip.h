@@ -302,6 +302,11 @@ int mgr_window_skip_stats(mgr_ctx *ctx, int64_t *point_chunks, int64_t *skipped)
 /* Kernel time of the last mgr_compute_radiation_field, measured with HIP
  * events on the context's stream (ms).                                        */
 int mgr_last_radiation_field_ms(mgr_ctx *ctx, double *ms);
+/* Name of the radiation-field kernel that launch dispatched (the FP32 kernel
+ * has a one-candidate-line instantiation), NUL-terminated in at most
+ * `capacity` bytes; "" before the first radiation field.  No counterpart in
+ * the reference.                                                              */
+int mgr_last_radiation_field_kernel(mgr_ctx *ctx, char *name, int capacity);
 /* Sum over the cached traces of the expanded points of every ray pair
  * (sum of ndep) and the number of ray pairs with ndep > 1.                    */
 int mgr_trace_stats(mgr_ctx *ctx, int64_t *sum_ndep, int64_t *n_pairs_solved);

@@ -211,6 +211,8 @@ struct mgr_ctx {
     double  *J = nullptr;           // [N*F]
     double  *Lam = nullptr;         // [N*F]  sum_r 2 w_r Lambda_ar / chi
     double  *Ls = nullptr;          // [N*L]
+    // each radiation-field wave's share of the Lambda* sums: [N][ceil(F/64)][L]
+    double  *Lpart = nullptr;
     double  *Jlin = nullptr, *Jeff = nullptr, *Jdif = nullptr;  // [N*L]
     // Radiation::u with scattering: each ray pair's u at the origin,
     // [R/2][N*F] at (r N + p) F + f; null without scattering
@@ -218,6 +220,7 @@ struct mgr_ctx {
     int32_t *err_flag = nullptr;    // device error word
     bool     J_set = false;
     float    last_rf_ms = 0.0f;
+    const char *last_rf_kernel = "";   // name of the radiation-field kernel last launched
     // radiation-field launch shape per precision (FP32, FP64): waves per
     // workgroup and staged steps per chunk; 0 until chosen
     int      rf_nwg[2] = {0, 0}, rf_ch[2] = {0, 0};
@@ -246,6 +249,7 @@ struct mgr_ctx {
     // any of the step's sub-points; valid for the current trace and frequencies
     uint8_t *step_mask = nullptr;   // [total segments]
     bool     masks_valid = false;
+    float    hdz32_max = 0.0f;      // largest dz' of the cached steps (set with the masks)
     bool     rf_window_skip = true;
     // lane order of the radiation-field kernel (lane_bin), chosen with the
     // masks: root-major when no bin can meet a second line's window

@@ -6,6 +6,7 @@
 // the reference and must round exactly as its scalar code does.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 
@@ -567,6 +568,7 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
     if ((rc = dalloc(ctx, &ctx->J, N * F))) return rc;
     if ((rc = dalloc(ctx, &ctx->Lam, N * F))) return rc;
     if ((rc = dalloc(ctx, &ctx->Ls, N * L))) return rc;
+    if ((rc = dalloc(ctx, &ctx->Lpart, N * ((F + WAVE - 1) / WAVE) * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->Jlin, N * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->Jeff, N * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->Jdif, N * L))) return rc;
@@ -611,7 +613,7 @@ int free_model(mgr_ctx *ctx)
     dfree(ctx->nr_line); dfree(ctx->f_appears); dfree(ctx->f_k); dfree(ctx->f_z);
     dfree(ctx->d_lam_f_off); dfree(ctx->d_lam_f); dfree(ctx->I_bdy); dfree(ctx->pop);
     dfree(ctx->prev1); dfree(ctx->prev2); dfree(ctx->prev3); dfree(ctx->pop_tot); dfree(ctx->emis);
-    dfree(ctx->opac); dfree(ctx->J); dfree(ctx->Lam); dfree(ctx->Ls); dfree(ctx->Jlin);
+    dfree(ctx->opac); dfree(ctx->J); dfree(ctx->Lam); dfree(ctx->Ls); dfree(ctx->Lpart); dfree(ctx->Jlin);
     dfree(ctx->Jeff); dfree(ctx->Jdif); dfree(ctx->err_flag); dfree(ctx->seg_off); dfree(ctx->nseg);
     dfree(ctx->npts); dfree(ctx->seg_cell); dfree(ctx->seg_shift); dfree(ctx->seg_dZ);
     dfree(ctx->seg_f32); dfree(ctx->seg_hdz32); dfree(ctx->far_cell); dfree(ctx->far_shift);
@@ -919,6 +921,13 @@ int mgr_last_radiation_field_ms(mgr_ctx *ctx, double *ms)
 {
     if (!ctx || !ms) return MGR_ERR_ARGUMENT;
     *ms = ctx->last_rf_ms;
+    return MGR_OK;
+}
+
+int mgr_last_radiation_field_kernel(mgr_ctx *ctx, char *name, int capacity)
+{
+    if (!ctx || !name || capacity < 1) return MGR_ERR_ARGUMENT;
+    std::snprintf(name, (size_t)capacity, "%s", ctx->last_rf_kernel);
     return MGR_OK;
 }
 

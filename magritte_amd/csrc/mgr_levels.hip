@@ -24,30 +24,19 @@ constexpr int32_t ERR_SINGULAR = 16;
 
 // The Lambda* diagonal of RayPair::update_Lambda (raypair_lambda.tpp:94-149),
 // n_off_diag = 0: Ls(p,k) = sum over bins f whose back-map names transition k
-// of A_k w_z nu phi_k(nu) * [sum_r 2 w_r L_diag / chi](p,f).
-__global__ void k_lambda(int64_t N, int L, int F, const double *__restrict__ Lam,
-                         const double *__restrict__ qw, const double *__restrict__ nu,
-                         const double *__restrict__ pw, const double *__restrict__ A,
-                         const double *__restrict__ frequency, const int32_t *__restrict__ lam_f_off,
-                         const int32_t *__restrict__ lam_f, const int32_t *__restrict__ f_z,
-                         double *__restrict__ Ls)
+// of A_k w_z nu phi_k(nu) * [sum_r 2 w_r L_diag / chi](p,f).  The
+// radiation-field kernel leaves each 64-bin wave's share of every sum in
+// Lpart; here they are added in chunk order.
+__global__ void k_lambda_sum(int64_t N, int L, int nchunk, const double *__restrict__ Lpart,
+                             double *__restrict__ Ls)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N * L) return;
     const int64_t p = t / L;
-    const int k = (int)(t % L);
-    const double freq_line = frequency[k];
-    const double width = freq_line * pw[p];
-    const double iw = 1.0 / width;
+    const int k = (int)(t - p * L);
     double ls = 0.0;
-    for (int q = lam_f_off[k]; q < lam_f_off[k + 1]; q++) {
-        const int f = lam_f[q];
-        const double nuf = nu[p * F + f];
-        const double x = iw * (nuf - freq_line);
-        const double prof = iw * INVERSE_SQRT_PI * (1.0 / exp(x * x));
-        ls += A[k] * qw[f_z[f]] * (nuf * prof * Lam[p * F + f]);
-    }
-    Ls[p * L + k] = ls;
+    for (int c = 0; c < nchunk; c++) ls += Lpart[(p * nchunk + c) * L + k];
+    Ls[t] = ls;
 }
 
 // Simulation::compute_Jeff (sim_lines.tpp:176-220): Jlin = sum_z w_z J(p, nr_line),
@@ -389,9 +378,8 @@ namespace mgr {
 
 int launch_lambda(mgr_ctx *ctx)
 {
-    hipLaunchKernelGGL(k_lambda, dim3(grid_for(ctx->N * ctx->L, 256)), dim3(256), 0, ctx->stream,
-                       ctx->N, (int)ctx->L, (int)ctx->F, ctx->Lam, ctx->qweights, ctx->nu, ctx->pw,
-                       ctx->A, ctx->frequency, ctx->d_lam_f_off, ctx->d_lam_f, ctx->f_z, ctx->Ls);
+    hipLaunchKernelGGL(k_lambda_sum, dim3(grid_for(ctx->N * ctx->L, 256)), dim3(256), 0, ctx->stream,
+                       ctx->N, (int)ctx->L, (int)((ctx->F + WAVE - 1) / WAVE), ctx->Lpart, ctx->Ls);
     MGR_LAUNCH_CHECK();
     return MGR_OK;
 }

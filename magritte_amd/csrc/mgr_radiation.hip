@@ -61,6 +61,7 @@
 // the profile argument accurate in FP32.
 #include <algorithm>
 #include <cfloat>
+#include <cmath>
 #include <vector>
 
 #include "mgr_common.hpp"
@@ -141,6 +142,8 @@ struct RFArgs {
     uint64_t       L_magic;                   // ceil(2^32 / L): idx / L = (idx * L_magic) >> 32
     const double  *origin_shift;              // FP64 shift of the origin per entry (null: 1, co-moving)
     const uint8_t *step_mask;                 // per step, out-of-window bit per 64-bin chunk (or null)
+    float          noop_lo, noop_hi;          // an out-of-window sub-point after another is an exact no-op on
+                                              // (y, z, p) when noop_lo < y < noop_hi (noop_lo = inf: never)
     bool           root_major;                // lane order (lane_bin)
     const int32_t *cell2bdy, *antipod;        // antipod: global ray indices
     const int32_t *cell_model;                // [N] model of each cell
@@ -151,6 +154,11 @@ struct RFArgs {
     double         cand_lo, cand_hi;          // candidate-line bounds (FP32 variant)
     double         lstart_threshold;          // first line that can be in a window (FP64 variant)
     double        *J, *Lam;                   // [N*F]
+    // RayPair::update_Lambda's sums: back-map of bin f to transition f_k[f]
+    // (L: none) and root f_z[f], with the line data and profile widths
+    const int32_t *f_k, *f_z;
+    const double  *qw, *A_k, *freq_k, *pw;
+    double        *Lpart;                     // [N][nchunk][L] one sum per wave and transition
     double        *u_out;                     // [R/2][N*F] per-pair u (scattering), or null
     int32_t       *err_flag;
 };
@@ -211,17 +219,18 @@ __device__ __forceinline__ float rfl(float x)
 __device__ __forceinline__ void eta_chi32(const RFArgs<float> &a, const float4 *__restrict__ row,
                                           const Lane32 &ln, float delta, float &eta, float &chi)
 {
+#pragma clang fp contract(off)
     const char *base = reinterpret_cast<const char *>(row);
-    const float t = delta * ln.nuf;
-    const float nus = t + ln.nuf;
+    // nu_f (1 + delta) and (nu_f delta + nu_f - nu_l), each with one rounding
+    const float nus = fmaf(delta, ln.nuf, ln.nuf);
     const float4 v0 = *reinterpret_cast<const float4 *>(base + ln.off0);
-    const float x0 = (t + ln.A0) * v0.x;
+    const float x0 = fmaf(delta, ln.nuf, ln.A0) * v0.x;
     const float g0 = nus * __builtin_amdgcn_exp2f(-(x0 * x0));
     float e_acc = v0.y * g0;
     float c_acc = fmaf(v0.z, g0, (float)(CHI_FLOOR * CHI_SCALE));
     if (ln.has2) {
         const float4 v1 = *reinterpret_cast<const float4 *>(base + ln.off1);
-        const float x1 = (t + ln.A1) * v1.x;
+        const float x1 = fmaf(delta, ln.nuf, ln.A1) * v1.x;
         const float g1 = nus * __builtin_amdgcn_exp2f(-(x1 * x1));
         e_acc = fmaf(v1.y, g1, e_acc);
         c_acc = fmaf(v1.z, g1, c_acc);
@@ -229,7 +238,7 @@ __device__ __forceinline__ void eta_chi32(const RFArgs<float> &a, const float4 *
             for (int c = 2; c < ln.nc; c++) {
                 const int l = ln.la + c;
                 const float4 v = row[l];
-                const float x = (t + (float)(ln.nu_f64 - a.tb->line[l])) * v.x;
+                const float x = fmaf(delta, ln.nuf, (float)(ln.nu_f64 - a.tb->line[l])) * v.x;
                 const float g = nus * __builtin_amdgcn_exp2f(-(x * x));
                 e_acc = fmaf(v.y, g, e_acc);
                 c_acc = fmaf(v.z, g, c_acc);
@@ -298,6 +307,12 @@ __device__ __forceinline__ double rescale_I_bdy(const double *__restrict__ nup,
 
 // ---- Feautrier elimination state of one side ----------------------------------
 
+__device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// The recurrences are written with explicit fused multiply-adds (and
+// contraction off), so that their rounding does not depend on how the
+// compiler schedules or unrolls the loops around them.
 template <typename real>
 struct Sweep {
     real S_q, chi_q, hdz_q, cq;   // pending (outer) point: S, chi, its dz' and chi dz'
@@ -316,7 +331,8 @@ struct Sweep {
 
     __device__ __forceinline__ void pend(real S, real chi, real hdz)
     {
-        p = S * c + z;
+#pragma clang fp contract(off)
+        p = fmad(S, c, z);
         S_q = S;
         chi_q = chi;
         hdz_q = hdz;
@@ -327,23 +343,26 @@ struct Sweep {
     // interval takes the pending point's dz'
     __device__ __forceinline__ void step(real chi)
     {
-        c = chi * hdz_q + cq;
+#pragma clang fp contract(off)
+        c = fmad(chi, hdz_q, cq);
         close_row();
     }
 
     // outward sweeps: the interval takes the new (outer) point's dz'
     __device__ __forceinline__ void step_out(real chi, real hdz)
     {
+#pragma clang fp contract(off)
         c = (chi_q + chi) * hdz;
         close_row();
     }
 
     __device__ __forceinline__ void close_row()
     {
+#pragma clang fp contract(off)
         const real x = c + y;
-        const real q = Prec<real>::rcp(c * x + (real)1);
-        y = x * q + c;
-        z = (S_q * c + p) * q;
+        const real q = Prec<real>::rcp(fmad(c, x, (real)1));
+        y = fmad(x, q, c);
+        z = fmad(S_q, c, p) * q;
     }
 };
 
@@ -537,6 +556,14 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
     const int ns = a.nseg[q];
     const int L = a.L;
     const int CH = a.ch;
+    // Wave-uniform: the pending point is an out-of-window sub-point and y lies
+    // in (noop_lo, noop_hi) on every lane.  Then each further out-of-window
+    // sub-point closes its row with c <= 2 chi_floor dz'_max, below half an
+    // ulp of y and with c y below 2^-25, so that x = y, q = 1, y stays y and,
+    // with S = 0 pending and p = z, z and p stay as they are: only the pending
+    // point's dz' (and chi dz') moves on.  The result is the same bits as
+    // stepping every sub-point.
+    bool noop = false;
     for (int b_ = ns; b_ > 0; b_ -= CH) {
         const int a_ = b_ > CH ? b_ - CH : 0;
         const int cnt = b_ - a_ + 1;
@@ -554,12 +581,31 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
             if ((unsigned)nh & cbit) {
                 // no bin of this wave meets a line window in this step: every
                 // sub-point has eta = 0 and chi at the floor, as evaluated
-                for (int m = n; m >= 1; --m) {
+                if (!noop) {
+                    // the first closes the row of the last evaluated point
                     sw.step(chi_floor);
                     sw.pend((real)0, chi_floor, hdz);
+                    if constexpr (sizeof(real) == 4)
+                        noop = __all(sw.y > a.noop_lo && sw.y < a.noop_hi);
+                    if (!noop) {
+                        for (int m = n - 1; m >= 1; --m) {
+                            sw.step(chi_floor);
+                            sw.pend((real)0, chi_floor, hdz);
+                        }
+                        continue;
+                    }
                 }
+                // a no-op stretch: pass over the following out-of-window steps
+                // of the chunk, keeping the dz' of the last one
+                int last = e;
+                while (last > 1 && ((unsigned)rfl_int(st[last - 1].nh) & cbit)) last--;
+                const real hdz_l = last == e ? hdz : (real)rfl_real(st[last].dz);
+                sw.hdz_q = hdz_l;
+                sw.cq = chi_floor * hdz_l;
+                e = last;
                 continue;
             }
+            noop = false;
             // sub-points m >= half lie in the step's cell (row e), m < half in
             // the previous one (row e-1); two loops keep each row's address
             // loop-invariant
@@ -687,6 +733,45 @@ __device__ __forceinline__ void origin_point(const RFArgs<real> &a, double nu_f,
     S_o = eta_o * Prec<real>::rcp(chi_o);
 }
 
+// This wave's share of the Lambda* diagonal per transition
+// (RayPair::update_Lambda, raypair_lambda.tpp:94-149, n_off_diag 0):
+// Ls(p,k) = sum over the bins f of transition k of
+// A_k w_z nu_f phi_k(nu_f) Lam(p,f), with Lam = sum_r 2 w_r L_diag / chi as
+// the kernel accumulates it.  The wave sums its lanes' terms per transition
+// in a fixed butterfly order and lane k writes transition k's sum;
+// k_lambda_sum adds the waves of a cell in chunk order.
+template <typename real>
+__device__ __forceinline__ void lambda_partials(const RFArgs<real> &a, int32_t o, int chunk, bool active, int f,
+                                                double nu_f, double lam_f)
+{
+    const int L = a.L;
+    const int k = active ? a.f_k[f] : L;
+    double term = 0.0;
+    if (k < L) {
+        const double freq_line = a.freq_k[k];
+        const double iw = 1.0 / (freq_line * a.pw[o]);
+        const double x = iw * (nu_f - freq_line);
+        const double prof = iw * INVERSE_SQRT_PI * (1.0 / exp(x * x));
+        term = a.A_k[k] * a.qw[a.f_z[f]] * (nu_f * prof * lam_f);
+    }
+    const int lane = (int)(threadIdx.x & (WAVE - 1));
+    // transitions present in this wave: [kmin, kmax]
+    int kmin = k, kmax = k < L ? k : -1;
+    for (int off = WAVE / 2; off >= 1; off >>= 1) {
+        kmin = min(kmin, __shfl_xor(kmin, off));
+        kmax = max(kmax, __shfl_xor(kmax, off));
+    }
+    double mine = 0.0;
+    for (int kk = kmin; kk <= kmax; kk++) {
+        double v = (k == kk) ? term : 0.0;
+        for (int off = WAVE / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+        v = __shfl(v, 0);   // lane 0's order for every lane
+        if (lane == kk) mine = v;
+    }
+    if (chunk < a.nchunk && lane < L)
+        a.Lpart[((int64_t)o * a.nchunk + chunk) * L + lane] = mine;
+}
+
 // FP32: at least 8 waves per SIMD (at most 64 VGPRs), which hide the LDS
 // latency of the line rows; FP64 is left to the compiler.
 template <typename real, bool ONE = false>
@@ -770,10 +855,12 @@ void k_radiation_field(RFArgs<real> a)
         if (a.u_out && active) a.u_out[((int64_t)r * a.N + o) * F + f] = u64;   // radiation.u[R][ind]
     }
     if (err) atomicOr(a.err_flag, err);
+    const double lam_f = Lam_acc * CHI_SCALE / (double)chi_o;
     if (active) {
         a.J[(int64_t)o * F + f] = J_acc;
-        a.Lam[(int64_t)o * F + f] = Lam_acc * CHI_SCALE / (double)chi_o;
+        a.Lam[(int64_t)o * F + f] = lam_f;
     }
+    lambda_partials(a, o, chunk, active, f, nu_f, lam_f);
 }
 
 // Image of ray r (Simulation::compute_and_write_image, sim_radiation.tpp:345-478;
@@ -919,6 +1006,13 @@ int make_args(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     a.lstart_threshold = a.cand_lo;
     a.J = ctx->J;
     a.Lam = ctx->Lam;
+    a.f_k = ctx->f_k;
+    a.f_z = ctx->f_z;
+    a.qw = ctx->qweights;
+    a.A_k = ctx->A;
+    a.freq_k = ctx->frequency;
+    a.pw = ctx->pw;
+    a.Lpart = ctx->Lpart;
     a.err_flag = ctx->err_flag;
     if (nwg < 1 || nwg > MAX_WAVES || o_begin < 0 || o_count < 1 || o_begin + o_count > ctx->N)
         return ctx->fail(MGR_ERR_ARGUMENT, "radiation-field launch shape out of range");
@@ -966,6 +1060,17 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     a.u_out = ctx->u_store;
     // the out-of-window bits are exact for the FP32 profile only
     a.step_mask = (sizeof(real) == 4 && ctx->masks_valid && ctx->rf_window_skip) ? ctx->step_mask : nullptr;
+    if (a.step_mask && ctx->hdz32_max > 0.0f) {
+        // An out-of-window sub-point closes its row with c = chi_floor dz'_a + chi_floor dz'_b (each
+        // product rounded), at most cb = 2 chi_floor dz'_max (1 + 2^-20).  The no-op needs
+        // cb < y 2^-25 (then c is below half an ulp of y) and cb y < 2^-25 (then 1 + c y rounds to 1).
+        const double cb = 2.0 * (double)(float)(CHI_FLOOR * CHI_SCALE) * (double)ctx->hdz32_max * (1.0 + 0x1p-20);
+        const double lo = cb * 0x1p25 * (1.0 + 0x1p-20), hi = 0x1p-25 / cb * (1.0 - 0x1p-20);
+        if (lo < hi && lo < 3.0e38 && hi > 1.0e-37) {
+            a.noop_lo = std::nextafter((float)lo, INFINITY);   // float(lo) may round below lo
+            a.noop_hi = std::nextafter((float)hi, 0.0f);
+        }
+    }
     a.root_major = ctx->root_major;
     a.far32 = (sizeof(real) == 4 && ctx->far_cache && ctx->far_valid[0]) ? ctx->far_I32 : nullptr;
     a.far64 = (sizeof(real) == 8 && ctx->far_cache && ctx->far_valid[1]) ? ctx->far_I64 : nullptr;
@@ -975,13 +1080,17 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
         one = true;
         for (int l = 0; l + 1 < ctx->h_tables.L; l++)
             if (!(ctx->h_tables.line[l] * a.cand_lo <= ctx->h_tables.line[l + 1] * a.cand_hi)) one = false;
-        if (one)
+        if (one) {
             hipLaunchKernelGGL((k_radiation_field<real, true>), dim3(grid_of(a)), dim3(WAVE * a.nwg),
                                lds_bytes(ctx, a), ctx->stream, a);
+            ctx->last_rf_kernel = "k_radiation_field<float, true>";
+        }
     }
-    if (!one)
+    if (!one) {
         hipLaunchKernelGGL((k_radiation_field<real, false>), dim3(grid_of(a)), dim3(WAVE * a.nwg),
                            lds_bytes(ctx, a), ctx->stream, a);
+        ctx->last_rf_kernel = sizeof(real) == 4 ? "k_radiation_field<float, false>" : "k_radiation_field<double, false>";
+    }
     MGR_LAUNCH_CHECK();
     return MGR_OK;
 }

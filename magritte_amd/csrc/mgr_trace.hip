@@ -359,6 +359,24 @@ __global__ void k_mask_stats(int64_t T, int nchunk, const int4 *__restrict__ seg
     }
 }
 
+// Largest dz' = dZ HDZ_SCALE of the cached steps: non-negative floats order
+// as their bit patterns, so a block maximum of the bits and one atomicMax per
+// block.
+__global__ void k_max_hdz(int64_t T, const float *__restrict__ hdz, unsigned *__restrict__ out)
+{
+    __shared__ unsigned s_max[256];
+    unsigned v = 0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T; t += (int64_t)gridDim.x * blockDim.x)
+        v = max(v, __float_as_uint(hdz[t]));
+    s_max[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s_max[threadIdx.x] = max(s_max[threadIdx.x], s_max[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, s_max[0]);
+}
+
 }  // namespace
 
 namespace mgr {
@@ -490,11 +508,22 @@ int launch_step_masks(mgr_ctx *ctx)
                        (int)ctx->R, nchunk, ctx->d_cell_model, ctx->d_cell_qbase, ctx->d_m_R, ctx->pw,
                        ctx->dshift_max, ctx->seg_off,
                        ctx->nseg, ctx->seg_cell, ctx->seg_shift, win, cnt, ctx->step_mask);
-    const hipError_t le = hipGetLastError();
+    // the largest dz' bounds the row closings of out-of-window sub-points
+    unsigned *dmax = reinterpret_cast<unsigned *>(cnt);   // N * STEP_MASK_CHUNKS >= 4 bytes, no longer read
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess) le = hipMemsetAsync(dmax, 0, sizeof(unsigned), ctx->stream);
+    if (le == hipSuccess && ctx->total_segments > 0) {
+        hipLaunchKernelGGL(k_max_hdz, dim3(1024), dim3(256), 0, ctx->stream, ctx->total_segments, ctx->seg_hdz32,
+                           dmax);
+        le = hipGetLastError();
+    }
+    unsigned bits = 0;
     const hipError_t se = hipStreamSynchronize(ctx->stream);
+    if (le == hipSuccess && se == hipSuccess) le = hipMemcpy(&bits, dmax, sizeof(unsigned), hipMemcpyDeviceToHost);
     (void)hipFree(win);
     (void)hipFree(cnt);
     if (le != hipSuccess || se != hipSuccess) return ctx->fail(MGR_ERR_DEVICE, "step-mask kernels failed");
+    std::memcpy(&ctx->hdz32_max, &bits, sizeof(float));
     ctx->masks_valid = true;
     return MGR_OK;
 }

@@ -61,6 +61,7 @@ SIGNATURES = {
     "mgr_compute_spectral_discretisation_image": ([_vp, _d], _i),
     "mgr_compute_image": ([_vp, _i64, _i, _vp, _vp], _i),
     "mgr_last_radiation_field_ms": ([_vp, _vp], _i),
+    "mgr_last_radiation_field_kernel": ([_vp, C.c_char_p, _i], _i),
     "mgr_trace_stats": ([_vp, _vp, _vp], _i),
     "mgr_radiation_field_shape": ([_vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp], _i),
     "mgr_set_radiation_field_shape": ([_vp, _i, _i, _i], _i),
@@ -314,6 +315,11 @@ class Context:
         v = C.c_double(0.0)
         self._check(self.lib.mgr_last_radiation_field_ms(self.h, C.byref(v)), "last_radiation_field_ms")
         return v.value
+
+    def last_radiation_field_kernel(self):
+        buf = C.create_string_buffer(128)
+        self._check(self.lib.mgr_last_radiation_field_kernel(self.h, buf, 128), "last_radiation_field_kernel")
+        return buf.value.decode()
 
     def radiation_field_shape(self, precision="fp32"):
         """(waves per workgroup, steps per chunk) chosen by the autotuner, and

@@ -91,6 +91,43 @@ def test_io_round_trip(tmp_path, kind):
         np.testing.assert_allclose(rays[a], -rays[r], atol=1e-12)
 
 
+@pytest.mark.parametrize("kind", ["text", "hdf5"])
+def test_io_round_trip_of_the_iteration_state(tmp_path, kind):
+    """What a resumed ALI needs (LineProducingSpecies::read/write,
+    lineProducingSpecies.cpp:21-134, 179-206): population, population_prev1..3,
+    population_tot, J_lin and J_eff come back bit for bit, and the per-iteration
+    dumps (write_populations with a tag, sim_lines.tpp:78-83, 147-151) read back
+    through read_populations."""
+    m = _small()
+    sim = to_simulation(m)
+    ls = sim.lines.lineProducingSpecies[0]
+    V, K, N = m.linedata.nlev, m.nrad, m.ncells
+    rng = np.random.default_rng(17)
+    vals = {}
+    for name in ("population", "population_prev1", "population_prev2", "population_prev3"):
+        vals[name] = rng.lognormal(10.0, 3.0, N * V)
+        setattr(ls, name, vals[name].tolist())
+    vals["population_tot"] = rng.lognormal(10.0, 3.0, N)
+    ls.population_tot = vals["population_tot"].tolist()
+    vals["Jlin"] = rng.lognormal(-40.0, 2.0, (N, K))
+    vals["Jeff"] = rng.lognormal(-40.0, 2.0, (N, K))
+    ls.Jlin = vals["Jlin"].tolist()
+    ls.Jeff = vals["Jeff"].tolist()
+    io = _io(kind, tmp_path)
+    sim.write(io)
+    ls.write_populations(io, 0, "_rank_0_iteration_3")
+    back = core.Model()
+    back.read(io)
+    lb = back.lines.lineProducingSpecies[0]
+    for name in ("population", "population_prev1", "population_prev2", "population_prev3", "population_tot"):
+        np.testing.assert_array_equal(np.array(getattr(lb, name)), vals[name], err_msg=name)
+    np.testing.assert_array_equal(np.array(lb.Jlin), vals["Jlin"])
+    np.testing.assert_array_equal(np.array(lb.Jeff), vals["Jeff"])
+    lb.population = [0.0] * (N * V)
+    assert lb.read_populations(io, 0, "_rank_0_iteration_3") == 0
+    np.testing.assert_array_equal(np.array(lb.population), vals["population"])
+
+
 def test_hdf5_layout(tmp_path):
     """The HDF5 file has the reference's layout: "<group>/.<name>" are
     attributes (root attributes for the parameters), lists and arrays are
@@ -165,6 +202,55 @@ def test_simulation_matches_oracle(tmp_path):
     # per-iteration population files, tag "_rank_0_iteration_<k>" (sim_lines.tpp:149-151)
     written = os.listdir(str(tmp_path) + "/Lines/LineProducingSpecies_0")
     assert f"population_rank_0_iteration_{it}.txt" in written
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["text", "hdf5"])
+def test_checkpoint_resume_equals_uninterrupted_run(tmp_path, kind):
+    """An ALI stopped after five iterations (four normal ones and an Ng
+    step, after which the reference's schedule starts again), written with
+    Model::write, read into a fresh Simulation and continued, ends with the
+    populations, history and total iteration count of an uninterrupted run
+    (sim_lines.tpp:61-164; the state read back is lineProducingSpecies.cpp:21-134).
+    The model needs 11 iterations, so the resumed part runs a second Ng step."""
+    m = synthetic.envelope_model(n=10, n0=1.0e12, T0=1000.0, nside=1, seed=3, abundance=1.0e-3)
+    print(f"\n[host] checkpoint/resume through {kind}", flush=True)
+
+    def fresh():
+        sim = to_simulation(m)
+        assert sim.compute_spectral_discretisation() == 0
+        assert sim.compute_boundary_intensities() == 0
+        assert sim.compute_LTE_level_populations() == 0
+        sim.convergence_criterion = core.CRIT_MAX
+        return sim
+
+    def state(sim):
+        ls = sim.lines.lineProducingSpecies[0]
+        return [np.array(getattr(ls, n)) for n in
+                ("population", "population_prev1", "population_prev2", "population_prev3")]
+
+    whole = fresh()
+    it_whole = whole.compute_level_populations(core.IoText(str(tmp_path) + "/whole/"), True, 60)
+    assert it_whole >= 10
+
+    first = fresh()
+    assert first.compute_level_populations(core.IoText(str(tmp_path) + "/first/"), True, 5) == 5
+    (tmp_path / "checkpoint").mkdir()
+    io = _io(kind, tmp_path / "checkpoint")
+    first.write(io)
+    del first
+
+    resumed = core.Simulation()
+    resumed.read(io)
+    assert resumed.compute_spectral_discretisation() == 0
+    assert resumed.compute_boundary_intensities() == 0
+    resumed.convergence_criterion = core.CRIT_MAX
+    it_rest = resumed.compute_level_populations(core.IoText(str(tmp_path) + "/rest/"), True, 60)
+    print(f"[host] uninterrupted {it_whole} iterations, resumed 5 + {it_rest}", flush=True)
+    assert 5 + it_rest == it_whole
+    for a, b in zip(state(resumed), state(whole)):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(np.array(resumed.error_max), np.array(whole.error_max)[5:])
 
 
 def _read_text_array(path):
