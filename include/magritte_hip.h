@@ -201,6 +201,25 @@ int mgr_compute_level_populations_from_stateq(mgr_ctx *ctx, mgr_conv *conv);
 /* Lines::iteration_using_Ng_acceleration (src/Model/Lines/lines.cpp:131-149)  */
 int mgr_iteration_using_Ng_acceleration(mgr_ctx *ctx, mgr_conv *conv);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The Ng step and convergence statistics of
+ * mgr_iteration_using_Ng_acceleration (Lines::iteration_using_Ng_acceleration,
+ * lines.cpp:131-149, without the emissivity/opacity update) on caller-owned
+ * device arrays of one model on the current device: population, prev1..3
+ * [ncells*nlev] (index i + p*nlev) are updated in place as that entry point
+ * updates the context's, population_tot [ncells] is read, and conv (a host
+ * pointer, or NULL) receives the statistics after the stream is synchronised.
+ * stream is a hipStream_t (NULL: the null stream).  It calls no allocation
+ * routine: its scratch is a 196,640-byte static device array per GPU, which
+ * limits ncells to 4,194,304 (and nlev to 4096); larger sizes return
+ * MGR_ERR_ARGUMENT with the reason in mgr_last_error(NULL).  All calls on one
+ * GPU share that array, so calls that overlap on one GPU (other streams,
+ * other host threads) race.                                                   */
+int mgr_ng_acceleration_noalloc(int64_t ncells, int64_t nlev, double pop_prec, double *population,
+                                double *prev1, double *prev2, double *prev3, const double *population_tot,
+                                void *stream, mgr_conv *conv);
+
 /* Simulation::compute_level_populations(io, use_Ng, max_niterations)
  * (src/Simulation/sim_lines.tpp:61-164): the ALI loop, one hipGraph replay per
  * iteration.  iterations_out receives the iteration count (the reference's

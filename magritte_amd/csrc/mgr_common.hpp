@@ -305,6 +305,8 @@ namespace mgr {
 constexpr int RED_BLOCK = 256;
 constexpr int CHUNK_CELLS = 1024;   // cells per reduction chunk
 
+void set_context_free_error(const std::string &msg);
+
 // entry points shared between translation units
 int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels);
 int free_model(mgr_ctx *ctx);

@@ -15,8 +15,16 @@
 using namespace mgr;
 
 namespace {
-
 thread_local std::string g_create_error;
+}  // namespace
+
+namespace mgr {
+// The message mgr_last_error(NULL) returns: failures of the entry points
+// that take no context.
+void set_context_free_error(const std::string &msg) { g_create_error = msg; }
+}  // namespace mgr
+
+namespace {
 
 template <typename T>
 int dalloc(mgr_ctx *ctx, T **p, size_t n)

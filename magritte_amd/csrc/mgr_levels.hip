@@ -13,6 +13,8 @@
 // diagonal: here one thread solves one cell's V x V system with partial
 // pivoting in FP64 registers (V known at compile time for V <= 8).
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "mgr_common.hpp"
 
@@ -249,7 +251,7 @@ __global__ void k_conv_partial(int V, double pop_prec, const int64_t *__restrict
 {
     __shared__ double sh[RED_BLOCK];
     const int64_t c0 = chunk_cell[blockIdx.x], c1 = chunk_cell[blockIdx.x + 1];
-    const int32_t mdl = cell_model[c0];
+    const int32_t mdl = cell_model ? cell_model[c0] : 0;   // null: one model
     const double weight = 1.0 / (double)((m_cell_off[mdl + 1] - m_cell_off[mdl]) * V);
     double fnc = 0.0, rcm = 0.0, rmax = 0.0;
     for (int64_t p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
@@ -357,7 +359,7 @@ __global__ void k_ng_apply(int64_t N, int V, const int32_t *__restrict__ cell_mo
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N * V) return;
-    const int32_t m = cell_model[i / V];
+    const int32_t m = cell_model ? cell_model[i / V] : 0;   // null: one model
     if (!m_active[m] || coef[4 * m] == 0.0) return;
     const double a = coef[4 * m + 1], b = coef[4 * m + 2];
     const double xo = x[i];
@@ -368,6 +370,38 @@ __global__ void k_ng_apply(int64_t N, int V, const int32_t *__restrict__ cell_mo
 }
 
 __global__ void k_zero_flag(int32_t *flag) { *flag = 0; }
+
+// ---- Ng acceleration without allocation ---------------------------------------
+// mgr_ng_acceleration_noalloc runs the kernels above on caller-owned arrays of
+// one model.  Its scratch is one file-scope device array per GPU: the chunk
+// partial sums (5 per chunk), the chunk bounds and the few per-model words
+// the kernels read and write.  Every call overwrites what it reads first.
+constexpr int64_t NOALLOC_MAX_CHUNKS = 4096;   // ncells <= NOALLOC_MAX_CHUNKS * CHUNK_CELLS
+constexpr int64_t NA_PARTIALS = 0;                                     // double [5 * MAX_CHUNKS]
+constexpr int64_t NA_CHUNK_CELL = NA_PARTIALS + 5 * NOALLOC_MAX_CHUNKS;   // int64 [MAX_CHUNKS + 1]
+constexpr int64_t NA_WORDS = NA_CHUNK_CELL + NOALLOC_MAX_CHUNKS + 1;      // 8-byte words below
+constexpr int64_t NA_M_CELL_OFF = NA_WORDS + 0, NA_M_CHUNK_OFF = NA_WORDS + 2, NA_COEF = NA_WORDS + 4,
+                  NA_MCONV = NA_WORDS + 8, NA_REDUCED = NA_WORDS + 12, NA_FLAGS = NA_WORDS + 16;
+constexpr int64_t NA_SIZE = NA_WORDS + 24;   // 8-byte words: 196,640 bytes
+static __device__ __align__(256) double scratch[NA_SIZE];
+
+// Chunk bounds of one model of ncells cells, its cell and chunk offsets, its
+// activity flag (1) and a zero error word.
+__global__ void k_noalloc_setup(int64_t ncells, int64_t nch, double *base)
+{
+    int64_t *chunk_cell = reinterpret_cast<int64_t *>(base + NA_CHUNK_CELL);
+    for (int64_t c = threadIdx.x; c <= nch; c += blockDim.x) chunk_cell[c] = min(c * (int64_t)CHUNK_CELLS, ncells);
+    if (threadIdx.x == 0) {
+        int64_t *w = reinterpret_cast<int64_t *>(base);
+        w[NA_M_CELL_OFF] = 0;
+        w[NA_M_CELL_OFF + 1] = ncells;
+        w[NA_M_CHUNK_OFF] = 0;
+        w[NA_M_CHUNK_OFF + 1] = nch;
+        int32_t *flags = reinterpret_cast<int32_t *>(base + NA_FLAGS);
+        flags[0] = 1;   // m_active
+        flags[1] = 0;   // err_flag
+    }
+}
 
 unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
@@ -569,6 +603,79 @@ int mgr_iteration_using_Ng_acceleration(mgr_ctx *ctx, mgr_conv *conv)
     if ((rc = launch_emissivity_opacity(ctx))) return rc;
     if ((rc = enqueue_conv_copy(ctx))) return rc;
     return read_convergence(ctx, conv);
+}
+
+// Device address of this GPU's scratch array, looked up once per device
+// with that device current.
+static double *noalloc_scratch(int device)
+{
+    static std::mutex mtx;
+    static double *addr[64] = {};
+    if (device < 0 || device >= 64) return nullptr;
+    std::lock_guard<std::mutex> lock(mtx);
+    if (!addr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        addr[device] = static_cast<double *>(p);
+    }
+    return addr[device];
+}
+
+static int noalloc_fail(int code, const std::string &msg)
+{
+    mgr::set_context_free_error("mgr_ng_acceleration_noalloc: " + msg);
+    return code;
+}
+
+int mgr_ng_acceleration_noalloc(int64_t ncells, int64_t nlev, double pop_prec, double *population,
+                                double *prev1, double *prev2, double *prev3, const double *population_tot,
+                                void *stream, mgr_conv *conv)
+{
+    if (ncells < 1 || nlev < 1) return noalloc_fail(MGR_ERR_ARGUMENT, "ncells and nlev must be positive");
+    if (!population || !prev1 || !prev2 || !prev3 || !population_tot)
+        return noalloc_fail(MGR_ERR_ARGUMENT, "null array");
+    const int64_t nch = (ncells + CHUNK_CELLS - 1) / CHUNK_CELLS;
+    if (nch > NOALLOC_MAX_CHUNKS)
+        return noalloc_fail(MGR_ERR_ARGUMENT, "ncells " + std::to_string(ncells) + " exceeds the maximum of " +
+                                                  std::to_string(NOALLOC_MAX_CHUNKS * CHUNK_CELLS));
+    if (nlev > 4096) return noalloc_fail(MGR_ERR_ARGUMENT, "nlev exceeds 4096");
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess) return noalloc_fail(MGR_ERR_DEVICE, "no current HIP device");
+    double *base = noalloc_scratch(device);
+    if (!base) return noalloc_fail(MGR_ERR_DEVICE, "scratch address of the current device not found");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const int V = (int)nlev;
+    int64_t *chunk_cell = reinterpret_cast<int64_t *>(base + NA_CHUNK_CELL);
+    int64_t *m_cell_off = reinterpret_cast<int64_t *>(base + NA_M_CELL_OFF);
+    int64_t *m_chunk_off = reinterpret_cast<int64_t *>(base + NA_M_CHUNK_OFF);
+    int32_t *m_active = reinterpret_cast<int32_t *>(base + NA_FLAGS);
+    int32_t *err_flag = m_active + 1;
+    double *partials = base + NA_PARTIALS, *coef = base + NA_COEF, *mconv = base + NA_MCONV,
+           *reduced = base + NA_REDUCED;
+    hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(256), 0, st, ncells, nch, base);
+    // Lines::iteration_using_Ng_acceleration (lines.cpp:131-149) on one species:
+    // update_using_Ng_acceleration, then check_for_convergence
+    hipLaunchKernelGGL(k_ng_partial, dim3((unsigned)nch), dim3(RED_BLOCK), 0, st, V, chunk_cell, population, prev1,
+                       prev2, prev3, partials);
+    hipLaunchKernelGGL(k_ng_final, dim3(1), dim3(64), 0, st, (int64_t)1, m_chunk_off, partials, coef);
+    hipLaunchKernelGGL(k_ng_apply, dim3(grid_for(ncells * nlev, 256)), dim3(256), 0, st, ncells, V, nullptr,
+                       m_active, coef, population, prev1, prev2, prev3);
+    hipLaunchKernelGGL(k_conv_partial, dim3((unsigned)nch), dim3(RED_BLOCK), 0, st, V, pop_prec, chunk_cell,
+                       nullptr, m_cell_off, population, prev1, population_tot, partials);
+    hipLaunchKernelGGL(k_conv_final, dim3(1), dim3(RED_BLOCK), 0, st, (int64_t)1, m_chunk_off, partials, m_active,
+                       mconv, reduced, err_flag);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return noalloc_fail(MGR_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    if (conv) {
+        double h[3];
+        e = hipMemcpyAsync(h, reduced, sizeof(h), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return noalloc_fail(MGR_ERR_DEVICE, hipGetErrorString(e));
+        conv->fraction_not_converged = h[0];
+        conv->relative_change_mean = h[1];
+        conv->relative_change_max = h[2];
+    }
+    return MGR_OK;
 }
 
 static int ensure_graphs(mgr_ctx *ctx, int precision)

@@ -62,6 +62,7 @@ SIGNATURES = {
     "mgr_compute_image": ([_vp, _i64, _i, _vp, _vp], _i),
     "mgr_last_radiation_field_ms": ([_vp, _vp], _i),
     "mgr_last_radiation_field_kernel": ([_vp, C.c_char_p, _i], _i),
+    "mgr_ng_acceleration_noalloc": ([_i64, _i64, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "mgr_trace_stats": ([_vp, _vp, _vp], _i),
     "mgr_radiation_field_shape": ([_vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp], _i),
     "mgr_set_radiation_field_shape": ([_vp, _i, _i, _i], _i),
