@@ -569,63 +569,76 @@ __device__ __forceinline__ Sweep<real> sweep_side(const RFArgs<real> &a, const L
         const int cnt = b_ - a_ + 1;
         __syncthreads();   // the previous chunk is consumed by every wave
         stage_chunk<real, X64>(a, o, q, a_ - 1, cnt, dsm, st, rw);
-        for (int e = cnt - 1; e >= 1; --e) {
+        int e = cnt - 1;
+        while (e >= 1) {
             decltype(st->s0) s0, ds, dzs;
             int nhs;
             read_step(st, e, s0, ds, dzs, nhs);   // s0 and ds stay in VGPRs: one scalar operand per FMA
-            const int nh = rfl_int(nhs);   // one readfirstlane: n, half and the chunk bits
-            const int n = nh & STEP_N_MASK;
-            __builtin_assume(n >= 1);   // a traced step emits 1 .. 10000 points (0 fails the trace)
-            const int half = (nh >> STEP_HALF_SHIFT) & STEP_HALF_MASK;
-            const real hdz = (real)rfl_real(dzs);
-            if ((unsigned)nh & cbit) {
-                // no bin of this wave meets a line window in this step: every
-                // sub-point has eta = 0 and chi at the floor, as evaluated
-                if (!noop) {
-                    // the first closes the row of the last evaluated point
-                    sw.step(chi_floor);
-                    sw.pend((real)0, chi_floor, hdz);
-                    if constexpr (sizeof(real) == 4)
-                        noop = __all(sw.y > a.noop_lo && sw.y < a.noop_hi);
-                    if (!noop) {
-                        for (int m = n - 1; m >= 1; --m) {
-                            sw.step(chi_floor);
-                            sw.pend((real)0, chi_floor, hdz);
-                        }
-                        continue;
+            int nh = rfl_int(nhs);                // one readfirstlane: n, half and the chunk bits
+            if (!((unsigned)nh & cbit)) {
+                noop = false;
+                // a run of steps with points inside a line window, in a loop
+                // of its own so that the sweep state keeps its registers
+                for (;;) {
+                    const int n = nh & STEP_N_MASK;
+                    __builtin_assume(n >= 1);   // a traced step emits 1 .. 10000 points (0 fails the trace)
+                    const int half = (nh >> STEP_HALF_SHIFT) & STEP_HALF_MASK;
+                    const real hdz = (real)rfl_real(dzs);
+                    // sub-points m >= half lie in the step's cell (row e), m < half in
+                    // the previous one (row e-1); two loops keep each row's address
+                    // loop-invariant
+                    const row_t *rn = rw + e * L;
+                    const row_t *rc = rw + (e - 1) * L;
+                    const int mlo = half > 1 ? half : 1;
+                    for (int m = n; m >= mlo; --m) {
+                        real eta, chi;
+                        eval_at<real, X64>(a, lc, rn, sub_point_shift(s0, m, ds), eta, chi);
+                        const real S = eta * Prec<real>::rcp(chi);
+                        sw.step(chi);   // on the first point an exact no-op (c = 0, q = 1)
+                        sw.pend(S, chi, hdz);
                     }
+                    for (int m = (half - 1 < n ? half - 1 : n); m >= 1; --m) {
+                        real eta, chi;
+                        eval_at<real, X64>(a, lc, rc, sub_point_shift(s0, m, ds), eta, chi);
+                        const real S = eta * Prec<real>::rcp(chi);
+                        sw.step(chi);
+                        sw.pend(S, chi, hdz);
+                    }
+                    if (--e < 1) break;
+                    read_step(st, e, s0, ds, dzs, nhs);
+                    nh = rfl_int(nhs);
+                    if ((unsigned)nh & cbit) break;
                 }
-                // a no-op stretch: pass over the following out-of-window steps
-                // of the chunk, keeping the dz' of the last one
-                int last = e;
-                while (last > 1 && ((unsigned)rfl_int(st[last - 1].nh) & cbit)) last--;
-                const real hdz_l = last == e ? hdz : (real)rfl_real(st[last].dz);
-                sw.hdz_q = hdz_l;
-                sw.cq = chi_floor * hdz_l;
-                e = last;
-                continue;
+                if (e < 1) break;
             }
-            noop = false;
-            // sub-points m >= half lie in the step's cell (row e), m < half in
-            // the previous one (row e-1); two loops keep each row's address
-            // loop-invariant
-            const row_t *rn = rw + e * L;
-            const row_t *rc = rw + (e - 1) * L;
-            const int mlo = half > 1 ? half : 1;
-            for (int m = n; m >= mlo; --m) {
-                real eta, chi;
-                eval_at<real, X64>(a, lc, rn, sub_point_shift(s0, m, ds), eta, chi);
-                const real S = eta * Prec<real>::rcp(chi);
-                sw.step(chi);   // on the first point an exact no-op (c = 0, q = 1)
-                sw.pend(S, chi, hdz);
+            // step e: no bin of this wave meets a line window in it, so every
+            // sub-point has eta = 0 and chi at the floor, as evaluated
+            const int n = nh & STEP_N_MASK;
+            __builtin_assume(n >= 1);
+            const real hdz = (real)rfl_real(dzs);
+            if (!noop) {
+                // the first closes the row of the last evaluated point
+                sw.step(chi_floor);
+                sw.pend((real)0, chi_floor, hdz);
+                if constexpr (sizeof(real) == 4)
+                    noop = __all(sw.y > a.noop_lo && sw.y < a.noop_hi);
+                if (!noop) {
+                    for (int m = n - 1; m >= 1; --m) {
+                        sw.step(chi_floor);
+                        sw.pend((real)0, chi_floor, hdz);
+                    }
+                    --e;
+                    continue;
+                }
             }
-            for (int m = (half - 1 < n ? half - 1 : n); m >= 1; --m) {
-                real eta, chi;
-                eval_at<real, X64>(a, lc, rc, sub_point_shift(s0, m, ds), eta, chi);
-                const real S = eta * Prec<real>::rcp(chi);
-                sw.step(chi);
-                sw.pend(S, chi, hdz);
-            }
+            // a no-op stretch: pass over the following out-of-window steps of
+            // the chunk, keeping the dz' of the last one
+            int last = e;
+            while (last > 1 && ((unsigned)rfl_int(st[last - 1].nh) & cbit)) last--;
+            const real hdz_l = last == e ? hdz : (real)rfl_real(st[last].dz);
+            sw.hdz_q = hdz_l;
+            sw.cq = chi_floor * hdz_l;
+            e = last - 1;
         }
     }
     sw.step(chi_o);   // the origin closes the innermost row and is left pending
