@@ -1148,6 +1148,8 @@ static int launch_rf_shape(mgr_ctx *ctx, int nwg, int ch, int64_t o_begin, int64
         return launch_rf<float>(ctx, ctx->row32, nwg, ch, o_begin, o_count);
 }
 
+struct TuneCand { int nwg, ch; float ms; };
+
 // Launch-shape autotuner: waves (64-bin chunks) per workgroup and steps per
 // staged chunk, timed on up to 16384 origins of the model; the fastest shape
 // (or the one with the fewest waves among near-ties) is kept for the context
@@ -1203,9 +1205,11 @@ static int tune(mgr_ctx *ctx, int pi)
             if (ms < best) { best = ms; best_w = w; best_c = c; }
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc) return rc == MGR_ERR_DEVICE ? ctx->fail(MGR_ERR_DEVICE, "radiation-field tuning failed") : rc;
+    if (rc) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc == MGR_ERR_DEVICE ? ctx->fail(MGR_ERR_DEVICE, "radiation-field tuning failed") : rc;
+    }
     // Shapes within 2 % of the fastest are a tie on the slice; the one with
     // fewer waves per workgroup schedules the end of a full launch more evenly
     // (measured ~3 % faster on full launches where the slice could not tell).
@@ -1216,6 +1220,44 @@ static int tune(mgr_ctx *ctx, int pi)
             best_c = t.ch;
             pick = t.ms;
         }
+    // The slices do not always rank shapes as full launches do: when a full
+    // launch is cheap (under ~0.4 s by the slice timing), the fastest slice
+    // shape of each wave count within 5 % of the best is timed on the whole
+    // model once, and the fastest full launch is kept.
+    const double full_est = (double)best * (double)ctx->N / (double)(per * nslice);
+    if (nslice > 1 && full_est < 400.0) {
+        std::vector<TuneCand> cands;
+        for (int w : nwgs) {
+            float bw = 1e30f;
+            int bc = 0;
+            for (const auto &t : ctx->tune_log)
+                if (t.nwg == w && t.ms < bw) { bw = t.ms; bc = t.ch; }
+            if (bc && bw <= best * 1.05f) cands.push_back({w, bc, 0.0f});
+        }
+        if (cands.size() > 1) {
+            float full_best = 1e30f;
+            for (auto &cd : cands) {
+                float ms = 0.0f;
+                if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = MGR_ERR_DEVICE; break; }
+                rc = launch_rf_shape<real>(ctx, cd.nwg, cd.ch, 0, ctx->N);
+                if (rc) break;
+                if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                    rc = MGR_ERR_DEVICE;
+                    break;
+                }
+                // ties within 1 % go to the fewer waves (the candidates come in that order)
+                if (ms < full_best * 0.99f) {
+                    full_best = ms;
+                    best_w = cd.nwg;
+                    best_c = cd.ch;
+                }
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc == MGR_ERR_DEVICE ? ctx->fail(MGR_ERR_DEVICE, "radiation-field tuning failed") : rc;
     ctx->rf_nwg[pi] = best_w;
     ctx->rf_ch[pi] = best_c;
     return MGR_OK;
