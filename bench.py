@@ -33,6 +33,10 @@ per second over the timed steps.  The JSON line also carries:
 
 --workload batch measures the same on 512 small envelopes of uneven sizes
 and ray counts solved together in one context (one launch per kernel).
+
+--workload mlp measures training of the warm-start network (8 -> 128 -> 128
+-> 7, BF16 MFMA, FP32 accumulation) on synthetic rows: rows per second and
+the algorithmic matrix-core flop rate against the 2.5 PFLOP/s dense BF16 peak.
 """
 import argparse
 import hashlib
@@ -172,6 +176,45 @@ def time_radiation_field(ctx, steps, warmup, precision):
     return time.perf_counter() - t0, float(np.mean(kernel_ms))
 
 
+# Algorithmic matrix flops of one training row of the warm-start network
+# (8 -> 128 -> 128 -> 7): forward 2 (8*128 + 128*128 + 128*7), backward
+# through W3 and W2 2 (7*128 + 128*128), weight gradients 2 (8*128 + 128*128 + 128*7).
+MLP_FLOPS_PER_ROW = 2 * (8 * 128 + 128 * 128 + 128 * 7) + 2 * (7 * 128 + 128 * 128) + 2 * (8 * 128 + 128 * 128 + 128 * 7)
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def bench_mlp(args):
+    """Training throughput of the warm-start network on synthetic rows."""
+    from magritte_amd.hip import Mlp
+    rng = np.random.default_rng(0)
+    n = args.mlp_rows
+    X = rng.normal(0, 1, (n, 8)).astype(np.float32)
+    W = rng.normal(0, 0.5, (8, 7))
+    Y = (np.tanh(X @ W) + 0.1 * X[:, :7] ** 2).astype(np.float32)
+    net = Mlp(seed=0)
+    net.set_normalisation(X.mean(0), X.std(0), Y.mean(0), Y.std(0))
+    net.train(X[: args.mlp_batch * 4], Y[: args.mlp_batch * 4], max(1, args.warmup), args.mlp_batch, 4e-3, 0)
+    t0 = time.perf_counter()
+    losses = net.train(X, Y, args.steps, args.mlp_batch, 4e-3, 1)
+    dt = time.perf_counter() - t0
+    net.close()
+    rows = args.steps * n
+    tflops = rows * MLP_FLOPS_PER_ROW / dt / 1e12
+    out = {
+        "metric": "warm-start network training rows/s", "value": rows / dt, "unit": "rows/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": f"MLP 8-128-128-7 GELU, {n} rows per epoch, batch {args.mlp_batch}, Adam",
+                   "epochs": args.steps, "loss_first_last": [float(losses[0]), float(losses[-1])]},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tflops / PEAK_BF16_TFLOPS, 5), "traffic": None,
+                     "flops_per_row": MLP_FLOPS_PER_ROW},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def roofline_entry(sum_ndep, nfreqs, solves, k_ms, precision, kernel, counters_workload):
     points = sum_ndep * nfreqs                 # sum over solves of their points
     equiv_bytes = BYTES_PER_POINT_BIN * points + BYTES_PER_SOLVE * solves
@@ -211,8 +254,10 @@ def roofline_entry(sum_ndep, nfreqs, solves, k_ms, precision, kernel, counters_w
 def time_to_converged(make_ctx, precision, weights):
     """ALI from LTE, and from the learned warm start when trained weights are
     present; the clock starts after the one-off set-up and trace."""
-    out = {"criterion": "max relative change < 1e-6, Ng every 5th"}
+    out = {"criterion": "max relative change < 1e-6, Ng every 5th",
+           "clock": "after the per-model set-up (trace, masks, far-end cache, tuning, graphs)"}
     ctx = make_ctx()
+    ctx.prepare(precision)
     t0 = time.perf_counter()
     it = ctx.compute_level_populations(True, 400, "max", precision)
     out["lte"] = {"seconds": round(time.perf_counter() - t0, 3), "iterations": int(it)}
@@ -223,9 +268,9 @@ def time_to_converged(make_ctx, precision, weights):
         from magritte_amd import warmstart
         net = warmstart.load(weights)
         ctx = make_ctx()
+        ctx.prepare(precision)
         t0 = time.perf_counter()
-        ctx.compute_radiation_field(precision)
-        ctx.compute_Jeff()
+        ctx.first_pass(precision, net.first_pass_stride)
         ctx.warm_start(net)
         it = ctx.compute_level_populations(True, 400, "max", precision)
         dt = time.perf_counter() - t0
@@ -234,7 +279,8 @@ def time_to_converged(make_ctx, precision, weights):
         net.close()
         mask = P_lte > 1e-10 * P_lte.sum(axis=1, keepdims=True)
         diff = float(np.max(np.abs(P_nn - P_lte)[mask] / P_lte[mask]))
-        out["nn"] = {"seconds": round(dt, 3), "iterations": int(it), "pops_max_rel_diff_vs_lte": diff}
+        out["nn"] = {"seconds": round(dt, 3), "iterations": int(it), "pops_max_rel_diff_vs_lte": diff,
+                     "first_pass_ray_pair_stride": net.first_pass_stride}
         out["speedup_nn_vs_lte"] = round(out["lte"]["seconds"] / dt, 3)
         log(f"[bench] converged from the warm start in {it} iterations, {dt:.2f} s")
     return out
@@ -245,8 +291,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="envelope", choices=["envelope", "batch"],
-                    help="envelope: one n^3 model; batch: many small models in one context")
+    ap.add_argument("--workload", default="envelope", choices=["envelope", "batch", "mlp"],
+                    help="envelope: one n^3 model; batch: many small models in one context; "
+                         "mlp: training of the warm-start network (a step is one epoch)")
+    ap.add_argument("--mlp-rows", type=int, default=2_000_000)
+    ap.add_argument("--mlp-batch", type=int, default=16384)
     ap.add_argument("--n", type=int, default=64, help="cells per side of the envelope (64 = full workload)")
     ap.add_argument("--models", type=int, default=512, help="models of the batch workload")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
@@ -265,6 +314,8 @@ def main():
         return 2
     if args.copy_probe_gib > 0:
         copy_probe(args.copy_probe_gib)
+    if args.workload == "mlp":
+        return bench_mlp(args)
 
     from magritte_amd.hip import BatchContext, Context
     t0 = time.perf_counter()

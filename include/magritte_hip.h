@@ -228,6 +228,12 @@ int mgr_compute_level_populations(mgr_ctx *ctx, int use_Ng_acceleration,
                                   int64_t max_niterations, int criterion,
                                   int precision, int64_t *iterations_out);
 
+/* Builds what the first radiation field of a precision would build once per
+ * model -- the trace, the out-of-window masks, the far-end intensity cache,
+ * the launch-shape tuning and the ALI iteration graphs -- so that later calls
+ * time the iterations alone (no counterpart in the reference).               */
+int mgr_prepare(mgr_ctx *ctx, int precision);
+
 /* One iteration of that loop (sim_lines.tpp:101-132), replayed from the same
  * hipGraphs: ng != 0 runs Lines::iteration_using_Ng_acceleration, otherwise
  * compute_radiation_field + compute_Jeff + iteration_using_statistical_equilibrium.
@@ -372,6 +378,13 @@ int mgr_features(mgr_ctx *ctx, float *X);
 /* Features -> network -> populations (pop_tot 10^y_i / sum 10^y) in place of
  * the current populations; needs 7 levels.                                    */
 int mgr_warm_start(mgr_ctx *ctx, mgr_mlp *mlp);
+/* The warm start's first pass from the current populations: a radiation
+ * field over every pair_stride-th ray pair of each origin (ray weights scaled
+ * by pair_stride, an approximation of J at 1/pair_stride of the cost; 1 is
+ * mgr_compute_radiation_field) and J_eff, after which mgr_features and
+ * mgr_warm_start may be called.  The network must be trained on features of
+ * the same stride.                                                            */
+int mgr_first_pass(mgr_ctx *ctx, int precision, int pair_stride);
 
 #ifdef __cplusplus
 }

@@ -225,6 +225,7 @@ struct mgr_ctx {
     // workgroup and staged steps per chunk; 0 until chosen
     int      rf_nwg[2] = {0, 0}, rf_ch[2] = {0, 0};
     bool     rf_autotune = true;
+    int      rf_pair_stride = 1;    // every k-th ray pair (weights times k): the warm start's first pass only
     struct TuneEntry { int nwg, ch; float ms; };
     std::vector<TuneEntry> tune_log;
 

@@ -699,6 +699,21 @@ static int ensure_graphs(mgr_ctx *ctx, int precision)
     return MGR_OK;
 }
 
+int mgr_prepare(mgr_ctx *ctx, int precision)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (!ctx->spectral_set || ctx->spectral_image)
+        return ctx->fail(MGR_ERR_STATE, "Spectral discretisation was not set for Lines");
+    if (!ctx->bdy_set) return ctx->fail(MGR_ERR_STATE, "compute_boundary_intensities first");
+    if (!ctx->pops_set) return ctx->fail(MGR_ERR_STATE, "level populations not set");
+    if (precision != MGR_FP32 && precision != MGR_FP64) return ctx->fail(MGR_ERR_ARGUMENT, "bad precision");
+    MGR_HIP(hipSetDevice(ctx->device));
+    int rc = ensure_graphs(ctx, precision);
+    if (rc) return rc;
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    return MGR_OK;
+}
+
 int mgr_ali_iteration(mgr_ctx *ctx, int ng, int precision, mgr_conv *conv)
 {
     if (!ctx) return MGR_ERR_ARGUMENT;

@@ -8,10 +8,19 @@
 // warm start is unchanged, so converged populations are those of the plain
 // method.
 //
-// Arithmetic: every matrix product runs on the CDNA matrix cores
-// (v_mfma_f32_16x16x16_bf16) with bfloat16 operands, round to nearest even,
-// and FP32 accumulation; biases, GELU, the loss and Adam run in FP32 on the
-// FP32 master weights.
+// Arithmetic: every matrix product runs on the CDNA4 matrix cores with
+// bfloat16 operands, round to nearest even, and FP32 accumulation:
+// v_mfma_f32_16x16x32_bf16 wherever K is a multiple of 32 (the hidden layer,
+// the output layer, the back-propagation through W2 and the weight gradients,
+// whose K is the tile's 64 rows), v_mfma_f32_16x16x16_bf16 for the two
+// products with K = 16 (the 8 inputs and the 7 outputs, padded); biases,
+// GELU, the loss and Adam run in FP32 on the FP32 master weights.
+//
+// Training runs without host round trips: minibatches are consecutive slices
+// of a pseudo-random permutation of the rows computed on the device (a
+// four-round Feistel network over the next even power of two with cycle
+// walking, keyed per epoch), Adam's step count and the epoch's loss live in
+// device memory, and each epoch is one hipGraph replay.
 //
 // Work mapping: a workgroup of 4 wavefronts takes tiles of 64 rows (cells);
 // wave w owns rows 16w..16w+15 of the tile for the forward pass and the
@@ -22,8 +31,11 @@
 // tiles a workgroup takes.  Per-workgroup partial gradients are reduced in a
 // fixed order by the Adam kernel, so training is deterministic.
 //
-// MFMA 16x16x16 operand layout (lane l, r = l % 16, g = l / 16, j = 0..3):
-//   A[m = r][k = 4g + j],  B[k = 4g + j][n = r],  D[m = 4g + j][n = r].
+// MFMA operand layouts (lane l, r = l % 16, g = l / 16):
+//   16x16x16: A[m = r][k = 4g + j], B[k = 4g + j][n = r], j = 0..3;
+//   16x16x32: A[m = r][k = 8g + j], B[k = 8g + j][n = r], j = 0..7;
+//   both:     D[m = 4g + j][n = r], j = 0..3.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -76,6 +88,8 @@ constexpr int L_TOTAL = L_DZ1 + ROWS * HID;
 constexpr size_t LDS_TRAIN = (size_t)L_TOTAL * 2;   // 147,456 bytes
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float x)
@@ -87,18 +101,20 @@ __device__ __forceinline__ uint16_t f2bf(float x)
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ __forceinline__ s4 ld4(const uint16_t *p) { return *reinterpret_cast<const s4 *>(p); }
-__device__ __forceinline__ s4 ld4s(const uint16_t *p, int stride)
+__device__ __forceinline__ s8 ld8(const uint16_t *p) { return *reinterpret_cast<const s8 *>(p); }
+__device__ __forceinline__ s8 ld8s(const uint16_t *p, int stride)
 {
-    s4 r;
-    r[0] = (short)p[0];
-    r[1] = (short)p[stride];
-    r[2] = (short)p[2 * stride];
-    r[3] = (short)p[3 * stride];
+    s8 r;
+    for (int j = 0; j < 8; j++) r[j] = (short)p[j * stride];
     return r;
 }
 __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma32(s8 a, s8 b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
 }
 
 constexpr float SQRT_2_OVER_PI = 0.7978845608028654f;
@@ -125,6 +141,66 @@ __device__ __forceinline__ float gelu_grad(float z)
 struct Norm {
     float xm[IN], xs[IN], ym[OUT], ys[OUT];
 };
+
+// Device state of a training run: the epoch (counted from -1, advanced by
+// k_epoch_begin at the start of every epoch replay), its permutation key
+// and the shape of the permutation domain.
+struct TrainState {
+    int64_t  epoch;
+    uint64_t seed, key;
+    uint32_t n;              // rows permuted
+    int      half;           // bits of each Feistel half
+    uint32_t mask;           // (1 << half) - 1
+};
+
+// A four-round Feistel network on [0, 2^(2 half)): a bijection for any key.
+__device__ __forceinline__ uint32_t feistel(uint32_t x, int half, uint32_t mask, uint64_t key)
+{
+    uint32_t L = x >> half, R = x & mask;
+    for (int i = 0; i < 4; i++) {
+        uint32_t f = R * 0x85EBCA6Bu + ((uint32_t)(key >> (16 * i)) ^ (0x9E3779B9u * (uint32_t)(i + 1)));
+        f ^= f >> 13;
+        f *= 0xC2B2AE35u;
+        f ^= f >> 16;
+        const uint32_t nl = R;
+        R = (L ^ f) & mask;
+        L = nl;
+    }
+    return (L << half) | R;
+}
+
+// Row at position i of the epoch's permutation of [0, n): the Feistel
+// bijection, walked along its cycle until it lands inside [0, n) (the
+// restriction of a permutation to a subset by cycle walking is a
+// permutation of the subset).
+__device__ __forceinline__ int64_t permuted_row(const TrainState &st, int64_t i)
+{
+    uint32_t x = (uint32_t)i;
+    do { x = feistel(x, st.half, st.mask, st.key); } while (x >= st.n);
+    return (int64_t)x;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_epoch_begin(TrainState *st)
+{
+    st->epoch++;
+    st->key = mix64(st->seed ^ mix64((uint64_t)st->epoch));
+}
+
+// Source row of batch row `row` (b0 + row of the epoch order, or idx[row],
+// or row itself).
+__device__ __forceinline__ int64_t source_row(const TrainState *st, const int64_t *idx, int64_t b0, int64_t row)
+{
+    if (st) return permuted_row(*st, b0 + row);
+    return idx ? idx[row] : row;
+}
 
 // Copy the bf16 weight layouts (all of them, or the forward ones) into LDS.
 __device__ __forceinline__ void load_weights(uint16_t *s, const uint16_t *__restrict__ wbf, bool fwd_only)
@@ -162,9 +238,9 @@ __device__ __forceinline__ f4 forward_rows(const uint16_t *s, uint16_t *sH1, uin
     __syncthreads();
     for (int nt = 0; nt < 8; nt++) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
-        for (int kt = 0; kt < 8; kt++)
-            acc = mfma(ld4(sH1 + (R0 + r) * HID + kt * 16 + 4 * g),
-                       ld4(s + BF_W2F + (nt * 16 + r) * HID + kt * 16 + 4 * g), acc);
+        for (int kt = 0; kt < 4; kt++)
+            acc = mfma32(ld8(sH1 + (R0 + r) * HID + kt * 32 + 8 * g),
+                         ld8(s + BF_W2F + (nt * 16 + r) * HID + kt * 32 + 8 * g), acc);
         const float b = P[OFF_B2 + nt * 16 + r];
         for (int j = 0; j < 4; j++) {
             acc[j] += b;
@@ -174,8 +250,8 @@ __device__ __forceinline__ f4 forward_rows(const uint16_t *s, uint16_t *sH1, uin
     }
     __syncthreads();
     f4 y = {0.f, 0.f, 0.f, 0.f};
-    for (int kt = 0; kt < 8; kt++)
-        y = mfma(ld4(sH2 + (R0 + r) * HID + kt * 16 + 4 * g), ld4(s + BF_W3F + r * HID + kt * 16 + 4 * g), y);
+    for (int kt = 0; kt < 4; kt++)
+        y = mfma32(ld8(sH2 + (R0 + r) * HID + kt * 32 + 8 * g), ld8(s + BF_W3F + r * HID + kt * 32 + 8 * g), y);
     const float b3 = r < OUT ? P[OFF_B3 + r] : 0.0f;
     for (int j = 0; j < 4; j++) y[j] += b3;
     return y;
@@ -183,26 +259,30 @@ __device__ __forceinline__ f4 forward_rows(const uint16_t *s, uint16_t *sH1, uin
 
 // Stage the normalised inputs of tile rows into sX (zeros past n and in the
 // padding columns).
-__device__ __forceinline__ void stage_inputs(uint16_t *sX, const float *__restrict__ X, const int64_t *__restrict__ idx,
-                                             int64_t row0, int64_t n, const Norm &nm)
+__device__ __forceinline__ void stage_inputs(uint16_t *sX, const float *__restrict__ X, const TrainState *st,
+                                             const int64_t *__restrict__ idx, int64_t b0, int64_t row0, int64_t n,
+                                             const Norm &nm)
 {
     for (int e = threadIdx.x; e < ROWS * KIN; e += blockDim.x) {
         const int rr = e / KIN, c = e - rr * KIN;
         const int64_t row = row0 + rr;
         float v = 0.0f;
         if (row < n && c < IN) {
-            const int64_t src = idx ? idx[row] : row;
+            const int64_t src = source_row(st, idx, b0, row);
             v = (X[src * IN + c] - nm.xm[c]) / nm.xs[c];
         }
         sX[e] = f2bf(v);
     }
 }
 
-// Loss and gradients over rows [0, n) of X/T (through idx when given):
-// grid-stride over 64-row tiles; per-workgroup partial gradients (packed like
-// the parameters) and partial loss go to part[blockIdx.x * (NPAR + 1)].
+// Loss and gradients over rows [0, n) of a batch of X/T (rows b0 .. b0+n-1 of
+// the epoch's permutation with st, else through idx when given, else the
+// rows themselves): grid-stride over 64-row tiles; per-workgroup partial
+// gradients (packed like the parameters) and partial loss go to
+// part[blockIdx.x * (NPAR + 1)].
 __global__ __launch_bounds__(THREADS) void k_mlp_grad(const float *__restrict__ X, const float *__restrict__ T,
-                                                      const int64_t *__restrict__ idx, int64_t n, int64_t n_total,
+                                                      const TrainState *st, const int64_t *__restrict__ idx,
+                                                      int64_t b0, int64_t n, int64_t n_total,
                                                       const uint16_t *__restrict__ wbf, const float *__restrict__ P,
                                                       const Norm *__restrict__ norm, float *__restrict__ part)
 {
@@ -224,7 +304,7 @@ __global__ __launch_bounds__(THREADS) void k_mlp_grad(const float *__restrict__ 
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int64_t row0 = tile * ROWS;
         __syncthreads();   // the previous tile's activations are consumed
-        stage_inputs(sX, X, idx, row0, n, nm);
+        stage_inputs(sX, X, st, idx, b0, row0, n, nm);
         __syncthreads();
         f4 z1[8], z2[8];
         const f4 y = forward_rows(s, sH1, sH2, R0, P, z1, z2);
@@ -234,7 +314,7 @@ __global__ __launch_bounds__(THREADS) void k_mlp_grad(const float *__restrict__ 
             const int64_t row = row0 + rr;
             float d = 0.0f;
             if (row < n && r < OUT) {
-                const int64_t src = idx ? idx[row] : row;
+                const int64_t src = source_row(st, idx, b0, row);
                 const float t = (T[src * OUT + r] - nm.ym[r]) / nm.ys[r];
                 d = y[j] - t;
                 loss += d * d;
@@ -257,23 +337,23 @@ __global__ __launch_bounds__(THREADS) void k_mlp_grad(const float *__restrict__ 
         // dH1 = dZ2 W2^T, dZ1 = dH1 * gelu'(Z1)
         for (int nt = 0; nt < 8; nt++) {
             f4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (int kt = 0; kt < 8; kt++)
-                acc = mfma(ld4(sDZ2 + (R0 + r) * HID + kt * 16 + 4 * g),
-                           ld4(s + BF_W2B + (nt * 16 + r) * HID + kt * 16 + 4 * g), acc);
+            for (int kt = 0; kt < 4; kt++)
+                acc = mfma32(ld8(sDZ2 + (R0 + r) * HID + kt * 32 + 8 * g),
+                             ld8(s + BF_W2B + (nt * 16 + r) * HID + kt * 32 + 8 * g), acc);
             for (int j = 0; j < 4; j++)
                 sDZ1[(R0 + 4 * g + j) * HID + nt * 16 + r] = f2bf(acc[j] * gelu_grad(z1[nt][j]));
         }
         __syncthreads();
         // weight gradients over the 64 rows of the tile (MFMA K = rows)
-        for (int ks = 0; ks < 4; ks++) {
-            const int rb = 16 * ks + 4 * g;
+        for (int ks = 0; ks < 2; ks++) {
+            const int rb = 32 * ks + 8 * g;
             for (int q = 0; q < 2; q++) {
                 const int it = 2 * wave + q;
-                const s4 aH1 = ld4s(sH1 + rb * HID + it * 16 + r, HID);
+                const s8 aH1 = ld8s(sH1 + rb * HID + it * 16 + r, HID);
                 for (int ot = 0; ot < 8; ot++)
-                    gW2[q * 8 + ot] = mfma(aH1, ld4s(sDZ2 + rb * HID + ot * 16 + r, HID), gW2[q * 8 + ot]);
-                gW1[q] = mfma(ld4s(sX + rb * KIN + r, KIN), ld4s(sDZ1 + rb * HID + it * 16 + r, HID), gW1[q]);
-                gW3[q] = mfma(ld4s(sH2 + rb * HID + it * 16 + r, HID), ld4s(sDY + rb * NOUT + r, NOUT), gW3[q]);
+                    gW2[q * 8 + ot] = mfma32(aH1, ld8s(sDZ2 + rb * HID + ot * 16 + r, HID), gW2[q * 8 + ot]);
+                gW1[q] = mfma32(ld8s(sX + rb * KIN + r, KIN), ld8s(sDZ1 + rb * HID + it * 16 + r, HID), gW1[q]);
+                gW3[q] = mfma32(ld8s(sH2 + rb * HID + it * 16 + r, HID), ld8s(sDY + rb * NOUT + r, NOUT), gW3[q]);
             }
         }
         // bias gradients: column sums of dZ1 (threads 0..127), dZ2 (128..255), dY (0..6)
@@ -310,23 +390,64 @@ __global__ __launch_bounds__(THREADS) void k_mlp_grad(const float *__restrict__ 
     if (threadIdx.x == 0) out[NPAR] = (wl[0] + wl[1]) + (wl[2] + wl[3]);
 }
 
-// Sum the partials in workgroup order; optionally apply one Adam step.
-__global__ void k_mlp_reduce_adam(const float *__restrict__ part, int G, float *__restrict__ grad_out,
-                                  float *__restrict__ P, float *__restrict__ m, float *__restrict__ v,
-                                  int do_adam, float lr, float c1, float c2)
+// bf16 copies of master parameter p in the weight layouts (the padding
+// entries stay zero from the first k_mlp_pack).
+__device__ __forceinline__ void repack_param(int p, float x, uint16_t *__restrict__ wbf)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > NPAR) return;
-    float gsum = 0.0f;
-    for (int w = 0; w < G; w++) gsum += part[(int64_t)w * (NPAR + 1) + p];
+    const uint16_t h = f2bf(x);
+    if (p < OFF_B1) {                        // W1 [i][o]
+        const int i = (p - OFF_W1) / HID, o = (p - OFF_W1) % HID;
+        wbf[BF_W1F + o * KIN + i] = h;
+    } else if (p >= OFF_W2 && p < OFF_B2) {  // W2 [i][o]
+        const int k = p - OFF_W2, i = k / HID, o = k % HID;
+        wbf[BF_W2F + o * HID + i] = h;
+        wbf[BF_W2B + k] = h;
+    } else if (p >= OFF_W3 && p < OFF_B3) {  // W3 [i][o]
+        const int k = p - OFF_W3, i = k / OUT, o = k % OUT;
+        wbf[BF_W3F + o * HID + i] = h;
+        wbf[BF_W3B + i * NOUT + o] = h;
+    }
+}
+
+// Sum the G partials of each parameter in a fixed order (8 interleaved
+// sequential sums, then a fixed tree: deterministic), optionally apply one
+// Adam step and refresh the parameter's bf16 copies.  Blocks of 256 threads
+// cover 32 parameters (p = NPAR is the loss).  In a training run (st given)
+// the step is step0 + epoch * nbatch + batch + 1 and the loss sum is added to
+// loss_acc[epoch]; otherwise c1, c2 are given.
+constexpr int RED_P = 32, RED_W = 8;
+__global__ __launch_bounds__(RED_P * RED_W) void k_mlp_reduce_adam(
+    const float *__restrict__ part, int G, float *__restrict__ grad_out, float *__restrict__ P,
+    float *__restrict__ m, float *__restrict__ v, uint16_t *__restrict__ wbf, int do_adam, float lr, float c1,
+    float c2, const TrainState *st, int64_t step0, int64_t nbatch, int64_t batch, float *__restrict__ loss_acc)
+{
+    __shared__ float sh[RED_W][RED_P];
+    const int pi = threadIdx.x % RED_P, wi = threadIdx.x / RED_P;
+    const int p = blockIdx.x * RED_P + pi;
+    float s = 0.0f;
+    if (p <= NPAR)
+        for (int w = wi; w < G; w += RED_W) s += part[(int64_t)w * (NPAR + 1) + p];
+    sh[wi][pi] = s;
+    __syncthreads();
+    if (wi != 0 || p > NPAR) return;
+    const float gsum = ((sh[0][pi] + sh[1][pi]) + (sh[2][pi] + sh[3][pi])) +
+                       ((sh[4][pi] + sh[5][pi]) + (sh[6][pi] + sh[7][pi]));
     if (grad_out) grad_out[p] = gsum;
+    if (st && p == NPAR && loss_acc) loss_acc[st->epoch] += gsum;
     if (!do_adam || p == NPAR) return;
+    if (st) {
+        const float t = (float)(step0 + st->epoch * nbatch + batch + 1);
+        c1 = 1.0f - powf(0.9f, t);
+        c2 = 1.0f - powf(0.999f, t);
+    }
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float mm = b1 * m[p] + (1.0f - b1) * gsum;
     const float vv = b2 * v[p] + (1.0f - b2) * gsum * gsum;
     m[p] = mm;
     v[p] = vv;
-    P[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    const float x = P[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    P[p] = x;
+    if (wbf) repack_param(p, x, wbf);
 }
 
 // FP32 master parameters -> the bf16 layouts.
@@ -373,7 +494,7 @@ __global__ __launch_bounds__(THREADS) void k_mlp_forward(const float *__restrict
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int64_t row0 = tile * ROWS;
         __syncthreads();
-        stage_inputs(sX, X, nullptr, row0, n, nm);
+        stage_inputs(sX, X, nullptr, nullptr, 0, row0, n, nm);
         __syncthreads();
         f4 z1[8], z2[8];
         const f4 y = forward_rows(s, sH1, sH2, R0, P, z1, z2);
@@ -617,24 +738,18 @@ extern "C" int mgr_mlp_forward(mgr_mlp *mlp, const float *X, int64_t n, float *Y
     return MGR_OK;
 }
 
-// one gradient evaluation (and Adam step when lr > 0) on device-resident data
-static int mlp_grad_device(mgr_mlp *mlp, const float *dX, const float *dT, const int64_t *didx, int64_t n,
-                           int64_t n_total, float *dpart, int G, float *dgrad, float lr)
+// one gradient evaluation of rows [0, n) of device-resident X/T (no Adam step)
+static int mlp_grad_device(mgr_mlp *mlp, const float *dX, const float *dT, int64_t n, float *dpart, int G,
+                           float *dgrad)
 {
-    hipLaunchKernelGGL(k_mlp_grad, dim3(G), dim3(THREADS), LDS_TRAIN, mlp->stream, dX, dT, didx, n, n_total, mlp->wbf,
-                       mlp->P, mlp->norm, dpart);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(G), dim3(THREADS), LDS_TRAIN, mlp->stream, dX, dT,
+                       (const TrainState *)nullptr, (const int64_t *)nullptr, (int64_t)0, n, n, mlp->wbf, mlp->P,
+                       mlp->norm, dpart);
     MLP_HIP(hipGetLastError());
-    float c1 = 1.0f, c2 = 1.0f;
-    const int do_adam = lr > 0.0f;
-    if (do_adam) {
-        mlp->step++;
-        c1 = 1.0f - std::pow(0.9f, (float)mlp->step);
-        c2 = 1.0f - std::pow(0.999f, (float)mlp->step);
-    }
-    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPAR + 1 + 255) / 256), dim3(256), 0, mlp->stream, dpart, G, dgrad,
-                       mlp->P, mlp->m, mlp->v, do_adam, lr, c1, c2);
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPAR + 1 + RED_P - 1) / RED_P), dim3(RED_P * RED_W), 0, mlp->stream,
+                       dpart, G, dgrad, mlp->P, mlp->m, mlp->v, (uint16_t *)nullptr, 0, 0.0f, 1.0f, 1.0f,
+                       (const TrainState *)nullptr, (int64_t)0, (int64_t)0, (int64_t)0, (float *)nullptr);
     MLP_HIP(hipGetLastError());
-    if (do_adam) return mlp_pack(mlp);
     return MGR_OK;
 }
 
@@ -652,7 +767,7 @@ extern "C" int mgr_mlp_loss_grad(mgr_mlp *mlp, const float *X, const float *T, i
     if (!rc && (hipMemcpy(dX, X, n * IN * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(dT, T, n * OUT * sizeof(float), hipMemcpyHostToDevice) != hipSuccess))
         rc = MGR_ERR_DEVICE;
-    if (!rc) rc = mlp_grad_device(mlp, dX, dT, nullptr, n, n, dpart, G, dgrad, 0.0f);
+    if (!rc) rc = mlp_grad_device(mlp, dX, dT, n, dpart, G, dgrad);
     std::vector<float> h(NPAR + 1);
     if (!rc && (hipStreamSynchronize(mlp->stream) != hipSuccess ||
                 hipMemcpy(h.data(), dgrad, (NPAR + 1) * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess))
@@ -664,53 +779,71 @@ extern "C" int mgr_mlp_loss_grad(mgr_mlp *mlp, const float *X, const float *T, i
     return MGR_OK;
 }
 
+// One epoch = minibatches b0 = 0, batch, 2 batch, ... of the epoch's row
+// permutation, each a gradient and an Adam step with the bf16 repack; captured
+// once as a hipGraph and replayed per epoch, with no host synchronisation
+// until the end.
 extern "C" int mgr_mlp_train(mgr_mlp *mlp, const float *X, const float *T, int64_t n, int64_t epochs, int64_t batch,
                              float lr, uint64_t seed, float *loss_per_epoch)
 {
     if (!mlp || !X || !T || n <= 0 || epochs <= 0 || batch <= 0 || !(lr > 0.0f)) return MGR_ERR_ARGUMENT;
+    if (n > (int64_t)1 << 31) return mlp->fail(MGR_ERR_ARGUMENT, "more than 2^31 training rows");
     MLP_HIP(hipSetDevice(mlp->device));
-    const int Gmax = grid_for_rows(batch);
-    float *dX = nullptr, *dT = nullptr, *dpart = nullptr, *dgrad = nullptr;
-    int64_t *didx = nullptr;
+    const int64_t nbatch = (n + batch - 1) / batch;
+    const int Gmax = grid_for_rows(std::min(batch, n));
+    float *dX = nullptr, *dT = nullptr, *dpart = nullptr, *dloss = nullptr;
+    TrainState *dst = nullptr;
     int rc = MGR_OK;
     if (hipMalloc(&dX, n * IN * sizeof(float)) != hipSuccess || hipMalloc(&dT, n * OUT * sizeof(float)) != hipSuccess ||
-        hipMalloc(&didx, n * sizeof(int64_t)) != hipSuccess ||
         hipMalloc(&dpart, (size_t)Gmax * (NPAR + 1) * sizeof(float)) != hipSuccess ||
-        hipMalloc(&dgrad, (NPAR + 1) * sizeof(float)) != hipSuccess)
+        hipMalloc(&dloss, epochs * sizeof(float)) != hipSuccess || hipMalloc(&dst, sizeof(TrainState)) != hipSuccess)
         rc = MGR_ERR_NOMEM;
+    TrainState h{};
+    h.epoch = -1;
+    h.seed = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    h.n = (uint32_t)n;
+    int bits = 2;
+    while (((int64_t)1 << bits) < n) bits += 2;
+    h.half = bits / 2;
+    h.mask = (1u << h.half) - 1u;
     if (!rc && (hipMemcpy(dX, X, n * IN * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(dT, T, n * OUT * sizeof(float), hipMemcpyHostToDevice) != hipSuccess))
+                hipMemcpy(dT, T, n * OUT * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dst, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemset(dloss, 0, epochs * sizeof(float)) != hipSuccess))
         rc = MGR_ERR_DEVICE;
-    std::vector<int64_t> perm(n);
-    uint64_t st = seed * 2862933555777941757ULL + 3037000493ULL;
-    for (int64_t e = 0; !rc && e < epochs; e++) {
-        for (int64_t i = 0; i < n; i++) perm[i] = i;
-        for (int64_t i = n - 1; i > 0; i--) {   // Fisher-Yates
-            st = st * 6364136223846793005ULL + 1442695040888963407ULL;
-            const int64_t j = (int64_t)((st >> 33) % (uint64_t)(i + 1));
-            std::swap(perm[i], perm[j]);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (!rc && hipStreamBeginCapture(mlp->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = MGR_ERR_DEVICE;
+    if (!rc) {
+        hipLaunchKernelGGL(k_epoch_begin, dim3(1), dim3(1), 0, mlp->stream, dst);
+        for (int64_t b = 0; b < nbatch; b++) {
+            const int64_t b0 = b * batch, nb = std::min(batch, n - b0);
+            const int G = grid_for_rows(nb);
+            hipLaunchKernelGGL(k_mlp_grad, dim3(G), dim3(THREADS), LDS_TRAIN, mlp->stream, dX, dT,
+                               (const TrainState *)dst, (const int64_t *)nullptr, b0, nb, nb, mlp->wbf, mlp->P,
+                               mlp->norm, dpart);
+            hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPAR + 1 + RED_P - 1) / RED_P), dim3(RED_P * RED_W), 0,
+                               mlp->stream, dpart, G, (float *)nullptr, mlp->P, mlp->m, mlp->v, mlp->wbf, 1, lr, 1.0f,
+                               1.0f, (const TrainState *)dst, mlp->step, nbatch, b, dloss);
         }
-        if (hipMemcpyAsync(didx, perm.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, mlp->stream) != hipSuccess) {
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(mlp->stream, &graph);
+        if (le != hipSuccess || ce != hipSuccess ||
+            hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
             rc = MGR_ERR_DEVICE;
-            break;
-        }
-        double loss_sum = 0.0;
-        std::vector<float> h(NPAR + 1);
-        for (int64_t b0 = 0; !rc && b0 < n; b0 += batch) {
-            const int64_t nb = (n - b0) < batch ? (n - b0) : batch;
-            rc = mlp_grad_device(mlp, dX, dT, didx + b0, nb, nb, dpart, grid_for_rows(nb), dgrad, lr);
-            if (!rc && loss_per_epoch) {
-                float lsum = 0.0f;
-                if (hipMemcpyAsync(&lsum, dgrad + NPAR, sizeof(float), hipMemcpyDeviceToHost, mlp->stream) != hipSuccess ||
-                    hipStreamSynchronize(mlp->stream) != hipSuccess)
-                    rc = MGR_ERR_DEVICE;
-                loss_sum += lsum;
-            }
-        }
-        if (!rc && hipStreamSynchronize(mlp->stream) != hipSuccess) rc = MGR_ERR_DEVICE;
-        if (!rc && loss_per_epoch) loss_per_epoch[e] = (float)(loss_sum / ((double)n * OUT));
     }
-    (void)hipFree(dX); (void)hipFree(dT); (void)hipFree(didx); (void)hipFree(dpart); (void)hipFree(dgrad);
+    for (int64_t e = 0; !rc && e < epochs; e++)
+        if (hipGraphLaunch(exec, mlp->stream) != hipSuccess) rc = MGR_ERR_DEVICE;
+    if (!rc && hipStreamSynchronize(mlp->stream) != hipSuccess) rc = MGR_ERR_DEVICE;
+    if (!rc) mlp->step += epochs * nbatch;
+    if (!rc && loss_per_epoch) {
+        std::vector<float> l(epochs);
+        if (hipMemcpy(l.data(), dloss, epochs * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) rc = MGR_ERR_DEVICE;
+        for (int64_t e = 0; !rc && e < epochs; e++) loss_per_epoch[e] = (float)((double)l[e] / ((double)n * OUT));
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipFree(dX); (void)hipFree(dT); (void)hipFree(dpart); (void)hipFree(dloss); (void)hipFree(dst);
     if (rc) return mlp->fail(rc, "train failed");
     return MGR_OK;
 }

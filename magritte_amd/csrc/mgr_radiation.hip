@@ -125,6 +125,7 @@ template <typename real>
 struct RFArgs {
     int64_t N;
     int     F, L, nchunk, nwg, ngroups, grid_real, ch;
+    int     pair_stride;                      // every pair_stride-th ray pair, weights times pair_stride
     int32_t o_begin;                          // first origin of this launch
     size_t  row_offset;                       // byte offset of the row region in LDS
     const LineTables *tb;
@@ -827,9 +828,10 @@ void k_radiation_field(RFArgs<real> a)
     const int rb = a.m_rbase[mdl];
     const int Rh = a.m_R[mdl] / 2;
     const int64_t qb = a.cell_qbase[o];
-    for (int r = 0; r < Rh; r++) {
+    const double wsc = 2.0 * (double)a.pair_stride;
+    for (int r = 0; r < Rh; r += a.pair_stride) {
         const int ar = a.antipod[rb + r];
-        const double w2 = 2.0 * a.ray_w[rb + r];
+        const double w2 = wsc * a.ray_w[rb + r];
         const int64_t qr = qb + r;
         const int64_t qa = qb + (ar - rb);
         const int n_r = a.npts[qr];
@@ -1002,6 +1004,7 @@ int make_args(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     a.nwg = nwg;
     a.ngroups = (a.nchunk + a.nwg - 1) / a.nwg;
     a.o_begin = (int32_t)o_begin;
+    a.pair_stride = 1;
     a.tb = ctx->tables;
     a.nu = ctx->nu;
     a.rows = rows;
@@ -1070,6 +1073,7 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     a.far_cell = ctx->far_cell;
     a.far_shift = ctx->far_shift;
     a.origin_shift = nullptr;
+    a.pair_stride = ctx->rf_pair_stride;
     a.u_out = ctx->u_store;
     // the out-of-window bits are exact for the FP32 profile only
     a.step_mask = (sizeof(real) == 4 && ctx->masks_valid && ctx->rf_window_skip) ? ctx->step_mask : nullptr;
@@ -1326,6 +1330,41 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
     ctx->last_rf_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    int32_t errv = 0;
+    MGR_HIP(hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (errv) return ctx->fail(MGR_ERR_TRACE, "a ray pair ended off the boundary or an interior origin has an empty ray");
+    ctx->J_set = true;
+    return MGR_OK;
+}
+
+// The warm start's first pass: a radiation field over every pair_stride-th
+// ray pair of each origin (weights scaled by pair_stride) and J_eff.
+extern "C" int mgr_first_pass(mgr_ctx *ctx, int precision, int pair_stride)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (precision != MGR_FP32 && precision != MGR_FP64) return ctx->fail(MGR_ERR_ARGUMENT, "precision must be MGR_FP32 or MGR_FP64");
+    if (pair_stride < 1) return ctx->fail(MGR_ERR_ARGUMENT, "pair_stride must be at least 1");
+    if (!ctx->spectral_set) return ctx->fail(MGR_ERR_STATE, "compute_spectral_discretisation first");
+    if (!ctx->bdy_set) return ctx->fail(MGR_ERR_STATE, "compute_boundary_intensities first");
+    if (!ctx->pops_set) return ctx->fail(MGR_ERR_STATE, "level populations not set");
+    MGR_HIP(hipSetDevice(ctx->device));
+    if (!ctx->traced) {
+        int rc = launch_trace(ctx);
+        if (rc) return rc;
+    }
+    int rc = set_active(ctx, true);
+    if (!rc && precision == MGR_FP32) rc = launch_step_masks(ctx);
+    if (!rc) rc = launch_far_cache(ctx, precision);
+    if (!rc) rc = tune_radiation_field(ctx, precision);
+    if (rc) return rc;
+    MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
+    ctx->rf_pair_stride = pair_stride;
+    rc = launch_rf_kernel(ctx, precision);
+    ctx->rf_pair_stride = 1;
+    if (!rc) rc = launch_lambda(ctx);
+    if (!rc) rc = launch_jeff(ctx);
+    if (rc) return rc;
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
     int32_t errv = 0;
     MGR_HIP(hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (errv) return ctx->fail(MGR_ERR_TRACE, "a ray pair ended off the boundary or an interior origin has an empty ray");

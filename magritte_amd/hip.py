@@ -62,6 +62,8 @@ SIGNATURES = {
     "mgr_compute_image": ([_vp, _i64, _i, _vp, _vp], _i),
     "mgr_last_radiation_field_ms": ([_vp, _vp], _i),
     "mgr_last_radiation_field_kernel": ([_vp, C.c_char_p, _i], _i),
+    "mgr_first_pass": ([_vp, _i, _i], _i),
+    "mgr_prepare": ([_vp, _i], _i),
     "mgr_ng_acceleration_noalloc": ([_i64, _i64, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "mgr_trace_stats": ([_vp, _vp, _vp], _i),
     "mgr_radiation_field_shape": ([_vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp], _i),
@@ -349,6 +351,14 @@ class Context:
         a = np.zeros((self.N, MLP_IN), np.float32)
         self._check(self.lib.mgr_features(self.h, _p(a)), "features")
         return a
+
+    def prepare(self, precision="fp32"):
+        """Per-model caches, tuning and ALI graphs of a precision, built ahead."""
+        self._check(self.lib.mgr_prepare(self.h, _prec(precision)), "prepare")
+
+    def first_pass(self, precision="fp32", pair_stride=1):
+        """Radiation field over every pair_stride-th ray pair, then J_eff."""
+        self._check(self.lib.mgr_first_pass(self.h, _prec(precision), int(pair_stride)), "first_pass")
 
     def warm_start(self, mlp):
         """Replace the populations by the network's prediction."""

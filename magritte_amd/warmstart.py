@@ -3,10 +3,12 @@
 The network (include/magritte_hip.h, mgr_mlp_*) maps per-cell features --
 log10 collision-partner density, temperature, species density, neighbour
 velocity gradient and log10 Jlin/B_nu(T) of the first four transitions after
-one radiation field from LTE -- to log10 level-population fractions.  Its
-training data are populations the plain LTE-started ALI iteration converged
-on many small envelope models; training and inference run in the project's
-own HIP kernels.
+a first pass from LTE -- to log10 level-population fractions.  The first pass
+is a radiation field over every FIRST_PASS_STRIDE-th ray pair (mgr_first_pass),
+a quarter of a full one, so that the warm start costs less than the ALI
+iteration it saves.  Its training data are populations the plain LTE-started
+ALI iteration converged on many small envelope models; training and inference
+run in the project's own HIP kernels.
 
     python -m magritte_amd.warmstart train --models 1000 --out weights.npz
     python -m magritte_amd.warmstart evaluate --weights weights.npz --models 8
@@ -27,6 +29,7 @@ from .setup import synthetic
 DEFAULT_WEIGHTS = __file__.replace("warmstart.py", "data/warmstart_mlp.npz")
 CRITERION = "max"
 MAX_ITER = 400
+FIRST_PASS_STRIDE = 4
 
 
 def training_model(rng):
@@ -42,12 +45,12 @@ def training_model(rng):
     return synthetic.envelope_model(n=n, n0=n0, T0=T0, nside=nside, seed=seed)
 
 
-def first_pass(model):
-    """Context after set-up, trace, one radiation field and J_eff from LTE."""
+def first_pass(model, stride=FIRST_PASS_STRIDE):
+    """Context after set-up, trace and the warm start's first pass from LTE
+    (a radiation field over every stride-th ray pair, and J_eff)."""
     ctx = Context(model).setup()
     ctx.trace()
-    ctx.compute_radiation_field("fp32")
-    ctx.compute_Jeff()
+    ctx.first_pass("fp32", stride)
     return ctx
 
 
@@ -59,7 +62,7 @@ def targets(model, pops):
 TRAIN_MAX_ITER = 100
 
 
-def build_dataset(n_models, seed=0, log=print):
+def build_dataset(n_models, seed=0, log=print, stride=FIRST_PASS_STRIDE):
     """Features and converged log10 fractions of `n_models` training models.
     A model whose LTE-started iteration does not meet the criterion within
     TRAIN_MAX_ITER iterations, or ends with a non-positive population, gives
@@ -70,7 +73,7 @@ def build_dataset(n_models, seed=0, log=print):
     t0 = time.perf_counter()
     for i in range(n_models):
         m = training_model(rng)
-        ctx = first_pass(m)
+        ctx = first_pass(m, stride)
         X = ctx.features()
         it = ctx.compute_level_populations(True, TRAIN_MAX_ITER, CRITERION, "fp32")
         P = ctx.populations()
@@ -87,7 +90,7 @@ def build_dataset(n_models, seed=0, log=print):
     return np.concatenate(Xs), np.concatenate(Ys), np.array(its), skipped
 
 
-def fit(X, Y, epochs=30, batch=4096, lr=2e-3, seed=0, log=print):
+def fit(X, Y, epochs=30, batch=16384, lr=4e-3, seed=0, log=print):
     """Adam on the normalised data in three phases of falling step size
     (lr for `epochs`, lr/5 for epochs/2, lr/25 for epochs/4)."""
     mlp = Mlp(seed=seed)
@@ -111,24 +114,29 @@ def save(path, mlp, norm, meta):
 
 
 def load(path, device=0):
+    """The network of a weights file; its first-pass stride (1 for files
+    without one: a full radiation field) is kept as mlp.first_pass_stride."""
     d = np.load(path, allow_pickle=False)
     mlp = Mlp(seed=0, device=device)
     mlp.set_params({k: d[k] for k in ("W1", "b1", "W2", "b2", "W3", "b3")})
     mlp.set_normalisation(d["x_mean"], d["x_std"], d["y_mean"], d["y_std"])
+    meta = json.loads(str(d["meta"])) if "meta" in d.files else {}
+    mlp.first_pass_stride = int(meta.get("first_pass_stride", 1))
     return mlp
 
 
 def converge(model, mlp=None, precision="fp32", max_iter=MAX_ITER):
     """Iterations and seconds to the max-change criterion from LTE (mlp None)
     or from the network's populations.  The clock starts after the one-off
-    set-up and trace and includes, for the warm start, the first radiation
-    field, J_eff, the features and the inference."""
+    per-model set-up (trace, masks, far-end cache, launch tuning and the ALI
+    graphs: mgr_prepare) and includes, for the warm start, the first pass,
+    the features and the inference."""
     ctx = Context(model).setup()
     ctx.trace()
+    ctx.prepare(precision)
     t0 = time.perf_counter()
     if mlp is not None:
-        ctx.compute_radiation_field(precision)
-        ctx.compute_Jeff()
+        ctx.first_pass(precision, getattr(mlp, "first_pass_stride", 1))
         ctx.warm_start(mlp)
     it = ctx.compute_level_populations(True, max_iter, CRITERION, precision)
     dt = time.perf_counter() - t0
@@ -158,8 +166,9 @@ def main(argv=None):
     tr = sub.add_parser("train")
     tr.add_argument("--models", type=int, default=1000)
     tr.add_argument("--epochs", type=int, default=30)
-    tr.add_argument("--batch", type=int, default=4096)
-    tr.add_argument("--lr", type=float, default=2e-3)
+    tr.add_argument("--batch", type=int, default=16384)
+    tr.add_argument("--lr", type=float, default=4e-3)
+    tr.add_argument("--stride", type=int, default=FIRST_PASS_STRIDE, help="ray-pair stride of the first pass")
     tr.add_argument("--seed", type=int, default=0)
     tr.add_argument("--out", required=True)
     ev = sub.add_parser("evaluate")
@@ -175,9 +184,15 @@ def main(argv=None):
         print(msg, file=sys.stderr, flush=True)
 
     if a.cmd == "train":
-        X, Y, its, skipped = build_dataset(a.models, a.seed, log)
+        t0 = time.perf_counter()
+        X, Y, its, skipped = build_dataset(a.models, a.seed, log, a.stride)
+        t_data = time.perf_counter() - t0
+        t0 = time.perf_counter()
         mlp, norm, losses = fit(X, Y, a.epochs, a.batch, a.lr, a.seed, log)
-        meta = {"models": a.models, "models_not_converged": int(skipped), "cells": int(X.shape[0]), "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
+        t_fit = time.perf_counter() - t0
+        meta = {"models": a.models, "models_not_converged": int(skipped), "cells": int(X.shape[0]),
+                "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "first_pass_stride": a.stride,
+                "seconds_data": round(t_data, 1), "seconds_training": round(t_fit, 1),
                 "seed": a.seed, "mean_iterations_lte": float(its.mean()), "final_loss": float(losses[-1]),
                 "features": ["log10 n_partner", "log10 T", "log10 n_species", "log10 max |dv/dx| (1/s)",
                              "log10 Jlin_0/B_0", "log10 Jlin_1/B_1", "log10 Jlin_2/B_2", "log10 Jlin_3/B_3"],

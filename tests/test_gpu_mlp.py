@@ -144,7 +144,10 @@ def test_pretrained_warm_start_gains_over_lte():
     """The committed network (trained on the GPU by these kernels on
     envelopes the plain method solved) against the LTE start on held-out
     envelopes drawn from the same distribution: the same converged
-    populations (within 1e-5) in fewer ALI iterations in total."""
+    populations (within 1e-5), at most one iteration more on every model and
+    fewer ALI iterations in total (on these small models a few milliseconds
+    per iteration, so wall time is checked on 64^3 models in
+    test_gpu_workloads)."""
     import os
     from magritte_amd import warmstart
     if not os.path.exists(warmstart.DEFAULT_WEIGHTS):
@@ -159,6 +162,7 @@ def test_pretrained_warm_start_gains_over_lte():
         if not r["converged"]:
             continue
         assert r["pops_max_rel_diff"] < 1e-5
+        assert r["nn_iterations"] <= r["lte_iterations"] + 1
         lte += r["lte_iterations"]
         nn += r["nn_iterations"]
     print(f"[mlp] total iterations: LTE start {lte}, warm start {nn}", flush=True)
