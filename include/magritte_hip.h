@@ -253,8 +253,19 @@ int mgr_set_population_history(mgr_ctx *ctx, const double *prev1, const double *
                                const double *prev3);
 /* Radiation::J [ncells*nfreqs], index f + p*nfreqs (radiation.tpp:11-15).    */
 int mgr_get_J(mgr_ctx *ctx, double *J);
-/* Lambda::Ls for n_off_diag = 0: one element per (p,k) at nr = p, [ncells*nrad]. */
+/* Lambda::Ls element nr = p of every (p,k), [ncells*nrad]: the whole operator
+ * for n_off_diag <= 1.                                                        */
 int mgr_get_lambda(mgr_ctx *ctx, double *Ls);
+/* The other elements of Lambda (n_off_diag >= 2, RayPair::update_Lambda's
+ * bands, raypair_lambda.tpp:129-146; Lambda::get_nr / get_Ls,
+ * lambda.tpp:82-123): per cell `slots` = (nrays/2) 2 (n_off_diag-1) slots,
+ * slot (r 2 + side) (n_off_diag-1) + m-1 for ray pair r and band m, side 0 on
+ * the pair's longer ray (point n_ar - m), 1 on the shorter (n_ar + m).
+ * cells [ncells*slots]: the slot's emitting cell nr, -1 where the element
+ * does not exist; Ls [ncells*slots*nrad].  Elements of one (p, k) that name
+ * the same cell add (Lambda::add_element, lambda.tpp:130-143).              */
+int mgr_lambda_band_slots(mgr_ctx *ctx, int64_t *slots);
+int mgr_get_lambda_band(mgr_ctx *ctx, int32_t *cells, double *Ls);
 /* Jlin, Jeff, Jdif [ncells*nrad]; any pointer may be NULL.                    */
 int mgr_get_Jeff(mgr_ctx *ctx, double *Jlin, double *Jeff, double *Jdif);
 /* Frequencies::nu [ncells*nfreqs] and LineProducingSpecies::nr_line

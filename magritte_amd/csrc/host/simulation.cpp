@@ -287,7 +287,7 @@ void Simulation::pull_radiation()
     check(mgr_get_J(ctx_, radiation.J.data()), "get_J");
     Double1 Ls(N * K);
     check(mgr_get_lambda(ctx_, Ls.data()), "get_lambda");
-    // n_off_diag = 0: one ALO element per (p, k), at nr = p, when any was added
+    // the element nr = p of every (p, k), when any was added
     ls.lambda.Ls.assign(N, Double2(K));
     ls.lambda.nr.assign(N, Long2(K));
     ls.lambda.size.assign(N * K, 0);
@@ -296,8 +296,42 @@ void Simulation::pull_radiation()
             if (Ls[p * K + k] != 0.0) {
                 ls.lambda.Ls[p][k] = {Ls[p * K + k]};
                 ls.lambda.nr[p][k] = {(long)p};
-                ls.lambda.size[k + K * p] = 1;
             }
+    // n_off_diag >= 2: the band elements, merged per emitting cell in the
+    // order RayPair::update_Lambda first adds them (ray pairs, then bands m,
+    // lower before upper; raypair_lambda.tpp:129-146, lambda.tpp:130-143)
+    if (parameters.n_off_diag >= 2) {
+        int64_t S = 0;
+        check(mgr_lambda_band_slots(ctx_, &S), "lambda_band_slots");
+        std::vector<int32_t> cell(N * S);
+        Double1 Lb(N * S * K);
+        check(mgr_get_lambda_band(ctx_, cell.data(), Lb.data()), "get_lambda_band");
+        const int64_t nb = parameters.n_off_diag - 1;
+        for (size_t p = 0; p < N; p++)
+            for (int64_t r = 0; r < S / (2 * nb); r++)
+                for (int64_t m = 0; m < nb; m++)
+                    for (int64_t side = 0; side < 2; side++) {
+                        const int64_t sl = p * S + (r * 2 + side) * nb + m;
+                        if (cell[sl] < 0) continue;
+                        for (size_t k = 0; k < K; k++) {
+                            Double1 &ls_pk = ls.lambda.Ls[p][k];
+                            Long1 &nr_pk = ls.lambda.nr[p][k];
+                            if (nr_pk.empty()) {   // the diagonal element comes first
+                                ls_pk.push_back(0.0);
+                                nr_pk.push_back((long)p);
+                            }
+                            size_t e = 0;
+                            while (e < nr_pk.size() && nr_pk[e] != cell[sl]) e++;
+                            if (e == nr_pk.size()) {
+                                ls_pk.push_back(0.0);
+                                nr_pk.push_back(cell[sl]);
+                            }
+                            ls_pk[e] += Lb[sl * K + k];
+                        }
+                    }
+    }
+    for (size_t p = 0; p < N; p++)
+        for (size_t k = 0; k < K; k++) ls.lambda.size[k + K * p] = ls.lambda.nr[p][k].size();
     // Radiation::u with scattering (radiation.cpp:117-126), u[R][f + p*F]
     if (use_scattering()) {
         const size_t Rr = geometry.rays.rays.size() / 2;

@@ -214,6 +214,25 @@ struct mgr_ctx {
     // each radiation-field wave's share of the Lambda* sums: [N][ceil(F/64)][L]
     double  *Lpart = nullptr;
     double  *Jlin = nullptr, *Jeff = nullptr, *Jdif = nullptr;  // [N*L]
+    // Banded Lambda* (n_off_diag = nband + 1 >= 2, one model): per origin,
+    // band_S = (R/2) 2 nband slots, slot (r 2 + side) nband + m - 1 for ray
+    // pair r, band m, side 0 the lower band (on the longer ray), 1 the upper
+    // band (on the shorter ray).  band_cell: the slot's cell, -1 when the
+    // element does not exist; band_Ls: its sum per transition.
+    int      nband = 0;
+    int64_t  band_S = 0;
+    int32_t *band_cell = nullptr;   // [N*band_S], set at trace time
+    double  *band_part = nullptr;   // [N][ceil(F/64)][band_S][L] each wave's share
+    double  *band_Ls = nullptr;     // [N*band_S*L]
+    // slots whose cell is c, in ascending slot order: band_rev[band_rev_off[c] ..]
+    int64_t *band_rev_off = nullptr;   // [N+1]
+    int64_t *band_rev = nullptr;       // [band_rev_off[N]]
+    // the rate equations with band elements: per-cell LU of the block
+    // diagonal, the block-Jacobi iterate and the sweep flags
+    double  *band_lu = nullptr;     // [N*V*V]
+    int32_t *band_piv = nullptr;    // [N*V]
+    double  *band_x = nullptr;      // [N*V]
+    int32_t *band_flags = nullptr;  // [BAND_SWEEPS + 1]
     // Radiation::u with scattering: each ray pair's u at the origin,
     // [R/2][N*F] at (r N + p) F + f; null without scattering
     double  *u_store = nullptr;
@@ -304,6 +323,8 @@ struct mgr_ctx {
 
 namespace mgr {
 constexpr int RED_BLOCK = 256;
+constexpr int MAX_OFF_DIAG = 8;     // widest Lambda* band (n_off_diag)
+constexpr int BAND_SWEEPS = 16;     // block-Jacobi sweeps of the banded rate equations
 constexpr int CHUNK_CELLS = 1024;   // cells per reduction chunk
 
 void set_context_free_error(const std::string &msg);
@@ -319,6 +340,8 @@ int launch_far_cache(mgr_ctx *ctx, int precision);   // outside graph capture
 int launch_rf_kernel(mgr_ctx *ctx, int precision);
 int tune_radiation_field(mgr_ctx *ctx, int precision);   // outside graph capture
 int launch_lambda(mgr_ctx *ctx);
+int launch_band(mgr_ctx *ctx);         // band elements of Lambda* (nband > 0)
+int launch_band_cells(mgr_ctx *ctx);   // band cells and their reverse map (at trace time)
 int launch_jeff(mgr_ctx *ctx);
 int launch_stateq(mgr_ctx *ctx);
 int launch_convergence(mgr_ctx *ctx);

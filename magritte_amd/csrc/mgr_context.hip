@@ -409,9 +409,14 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
     // n_off_diag = 1 is the diagonal operator: RayPair::update_Lambda adds
     // bands m = 1 .. n_off_diag-1 only (raypair_lambda.tpp:129-146), and the
     // back-substitution branch's L_diag (raypair_solver.tpp:358-370) is the
-    // same diagonal of the inverse.  Wider bands are not built.
-    if (m->n_off_diag < 0 || m->n_off_diag > 1)
-        return ctx->fail(MGR_ERR_UNSUPPORTED, "n_off_diag > 1 (banded ALO) is not supported");
+    // same diagonal of the inverse.  Wider bands couple the cells' rate
+    // equations; they are built for one model at a time.
+    if (m->n_off_diag < 0 || m->n_off_diag > MAX_OFF_DIAG)
+        return ctx->fail(MGR_ERR_UNSUPPORTED, "n_off_diag above " + std::to_string(MAX_OFF_DIAG) + " is not supported");
+    if (m->n_off_diag >= 2 && Mm > 1)
+        return ctx->fail(MGR_ERR_UNSUPPORTED, "n_off_diag >= 2 (banded ALO) in a batch of models is not supported");
+    ctx->nband = m->n_off_diag >= 2 ? (int)m->n_off_diag - 1 : 0;
+    ctx->band_S = (ctx->R / 2) * 2 * ctx->nband;
     if (m->num < 0 || m->num >= m->nspecs) return ctx->fail(MGR_ERR_ARGUMENT, "line species index out of range");
 
     // Lines::setup (src/Model/Lines/lines.cpp:43-87): sorted line list.
@@ -580,6 +585,17 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
     if ((rc = dalloc(ctx, &ctx->Jlin, N * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->Jeff, N * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->Jdif, N * L))) return rc;
+    if (ctx->nband > 0) {
+        const int64_t S = ctx->band_S;
+        if ((rc = dalloc(ctx, &ctx->band_cell, N * S))) return rc;
+        if ((rc = dalloc(ctx, &ctx->band_part, N * ((F + WAVE - 1) / WAVE) * S * L))) return rc;
+        if ((rc = dalloc(ctx, &ctx->band_Ls, N * S * L))) return rc;
+        if ((rc = dalloc(ctx, &ctx->band_lu, N * V * V))) return rc;
+        if ((rc = dalloc(ctx, &ctx->band_piv, N * V))) return rc;
+        if ((rc = dalloc(ctx, &ctx->band_x, N * V))) return rc;
+        if ((rc = dalloc(ctx, &ctx->band_flags, BAND_SWEEPS + 1))) return rc;
+        MGR_HIP(hipMemset(ctx->band_Ls, 0, N * S * L * sizeof(double)));
+    }
     if ((rc = dalloc(ctx, &ctx->err_flag, 1))) return rc;
     if ((rc = dalloc(ctx, &ctx->partials, (size_t)ctx->nchunks * 8))) return rc;
     if ((rc = dalloc(ctx, &ctx->reduced, 8))) return rc;
@@ -634,6 +650,8 @@ int free_model(mgr_ctx *ctx)
     dfree(ctx->d_m_cell_off);
     dfree(ctx->d_mng);
     dfree(ctx->u_store);
+    dfree(ctx->band_cell); dfree(ctx->band_part); dfree(ctx->band_Ls); dfree(ctx->band_rev_off);
+    dfree(ctx->band_rev); dfree(ctx->band_lu); dfree(ctx->band_piv); dfree(ctx->band_x); dfree(ctx->band_flags);
     free_image(ctx);
     if (ctx->h_reduced) (void)hipHostFree(ctx->h_reduced);
     if (ctx->h_m_active) (void)hipHostFree(ctx->h_m_active);
@@ -885,6 +903,23 @@ int mgr_get_lambda(mgr_ctx *ctx, double *Ls)
 {
     if (!ctx) return MGR_ERR_ARGUMENT;
     return copy_out(ctx, Ls, ctx->Ls, ctx->N * ctx->L * sizeof(double));
+}
+
+int mgr_lambda_band_slots(mgr_ctx *ctx, int64_t *slots)
+{
+    if (!ctx || !slots) return MGR_ERR_ARGUMENT;
+    *slots = ctx->band_S;
+    return MGR_OK;
+}
+
+int mgr_get_lambda_band(mgr_ctx *ctx, int32_t *cells, double *Ls)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (ctx->nband <= 0) return ctx->fail(MGR_ERR_STATE, "the model has no Lambda* bands (n_off_diag < 2)");
+    if (!ctx->traced) return ctx->fail(MGR_ERR_STATE, "mgr_trace first");
+    int rc;
+    if ((rc = copy_out(ctx, cells, ctx->band_cell, ctx->N * ctx->band_S * sizeof(int32_t)))) return rc;
+    return copy_out(ctx, Ls, ctx->band_Ls, ctx->N * ctx->band_S * ctx->L * sizeof(double));
 }
 
 int mgr_get_Jeff(mgr_ctx *ctx, double *Jlin, double *Jeff, double *Jdif)

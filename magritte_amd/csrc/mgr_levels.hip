@@ -43,11 +43,14 @@ __global__ void k_lambda_sum(int64_t N, int L, int nchunk, const double *__restr
 
 // Simulation::compute_Jeff (sim_lines.tpp:176-220): Jlin = sum_z w_z J(p, nr_line),
 // Jeff = Jlin - h/(4 pi) Ls n_i, Jdif = h/(4 pi) Ls n_i.
+// With band elements (band_cell non-null) every element of (p, k) enters:
+// diff = Ls(p,k) n_i(p) + sum over the slots of p of Ls(slot, k) n_i(cell).
 __global__ void k_jeff(int64_t N, int L, int Q, int F, int V, const double *__restrict__ J,
                        const int32_t *__restrict__ nr_line, const double *__restrict__ qw,
                        const int32_t *__restrict__ irad, const double *__restrict__ pop,
                        const double *__restrict__ Ls, double *__restrict__ Jlin,
-                       double *__restrict__ Jeff, double *__restrict__ Jdif)
+                       double *__restrict__ Jeff, double *__restrict__ Jdif, int64_t S,
+                       const int32_t *__restrict__ band_cell, const double *__restrict__ band_Ls)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N * L) return;
@@ -55,7 +58,12 @@ __global__ void k_jeff(int64_t N, int L, int Q, int F, int V, const double *__re
     const int k = (int)(t % L);
     double jl = 0.0;
     for (int z = 0; z < Q; z++) jl += qw[z] * J[p * F + nr_line[(p * L + k) * Q + z]];
-    const double diff = Ls[p * L + k] * pop[p * V + irad[k]];
+    double diff = Ls[p * L + k] * pop[p * V + irad[k]];
+    if (band_cell)
+        for (int64_t sl = p * S; sl < (p + 1) * S; sl++) {
+            const int32_t c = band_cell[sl];
+            if (c >= 0) diff += band_Ls[sl * L + k] * pop[(int64_t)c * V + irad[k]];
+        }
     Jlin[p * L + k] = jl;
     Jeff[p * L + k] = jl - HH_OVER_FOUR_PI * diff;
     Jdif[p * L + k] = HH_OVER_FOUR_PI * diff;
@@ -88,8 +96,10 @@ struct StateqArgs {
     const int32_t *cell_model, *m_active;   // converged models of a batch keep their populations
 };
 
+// History shift and the cell's own triplets (lineProducingSpecies.tpp:260-388)
+// in the transpose convention of the reference, last row = conservation.
 template <int VV>
-__device__ __forceinline__ void stateq_cell(const StateqArgs &a, int64_t p, int V, double *M, double *y)
+__device__ __forceinline__ void stateq_assemble(const StateqArgs &a, int64_t p, int V, double *M, double *y)
 {
     const int Vn = (VV > 0) ? VV : V;
     // history: prev3 <- prev2 <- prev1 <- population (lineProducingSpecies.tpp:260-262)
@@ -153,7 +163,13 @@ __device__ __forceinline__ void stateq_cell(const StateqArgs &a, int64_t p, int 
     // conservation replaces the last level's equation
     for (int i = 0; i < Vn; i++) M[(Vn - 1) * Vn + i] += 1.0;
     y[Vn - 1] = a.pop_tot[p];
+}
 
+template <int VV>
+__device__ __forceinline__ void stateq_cell(const StateqArgs &a, int64_t p, int V, double *M, double *y)
+{
+    const int Vn = (VV > 0) ? VV : V;
+    stateq_assemble<VV>(a, p, Vn, M, y);
     // LU with partial pivoting
     for (int c = 0; c < Vn; c++) {
         int piv = c;
@@ -202,6 +218,7 @@ __global__ void k_stateq(StateqArgs a)
 }
 
 constexpr int VMAX_GENERIC = 32;
+constexpr int VMAX_BAND = 32;
 __global__ void k_stateq_generic(StateqArgs a)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -209,6 +226,167 @@ __global__ void k_stateq_generic(StateqArgs a)
     if (!a.m_active[a.cell_model[p]]) return;
     double M[VMAX_GENERIC * VMAX_GENERIC], y[VMAX_GENERIC];
     stateq_cell<0>(a, p, a.V, M, y);
+}
+
+// ---- rate equations with band elements ----------------------------------------
+// An element (p, k, nr) of Lambda* adds +v at (index(p, j_k), index(nr, i_k))
+// and -v at (index(nr, i_k), index(nr, i_k)), v = -Ls opacity(p, k)
+// (lineProducingSpecies.tpp:317-340): the second lies in cell nr's block, the
+// first couples row p to cell nr.  The system is solved by block Jacobi: the
+// block diagonal D (each cell's own triplets and the second kind of every
+// element naming it) is factorised once, then each sweep solves
+// D x' = y - O x for all cells from the last iterate x.  Sweeps stop when no
+// population moves by more than BAND_TOL relative; not converging in
+// BAND_SWEEPS sweeps is an error.
+constexpr int32_t ERR_BAND = 32;
+constexpr double BAND_TOL = 1.0e-14;
+
+struct BandArgs {
+    int64_t S;
+    const int32_t *cell;            // [N*S]
+    const double  *Ls;              // [N*S*L]
+    const int64_t *rev_off, *rev;   // slots naming each cell
+    double  *lu;                    // [N*V*V]
+    int32_t *piv;                   // [N*V]
+    double  *x;                     // [N*V]
+    int32_t *flags;                 // [BAND_SWEEPS + 1]
+};
+
+__device__ __forceinline__ double opacity_of(const StateqArgs &a, const double *pop, int k)
+{
+    return HH_OVER_FOUR_PI * (pop[a.jrad[k]] * a.Ba[k] - pop[a.irad[k]] * a.Bs[k]);
+}
+
+// History shift and factorisation of the cell's block of D (partial pivoting;
+// multipliers below the diagonal, the pivot row of each column in piv).
+template <int VV>
+__device__ __forceinline__ void band_factor_cell(const StateqArgs &a, const BandArgs &b, int64_t p, int V,
+                                                 double *M, double *y)
+{
+    const int Vn = (VV > 0) ? VV : V;
+    stateq_assemble<VV>(a, p, Vn, M, y);
+    // the diagonal terms of the elements of other cells that name this cell,
+    // with their opacity from the populations before the solve
+    for (int64_t e = b.rev_off[p]; e < b.rev_off[p + 1]; e++) {
+        const int64_t sl = b.rev[e];
+        const int64_t o = sl / b.S;
+        const double *pop_o = a.pop + o * Vn;
+        for (int k = 0; k < a.L; k++) {
+            const int I = a.irad[k];
+            if (I != Vn - 1) M[I * Vn + I] += b.Ls[sl * a.L + k] * opacity_of(a, pop_o, k);
+        }
+    }
+    for (int c = 0; c < Vn; c++) {
+        int pv = c;
+        double best = fabs(M[c * Vn + c]);
+        for (int r = c + 1; r < Vn; r++) {
+            const double v = fabs(M[r * Vn + c]);
+            if (v > best) { best = v; pv = r; }
+        }
+        if (best == 0.0) {
+            atomicOr(a.err_flag, ERR_SINGULAR);
+            pv = c;
+        }
+        b.piv[p * Vn + c] = pv;
+        if (pv != c)
+            for (int j = 0; j < Vn; j++) {
+                const double tsw = M[c * Vn + j];
+                M[c * Vn + j] = M[pv * Vn + j];
+                M[pv * Vn + j] = tsw;
+            }
+        const double inv = best == 0.0 ? 0.0 : 1.0 / M[c * Vn + c];
+        for (int r = c + 1; r < Vn; r++) {
+            const double l = M[r * Vn + c] * inv;
+            M[r * Vn + c] = l;
+            for (int j = c + 1; j < Vn; j++) M[r * Vn + j] -= l * M[c * Vn + j];
+        }
+    }
+    for (int i = 0; i < Vn * Vn; i++) b.lu[p * Vn * Vn + i] = M[i];
+}
+
+// One block-Jacobi sweep for the cell: right-hand side y - O x with x the
+// current populations, solved with the stored factors into b.x.
+template <int VV>
+__device__ __forceinline__ void band_sweep_cell(const StateqArgs &a, const BandArgs &b, int64_t p, int V,
+                                                double *M, double *y)
+{
+    const int Vn = (VV > 0) ? VV : V;
+    for (int i = 0; i < Vn; i++) y[i] = 0.0;
+    y[Vn - 1] = a.pop_tot[p];
+    const double *pre = a.prev1 + p * Vn;   // the populations before the solve
+    for (int64_t sl = p * b.S; sl < (p + 1) * b.S; sl++) {
+        const int32_t c = b.cell[sl];
+        if (c < 0) continue;
+        for (int k = 0; k < a.L; k++) {
+            const int Jl = a.jrad[k];
+            if (Jl == Vn - 1) continue;
+            const double v = -b.Ls[sl * a.L + k] * opacity_of(a, pre, k);
+            y[Jl] -= v * a.pop[(int64_t)c * Vn + a.irad[k]];
+        }
+    }
+    for (int i = 0; i < Vn * Vn; i++) M[i] = b.lu[p * Vn * Vn + i];
+    // the factorisation swapped whole rows (multipliers included): every
+    // interchange first, then the unit lower triangle
+    for (int c = 0; c < Vn; c++) {
+        const int pv = b.piv[p * Vn + c];
+        if (pv != c) {
+            const double tsw = y[c];
+            y[c] = y[pv];
+            y[pv] = tsw;
+        }
+    }
+    for (int c = 0; c < Vn; c++)
+        for (int r = c + 1; r < Vn; r++) y[r] -= M[r * Vn + c] * y[c];
+    for (int r = Vn - 1; r >= 0; r--) {
+        double acc = y[r];
+        for (int j = r + 1; j < Vn; j++) acc -= M[r * Vn + j] * y[j];
+        y[r] = acc / M[r * Vn + r];
+    }
+    for (int i = 0; i < Vn; i++) b.x[p * Vn + i] = y[i];
+}
+
+template <int VV>
+__global__ void k_band_factor(StateqArgs a, BandArgs b)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.N) return;
+    constexpr int SZ = VV > 0 ? VV : VMAX_BAND;
+    double M[SZ * SZ], y[SZ];
+    band_factor_cell<VV>(a, b, p, a.V, M, y);
+}
+
+template <int VV>
+__global__ void k_band_sweep(StateqArgs a, BandArgs b, int t)
+{
+    if (b.flags[t] == 0) return;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.N) return;
+    constexpr int SZ = VV > 0 ? VV : VMAX_BAND;
+    double M[SZ * SZ], y[SZ];
+    band_sweep_cell<VV>(a, b, p, a.V, M, y);
+}
+
+// Sweep t's iterate becomes the populations; a move above BAND_TOL asks for
+// sweep t + 1.
+__global__ void k_band_commit(int64_t NV, double *__restrict__ pop, const double *__restrict__ x,
+                              int32_t *__restrict__ flags, int t)
+{
+    if (flags[t] == 0) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NV) return;
+    const double xn = x[i];
+    if (!(fabs(xn - pop[i]) <= BAND_TOL * fabs(xn))) atomicOr(&flags[t + 1], 1);
+    pop[i] = xn;
+}
+
+__global__ void k_band_flags(int32_t *flags, int n)
+{
+    for (int i = threadIdx.x; i <= n; i += blockDim.x) flags[i] = (i == 0);
+}
+
+__global__ void k_band_check(const int32_t *flags, int n, int32_t *err_flag)
+{
+    if (flags[n]) atomicOr(err_flag, ERR_BAND);
 }
 
 __device__ __forceinline__ double block_sum(double v, double *sh)
@@ -423,7 +601,53 @@ int launch_jeff(mgr_ctx *ctx)
     hipLaunchKernelGGL(k_jeff, dim3(grid_for(ctx->N * ctx->L, 256)), dim3(256), 0, ctx->stream,
                        ctx->N, (int)ctx->L, (int)ctx->Q, (int)ctx->F, (int)ctx->V, ctx->J,
                        ctx->nr_line, ctx->qweights, ctx->irad, ctx->pop, ctx->Ls, ctx->Jlin,
-                       ctx->Jeff, ctx->Jdif);
+                       ctx->Jeff, ctx->Jdif, ctx->band_S, ctx->nband > 0 ? ctx->band_cell : nullptr,
+                       ctx->band_Ls);
+    MGR_LAUNCH_CHECK();
+    return MGR_OK;
+}
+
+// The rate equations with band elements (one model): factorise the block
+// diagonal, then BAND_SWEEPS block-Jacobi sweeps, each a no-op once an
+// earlier one has converged (graph-capturable: the flags are set by kernels).
+static int launch_stateq_band(mgr_ctx *ctx, const StateqArgs &a)
+{
+    if (ctx->V > VMAX_BAND) return ctx->fail(MGR_ERR_UNSUPPORTED, "more than 32 levels");
+    BandArgs bb;
+    bb.S = ctx->band_S;
+    bb.cell = ctx->band_cell;
+    bb.Ls = ctx->band_Ls;
+    bb.rev_off = ctx->band_rev_off;
+    bb.rev = ctx->band_rev;
+    bb.lu = ctx->band_lu;
+    bb.piv = ctx->band_piv;
+    bb.x = ctx->band_x;
+    bb.flags = ctx->band_flags;
+    const dim3 g(grid_for(ctx->N, 128)), b(128);
+    const int64_t NV = ctx->N * ctx->V;
+    hipLaunchKernelGGL(k_band_flags, dim3(1), dim3(64), 0, ctx->stream, bb.flags, BAND_SWEEPS);
+    MGR_LAUNCH_CHECK();
+#define MGR_BAND_V(VV)                                                                                       \
+    hipLaunchKernelGGL(k_band_factor<VV>, g, b, 0, ctx->stream, a, bb);                                       \
+    for (int t = 0; t < BAND_SWEEPS; t++) {                                                                   \
+        hipLaunchKernelGGL(k_band_sweep<VV>, g, b, 0, ctx->stream, a, bb, t);                                 \
+        hipLaunchKernelGGL(k_band_commit, dim3(grid_for(NV, 256)), dim3(256), 0, ctx->stream, NV, ctx->pop,   \
+                           (const double *)bb.x, bb.flags, t);                                                \
+    }
+    switch (ctx->V) {
+        case 2: MGR_BAND_V(2) break;
+        case 3: MGR_BAND_V(3) break;
+        case 4: MGR_BAND_V(4) break;
+        case 5: MGR_BAND_V(5) break;
+        case 6: MGR_BAND_V(6) break;
+        case 7: MGR_BAND_V(7) break;
+        case 8: MGR_BAND_V(8) break;
+        default: MGR_BAND_V(0) break;
+    }
+#undef MGR_BAND_V
+    MGR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_band_check, dim3(1), dim3(1), 0, ctx->stream, (const int32_t *)bb.flags, BAND_SWEEPS,
+                       ctx->err_flag);
     MGR_LAUNCH_CHECK();
     return MGR_OK;
 }
@@ -444,6 +668,7 @@ int launch_stateq(mgr_ctx *ctx)
     a.cell_model = ctx->d_cell_model;
     a.m_active = ctx->d_m_active;
     const dim3 g(grid_for(ctx->N, 128)), b(128);
+    if (ctx->nband > 0) return launch_stateq_band(ctx, a);
     switch (ctx->V) {
         case 2: hipLaunchKernelGGL(k_stateq<2>, g, b, 0, ctx->stream, a); break;
         case 3: hipLaunchKernelGGL(k_stateq<3>, g, b, 0, ctx->stream, a); break;
@@ -521,6 +746,8 @@ int read_convergence(mgr_ctx *ctx, mgr_conv *conv)
         conv->relative_change_max = ctx->h_reduced[2];
     }
     if (errv & ERR_SINGULAR) return ctx->fail(MGR_ERR_NUMERIC, "singular rate matrix in a cell");
+    if (errv & ERR_BAND)
+        return ctx->fail(MGR_ERR_NUMERIC, "rate equations with Lambda* bands: block-Jacobi sweeps did not converge");
     if (errv) return ctx->fail(MGR_ERR_TRACE, "a ray pair ended off the boundary or an interior origin has an empty ray");
     return MGR_OK;
 }

@@ -162,6 +162,9 @@ struct RFArgs {
     double        *Lpart;                     // [N][nchunk][L] one sum per wave and transition
     double        *u_out;                     // [R/2][N*F] per-pair u (scattering), or null
     int32_t       *err_flag;
+    int            nband;                     // Lambda* bands (n_off_diag - 1) of k_lambda_band
+    int64_t        band_S;                    // band slots per origin
+    double        *band_part;                 // [N][nchunk][band_S][L]
 };
 
 // ---- line profiles ----------------------------------------------------------
@@ -972,6 +975,265 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_image(RFArgs<real> a, doub
     }
 }
 
+// ---- banded Lambda* (n_off_diag >= 2) ------------------------------------------
+
+// Sub-points of traced step j of the entry at base: m = 1 .. n at shift
+// s0 + m ds with dZ dz, in cell c_prev for m < half and c_next otherwise
+// (Geometry::set_data, geometry.tpp:100-207).
+struct StepPoints {
+    int     n, half;
+    double  s0, ds, dz;
+    int32_t c_prev, c_next;
+};
+
+__device__ __forceinline__ StepPoints step_points(const RFArgs<double> &a, int64_t base, int j, int32_t o,
+                                                  double dsm)
+{
+    StepPoints p;
+    const double sn = a.seg_shift[base + j];
+    const double sc = j > 0 ? a.seg_shift[base + j - 1] : 1.0;   // co-moving: the origin's shift is 1
+    const StepSplit sp = split_step(sc, sn, a.seg_dZ[base + j], dsm);
+    p.n = sp.n;
+    p.half = sp.interp ? sp.half : 0;
+    p.s0 = sp.interp ? sc : sn;
+    p.ds = sp.dshift;
+    p.dz = sp.dZ;
+    p.c_prev = j > 0 ? a.seg_cell[base + j - 1] : o;
+    p.c_next = a.seg_cell[base + j];
+    return p;
+}
+
+// The points of a ray pair in order: entry qx from its far end inwards, the
+// origin, then entry qy outwards, at most ny points of it.  visit(t, cell,
+// shift, dz) gets the visit count t and the dZ of the interval between the
+// point and the one visited before it, which is the dZ of the interval's
+// outer point (Simulation::setup, simulation.tpp:233-240, 292-299).
+template <typename Visit>
+__device__ __forceinline__ void walk_pair(const RFArgs<double> &a, int32_t o, int64_t qx, int64_t qy, int ny,
+                                          double dsm, Visit &&visit)
+{
+    int t = 0;
+    double dz_prev = 0.0;
+    const int64_t bx = a.seg_off[qx];
+    for (int j = a.nseg[qx] - 1; j >= 0; --j) {
+        const StepPoints sp = step_points(a, bx, j, o, dsm);
+        for (int m = sp.n; m >= 1; --m) {
+            visit(t++, m < sp.half ? sp.c_prev : sp.c_next, sub_point_shift(sp.s0, m, sp.ds), dz_prev);
+            dz_prev = sp.dz;
+        }
+    }
+    visit(t++, o, 1.0, dz_prev);
+    const int64_t by = a.seg_off[qy];
+    const int nsy = a.nseg[qy];
+    for (int j = 0; j < nsy && ny > 0; ++j) {
+        const StepPoints sp = step_points(a, by, j, o, dsm);
+        for (int m = 1; m <= sp.n && ny > 0; ++m, --ny)
+            visit(t++, m < sp.half ? sp.c_prev : sp.c_next, sub_point_shift(sp.s0, m, sp.ds), sp.dz);
+    }
+}
+
+// Thermodynamics::profile (thermodynamics.tpp:16-29) of width `width` at
+// freq_diff, with the reference's scalar 1/exp.
+__device__ __forceinline__ double profile64(double width, double freq_diff)
+{
+#pragma clang fp contract(off)
+    const double iw = 1.0 / width;
+    const double x = iw * freq_diff;
+    return iw * INVERSE_SQRT_PI * (1.0 / exp(x * x));
+}
+
+// Per-lane window of the band kernel in LDS: array k, pair index n at
+// ((k W) + n - (n_ar - nband)) WAVE + lane, W = 3 (nband + 1), covering
+// n_ar - nband .. n_ar + 2 nband + 2.
+constexpr int BW_F = 0, BW_IC = 1, BW_G = 2, BW_FP = 3, BW_CHI = 4, BW_ARRAYS = 5;
+
+// The band elements of Lambda* (RayPair::update_Lambda, raypair_lambda.tpp:129-146,
+// with get_L_lower / get_L_upper, raypair_lambda.tpp:44-84) in FP64.  One wave
+// of 64 bins per (origin, chunk of bins).  Per ray pair the longer ray is the
+// low side (n_ar = nA points, sim_radiation.tpp:164-171), and the reference's
+// own variables are formed with its operation order (raypair_solver.tpp:200-388):
+//   pass 1, from index 0 up: dtau, F and 1/C up to index n_ar + 2 nband + 1;
+//   pass 2, from index last down: G down to index n_ar - nband + 1;
+// then for band m = 1 .. nband
+//   lower: L_lower[m][n_ar-m] = L_diag[n_ar-m] prod_{q=n_ar-m+1}^{n_ar+1} 1/(1+G_q),
+//   upper: L_upper[m][n_ar+m] = L_diag[n_ar+2m+1] prod_{q=n_ar+2m}^{n_ar+m} 1/(1+F_q),
+// each times A_k (w_z 2 w_r) nu phi(nu) / chi at its own point, where the
+// pair's solve wrote it (the slots k_band_cells marks).  The wave sums its
+// lanes per transition; k_band_sum adds the waves of an origin.
+__global__ __launch_bounds__(WAVE) void k_lambda_band(RFArgs<double> a)
+{
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *win = reinterpret_cast<double *>(smem);
+    const int G = gridDim.x;
+    const int per = G >> 3;
+    const int bid = blockIdx.x;
+    const int logical = (bid & 7) * per + (bid >> 3);
+    if (logical >= a.grid_real) return;
+    const int32_t o = (int32_t)(logical / a.nchunk);
+    const int chunk = logical - o * a.nchunk;
+    const int lane = (int)threadIdx.x;
+    const int F = a.F, L = a.L, nb = a.nband;
+    const int W = 3 * (nb + 1);
+    const int f = chunk * WAVE + lane;
+    const bool active = f < F;
+    const int fa = active ? f : F - 1;
+    const int k = active ? a.f_k[fa] : L;
+    const double nu_f = a.nu[(int64_t)o * F + fa];
+    const double dsm = a.dshift_max[o];
+    LaneCtx<double> lc;
+    double S_o, chi_o;
+    origin_point<double>(a, nu_f, a.rows + (int64_t)o * L, lc, S_o, chi_o);
+    const double freq_line = k < L ? a.freq_k[k] : 1.0;
+    const double A_k = k < L ? a.A_k[k] : 0.0;
+    const double w_z = k < L ? a.qw[a.f_z[fa]] : 0.0;
+    auto wref = [&](int arr, int w) -> double & { return win[(arr * W + w) * WAVE + lane]; };
+    const int64_t qb = a.cell_qbase[o];
+    const int Rh = a.m_R[0] / 2;
+    double *part = a.band_part + ((int64_t)o * a.nchunk + chunk) * a.band_S * L;
+
+    for (int r = 0; r < Rh; r++) {
+        const int64_t qr = qb + r, qa = qb + a.antipod[r];
+        const int n_r = a.npts[qr], n_a = a.npts[qa];
+        const bool r_longer = n_r > n_a;
+        const int64_t qA = r_longer ? qr : qa, qB = r_longer ? qa : qr;
+        const int nA = r_longer ? n_r : n_a, nB = r_longer ? n_a : n_r;
+        const int last = nA + nB;
+        const int w0 = nA - nb;
+        double dt0 = 0.0, dtL = 0.0;
+        if (nB >= 1) {
+            // pass 1: from index 0 (the far end of the longer ray) upwards
+            double chi_p = 0.0, dt_p = 0.0, F_p = 0.0, i1F_p = 0.0;
+            const int top = nA + 2 * nb + 2 < last ? nA + 2 * nb + 2 : last;
+            walk_pair(a, o, qA, qB, top - nA, dsm, [&](int n, int32_t c, double s, double dz) {
+                double eta, chi;
+                eval_point<double>(a, lc, a.rows + (int64_t)c * L, s, eta, chi);
+                chi *= 1.0 / CHI_SCALE;
+                const int w = n - w0;
+                if (w >= 0 && w < W) {
+                    // nu phi(nu) at the point's own cell and frequency, and chi
+                    const double fs = s * nu_f;
+                    wref(BW_FP, w) = k < L ? fs * profile64(freq_line * a.pw[c], fs - freq_line) : 0.0;
+                    wref(BW_CHI, w) = chi;
+                }
+                if (n >= 1) {
+                    const double dt = 0.5 * (chi_p + chi) * dz;   // dtau[n-1]
+                    if (n == 1) {
+                        dt0 = dt;
+                        const double B0_min_C0 = 1.0 + 2.0 * (1.0 / dt);
+                        F_p = 0.5 * B0_min_C0 * dt * dt;
+                        i1F_p = 1.0 / (1.0 + F_p);
+                        if (w - 1 >= 0 && w - 1 < W) wref(BW_F, w - 1) = F_p;
+                    } else {
+                        const double iA = 0.5 * (dt_p + dt) * dt_p;   // inverse_A[n-1]
+                        const double iC = 0.5 * (dt_p + dt) * dt;     // inverse_C[n-1]
+                        F_p = (1.0 + (1.0 / iA) * F_p * i1F_p) * iC;  // F[n-1]
+                        i1F_p = 1.0 / (1.0 + F_p);
+                        if (w - 1 >= 0 && w - 1 < W) {
+                            wref(BW_F, w - 1) = F_p;
+                            wref(BW_IC, w - 1) = iC;
+                        }
+                    }
+                    dt_p = dt;
+                }
+                chi_p = chi;
+            });
+            // pass 2: from index last (the far end of the shorter ray) downwards
+            double G_p = 0.0, i1G_p = 0.0;
+            chi_p = 0.0;
+            dt_p = 0.0;
+            const int bottom = nA - nb > 0 ? nA - nb : 0;
+            walk_pair(a, o, qB, qA, nA - bottom, dsm, [&](int t, int32_t c, double s, double dz) {
+                const int n = last - t;
+                double eta, chi;
+                eval_point<double>(a, lc, a.rows + (int64_t)c * L, s, eta, chi);
+                chi *= 1.0 / CHI_SCALE;
+                if (t >= 1) {
+                    const double dt = 0.5 * (chi + chi_p) * dz;   // dtau[n]
+                    if (n == last - 1) {
+                        dtL = dt;
+                        const double Bd_min_Ad = 1.0 + 2.0 * (1.0 / dt);
+                        G_p = 0.5 * Bd_min_Ad * dt * dt;              // G[last]
+                    } else {
+                        const double iA = 0.5 * (dt + dt_p) * dt;     // inverse_A[n+1]
+                        const double iC = 0.5 * (dt + dt_p) * dt_p;   // inverse_C[n+1]
+                        G_p = (1.0 + (1.0 / iC) * G_p * i1G_p) * iA;  // G[n+1]
+                    }
+                    i1G_p = 1.0 / (1.0 + G_p);
+                    const int w = n + 1 - w0;
+                    if (w >= 0 && w < W) wref(BW_G, w) = G_p;
+                    dt_p = dt;
+                }
+                chi_p = chi;
+            });
+        }
+        // per slot of the pair: this lane's element, then the wave's sum per transition
+        int kmin = k, kmax = k < L ? k : -1;
+        for (int off = WAVE / 2; off >= 1; off >>= 1) {
+            kmin = min(kmin, __shfl_xor(kmin, off));
+            kmax = max(kmax, __shfl_xor(kmax, off));
+        }
+        const double factor = A_k * (w_z * 2.0 * a.ray_w[r]);
+        for (int i = 0; i < 2 * nb; i++) {
+            const bool upper = i >= nb;
+            const int m = upper ? i - nb + 1 : i + 1;
+            double v = 0.0;
+            if (!upper && nB >= 1 && nA >= m) {
+                // L_lower[m][n] = T^-1[n_ar+1][n], n = n_ar - m
+                const int n = nA - m, w = n - w0;
+                double T;
+                if (n == 0) {
+                    const double B0_min_C0 = 1.0 + 2.0 * (1.0 / dt0);
+                    const double B0 = B0_min_C0 + 2.0 * (1.0 / dt0) * (1.0 / dt0);
+                    const double G1 = wref(BW_G, 1 - w0);
+                    T = (1.0 + G1) / (B0_min_C0 + B0 * G1);
+                } else {
+                    const double G1 = wref(BW_G, w + 1);
+                    T = wref(BW_IC, w) / (wref(BW_F, w) + G1 * (1.0 / (1.0 + G1)));
+                }
+                for (int q = n + 1; q <= nA + 1; q++) T = T * (1.0 / (1.0 + wref(BW_G, q - w0)));
+                v = factor * (wref(BW_FP, w) * T / wref(BW_CHI, w));
+            } else if (upper && nB > 2 * m) {
+                // L_upper[m][n] = T^-1[n][n_ar+2m+1], n = n_ar + m
+                const int n = nA + m, j = nA + 2 * m + 1;
+                double T;
+                if (j == last) {
+                    const double Fl = wref(BW_F, j - 1 - w0);
+                    const double Bd_min_Ad = 1.0 + 2.0 * (1.0 / dtL);
+                    const double Bd = Bd_min_Ad + 2.0 * (1.0 / dtL) * (1.0 / dtL);
+                    T = (1.0 + Fl) / (Bd_min_Ad + Bd * Fl);
+                } else {
+                    const double Gj = wref(BW_G, j + 1 - w0);
+                    T = wref(BW_IC, j - w0) / (wref(BW_F, j - w0) + Gj * (1.0 / (1.0 + Gj)));
+                }
+                for (int q = j - 1; q >= n; q--) T = T * (1.0 / (1.0 + wref(BW_F, q - w0)));
+                v = factor * (wref(BW_FP, n - w0) * T / wref(BW_CHI, n - w0));
+            }
+            if (k >= L) v = 0.0;
+            double mine = 0.0;
+            for (int kk = kmin; kk <= kmax; kk++) {
+                double x = (k == kk) ? v : 0.0;
+                for (int off = WAVE / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off);
+                x = __shfl(x, 0);   // lane 0's order for every lane
+                if (lane == kk) mine = x;
+            }
+            if (lane < L) part[((int64_t)r * 2 * nb + i) * L + lane] = mine;
+        }
+    }
+}
+
+// Band sums per (origin, slot, transition): the waves of an origin in chunk order.
+__global__ void k_band_sum(int64_t N, int64_t SL, int nchunk, const double *__restrict__ part,
+                           double *__restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * SL) return;
+    const int64_t o = t / SL, i = t - o * SL;
+    double acc = 0.0;
+    for (int c = 0; c < nchunk; c++) acc += part[(o * nchunk + c) * SL + i];
+    out[t] = acc;
+}
+
 // Largest chunk of staged steps that fits the LDS budget with the cells'
 // line rows, at most CH_MAX.
 template <typename real, bool X64 = false>
@@ -1143,6 +1405,33 @@ int launch_image(mgr_ctx *ctx, const typename Prec<real>::row_t *rows)
 
 namespace mgr {
 
+// Band elements of Lambda* for the current populations (n_off_diag >= 2):
+// k_lambda_band into per-wave shares, then their sums per slot and transition.
+int launch_band(mgr_ctx *ctx)
+{
+    if (ctx->nband <= 0) return MGR_OK;
+    RFArgs<double> a;
+    int rc = make_args<double>(ctx, ctx->row64, 1, 2, 0, ctx->N, ctx->shift_min, ctx->shift_max, a);
+    if (rc) return rc;
+    a.seg_off = ctx->seg_off;
+    a.nseg = ctx->nseg;
+    a.npts = ctx->npts;
+    a.seg_cell = ctx->seg_cell;
+    a.seg_shift = ctx->seg_shift;
+    a.seg_dZ = ctx->seg_dZ;
+    a.nband = ctx->nband;
+    a.band_S = ctx->band_S;
+    a.band_part = ctx->band_part;
+    const size_t lds = (size_t)BW_ARRAYS * 3 * (ctx->nband + 1) * WAVE * sizeof(double);
+    hipLaunchKernelGGL(k_lambda_band, dim3(grid_of(a)), dim3(WAVE), lds, ctx->stream, a);
+    MGR_LAUNCH_CHECK();
+    const int64_t SL = ctx->band_S * ctx->L;
+    hipLaunchKernelGGL(k_band_sum, dim3((unsigned)((ctx->N * SL + 255) / 256)), dim3(256), 0, ctx->stream, ctx->N,
+                       SL, a.nchunk, ctx->band_part, ctx->band_Ls);
+    MGR_LAUNCH_CHECK();
+    return MGR_OK;
+}
+
 template <typename real>
 static int launch_rf_shape(mgr_ctx *ctx, int nwg, int ch, int64_t o_begin, int64_t o_count)
 {
@@ -1288,7 +1577,8 @@ int launch_rf_kernel(mgr_ctx *ctx, int precision)
 
 int launch_radiation_field(mgr_ctx *ctx, int precision)
 {
-    const int rc = launch_rf_kernel(ctx, precision);
+    int rc = launch_rf_kernel(ctx, precision);
+    if (!rc) rc = launch_band(ctx);
     if (rc) return rc;
     return launch_lambda(ctx);   // update_Lambda's sums over the line bins
 }
@@ -1321,7 +1611,8 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
     int rc = launch_rf_kernel(ctx, precision);
     if (!rc) {
         MGR_HIP(hipEventRecord(e1, ctx->stream));
-        rc = launch_lambda(ctx);
+        rc = launch_band(ctx);
+        if (!rc) rc = launch_lambda(ctx);
     }
     if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
     MGR_HIP(hipEventSynchronize(e1));
@@ -1361,6 +1652,7 @@ extern "C" int mgr_first_pass(mgr_ctx *ctx, int precision, int pair_stride)
     ctx->rf_pair_stride = pair_stride;
     rc = launch_rf_kernel(ctx, precision);
     ctx->rf_pair_stride = 1;
+    if (!rc) rc = launch_band(ctx);
     if (!rc) rc = launch_lambda(ctx);
     if (!rc) rc = launch_jeff(ctx);
     if (rc) return rc;

@@ -377,9 +377,93 @@ __global__ void k_max_hdz(int64_t T, const float *__restrict__ hdz, unsigned *__
     if (threadIdx.x == 0) atomicMax(out, s_max[0]);
 }
 
+// Cell of sub-point i (0: nearest the origin) of entry q: the steps' split
+// (Geometry::set_data, geometry.tpp:100-207) assigns sub-points m < half of a
+// step to the step's current cell and the others to its next cell.
+__device__ int32_t sub_point_cell(const int64_t *__restrict__ seg_off, const int32_t *__restrict__ nseg,
+                                  const int32_t *__restrict__ seg_cell, const double *__restrict__ seg_shift,
+                                  const double *__restrict__ seg_dZ, int64_t q, int32_t o, double dsm, int i)
+{
+    const int64_t b = seg_off[q];
+    const int ns = nseg[q];
+    for (int j = 0; j < ns; j++) {
+        const double sc = j > 0 ? seg_shift[b + j - 1] : 1.0;
+        const StepSplit sp = split_step(sc, seg_shift[b + j], seg_dZ[b + j], dsm);
+        if (i < sp.n) return (i + 1 < (sp.interp ? sp.half : 0)) ? (j > 0 ? seg_cell[b + j - 1] : o) : seg_cell[b + j];
+        i -= sp.n;
+    }
+    return -1;
+}
+
+// Cells of the Lambda* band elements of every (origin, ray pair), one model
+// (RayPair::update_Lambda, raypair_lambda.tpp:129-146).  The longer ray of
+// the pair is the low-index side (sim_radiation.tpp:164-171), ties to the
+// antipode.  Band m's lower element is at point n_ar - m, sub-point m-1 of the
+// longer ray; it exists when the shorter ray has a point, so that the pair's
+// solve wrote L_lower[m][n_ar-m] (raypair_solver.tpp:381-388).  Its upper
+// element is at point n_ar + m, sub-point m-1 of the shorter ray; it exists
+// when that ray has more than 2m points (L_upper[m][n_ar+m] written).  Other
+// slots get -1.
+__global__ void k_band_cells(int64_t N, int Rh, int nband, const int64_t *__restrict__ cell_qbase,
+                             const int32_t *__restrict__ antipod, const int64_t *__restrict__ seg_off,
+                             const int32_t *__restrict__ nseg, const int32_t *__restrict__ npts,
+                             const int32_t *__restrict__ seg_cell, const double *__restrict__ seg_shift,
+                             const double *__restrict__ seg_dZ, const double *__restrict__ dshift_max,
+                             int32_t *__restrict__ band_cell)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * Rh) return;
+    const int32_t o = (int32_t)(t / Rh);
+    const int r = (int)(t - (int64_t)o * Rh);
+    const int64_t qr = cell_qbase[o] + r, qa = cell_qbase[o] + antipod[r];
+    const int n_r = npts[qr], n_a = npts[qa];
+    const bool r_longer = n_r > n_a;
+    const int64_t qA = r_longer ? qr : qa, qB = r_longer ? qa : qr;
+    const int nA = r_longer ? n_r : n_a, nB = r_longer ? n_a : n_r;
+    const double dsm = dshift_max[o];
+    int32_t *out = band_cell + t * 2 * nband;
+    for (int m = 1; m <= nband; m++) {
+        out[m - 1] = (nB >= 1 && nA >= m)
+                         ? sub_point_cell(seg_off, nseg, seg_cell, seg_shift, seg_dZ, qA, o, dsm, m - 1) : -1;
+        out[nband + m - 1] = (nB > 2 * m)
+                         ? sub_point_cell(seg_off, nseg, seg_cell, seg_shift, seg_dZ, qB, o, dsm, m - 1) : -1;
+    }
+}
+
 }  // namespace
 
 namespace mgr {
+
+// Band cells of the current trace and, on the host, their reverse map: for
+// every cell the slots (origin S + slot) that name it, in ascending order.
+int launch_band_cells(mgr_ctx *ctx)
+{
+    const int64_t N = ctx->N, S = ctx->band_S;
+    const int Rh = (int)(ctx->R / 2);
+    hipLaunchKernelGGL(k_band_cells, dim3((unsigned)((N * Rh + 127) / 128)), dim3(128), 0, ctx->stream, N, Rh,
+                       ctx->nband, ctx->d_cell_qbase, ctx->d_antipod, ctx->seg_off, ctx->nseg, ctx->npts,
+                       ctx->seg_cell, ctx->seg_shift, ctx->seg_dZ, ctx->dshift_max, ctx->band_cell);
+    MGR_LAUNCH_CHECK();
+    std::vector<int32_t> cell(N * S);
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    MGR_HIP(hipMemcpy(cell.data(), ctx->band_cell, N * S * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int64_t> off(N + 1, 0);
+    for (int64_t i = 0; i < N * S; i++)
+        if (cell[i] >= 0) off[cell[i] + 1]++;
+    for (int64_t c = 0; c < N; c++) off[c + 1] += off[c];
+    std::vector<int64_t> rev(std::max<int64_t>(off[N], 1)), pos(off.begin(), off.end() - 1);
+    for (int64_t i = 0; i < N * S; i++)
+        if (cell[i] >= 0) rev[pos[cell[i]]++] = i;
+    if (ctx->band_rev_off) (void)hipFree(ctx->band_rev_off);
+    if (ctx->band_rev) (void)hipFree(ctx->band_rev);
+    ctx->band_rev_off = nullptr;
+    ctx->band_rev = nullptr;
+    MGR_HIP(hipMalloc((void **)&ctx->band_rev_off, (N + 1) * sizeof(int64_t)));
+    MGR_HIP(hipMalloc((void **)&ctx->band_rev, rev.size() * sizeof(int64_t)));
+    MGR_HIP(hipMemcpy(ctx->band_rev_off, off.data(), (N + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    MGR_HIP(hipMemcpy(ctx->band_rev, rev.data(), rev.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    return MGR_OK;
+}
 
 static int trace_error(mgr_ctx *ctx, int32_t errv)
 {
@@ -469,6 +553,10 @@ int launch_trace(mgr_ctx *ctx)
                        ctx->seg_f32, ctx->seg_hdz32, ctx->far_cell, ctx->far_shift, nullptr, ctx->err_flag);
     MGR_LAUNCH_CHECK();
     MGR_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->nband > 0) {
+        const int rc = launch_band_cells(ctx);
+        if (rc) return rc;
+    }
     ctx->traced = true;
     return MGR_OK;
 }

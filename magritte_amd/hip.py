@@ -46,6 +46,8 @@ SIGNATURES = {
     "mgr_set_population_history": ([_vp, _vp, _vp, _vp], _i),
     "mgr_get_J": ([_vp, _vp], _i),
     "mgr_get_lambda": ([_vp, _vp], _i),
+    "mgr_lambda_band_slots": ([_vp, _vp], _i),
+    "mgr_get_lambda_band": ([_vp, _vp, _vp], _i),
     "mgr_get_Jeff": ([_vp, _vp, _vp, _vp], _i),
     "mgr_get_frequencies": ([_vp, _vp, _vp], _i),
     "mgr_get_boundary_intensities": ([_vp, _vp], _i),
@@ -220,6 +222,34 @@ class Context:
         a = np.zeros(self.N * self.L)
         self._check(self.lib.mgr_get_lambda(self.h, _p(a)), "get_lambda")
         return a.reshape(self.N, self.L)
+
+    def lambda_band_raw(self):
+        """Band slots of Lambda* (n_off_diag >= 2): cells [N, S] (-1: no
+        element) and Ls [N, S, L]."""
+        S = C.c_int64()
+        self._check(self.lib.mgr_lambda_band_slots(self.h, C.byref(S)), "lambda_band_slots")
+        S = S.value
+        cells = np.zeros(self.N * S, dtype=np.int32)
+        Ls = np.zeros(self.N * S * self.L)
+        self._check(self.lib.mgr_get_lambda_band(self.h, _p(cells), _p(Ls)), "get_lambda_band")
+        return cells.reshape(self.N, S), Ls.reshape(self.N, S, self.L)
+
+    def lambda_band(self):
+        """Band elements merged per (p, k) and emitting cell, as
+        Lambda::add_element keeps them: {(p, k): {nr: Ls}} for nr != p; an
+        element naming p itself is added to lambda_diag()'s (returned
+        separately as the second value)."""
+        cells, Ls = self.lambda_band_raw()
+        out, diag_add = {}, np.zeros((self.N, self.L))
+        for p, s in zip(*np.nonzero(cells >= 0)):
+            c = int(cells[p, s])
+            for k in range(self.L):
+                if c == p:
+                    diag_add[p, k] += Ls[p, s, k]
+                else:
+                    d = out.setdefault((int(p), k), {})
+                    d[c] = d.get(c, 0.0) + Ls[p, s, k]
+        return out, diag_add
 
     def Jeff(self):
         a, b, c = (np.zeros(self.N * self.L) for _ in range(3))

@@ -22,7 +22,16 @@
  *    points, as set_data's early return does (geometry.tpp:138-147);
  *  - a ray with no forward neighbour stops and is reported (-1) instead of
  *    reading velocity[-1] (geometry.tpp:68-77).
- *  - Lambda* is stored for n_off_diag = 0 only (one element per (p,k) at nr = p).
+ *  - with n_off_diag >= 2 the band elements (raypair_lambda.tpp:129-146) are
+ *    added only where the solve of the same pair wrote them: the lower band m
+ *    reads L_lower[m][n_ar-m], written iff the shorter ray has a point
+ *    (n_r >= 1), and the upper band m reads L_upper[m][n_ar+m], written iff
+ *    n_r > 2m (raypair_solver.tpp:381-388).  Elsewhere the reference reads what
+ *    an earlier pair of the same thread left in its buffers; those elements
+ *    are left out;
+ *  - with n_off_diag >= 2 the global rate equations (cells coupled through the
+ *    band elements) are solved with one dense LU with partial pivoting over
+ *    all N V unknowns in place of SparseLU<COLAMD>.
  */
 #include "oracle.h"
 
@@ -63,7 +72,13 @@ struct orc_state {
     double  *pop, *pop_tot, *prev1, *prev2, *prev3;       /* [N*V], [N] */
     double  *emissivity, *opacity;                        /* [N*L] */
     double  *J;                                           /* [N*F] */
-    double  *Ls;                                          /* [N*L] */
+    double  *Ls;                                          /* [N*L] element nr = p */
+    /* Lambda* elements with nr != p (n_off_diag >= 2), per (p,k) in the order
+     * Lambda::add_element first met them: band_n[p*L+k] entries at
+     * (p*L+k)*band_cap */
+    int64_t  band_cap;
+    int64_t *band_n, *band_nr;
+    double  *band_Ls;
     double  *Jlin, *Jeff, *Jdif;                          /* [N*L] */
     double  *u;                                           /* [R/2][N*F] with scattering, else NULL */
     double   fnc, rcm, rcmax;
@@ -158,6 +173,13 @@ orc_state *orc_create(const mgr_model_view *m)
     s->Jlin     = (double *)xcalloc(s->N * s->L, sizeof(double));
     s->Jeff     = (double *)xcalloc(s->N * s->L, sizeof(double));
     s->Jdif     = (double *)xcalloc(s->N * s->L, sizeof(double));
+    if (m->n_off_diag >= 2) {
+        /* at most two elements per band and ray pair (raypair_lambda.tpp:129-146) */
+        s->band_cap = (s->R / 2) * 2 * (m->n_off_diag - 1);
+        s->band_n   = (int64_t *)xcalloc(s->N * s->L, sizeof(int64_t));
+        s->band_nr  = (int64_t *)xcalloc(s->N * s->L * s->band_cap, sizeof(int64_t));
+        s->band_Ls  = (double *)xcalloc(s->N * s->L * s->band_cap, sizeof(double));
+    }
     return s;
 }
 
@@ -169,7 +191,7 @@ void orc_destroy(orc_state *s)
     free(s->corr_l); free(s->corr_k); free(s->corr_z); free(s->I_bdy); free(s->pop);
     free(s->prev1); free(s->prev2); free(s->prev3); free(s->pop_tot); free(s->emissivity);
     free(s->opacity); free(s->J); free(s->Ls); free(s->Jlin); free(s->Jeff); free(s->Jdif);
-    free(s->u);
+    free(s->u); free(s->band_n); free(s->band_nr); free(s->band_Ls);
     free(s);
 }
 
@@ -685,16 +707,17 @@ void orc_raypair_solve(int64_t ndep, int64_t n_ar, int64_t first, int64_t last,
 /* ---- Radiation field ---------------------------------------------------- */
 
 typedef struct {
-    int64_t cap;
+    int64_t cap, nband;   /* nband: rows of L_upper / L_lower */
     int64_t *cell_r, *cell_ar; double *shift_r, *shift_ar, *dZ_r, *dZ_ar;
     int64_t *lnotch_r, *lnotch_ar;
     double *term1, *chi, *frs, *dtau, *Su, *L_diag, *L_upper, *L_lower;
     int64_t *nrs;
 } o_pair_buf;
 
-static void o_buf_alloc(o_pair_buf *b, int64_t cap)
+static void o_buf_alloc(o_pair_buf *b, int64_t cap, int64_t nband)
 {
     b->cap = cap;
+    b->nband = nband > 1 ? nband : 1;
     b->cell_r = (int64_t *)xcalloc(cap, 8); b->cell_ar = (int64_t *)xcalloc(cap, 8);
     b->shift_r = (double *)xcalloc(cap, 8); b->shift_ar = (double *)xcalloc(cap, 8);
     b->dZ_r = (double *)xcalloc(cap, 8);    b->dZ_ar = (double *)xcalloc(cap, 8);
@@ -703,7 +726,7 @@ static void o_buf_alloc(o_pair_buf *b, int64_t cap)
     b->term1 = (double *)xcalloc(d, 8); b->chi = (double *)xcalloc(d, 8);
     b->frs = (double *)xcalloc(d, 8);   b->dtau = (double *)xcalloc(d, 8);
     b->Su = (double *)xcalloc(d, 8);    b->L_diag = (double *)xcalloc(d, 8);
-    b->L_upper = (double *)xcalloc(d, 8); b->L_lower = (double *)xcalloc(d, 8);
+    b->L_upper = (double *)xcalloc(b->nband * d, 8); b->L_lower = (double *)xcalloc(b->nband * d, 8);
     b->nrs = (int64_t *)xcalloc(d, 8);
 }
 static void o_buf_free(o_pair_buf *b)
@@ -779,9 +802,9 @@ static void o_setup(const orc_state *s, int64_t R, int64_t origin, int64_t f,
 }
 
 /* Simulation::setup followed by RayPair::solve with the model's n_off_diag
- * (0, or 1: the back-substitution branch, raypair_solver.tpp:336-388, whose
- * first band update_Lambda never reads).  Side "ar" is the longer ray.
- * Returns u at the origin. */
+ * (0; >= 1: the back-substitution branch, raypair_solver.tpp:336-388, which
+ * also fills the bands).  Side "ar" is the longer ray.  Returns u at the
+ * origin. */
 static double o_setup_and_solve(const orc_state *s, int64_t R, int64_t origin, int64_t f,
                                 int64_t n_ar, const int64_t *c_ar, const double *s_ar,
                                 const double *z_ar, int64_t *ln_ar,
@@ -862,7 +885,7 @@ int orc_compute_image(orc_state *s, int64_t r, double *I_m, double *I_p)
     {
         o_pair_buf b;
         int64_t cap = 256;
-        o_buf_alloc(&b, cap);
+        o_buf_alloc(&b, cap, 1);
         #pragma omp for schedule(static)
         for (int64_t o = 0; o < s->N; o++) {
             const int64_t ar = s->antipod[r];
@@ -873,7 +896,7 @@ int orc_compute_image(orc_state *s, int64_t r, double *I_m, double *I_p)
                 n_ar = orc_trace_ray_rest(s, o, ar, dshift_max, b.cap, NULL, b.cell_ar, b.shift_ar, b.dZ_ar);
                 if (n_r > b.cap || n_ar > b.cap) {
                     cap = 2 * (n_r > n_ar ? n_r : n_ar);
-                    o_buf_free(&b); o_buf_alloc(&b, cap);
+                    o_buf_free(&b); o_buf_alloc(&b, cap, 1);
                     continue;
                 }
                 break;
@@ -928,9 +951,6 @@ void orc_get_u(const orc_state *s, double *u)
 
 int orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, int64_t o_end)
 {
-    /* n_off_diag >= 2 adds off-diagonal elements (raypair_lambda.tpp:129-146)
-     * that are not restated here */
-    if (s->m.n_off_diag > 1) return -1;
     if (o_begin < 0) o_begin = 0;
     if (o_end > s->N || o_end < 0) o_end = s->N;
     return o_radiation_field(s, nthreads, NULL, o_begin, o_end);
@@ -938,8 +958,47 @@ int orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, int
 
 int orc_compute_radiation_field_list(orc_state *s, int nthreads, const int64_t *origins, int64_t n)
 {
-    if (s->m.n_off_diag > 1) return -1;
     return o_radiation_field(s, nthreads, origins, 0, n);
+}
+
+/* Lambda::add_element (lambda.tpp:130-143): an element of (p, k) from cell
+ * nr adds to the element of the same cell if there is one, else it is
+ * appended.  The element nr = p is held in Ls. */
+static void o_add_element(orc_state *s, int64_t p, int64_t k, int64_t nr, double Lel)
+{
+    if (nr == p) { s->Ls[p*s->L + k] += Lel; return; }
+    const int64_t pk = p*s->L + k;
+    int64_t *nrs = s->band_nr + pk*s->band_cap;
+    double *Ls = s->band_Ls + pk*s->band_cap;
+    for (int64_t e = 0; e < s->band_n[pk]; e++)
+        if (nrs[e] == nr) { Ls[e] += Lel; return; }
+    nrs[s->band_n[pk]] = nr;
+    Ls[s->band_n[pk]] = Lel;
+    s->band_n[pk]++;
+}
+
+/* RayPair::update_Lambda bands m = 1 .. n_off_diag-1 (raypair_lambda.tpp:129-146,
+ * with get_L_lower / get_L_upper, raypair_lambda.tpp:44-84) for origin o and
+ * bin f of transition k; the pair was solved with the longer ray (nA points)
+ * on the low side.  Only elements this solve wrote are added (see the header). */
+static void o_update_bands(orc_state *s, int64_t o, int64_t k, double factor, double freq_line,
+                           int64_t nA, int64_t nB, const o_pair_buf *b)
+{
+    const int64_t ndep = nA + nB + 1;
+    for (int64_t m = 1; m < s->m.n_off_diag; m++) {
+        if (nA - m >= 0 && nB >= 1) {
+            const int64_t n = nA - m;
+            const double prof = orc_profile(freq_line * orc_profile_width(s, b->nrs[n]), b->frs[n] - freq_line);
+            const double Lel = factor * (b->frs[n] * prof * b->L_lower[m*ndep + n] / b->chi[n]);
+            o_add_element(s, o, k, b->nrs[n], Lel);
+        }
+        if (nA + m < ndep - m && nB > 2*m) {
+            const int64_t n = nA + m;
+            const double prof = orc_profile(freq_line * orc_profile_width(s, b->nrs[n]), b->frs[n] - freq_line);
+            const double Lel = factor * (b->frs[n] * prof * b->L_upper[m*ndep + n] / b->chi[n]);
+            o_add_element(s, o, k, b->nrs[n], Lel);
+        }
+    }
 }
 
 /* list == NULL: origins o_begin..o_end-1; otherwise origins list[o_begin..o_end-1] */
@@ -951,7 +1010,9 @@ static int o_radiation_field(orc_state *s, int nthreads, const int64_t *list, in
         const int64_t p = list ? list[i] : i;
         for (int64_t f = 0; f < F; f++) s->J[p*F + f] = 0.0;    /* initialize_J */
         for (int64_t k = 0; k < L; k++) s->Ls[p*L + k] = 0.0;   /* lambda.clear */
+        if (s->band_n) for (int64_t k = 0; k < L; k++) s->band_n[p*L + k] = 0;
     }
+    const int64_t nband = s->m.n_off_diag;
     int err = 0;
     if (nthreads <= 0) nthreads = orc_nthreads_max();
     for (int64_t r = 0; r < s->R / 2; r++) {
@@ -960,7 +1021,7 @@ static int o_radiation_field(orc_state *s, int nthreads, const int64_t *list, in
         {
             o_pair_buf b;
             int64_t cap = 256;
-            o_buf_alloc(&b, cap);
+            o_buf_alloc(&b, cap, nband);
             int tid = 0, nth = 1;
 #ifdef _OPENMP
             tid = omp_get_thread_num(); nth = omp_get_num_threads();
@@ -976,7 +1037,7 @@ static int o_radiation_field(orc_state *s, int nthreads, const int64_t *list, in
                     n_ar_t = orc_trace_ray(s, o, ar, dshift_max, b.cap, NULL, b.cell_ar, b.shift_ar, b.dZ_ar);
                     if (n_r > b.cap || n_ar_t > b.cap) {
                         cap = 2 * (n_r > n_ar_t ? n_r : n_ar_t);
-                        o_buf_free(&b); o_buf_alloc(&b, cap);
+                        o_buf_free(&b); o_buf_alloc(&b, cap, nband);
                         continue;
                     }
                     break;
@@ -1009,6 +1070,7 @@ static int o_radiation_field(orc_state *s, int nthreads, const int64_t *list, in
                                                             frs - freq_line);
                             const double Lel = factor * (frs * prof * Lo / chio);
                             s->Ls[o*L + k] += Lel;
+                            if (nband >= 2) o_update_bands(s, o, k, factor, freq_line, nA, nB, &b);
                         }
                         s->J[o*F + f] += 2.0 * weight_ang * u;
                         if (s->u) s->u[(R*s->N + o)*F + f] = u;   /* radiation.u[R][ind] */
@@ -1031,7 +1093,8 @@ static int o_radiation_field(orc_state *s, int nthreads, const int64_t *list, in
 
 /* ---- Level populations -------------------------------------------------- */
 
-/* Simulation::compute_Jeff (src/Simulation/sim_lines.tpp:176-220), n_off_diag = 0 */
+/* Simulation::compute_Jeff (src/Simulation/sim_lines.tpp:176-220): every Lambda*
+ * element of (p, k), nr = p first, then the band elements in list order */
 void orc_compute_Jeff(orc_state *s)
 {
     const int64_t F = s->F, L = s->L, Q = s->Q, V = s->V;
@@ -1043,6 +1106,10 @@ void orc_compute_Jeff(orc_state *s)
                 Jlin += s->m.qweights[z] * s->J[p*F + s->nr_line[(p*L + k)*Q + z]];
             double diff = 0.0;
             diff += s->Ls[p*L + k] * s->pop[p*V + s->m.irad[k]];
+            if (s->band_n)
+                for (int64_t e = 0; e < s->band_n[p*L + k]; e++)
+                    diff += s->band_Ls[(p*L + k)*s->band_cap + e]
+                          * s->pop[s->band_nr[(p*L + k)*s->band_cap + e]*V + s->m.irad[k]];
             s->Jlin[p*L + k] = Jlin;
             s->Jeff[p*L + k] = Jlin - HH_OVER_FOUR_PI * diff;
             s->Jdif[p*L + k] = HH_OVER_FOUR_PI * diff;
@@ -1075,10 +1142,104 @@ static int o_lu_solve(int64_t n, double *M, double *y)
     return 0;
 }
 
+/* The same elimination for one large system, rows updated in parallel. */
+static int o_lu_solve_parallel(int64_t n, double *M, double *y)
+{
+    for (int64_t c = 0; c < n; c++) {
+        int64_t piv = c; double best = fabs(M[c*n + c]);
+        for (int64_t r = c+1; r < n; r++) if (fabs(M[r*n + c]) > best) { best = fabs(M[r*n + c]); piv = r; }
+        if (best == 0.0) return -1;
+        if (piv != c) {
+            for (int64_t j = 0; j < n; j++) { const double t = M[c*n+j]; M[c*n+j] = M[piv*n+j]; M[piv*n+j] = t; }
+            const double t = y[c]; y[c] = y[piv]; y[piv] = t;
+        }
+        #pragma omp parallel for schedule(static) if (n - c > 256)
+        for (int64_t r = c+1; r < n; r++) {
+            const double l = M[r*n + c] / M[c*n + c];
+            M[r*n + c] = l;
+            if (l != 0.0)
+                for (int64_t j = c+1; j < n; j++) M[r*n + j] -= l * M[c*n + j];
+            y[r] -= l * y[c];
+        }
+    }
+    for (int64_t r = n-1; r >= 0; r--) {
+        double acc = y[r];
+        for (int64_t j = r+1; j < n; j++) acc -= M[r*n + j] * y[j];
+        y[r] = acc / M[r*n + r];
+    }
+    return 0;
+}
+
+/* Largest N V for which the global rate equations are solved densely. */
+#define O_GLOBAL_MAX 8192
+
+/* update_using_statistical_equilibrium with band elements: the triplets of
+ * lineProducingSpecies.tpp:270-388 over all cells, where an element (p, k, nr)
+ * adds +v at (index(p, j_k), index(nr, i_k)) and -v at (index(nr, i_k),
+ * index(nr, i_k)), v = -Ls opacity(p, k) (lineProducingSpecies.tpp:317-340),
+ * solved as one system (history already shifted). */
+static int o_stateq_global(orc_state *s)
+{
+    const mgr_model_view *m = &s->m;
+    const int64_t V = s->V, L = s->L, NV = s->N * s->V;
+    if (NV > O_GLOBAL_MAX) return -2;
+    double *M = (double *)xcalloc((size_t)NV * NV, sizeof(double));
+    double *y = (double *)xcalloc(NV, sizeof(double));
+    int64_t ncol_max = 1;
+    for (int64_t c = 0; c < m->ncolpar; c++) if (m->colpar_ncol[c] > ncol_max) ncol_max = m->colpar_ncol[c];
+    double *Ce = (double *)xcalloc(ncol_max, sizeof(double));
+    double *Cd = (double *)xcalloc(ncol_max, sizeof(double));
+    for (int64_t p = 0; p < s->N; p++) {
+        const double *pop = s->prev1 + p*V;   /* == population before the solve */
+        const int64_t b = p*V;
+        for (int64_t k = 0; k < L; k++) {
+            const double Jeff = s->Jeff[p*L + k];
+            const double v_IJ = m->A[k] + m->Bs[k] * Jeff;
+            const double v_JI = m->Ba[k] * Jeff;
+            const int64_t I = b + m->irad[k], J = b + m->jrad[k];
+            if (m->jrad[k] != V-1) { M[J*NV + I] += v_IJ; M[J*NV + J] += -v_JI; }
+            if (m->irad[k] != V-1) { M[I*NV + J] += v_JI; M[I*NV + I] += -v_IJ; }
+        }
+        for (int64_t k = 0; k < L; k++) {
+            const double opacity = HH_OVER_FOUR_PI * (pop[m->jrad[k]] * m->Ba[k] - pop[m->irad[k]] * m->Bs[k]);
+            const int64_t J = b + m->jrad[k];
+            const int64_t pk = p*L + k;
+            for (int64_t e = -1; e < s->band_n[pk]; e++) {
+                const int64_t nr = e < 0 ? p : s->band_nr[pk*s->band_cap + e];
+                const double Lel = e < 0 ? s->Ls[pk] : s->band_Ls[pk*s->band_cap + e];
+                const double v_IJ = -Lel * opacity;
+                const int64_t I = nr*V + m->irad[k];
+                if (m->jrad[k] != V-1) M[J*NV + I] += v_IJ;
+                if (m->irad[k] != V-1) M[I*NV + I] += -v_IJ;
+            }
+        }
+        int64_t koff = 0;
+        for (int64_t c = 0; c < m->ncolpar; c++) {
+            double factor;
+            orc_collision_rates(s, c, m->temperature[p], Ce, Cd, &factor);
+            const double abn = m->abundance[p*m->nspecs + m->colpar_num[c]] * factor;
+            for (int64_t k = 0; k < m->colpar_ncol[c]; k++) {
+                const double v_IJ = Cd[k] * abn, v_JI = Ce[k] * abn;
+                const int64_t I = b + m->icol[koff + k], J = b + m->jcol[koff + k];
+                if (m->jcol[koff + k] != V-1) { M[J*NV + I] += v_IJ; M[J*NV + J] += -v_JI; }
+                if (m->icol[koff + k] != V-1) { M[I*NV + J] += v_JI; M[I*NV + I] += -v_IJ; }
+            }
+            koff += m->colpar_ncol[c];
+        }
+        for (int64_t i = 0; i < V; i++) M[(b + V-1)*NV + b + i] += 1.0;
+        y[b + V-1] = s->pop_tot[p];
+    }
+    const int rc = o_lu_solve_parallel(NV, M, y);
+    if (rc == 0) memcpy(s->pop, y, NV * sizeof(double));
+    free(M); free(y); free(Ce); free(Cd);
+    return rc;
+}
+
 /* LineProducingSpecies::update_using_statistical_equilibrium
  * (lineProducingSpecies.tpp:249-495): history shift, per-cell assembly of the
  * same triplets (transpose convention, last row = conservation), one dense LU
- * per cell in place of the global SparseLU (block diagonal for n_off_diag = 0). */
+ * per cell in place of the global SparseLU (block diagonal without band
+ * elements); with band elements one global solve (o_stateq_global). */
 int orc_update_using_statistical_equilibrium(orc_state *s)
 {
     const mgr_model_view *m = &s->m;
@@ -1086,6 +1247,7 @@ int orc_update_using_statistical_equilibrium(orc_state *s)
     memcpy(s->prev3, s->prev2, NV * sizeof(double));
     memcpy(s->prev2, s->prev1, NV * sizeof(double));
     memcpy(s->prev1, s->pop,   NV * sizeof(double));
+    if (s->band_n) return o_stateq_global(s);
     int err = 0;
     int64_t ncol_max = 1;
     for (int64_t c = 0; c < m->ncolpar; c++) if (m->colpar_ncol[c] > ncol_max) ncol_max = m->colpar_ncol[c];
@@ -1237,6 +1399,15 @@ void orc_get_boundary_intensities(const orc_state *s, double *I_bdy)
 }
 void orc_get_J(const orc_state *s, double *J) { memcpy(J, s->J, s->N * s->F * sizeof(double)); }
 void orc_get_lambda(const orc_state *s, double *Ls) { memcpy(Ls, s->Ls, s->N * s->L * sizeof(double)); }
+int64_t orc_lambda_band_capacity(const orc_state *s) { return s->band_cap; }
+void orc_get_lambda_band(const orc_state *s, int64_t *count, int64_t *nr, double *Ls)
+{
+    if (!s->band_n) return;
+    const size_t n = s->N * s->L;
+    if (count) memcpy(count, s->band_n, n * sizeof(int64_t));
+    if (nr) memcpy(nr, s->band_nr, n * s->band_cap * sizeof(int64_t));
+    if (Ls) memcpy(Ls, s->band_Ls, n * s->band_cap * sizeof(double));
+}
 void orc_get_Jeff(const orc_state *s, double *Jlin, double *Jeff, double *Jdif)
 {
     const size_t n = s->N * s->L * sizeof(double);

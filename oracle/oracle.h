@@ -83,7 +83,8 @@ int  orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, in
 int  orc_compute_radiation_field_list(orc_state *s, int nthreads, const int64_t *origins, int64_t n);
 /* src/Simulation/sim_lines.tpp:176-220 */
 void orc_compute_Jeff(orc_state *s);
-/* lineProducingSpecies.tpp:249-495 (per-cell dense LU, see oracle.c) */
+/* lineProducingSpecies.tpp:249-495 (per-cell dense LU; with band elements one
+ * dense LU over all N V unknowns, at most 8192; see oracle.c) */
 int  orc_update_using_statistical_equilibrium(orc_state *s);
 /* lineProducingSpecies.tpp:120-171 */
 void orc_check_for_convergence(orc_state *s, double pop_prec);
@@ -97,7 +98,11 @@ int64_t orc_compute_level_populations(orc_state *s, int use_Ng, int64_t max_nite
 void orc_get_frequencies(const orc_state *s, double *nu, int64_t *nr_line);
 void orc_get_boundary_intensities(const orc_state *s, double *I_bdy /* [nboundary*F] for R=0 */);
 void orc_get_J(const orc_state *s, double *J);
-void orc_get_lambda(const orc_state *s, double *Ls);
+void orc_get_lambda(const orc_state *s, double *Ls);   /* element nr = p of every (p, k) */
+/* n_off_diag >= 2: the other elements of (p, k), count[p*L+k] of them at
+ * (p*L+k)*capacity in nr / Ls (Lambda::add_element order, lambda.tpp:130-143) */
+int64_t orc_lambda_band_capacity(const orc_state *s);
+void orc_get_lambda_band(const orc_state *s, int64_t *count, int64_t *nr, double *Ls);
 void orc_get_Jeff(const orc_state *s, double *Jlin, double *Jeff, double *Jdif);
 void orc_get_populations(const orc_state *s, double *pop);
 void orc_set_populations(orc_state *s, const double *pop);

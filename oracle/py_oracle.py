@@ -88,6 +88,9 @@ def lib():
             getattr(L, name).argtypes = [vp, vp]
         L.orc_get_frequencies.argtypes = [vp, vp, vp]
         L.orc_get_Jeff.argtypes = [vp, vp, vp, vp]
+        L.orc_lambda_band_capacity.argtypes = [vp]
+        L.orc_lambda_band_capacity.restype = C.c_int64
+        L.orc_get_lambda_band.argtypes = [vp, vp, vp, vp]
         L.orc_set_scattering.argtypes = [vp, C.c_int]
         L.orc_get_u.argtypes = [vp, vp]
         L.orc_get_population_history.argtypes = [vp, vp, vp, vp]
@@ -256,6 +259,23 @@ class Oracle:
         a = np.zeros(self.N * self.L)
         lib().orc_get_lambda(self._s, _p(a))
         return a.reshape(self.N, self.L)
+
+    def lambda_band(self):
+        """Lambda* elements with nr != p (n_off_diag >= 2): a list per (p, k) of
+        (nr, Ls) in Lambda::add_element order."""
+        cap = lib().orc_lambda_band_capacity(self._s)
+        out = [[[] for _ in range(self.L)] for _ in range(self.N)]
+        if cap == 0:
+            return out
+        cnt = np.zeros(self.N * self.L, dtype=np.int64)
+        nr = np.zeros(self.N * self.L * cap, dtype=np.int64)
+        Ls = np.zeros(self.N * self.L * cap)
+        lib().orc_get_lambda_band(self._s, _p(cnt), _p(nr), _p(Ls))
+        for p in range(self.N):
+            for k in range(self.L):
+                pk = p * self.L + k
+                out[p][k] = [(int(nr[pk * cap + e]), float(Ls[pk * cap + e])) for e in range(cnt[pk])]
+        return out
 
     def Jeff(self):
         a, b, c = (np.zeros(self.N * self.L) for _ in range(3))
