@@ -266,6 +266,12 @@ int mgr_get_lambda(mgr_ctx *ctx, double *Ls);
  * the same cell add (Lambda::add_element, lambda.tpp:130-143).              */
 int mgr_lambda_band_slots(mgr_ctx *ctx, int64_t *slots);
 int mgr_get_lambda_band(mgr_ctx *ctx, int32_t *cells, double *Ls);
+/* With band elements the rate equations (lineProducingSpecies.tpp:249-495,
+ * one SparseLU over all cells in the reference) are solved by block-Jacobi
+ * sweeps.  Of the last solve: the sweeps that moved a population (at most 16)
+ * and, per sweep, the largest move over the cell's total population
+ * (changes [16], may be NULL).  No counterpart in the reference.          */
+int mgr_band_sweeps(mgr_ctx *ctx, int *sweeps, double *changes);
 /* Jlin, Jeff, Jdif [ncells*nrad]; any pointer may be NULL.                    */
 int mgr_get_Jeff(mgr_ctx *ctx, double *Jlin, double *Jeff, double *Jdif);
 /* Frequencies::nu [ncells*nfreqs] and LineProducingSpecies::nr_line

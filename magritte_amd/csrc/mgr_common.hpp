@@ -233,6 +233,7 @@ struct mgr_ctx {
     int32_t *band_piv = nullptr;    // [N*V]
     double  *band_x = nullptr;      // [N*V]
     int32_t *band_flags = nullptr;  // [BAND_SWEEPS + 1]
+    unsigned long long *band_change = nullptr;   // [BAND_SWEEPS] largest move of each sweep / n_tot (bits)
     // Radiation::u with scattering: each ray pair's u at the origin,
     // [R/2][N*F] at (r N + p) F + f; null without scattering
     double  *u_store = nullptr;

@@ -594,6 +594,7 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
         if ((rc = dalloc(ctx, &ctx->band_piv, N * V))) return rc;
         if ((rc = dalloc(ctx, &ctx->band_x, N * V))) return rc;
         if ((rc = dalloc(ctx, &ctx->band_flags, BAND_SWEEPS + 1))) return rc;
+        if ((rc = dalloc(ctx, &ctx->band_change, BAND_SWEEPS))) return rc;
         MGR_HIP(hipMemset(ctx->band_Ls, 0, N * S * L * sizeof(double)));
     }
     if ((rc = dalloc(ctx, &ctx->err_flag, 1))) return rc;
@@ -652,6 +653,7 @@ int free_model(mgr_ctx *ctx)
     dfree(ctx->u_store);
     dfree(ctx->band_cell); dfree(ctx->band_part); dfree(ctx->band_Ls); dfree(ctx->band_rev_off);
     dfree(ctx->band_rev); dfree(ctx->band_lu); dfree(ctx->band_piv); dfree(ctx->band_x); dfree(ctx->band_flags);
+    dfree(ctx->band_change);
     free_image(ctx);
     if (ctx->h_reduced) (void)hipHostFree(ctx->h_reduced);
     if (ctx->h_m_active) (void)hipHostFree(ctx->h_m_active);
@@ -909,6 +911,25 @@ int mgr_lambda_band_slots(mgr_ctx *ctx, int64_t *slots)
 {
     if (!ctx || !slots) return MGR_ERR_ARGUMENT;
     *slots = ctx->band_S;
+    return MGR_OK;
+}
+
+int mgr_band_sweeps(mgr_ctx *ctx, int *sweeps, double *changes)
+{
+    if (!ctx || !sweeps) return MGR_ERR_ARGUMENT;
+    if (ctx->nband <= 0) return ctx->fail(MGR_ERR_STATE, "the model has no Lambda* bands (n_off_diag < 2)");
+    MGR_HIP(hipSetDevice(ctx->device));
+    MGR_HIP(hipStreamSynchronize(ctx->stream));
+    unsigned long long bits[BAND_SWEEPS];
+    MGR_HIP(hipMemcpy(bits, ctx->band_change, sizeof(bits), hipMemcpyDeviceToHost));
+    int n = 0;
+    for (int t = 0; t < BAND_SWEEPS; t++) {
+        double v;
+        std::memcpy(&v, &bits[t], sizeof(v));
+        if (changes) changes[t] = v;
+        if (bits[t]) n = t + 1;
+    }
+    *sweeps = n;
     return MGR_OK;
 }
 

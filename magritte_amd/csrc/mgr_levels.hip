@@ -236,10 +236,13 @@ __global__ void k_stateq_generic(StateqArgs a)
 // block diagonal D (each cell's own triplets and the second kind of every
 // element naming it) is factorised once, then each sweep solves
 // D x' = y - O x for all cells from the last iterate x.  Sweeps stop when no
-// population moves by more than BAND_TOL relative; not converging in
-// BAND_SWEEPS sweeps is an error.
+// population moves by more than BAND_TOL of itself or BAND_TOL_TOT of its
+// cell's total (levels far below the total carry the rounding of the large
+// ones and settle only to that); not converging in BAND_SWEEPS sweeps is an
+// error.
 constexpr int32_t ERR_BAND = 32;
 constexpr double BAND_TOL = 1.0e-14;
+constexpr double BAND_TOL_TOT = 1.0e-15;
 
 struct BandArgs {
     int64_t S;
@@ -366,22 +369,40 @@ __global__ void k_band_sweep(StateqArgs a, BandArgs b, int t)
     band_sweep_cell<VV>(a, b, p, a.V, M, y);
 }
 
-// Sweep t's iterate becomes the populations; a move above BAND_TOL asks for
-// sweep t + 1.
-__global__ void k_band_commit(int64_t NV, double *__restrict__ pop, const double *__restrict__ x,
-                              int32_t *__restrict__ flags, int t)
+// Sweep t's iterate becomes the populations; a move above the tolerance
+// asks for sweep t + 1.  change[t] keeps the sweep's largest move over the
+// cell's total (as the bits of a non-negative double).
+__global__ void k_band_commit(int64_t NV, int V, double *__restrict__ pop, const double *__restrict__ x,
+                              const double *__restrict__ pop_tot, int32_t *__restrict__ flags,
+                              unsigned long long *__restrict__ change, int t)
 {
     if (flags[t] == 0) return;
+    __shared__ double sh[256];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= NV) return;
-    const double xn = x[i];
-    if (!(fabs(xn - pop[i]) <= BAND_TOL * fabs(xn))) atomicOr(&flags[t + 1], 1);
-    pop[i] = xn;
+    double d = 0.0;
+    if (i < NV) {
+        const double xn = x[i];
+        const double mv = fabs(xn - pop[i]);
+        const double tot = pop_tot[i / V];
+        d = mv / tot;
+        if (!(mv <= BAND_TOL * fabs(xn) || mv <= BAND_TOL_TOT * tot)) atomicOr(&flags[t + 1], 1);
+        pop[i] = xn;
+    }
+    sh[threadIdx.x] = d;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && !(sh[0] <= 0.0)) atomicMax(&change[t], (unsigned long long)__double_as_longlong(sh[0]));
 }
 
-__global__ void k_band_flags(int32_t *flags, int n)
+__global__ void k_band_flags(int32_t *flags, unsigned long long *change, int n)
 {
-    for (int i = threadIdx.x; i <= n; i += blockDim.x) flags[i] = (i == 0);
+    for (int i = threadIdx.x; i <= n; i += blockDim.x) {
+        flags[i] = (i == 0);
+        if (i < n) change[i] = 0ull;
+    }
 }
 
 __global__ void k_band_check(const int32_t *flags, int n, int32_t *err_flag)
@@ -625,14 +646,15 @@ static int launch_stateq_band(mgr_ctx *ctx, const StateqArgs &a)
     bb.flags = ctx->band_flags;
     const dim3 g(grid_for(ctx->N, 128)), b(128);
     const int64_t NV = ctx->N * ctx->V;
-    hipLaunchKernelGGL(k_band_flags, dim3(1), dim3(64), 0, ctx->stream, bb.flags, BAND_SWEEPS);
+    hipLaunchKernelGGL(k_band_flags, dim3(1), dim3(64), 0, ctx->stream, bb.flags, ctx->band_change, BAND_SWEEPS);
     MGR_LAUNCH_CHECK();
 #define MGR_BAND_V(VV)                                                                                       \
     hipLaunchKernelGGL(k_band_factor<VV>, g, b, 0, ctx->stream, a, bb);                                       \
     for (int t = 0; t < BAND_SWEEPS; t++) {                                                                   \
         hipLaunchKernelGGL(k_band_sweep<VV>, g, b, 0, ctx->stream, a, bb, t);                                 \
-        hipLaunchKernelGGL(k_band_commit, dim3(grid_for(NV, 256)), dim3(256), 0, ctx->stream, NV, ctx->pop,   \
-                           (const double *)bb.x, bb.flags, t);                                                \
+        hipLaunchKernelGGL(k_band_commit, dim3(grid_for(NV, 256)), dim3(256), 0, ctx->stream, NV, (int)ctx->V, \
+                           ctx->pop, (const double *)bb.x, (const double *)ctx->pop_tot, bb.flags,            \
+                           ctx->band_change, t);                                                              \
     }
     switch (ctx->V) {
         case 2: MGR_BAND_V(2) break;

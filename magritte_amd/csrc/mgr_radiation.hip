@@ -1621,6 +1621,7 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
     ctx->last_rf_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    MGR_HIP(hipStreamSynchronize(ctx->stream));   // the band elements and Lambda* sums too
     int32_t errv = 0;
     MGR_HIP(hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (errv) return ctx->fail(MGR_ERR_TRACE, "a ray pair ended off the boundary or an interior origin has an empty ray");

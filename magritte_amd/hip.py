@@ -48,6 +48,7 @@ SIGNATURES = {
     "mgr_get_lambda": ([_vp, _vp], _i),
     "mgr_lambda_band_slots": ([_vp, _vp], _i),
     "mgr_get_lambda_band": ([_vp, _vp, _vp], _i),
+    "mgr_band_sweeps": ([_vp, _vp, _vp], _i),
     "mgr_get_Jeff": ([_vp, _vp, _vp, _vp], _i),
     "mgr_get_frequencies": ([_vp, _vp, _vp], _i),
     "mgr_get_boundary_intensities": ([_vp, _vp], _i),
@@ -233,6 +234,14 @@ class Context:
         Ls = np.zeros(self.N * S * self.L)
         self._check(self.lib.mgr_get_lambda_band(self.h, _p(cells), _p(Ls)), "get_lambda_band")
         return cells.reshape(self.N, S), Ls.reshape(self.N, S, self.L)
+
+    def band_sweeps(self):
+        """Block-Jacobi sweeps of the last rate-equation solve with bands and
+        the largest population move of each (over the cell's total)."""
+        n = C.c_int()
+        ch = np.zeros(16)
+        self._check(self.lib.mgr_band_sweeps(self.h, C.byref(n), _p(ch)), "band_sweeps")
+        return n.value, ch[:n.value]
 
     def lambda_band(self):
         """Band elements merged per (p, k) and emitting cell, as

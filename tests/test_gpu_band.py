@@ -113,6 +113,11 @@ def test_rate_equations_with_bands_match_oracle(name, n_off_diag):
     assert err < 1e-10
     h1, _, _ = ctx.population_history()
     assert np.array_equal(h1, pop0)
+    # block Jacobi: a few sweeps, the last one moving nothing above rounding
+    n, changes = ctx.band_sweeps()
+    print(f"[band] {n} block-Jacobi sweeps, moves {changes}", flush=True)
+    assert 2 <= n <= 10
+    assert changes[-1] < 1e-14
     tot = m.abundance[:, m.linedata.num]
     assert rel_err(p_g.sum(axis=1), tot) < 1e-12
 

@@ -253,6 +253,38 @@ def test_checkpoint_resume_equals_uninterrupted_run(tmp_path, kind):
     np.testing.assert_array_equal(np.array(resumed.error_max), np.array(whole.error_max)[5:])
 
 
+@pytest.mark.gpu
+def test_lambda_lists_with_bands_match_oracle():
+    """With n_off_diag = 3 the Simulation's Lambda lists hold, per (p, k), the
+    element nr = p first and then the band elements in the order
+    RayPair::update_Lambda first adds them (Lambda::add_element,
+    lambda.tpp:130-143), as the oracle's lists; the band values are FP64."""
+    from oracle.py_oracle import Oracle
+    m = synthetic.envelope_model(n=5, nside=1, nquads=11, nlev=4, seed=3)
+    m.n_off_diag = 3
+    print("\n[host] Lambda lists with bands", flush=True)
+    orc = Oracle(m).setup()
+    orc.compute_radiation_field()
+    sim = to_simulation(m)
+    assert sim.compute_spectral_discretisation() == 0
+    assert sim.compute_boundary_intensities() == 0
+    assert sim.compute_LTE_level_populations() == 0
+    assert sim.compute_radiation_field() == 0
+    lam = sim.lines.lineProducingSpecies[0].Lambda
+    band_o, diag_o = orc.lambda_band(), orc.lambda_diag()
+    n_band = 0
+    for p in range(m.ncells):
+        for k in range(m.nrad):
+            nr, Ls = list(lam.nr[p][k]), list(lam.Ls[p][k])
+            assert nr[0] == p and nr[1:] == [c for c, _ in band_o[p][k]], (p, k)
+            assert Ls[0] == pytest.approx(diag_o[p, k], rel=1e-5)
+            for got, (_, want) in zip(Ls[1:], band_o[p][k]):
+                assert got == pytest.approx(want, rel=1e-10)
+            assert lam.size[k + m.nrad * p] == len(nr)
+            n_band += len(nr) - 1
+    assert n_band > 0
+
+
 def _read_text_array(path):
     return np.loadtxt(path, ndmin=2)
 
