@@ -50,6 +50,37 @@ extern "C" {
 /* ------------------------------------------------------------------------ */
 /* Model view: what Model::read loads (src/Model/model.cpp:54-86)            */
 /* ------------------------------------------------------------------------ */
+
+/* One further line-producing species, Lines/LineProducingSpecies_<l> for
+ * l >= 1 (LineProducingSpecies::read, lineProducingSpecies.cpp:21-134): the
+ * same fields, with the same meaning, as species 0's in mgr_model_view. */
+typedef struct mgr_species_view {
+    int64_t        num;
+    double         inverse_mass;
+    int64_t        nlev;
+    int64_t        nrad;
+    const double  *energy;
+    const double  *weight;
+    const int64_t *irad;
+    const int64_t *jrad;
+    const double  *A;
+    const double  *Bs;
+    const double  *Ba;
+    const double  *frequency;
+    int64_t        ncolpar;
+    const int64_t *colpar_num;
+    const char    *colpar_orth_or_para;
+    const int64_t *colpar_ncol;
+    const int64_t *colpar_ntmp;
+    const int64_t *icol;
+    const int64_t *jcol;
+    const double  *tmp;
+    const double  *Cd;
+    const double  *Ce;
+    const double  *roots;        /* [nquads] this species' quadrature            */
+    const double  *qweights;     /* [nquads]                                      */
+} mgr_species_view;
+
 typedef struct mgr_model_view {
     /* Parameters (src/Model/parameters.hpp:18-96) */
     int64_t ncells;
@@ -111,6 +142,15 @@ typedef struct mgr_model_view {
     /* Lines/LineProducingSpecies_0/Quadrature (Quadrature/quadrature.cpp:19-40) */
     const double  *roots;        /* [nquads]                                      */
     const double  *qweights;     /* [nquads]                                      */
+
+    /* Parameters::nlspecs (parameters.hpp:18-96): the number of line-producing
+     * species; species 0 is described above and species 1 .. nlspecs-1 by
+     * more_species[0 .. nlspecs-2].  0 reads as 1.  The species' lines are
+     * merged into one sorted line list (Lines::setup, lines.cpp:43-87); level
+     * populations are then held species after species, each in its own
+     * layout index(p, i) = i + p*nlev (lineProducingSpecies.tpp:24-30).      */
+    int64_t                  nlspecs;
+    const mgr_species_view  *more_species;
 } mgr_model_view;
 
 /* Convergence statistics of one level-population update

@@ -14,7 +14,7 @@ Fixed choices (not given by the workload description) are recorded in
 """
 import numpy as np
 
-from ..model import Model
+from ..model import LineSpecies, Model
 from . import healpix
 from .linedata import CC, linear_rotor, split_rotor, two_level
 from .quadrature import gaussian_linspace
@@ -122,3 +122,21 @@ def blended_model(n=8, seed=21, **kw):
 def stress_model(n=64, seed=4321, **kw):
     kw.setdefault("n0", 1.0e17)
     return envelope_model(n=n, seed=seed, jump_fraction=0.2, **kw)
+
+
+def two_species_model(n=6, seed=5, second_nu=700.0e9, second_mass=28.0, second_abundance=3.0e-5,
+                      second_A=2.0e-6, **kw):
+    """The envelope with a 4-level rotor (lines at 115, 230 and 346 GHz) and a
+    second line-producing species: a two-level species at second_nu with its
+    own abundance (species index 3, collision partner H2), so
+    Lines/LineProducingSpecies_0 and _1."""
+    kw.setdefault("nlev", 4)
+    m = envelope_model(n=n, seed=seed, **kw)
+    extra = (second_abundance * m.abundance[:, 2])[:, None]
+    m.abundance = np.concatenate([m.abundance, extra], axis=1)
+    m.species = list(m.species) + ["X"]
+    ld = two_level(nu=second_nu, A=second_A, mass=second_mass, num=3, partner_num=2,
+                   Cd_cgs=3e-11, T_table=(10.0, 100.0, 1000.0, 3000.0))
+    m.more_species = [LineSpecies(ld, m.roots.copy(), m.qweights.copy())]
+    m.meta["species"] = "rotor(4) + two-level"
+    return m

@@ -31,7 +31,10 @@
  *    are left out;
  *  - with n_off_diag >= 2 the global rate equations (cells coupled through the
  *    band elements) are solved with one dense LU with partial pivoting over
- *    all N V unknowns in place of SparseLU<COLAMD>.
+ *    all N V unknowns in place of SparseLU<COLAMD>;
+ *  - several line-producing species: each species' rate equations are solved
+ *    per cell as above (the reference's global matrix is block diagonal over
+ *    species as well); bands are restated for one species only.
  */
 #include "oracle.h"
 
@@ -58,9 +61,20 @@ static const double TWO_HH_OVER_CC_SQUARED    = 2.0 * O_HH / O_C_SQUARED;
 static const double HH_OVER_KB                = O_HH / O_KB;
 static const double HH_OVER_FOUR_PI           = O_HH / O_FOUR_PI;
 
+/* One line-producing species (Lines/LineProducingSpecies_<l>): its fields,
+ * level and line counts, and where its populations (from N*voff, index
+ * p*V + i) and its lines (global line loff + k) start. */
+typedef struct {
+    mgr_species_view v;
+    int64_t V, L, voff, loff;
+    double  fnc, rcm, rcmax;
+} o_spec;
+
 struct orc_state {
     mgr_model_view m;
-    int64_t N, R, NB, F, L, V, Q;
+    int64_t N, R, NB, F, L, V, Q;   /* L, V: lines and levels summed over species */
+    int64_t S;                      /* line-producing species */
+    o_spec *sp;
     double  *line;        int64_t *line_index;            /* [L] sorted line list */
     int64_t *cell2boundary_nr; unsigned char *boundary;   /* [N] */
     int64_t *antipod;                                     /* [R] */
@@ -69,7 +83,8 @@ struct orc_state {
     unsigned char *appears; int64_t *corr_l, *corr_k, *corr_z; /* [F] */
     int      spec_set;
     double  *I_bdy;                                       /* [R/2][NB][F] */
-    double  *pop, *pop_tot, *prev1, *prev2, *prev3;       /* [N*V], [N] */
+    double  *pop, *prev1, *prev2, *prev3;                 /* [N*V], species after species */
+    double  *pop_tot;                                     /* [S*N] */
     double  *emissivity, *opacity;                        /* [N*L] */
     double  *J;                                           /* [N*F] */
     double  *Ls;                                          /* [N*L] element nr = p */
@@ -113,17 +128,40 @@ orc_state *orc_create(const mgr_model_view *m)
     s->N  = m->ncells;
     s->R  = m->nrays;
     s->NB = m->nboundary;
-    s->L  = m->nrad;
-    s->V  = m->nlev;
     s->Q  = m->nquads;
-    s->F  = m->nrad * m->nquads;   /* nfreqs = nlines * nquads (one species) */
+    /* species 0 from the model view, species 1 .. from more_species */
+    s->S  = m->nlspecs > 1 ? m->nlspecs : 1;
+    s->sp = (o_spec *)xcalloc(s->S, sizeof(o_spec));
+    {
+        mgr_species_view *v0 = &s->sp[0].v;
+        v0->num = m->num; v0->inverse_mass = m->inverse_mass; v0->nlev = m->nlev; v0->nrad = m->nrad;
+        v0->energy = m->energy; v0->weight = m->weight; v0->irad = m->irad; v0->jrad = m->jrad;
+        v0->A = m->A; v0->Bs = m->Bs; v0->Ba = m->Ba; v0->frequency = m->frequency;
+        v0->ncolpar = m->ncolpar; v0->colpar_num = m->colpar_num;
+        v0->colpar_orth_or_para = m->colpar_orth_or_para; v0->colpar_ncol = m->colpar_ncol;
+        v0->colpar_ntmp = m->colpar_ntmp; v0->icol = m->icol; v0->jcol = m->jcol; v0->tmp = m->tmp;
+        v0->Cd = m->Cd; v0->Ce = m->Ce; v0->roots = m->roots; v0->qweights = m->qweights;
+        for (int64_t l = 1; l < s->S; l++) s->sp[l].v = m->more_species[l-1];
+        s->V = 0; s->L = 0;
+        for (int64_t l = 0; l < s->S; l++) {
+            s->sp[l].V = s->sp[l].v.nlev; s->sp[l].L = s->sp[l].v.nrad;
+            s->sp[l].voff = s->V; s->sp[l].loff = s->L;
+            s->V += s->sp[l].V; s->L += s->sp[l].L;
+        }
+    }
+    s->F  = s->L * s->Q;   /* nfreqs = nlines * nquads */
 
-    /* Lines::setup (src/Model/Lines/lines.cpp:43-87): sorted line list */
+    /* Lines::setup (src/Model/Lines/lines.cpp:43-87): the lines of every
+     * species, species after species, sorted */
     s->line = (double *)xcalloc(s->L, sizeof(double));
     s->line_index = (int64_t *)xcalloc(s->L, sizeof(int64_t));
     {
         vi_pair *tmp = (vi_pair *)xcalloc(s->L, sizeof(vi_pair));
-        for (int64_t k = 0; k < s->L; k++) { tmp[k].v = m->frequency[k]; tmp[k].i = k; }
+        for (int64_t l = 0; l < s->S; l++)
+            for (int64_t k = 0; k < s->sp[l].L; k++) {
+                tmp[s->sp[l].loff + k].v = s->sp[l].v.frequency[k];
+                tmp[s->sp[l].loff + k].i = s->sp[l].loff + k;
+            }
         qsort(tmp, (size_t)s->L, sizeof(vi_pair), vi_cmp);
         for (int64_t k = 0; k < s->L; k++) { s->line[k] = tmp[k].v; s->line_index[k] = tmp[k].i; }
         free(tmp);
@@ -165,7 +203,7 @@ orc_state *orc_create(const mgr_model_view *m)
     s->prev1    = (double *)xcalloc(s->N * s->V, sizeof(double));
     s->prev2    = (double *)xcalloc(s->N * s->V, sizeof(double));
     s->prev3    = (double *)xcalloc(s->N * s->V, sizeof(double));
-    s->pop_tot  = (double *)xcalloc(s->N, sizeof(double));
+    s->pop_tot  = (double *)xcalloc(s->S * s->N, sizeof(double));
     s->emissivity = (double *)xcalloc(s->N * s->L, sizeof(double));
     s->opacity  = (double *)xcalloc(s->N * s->L, sizeof(double));
     s->J        = (double *)xcalloc(s->N * s->F, sizeof(double));
@@ -173,7 +211,7 @@ orc_state *orc_create(const mgr_model_view *m)
     s->Jlin     = (double *)xcalloc(s->N * s->L, sizeof(double));
     s->Jeff     = (double *)xcalloc(s->N * s->L, sizeof(double));
     s->Jdif     = (double *)xcalloc(s->N * s->L, sizeof(double));
-    if (m->n_off_diag >= 2) {
+    if (m->n_off_diag >= 2 && s->S == 1) {
         /* at most two elements per band and ray pair (raypair_lambda.tpp:129-146) */
         s->band_cap = (s->R / 2) * 2 * (m->n_off_diag - 1);
         s->band_n   = (int64_t *)xcalloc(s->N * s->L, sizeof(int64_t));
@@ -191,7 +229,7 @@ void orc_destroy(orc_state *s)
     free(s->corr_l); free(s->corr_k); free(s->corr_z); free(s->I_bdy); free(s->pop);
     free(s->prev1); free(s->prev2); free(s->prev3); free(s->pop_tot); free(s->emissivity);
     free(s->opacity); free(s->J); free(s->Ls); free(s->Jlin); free(s->Jeff); free(s->Jdif);
-    free(s->u); free(s->band_n); free(s->band_nr); free(s->band_Ls);
+    free(s->u); free(s->band_n); free(s->band_nr); free(s->band_Ls); free(s->sp);
     free(s);
 }
 
@@ -215,10 +253,15 @@ double orc_profile(double width, double freq_diff)
 }
 
 /* Thermodynamics::profile_width(inverse_mass, p) (thermodynamics.tpp:62-69) */
+static double o_profile_width(const orc_state *s, double inverse_mass, int64_t p)
+{
+    return sqrt(TWO_KB_OVER_AMU_C_SQUARED * inverse_mass * s->m.temperature[p] + s->m.vturb2[p]);
+}
+
+/* ... of species 0 */
 double orc_profile_width(const orc_state *s, int64_t p)
 {
-    return sqrt(TWO_KB_OVER_AMU_C_SQUARED * s->m.inverse_mass * s->m.temperature[p]
-                + s->m.vturb2[p]);
+    return o_profile_width(s, s->sp[0].v.inverse_mass, p);
 }
 
 /* search (src/Functions/interpolation.tpp:14-54) */
@@ -239,10 +282,9 @@ static long o_search(const double *x, long n, double value)
 /* CollisionPartner::adjust_abundance_for_ortho_or_para and
  * ::interpolate_collision_coefficients (collisionPartner.tpp:15-72).
  * Partner c's tables start at the concatenated offsets. */
-void orc_collision_rates(const orc_state *s, int64_t c, double temperature,
-                         double *Ce_intpld, double *Cd_intpld, double *abundance_factor)
+static void o_collision_rates(const mgr_species_view *m, int64_t c, double temperature,
+                              double *Ce_intpld, double *Cd_intpld, double *abundance_factor)
 {
-    const mgr_model_view *m = &s->m;
     int64_t koff = 0, toff = 0, ctoff = 0;
     for (int64_t q = 0; q < c; q++) {
         koff  += m->colpar_ncol[q];
@@ -278,6 +320,13 @@ void orc_collision_rates(const orc_state *s, int64_t c, double temperature,
     }
 }
 
+/* ... for partner c of species 0 */
+void orc_collision_rates(const orc_state *s, int64_t c, double temperature,
+                         double *Ce_intpld, double *Cd_intpld, double *abundance_factor)
+{
+    o_collision_rates(&s->sp[0].v, c, temperature, Ce_intpld, Cd_intpld, abundance_factor);
+}
+
 /* ---- Model set-up ------------------------------------------------------- */
 
 /* Simulation::compute_spectral_discretisation (src/Simulation/sim_spectrum.tpp:13-196) */
@@ -301,19 +350,24 @@ static int o_spectral_discretisation(orc_state *s, int image, double image_width
         int64_t *inv = (int64_t *)xcalloc(F, sizeof(int64_t));
         #pragma omp for schedule(static)
         for (int64_t p = 0; p < s->N; p++) {
-            const double pw = orc_profile_width(s, p);
-            int64_t index1 = 0;
-            for (int64_t k = 0; k < L; k++) {
-                /* uses the sorted list at the running index (sim_spectrum.tpp:31) */
-                const double freqs_line = s->line[k];
-                const double width = freqs_line * pw;
-                for (int64_t z = 0; z < Q; z++) {
-                    if (image)   /* freqs_line + freqs_line * width * root (sim_spectrum.tpp:227) */
-                        fr[index1].v = freqs_line + freqs_line * image_width * s->m.roots[z];
-                    else
-                        fr[index1].v = freqs_line + width * s->m.roots[z];
-                    fr[index1].i = index1;
-                    index1++;
+            int64_t index0 = 0, index1 = 0;
+            for (int64_t l = 0; l < s->S; l++) {
+                const mgr_species_view *sv = &s->sp[l].v;
+                const double pw = o_profile_width(s, sv->inverse_mass, p);
+                for (int64_t k = 0; k < sv->nrad; k++) {
+                    /* the sorted list at the running line index, with the width
+                     * of species l (sim_spectrum.tpp:23-45) */
+                    const double freqs_line = s->line[index0];
+                    const double width = freqs_line * pw;
+                    for (int64_t z = 0; z < Q; z++) {
+                        if (image)   /* freqs_line + freqs_line * width * root (sim_spectrum.tpp:227) */
+                            fr[index1].v = freqs_line + freqs_line * image_width * sv->roots[z];
+                        else
+                            fr[index1].v = freqs_line + width * sv->roots[z];
+                        fr[index1].i = index1;
+                        index1++;
+                    }
+                    index0++;
                 }
             }
             qsort(fr, (size_t)F, sizeof(vi_pair), vi_cmp);
@@ -335,12 +389,13 @@ static int o_spectral_discretisation(orc_state *s, int image, double image_width
         s->appears[fl] = 0; s->corr_l[fl] = F; s->corr_k[fl] = F; s->corr_z[fl] = F;
     }
     int64_t index2 = 0;
-    for (int64_t k = 0; k < L; k++)
-        for (int64_t z = 0; z < Q; z++) {
-            s->appears[index2] = 1; s->corr_l[index2] = 0;
-            s->corr_k[index2] = k;  s->corr_z[index2] = z;
-            index2++;
-        }
+    for (int64_t l = 0; l < s->S; l++)
+        for (int64_t k = 0; k < s->sp[l].L; k++)
+            for (int64_t z = 0; z < Q; z++) {
+                s->appears[index2] = 1; s->corr_l[index2] = l;
+                s->corr_k[index2] = k;  s->corr_z[index2] = z;
+                index2++;
+            }
     s->spec_set = 1;
     return 0;
 }
@@ -363,28 +418,37 @@ int orc_compute_boundary_intensities(orc_state *s, const double *temperatures)
  * and Lines::set_emissivity_and_opacity (lines.tpp:44-60). */
 void orc_set_emissivity_and_opacity(orc_state *s)
 {
-    const int64_t V = s->V, L = s->L;
-    for (int64_t p = 0; p < s->N; p++)
-        for (int64_t k = 0; k < L; k++) {
-            const int64_t i = s->m.irad[k], j = s->m.jrad[k];
-            s->emissivity[p*L + k] = HH_OVER_FOUR_PI * s->m.A[k] * s->pop[p*V + i];
-            s->opacity[p*L + k] = HH_OVER_FOUR_PI * (s->pop[p*V + j] * s->m.Ba[k]
-                                                     - s->pop[p*V + i] * s->m.Bs[k]);
-        }
+    const int64_t L = s->L;
+    for (int64_t l = 0; l < s->S; l++) {
+        const mgr_species_view *sv = &s->sp[l].v;
+        const int64_t V = s->sp[l].V, loff = s->sp[l].loff;
+        const double *pop = s->pop + s->N * s->sp[l].voff;
+        for (int64_t p = 0; p < s->N; p++)
+            for (int64_t k = 0; k < s->sp[l].L; k++) {
+                const int64_t i = sv->irad[k], j = sv->jrad[k];
+                s->emissivity[p*L + loff + k] = HH_OVER_FOUR_PI * sv->A[k] * pop[p*V + i];
+                s->opacity[p*L + loff + k] = HH_OVER_FOUR_PI * (pop[p*V + j] * sv->Ba[k]
+                                                                - pop[p*V + i] * sv->Bs[k]);
+            }
+    }
 }
 
 /* LineProducingSpecies::update_using_LTE (lineProducingSpecies.tpp:82-115) */
 int orc_compute_LTE_level_populations(orc_state *s)
 {
-    const int64_t V = s->V;
-    for (int64_t p = 0; p < s->N; p++) {
-        s->pop_tot[p] = s->m.abundance[p*s->m.nspecs + s->m.num];
-        double pf = 0.0;
-        for (int64_t i = 0; i < V; i++) {
-            s->pop[p*V + i] = s->m.weight[i] * exp(-s->m.energy[i] / (O_KB * s->m.temperature[p]));
-            pf += s->pop[p*V + i];
+    for (int64_t l = 0; l < s->S; l++) {
+        const mgr_species_view *sv = &s->sp[l].v;
+        const int64_t V = s->sp[l].V;
+        double *pop = s->pop + s->N * s->sp[l].voff, *pop_tot = s->pop_tot + l * s->N;
+        for (int64_t p = 0; p < s->N; p++) {
+            pop_tot[p] = s->m.abundance[p*s->m.nspecs + sv->num];
+            double pf = 0.0;
+            for (int64_t i = 0; i < V; i++) {
+                pop[p*V + i] = sv->weight[i] * exp(-sv->energy[i] / (O_KB * s->m.temperature[p]));
+                pf += pop[p*V + i];
+            }
+            for (int64_t i = 0; i < V; i++) pop[p*V + i] *= pop_tot[p] / pf;
         }
-        for (int64_t i = 0; i < V; i++) s->pop[p*V + i] *= s->pop_tot[p] / pf;
     }
     orc_set_emissivity_and_opacity(s);
     return 0;
@@ -392,12 +456,15 @@ int orc_compute_LTE_level_populations(orc_state *s)
 
 /* ---- Geometry ----------------------------------------------------------- */
 
-/* Simulation::get_dshift_max (src/Simulation/simulation.tpp:15-37), one species */
+/* Simulation::get_dshift_max (src/Simulation/simulation.tpp:15-37): the
+ * smallest over the species */
 double orc_get_dshift_max(const orc_state *s, int64_t o)
 {
     double dshift_max = DBL_MAX;
-    const double nd = s->m.max_width_fraction * orc_profile_width(s, o);
-    if (dshift_max > nd) dshift_max = nd;
+    for (int64_t l = 0; l < s->S; l++) {
+        const double nd = s->m.max_width_fraction * o_profile_width(s, s->sp[l].v.inverse_mass, o);
+        if (dshift_max > nd) dshift_max = nd;
+    }
     return dshift_max;
 }
 
@@ -556,23 +623,32 @@ int64_t orc_get_npoints_on_ray(const orc_state *s, int64_t o, int64_t r, double 
 
 /* ---- Line emissivity and opacity ---------------------------------------- */
 
-/* Simulation::get_eta_and_chi (src/Simulation/simulation.tpp:385-481) with
- * get_line_width (:363-373) for a single species.  lnotch is stateful. */
+/* Simulation::get_line_width (src/Simulation/simulation.tpp:363-373): the
+ * width of sorted line lindex, with the inverse mass of the species that the
+ * frequency back-map names at index line_index[lindex] (a bin index there). */
+static double o_line_width(const orc_state *s, int64_t p, int64_t lindex)
+{
+    const int64_t f = s->line_index[lindex];
+    const int64_t l = s->corr_l[f] < s->S ? s->corr_l[f] : 0;
+    return s->line[lindex] * o_profile_width(s, s->sp[l].v.inverse_mass, p);
+}
+
+/* Simulation::get_eta_and_chi (src/Simulation/simulation.tpp:385-481).
+ * lnotch is stateful. */
 void orc_get_eta_and_chi(const orc_state *s, double freq_scaled, int64_t p,
                          int64_t *lnotch, double *eta, double *chi)
 {
     const int64_t L = s->L;
-    const double pw = orc_profile_width(s, p);
     *eta = 0.0;
     *chi = 1.0E-26;
     const double lower = -11.0, upper = +11.0;
     double freq_diff = freq_scaled - s->line[*lnotch];
-    double width = s->line[*lnotch] * pw;
+    double width = o_line_width(s, p, *lnotch);
     while (freq_diff > upper*width) {
         if (*lnotch >= L-1) break;
         (*lnotch)++;
         freq_diff = freq_scaled - s->line[*lnotch];
-        width = s->line[*lnotch] * pw;
+        width = o_line_width(s, p, *lnotch);
     }
     int64_t lindex = *lnotch;
     while (freq_diff >= lower*width) {
@@ -583,7 +659,7 @@ void orc_get_eta_and_chi(const orc_state *s, double freq_scaled, int64_t p,
         if (lindex >= L-1) break;
         lindex++;
         freq_diff = freq_scaled - s->line[lindex];
-        width = s->line[lindex] * pw;
+        width = o_line_width(s, p, lindex);
     }
 }
 
@@ -951,6 +1027,7 @@ void orc_get_u(const orc_state *s, double *u)
 
 int orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, int64_t o_end)
 {
+    if (s->m.n_off_diag >= 2 && s->S > 1) return -1;   /* bands are restated for one species */
     if (o_begin < 0) o_begin = 0;
     if (o_end > s->N || o_end < 0) o_end = s->N;
     return o_radiation_field(s, nthreads, NULL, o_begin, o_end);
@@ -958,6 +1035,7 @@ int orc_compute_radiation_field(orc_state *s, int nthreads, int64_t o_begin, int
 
 int orc_compute_radiation_field_list(orc_state *s, int nthreads, const int64_t *origins, int64_t n)
 {
+    if (s->m.n_off_diag >= 2 && s->S > 1) return -1;
     return o_radiation_field(s, nthreads, origins, 0, n);
 }
 
@@ -1061,16 +1139,19 @@ static int o_radiation_field(orc_state *s, int nthreads, const int64_t *list, in
                                                            nB, cB, sB, zB, lB,
                                                            &lnotch_at_origin, &b, &Lo, &chio);
                         if (s->appears[f]) {
-                            const int64_t k = s->corr_k[f], z = s->corr_z[f];
-                            const double freq_line = s->m.frequency[k];
-                            const double weight = s->m.qweights[z] * 2.0 * weight_ang;
-                            const double factor = s->m.A[k] * weight;
+                            /* species l, its transition k (raypair_lambda.tpp:103-124) */
+                            const int64_t l = s->corr_l[f], k = s->corr_k[f], z = s->corr_z[f];
+                            const mgr_species_view *sv = &s->sp[l].v;
+                            const int64_t g = s->sp[l].loff + k;
+                            const double freq_line = sv->frequency[k];
+                            const double weight = sv->qweights[z] * 2.0 * weight_ang;
+                            const double factor = sv->A[k] * weight;
                             const double frs = s->nu[o*F + f];
-                            const double prof = orc_profile(freq_line * orc_profile_width(s, o),
+                            const double prof = orc_profile(freq_line * o_profile_width(s, sv->inverse_mass, o),
                                                             frs - freq_line);
                             const double Lel = factor * (frs * prof * Lo / chio);
-                            s->Ls[o*L + k] += Lel;
-                            if (nband >= 2) o_update_bands(s, o, k, factor, freq_line, nA, nB, &b);
+                            s->Ls[o*L + g] += Lel;
+                            if (nband >= 2) o_update_bands(s, o, g, factor, freq_line, nA, nB, &b);
                         }
                         s->J[o*F + f] += 2.0 * weight_ang * u;
                         if (s->u) s->u[(R*s->N + o)*F + f] = u;   /* radiation.u[R][ind] */
@@ -1097,23 +1178,29 @@ static int o_radiation_field(orc_state *s, int nthreads, const int64_t *list, in
  * element of (p, k), nr = p first, then the band elements in list order */
 void orc_compute_Jeff(orc_state *s)
 {
-    const int64_t F = s->F, L = s->L, Q = s->Q, V = s->V;
-    #pragma omp parallel for schedule(static)
-    for (int64_t p = 0; p < s->N; p++)
-        for (int64_t k = 0; k < L; k++) {
-            double Jlin = 0.0;
-            for (int64_t z = 0; z < Q; z++)
-                Jlin += s->m.qweights[z] * s->J[p*F + s->nr_line[(p*L + k)*Q + z]];
-            double diff = 0.0;
-            diff += s->Ls[p*L + k] * s->pop[p*V + s->m.irad[k]];
-            if (s->band_n)
-                for (int64_t e = 0; e < s->band_n[p*L + k]; e++)
-                    diff += s->band_Ls[(p*L + k)*s->band_cap + e]
-                          * s->pop[s->band_nr[(p*L + k)*s->band_cap + e]*V + s->m.irad[k]];
-            s->Jlin[p*L + k] = Jlin;
-            s->Jeff[p*L + k] = Jlin - HH_OVER_FOUR_PI * diff;
-            s->Jdif[p*L + k] = HH_OVER_FOUR_PI * diff;
-        }
+    const int64_t F = s->F, L = s->L, Q = s->Q;
+    for (int64_t l = 0; l < s->S; l++) {
+        const mgr_species_view *sv = &s->sp[l].v;
+        const int64_t V = s->sp[l].V, loff = s->sp[l].loff;
+        const double *pop = s->pop + s->N * s->sp[l].voff;
+        #pragma omp parallel for schedule(static)
+        for (int64_t p = 0; p < s->N; p++)
+            for (int64_t k = 0; k < s->sp[l].L; k++) {
+                const int64_t g = loff + k;
+                double Jlin = 0.0;
+                for (int64_t z = 0; z < Q; z++)
+                    Jlin += sv->qweights[z] * s->J[p*F + s->nr_line[(p*L + g)*Q + z]];
+                double diff = 0.0;
+                diff += s->Ls[p*L + g] * pop[p*V + sv->irad[k]];
+                if (s->band_n)
+                    for (int64_t e = 0; e < s->band_n[p*L + g]; e++)
+                        diff += s->band_Ls[(p*L + g)*s->band_cap + e]
+                              * pop[s->band_nr[(p*L + g)*s->band_cap + e]*V + sv->irad[k]];
+                s->Jlin[p*L + g] = Jlin;
+                s->Jeff[p*L + g] = Jlin - HH_OVER_FOUR_PI * diff;
+                s->Jdif[p*L + g] = HH_OVER_FOUR_PI * diff;
+            }
+    }
 }
 
 /* Dense LU with partial pivoting, row-major M [n][n], solves M x = y in place. */
@@ -1242,64 +1329,69 @@ static int o_stateq_global(orc_state *s)
  * elements); with band elements one global solve (o_stateq_global). */
 int orc_update_using_statistical_equilibrium(orc_state *s)
 {
-    const mgr_model_view *m = &s->m;
-    const int64_t V = s->V, L = s->L, NV = s->N * s->V;
+    const int64_t L = s->L, NV = s->N * s->V;
     memcpy(s->prev3, s->prev2, NV * sizeof(double));
     memcpy(s->prev2, s->prev1, NV * sizeof(double));
     memcpy(s->prev1, s->pop,   NV * sizeof(double));
     if (s->band_n) return o_stateq_global(s);
     int err = 0;
-    int64_t ncol_max = 1;
-    for (int64_t c = 0; c < m->ncolpar; c++) if (m->colpar_ncol[c] > ncol_max) ncol_max = m->colpar_ncol[c];
-    #pragma omp parallel reduction(|:err)
-    {
-        double *M = (double *)xcalloc(V*V, sizeof(double));
-        double *y = (double *)xcalloc(V, sizeof(double));
-        double *Ce = (double *)xcalloc(ncol_max, sizeof(double));
-        double *Cd = (double *)xcalloc(ncol_max, sizeof(double));
-        #pragma omp for schedule(static)
-        for (int64_t p = 0; p < s->N; p++) {
-            for (int64_t q = 0; q < V*V; q++) M[q] = 0.0;
-            for (int64_t q = 0; q < V; q++) y[q] = 0.0;
-            const double *pop = s->prev1 + p*V;   /* == population before the solve */
-            /* radiative transitions */
-            for (int64_t k = 0; k < L; k++) {
-                const double Jeff = s->Jeff[p*L + k];
-                const double v_IJ = m->A[k] + m->Bs[k] * Jeff;
-                const double v_JI = m->Ba[k] * Jeff;
-                const int64_t I = m->irad[k], J = m->jrad[k];
-                if (J != V-1) { M[J*V + I] += v_IJ; M[J*V + J] += -v_JI; }
-                if (I != V-1) { M[I*V + J] += v_JI; M[I*V + I] += -v_IJ; }
-            }
-            /* approximated Lambda operator, kappa from the current populations */
-            for (int64_t k = 0; k < L; k++) {
-                const int64_t I = m->irad[k], J = m->jrad[k];
-                const double opacity = HH_OVER_FOUR_PI * (pop[J] * m->Ba[k] - pop[I] * m->Bs[k]);
-                const double v_IJ = -s->Ls[p*L + k] * opacity;
-                if (J != V-1) M[J*V + I] += v_IJ;
-                if (I != V-1) M[I*V + I] += -v_IJ;
-            }
-            /* collisional transitions */
-            int64_t koff = 0;
-            for (int64_t c = 0; c < m->ncolpar; c++) {
-                double factor;
-                const double tmp = m->temperature[p];
-                orc_collision_rates(s, c, tmp, Ce, Cd, &factor);
-                const double abn = m->abundance[p*m->nspecs + m->colpar_num[c]] * factor;
-                for (int64_t k = 0; k < m->colpar_ncol[c]; k++) {
-                    const double v_IJ = Cd[k] * abn, v_JI = Ce[k] * abn;
-                    const int64_t I = m->icol[koff + k], J = m->jcol[koff + k];
+    for (int64_t l = 0; l < s->S; l++) {   /* Lines::iteration_using_statistical_equilibrium */
+        const mgr_species_view *m = &s->sp[l].v;
+        const int64_t V = s->sp[l].V, loff = s->sp[l].loff;
+        const double *prev1 = s->prev1 + s->N * s->sp[l].voff, *pop_tot = s->pop_tot + l * s->N;
+        double *popl = s->pop + s->N * s->sp[l].voff;
+        int64_t ncol_max = 1;
+        for (int64_t c = 0; c < m->ncolpar; c++) if (m->colpar_ncol[c] > ncol_max) ncol_max = m->colpar_ncol[c];
+        #pragma omp parallel reduction(|:err)
+        {
+            double *M = (double *)xcalloc(V*V, sizeof(double));
+            double *y = (double *)xcalloc(V, sizeof(double));
+            double *Ce = (double *)xcalloc(ncol_max, sizeof(double));
+            double *Cd = (double *)xcalloc(ncol_max, sizeof(double));
+            #pragma omp for schedule(static)
+            for (int64_t p = 0; p < s->N; p++) {
+                for (int64_t q = 0; q < V*V; q++) M[q] = 0.0;
+                for (int64_t q = 0; q < V; q++) y[q] = 0.0;
+                const double *pop = prev1 + p*V;   /* == population before the solve */
+                /* radiative transitions */
+                for (int64_t k = 0; k < m->nrad; k++) {
+                    const double Jeff = s->Jeff[p*L + loff + k];
+                    const double v_IJ = m->A[k] + m->Bs[k] * Jeff;
+                    const double v_JI = m->Ba[k] * Jeff;
+                    const int64_t I = m->irad[k], J = m->jrad[k];
                     if (J != V-1) { M[J*V + I] += v_IJ; M[J*V + J] += -v_JI; }
                     if (I != V-1) { M[I*V + J] += v_JI; M[I*V + I] += -v_IJ; }
                 }
-                koff += m->colpar_ncol[c];
+                /* approximated Lambda operator, kappa from the current populations */
+                for (int64_t k = 0; k < m->nrad; k++) {
+                    const int64_t I = m->irad[k], J = m->jrad[k];
+                    const double opacity = HH_OVER_FOUR_PI * (pop[J] * m->Ba[k] - pop[I] * m->Bs[k]);
+                    const double v_IJ = -s->Ls[p*L + loff + k] * opacity;
+                    if (J != V-1) M[J*V + I] += v_IJ;
+                    if (I != V-1) M[I*V + I] += -v_IJ;
+                }
+                /* collisional transitions */
+                int64_t koff = 0;
+                for (int64_t c = 0; c < m->ncolpar; c++) {
+                    double factor;
+                    const double tmp = s->m.temperature[p];
+                    o_collision_rates(m, c, tmp, Ce, Cd, &factor);
+                    const double abn = s->m.abundance[p*s->m.nspecs + m->colpar_num[c]] * factor;
+                    for (int64_t k = 0; k < m->colpar_ncol[c]; k++) {
+                        const double v_IJ = Cd[k] * abn, v_JI = Ce[k] * abn;
+                        const int64_t I = m->icol[koff + k], J = m->jcol[koff + k];
+                        if (J != V-1) { M[J*V + I] += v_IJ; M[J*V + J] += -v_JI; }
+                        if (I != V-1) { M[I*V + J] += v_JI; M[I*V + I] += -v_IJ; }
+                    }
+                    koff += m->colpar_ncol[c];
+                }
+                for (int64_t i = 0; i < V; i++) M[(V-1)*V + i] += 1.0;
+                y[V-1] = pop_tot[p];
+                if (o_lu_solve(V, M, y) != 0) { err = 1; continue; }
+                for (int64_t i = 0; i < V; i++) popl[p*V + i] = y[i];
             }
-            for (int64_t i = 0; i < V; i++) M[(V-1)*V + i] += 1.0;
-            y[V-1] = s->pop_tot[p];
-            if (o_lu_solve(V, M, y) != 0) { err = 1; continue; }
-            for (int64_t i = 0; i < V; i++) s->pop[p*V + i] = y[i];
+            free(M); free(y); free(Ce); free(Cd);
         }
-        free(M); free(y); free(Ce); free(Cd);
     }
     return err ? -1 : 0;
 }
@@ -1307,50 +1399,64 @@ int orc_update_using_statistical_equilibrium(orc_state *s)
 /* LineProducingSpecies::check_for_convergence (lineProducingSpecies.tpp:120-171) */
 void orc_check_for_convergence(orc_state *s, double pop_prec)
 {
-    const int64_t V = s->V;
-    const double weight = 1.0 / (double)(s->N * V);
-    double fnc = 0.0, rcm = 0.0, rcmax = 0.0;
-    #pragma omp parallel for reduction(+:fnc, rcm) reduction(max:rcmax)
-    for (int64_t p = 0; p < s->N; p++) {
-        const double min_pop = 1.0E-10 * s->pop_tot[p];
-        for (int64_t i = 0; i < V; i++) {
-            const int64_t ind = p*V + i;
-            if (s->pop[ind] > min_pop) {
-                double relative_change = 2.0;
-                relative_change *= fabs(s->pop[ind] - s->prev1[ind]);
-                relative_change /= (s->pop[ind] + s->prev1[ind]);
-                if (relative_change > pop_prec) fnc += weight;
-                rcm += (weight * relative_change);
-                if (relative_change > rcmax) rcmax = relative_change;
+    s->fnc = s->rcm = s->rcmax = 0.0;
+    for (int64_t l = 0; l < s->S; l++) {
+        const int64_t V = s->sp[l].V;
+        const double *pop = s->pop + s->N * s->sp[l].voff, *prev1 = s->prev1 + s->N * s->sp[l].voff;
+        const double *pop_tot = s->pop_tot + l * s->N;
+        const double weight = 1.0 / (double)(s->N * V);
+        double fnc = 0.0, rcm = 0.0, rcmax = 0.0;
+        #pragma omp parallel for reduction(+:fnc, rcm) reduction(max:rcmax)
+        for (int64_t p = 0; p < s->N; p++) {
+            const double min_pop = 1.0E-10 * pop_tot[p];
+            for (int64_t i = 0; i < V; i++) {
+                const int64_t ind = p*V + i;
+                if (pop[ind] > min_pop) {
+                    double relative_change = 2.0;
+                    relative_change *= fabs(pop[ind] - prev1[ind]);
+                    relative_change /= (pop[ind] + prev1[ind]);
+                    if (relative_change > pop_prec) fnc += weight;
+                    rcm += (weight * relative_change);
+                    if (relative_change > rcmax) rcmax = relative_change;
+                }
             }
         }
+        s->sp[l].fnc = fnc; s->sp[l].rcm = rcm; s->sp[l].rcmax = rcmax;
+        /* the ALI continues while any species is not converged (sim_lines.tpp:130-150):
+         * the driver's figures are the largest over the species */
+        if (fnc > s->fnc) s->fnc = fnc;
+        if (rcm > s->rcm) s->rcm = rcm;
+        if (rcmax > s->rcmax) s->rcmax = rcmax;
     }
-    s->fnc = fnc; s->rcm = rcm; s->rcmax = rcmax;
 }
 
-/* LineProducingSpecies::update_using_Ng_acceleration (lineProducingSpecies.tpp:181-238) */
+/* LineProducingSpecies::update_using_Ng_acceleration (lineProducingSpecies.tpp:181-238),
+ * for every species in turn (Lines::iteration_using_Ng_acceleration) */
 void orc_update_using_Ng_acceleration(orc_state *s)
 {
-    const int64_t NV = s->N * s->V;
-    double A1 = 0.0, A2 = 0.0, B2 = 0.0, C1 = 0.0, C2 = 0.0;
-    for (int64_t i = 0; i < NV; i++) {
-        const double x = s->pop[i], x1 = s->prev1[i], x2 = s->prev2[i], x3 = s->prev3[i];
-        const double Q1 = x - 2.0*x1 + x2;
-        const double Q2 = x - x1 - x2 + x3;
-        const double Q3 = x - x1;
-        A1 += Q1*Q1; A2 += Q1*Q2; B2 += Q2*Q2; C1 += Q1*Q3; C2 += Q2*Q3;
-    }
-    const double B1 = A2;
-    const double denominator = A1*B2 - A2*B1;
-    if (denominator != 0.0) {
-        const double a = (C1*B2 - C2*B1) / denominator;
-        const double b = (C2*A1 - C1*A2) / denominator;
+    for (int64_t l = 0; l < s->S; l++) {
+        const int64_t NV = s->N * s->sp[l].V, b0 = s->N * s->sp[l].voff;
+        double *pop = s->pop + b0, *prev1 = s->prev1 + b0, *prev2 = s->prev2 + b0, *prev3 = s->prev3 + b0;
+        double A1 = 0.0, A2 = 0.0, B2 = 0.0, C1 = 0.0, C2 = 0.0;
         for (int64_t i = 0; i < NV; i++) {
-            const double x = s->pop[i];
-            s->pop[i] = (1.0 - a - b)*x + a*s->prev1[i] + b*s->prev2[i];
-            s->prev3[i] = s->prev2[i];
-            s->prev2[i] = s->prev1[i];
-            s->prev1[i] = x;
+            const double x = pop[i], x1 = prev1[i], x2 = prev2[i], x3 = prev3[i];
+            const double Q1 = x - 2.0*x1 + x2;
+            const double Q2 = x - x1 - x2 + x3;
+            const double Q3 = x - x1;
+            A1 += Q1*Q1; A2 += Q1*Q2; B2 += Q2*Q2; C1 += Q1*Q3; C2 += Q2*Q3;
+        }
+        const double B1 = A2;
+        const double denominator = A1*B2 - A2*B1;
+        if (denominator != 0.0) {
+            const double a = (C1*B2 - C2*B1) / denominator;
+            const double b = (C2*A1 - C1*A2) / denominator;
+            for (int64_t i = 0; i < NV; i++) {
+                const double x = pop[i];
+                pop[i] = (1.0 - a - b)*x + a*prev1[i] + b*prev2[i];
+                prev3[i] = prev2[i];
+                prev2[i] = prev1[i];
+                prev1[i] = x;
+            }
         }
     }
 }
@@ -1419,7 +1525,9 @@ void orc_get_populations(const orc_state *s, double *pop) { memcpy(pop, s->pop, 
 void orc_set_populations(orc_state *s, const double *pop)
 {
     memcpy(s->pop, pop, s->N * s->V * sizeof(double));
-    for (int64_t p = 0; p < s->N; p++) s->pop_tot[p] = s->m.abundance[p*s->m.nspecs + s->m.num];
+    for (int64_t l = 0; l < s->S; l++)
+        for (int64_t p = 0; p < s->N; p++)
+            s->pop_tot[l*s->N + p] = s->m.abundance[p*s->m.nspecs + s->sp[l].v.num];
     orc_set_emissivity_and_opacity(s);
 }
 void orc_get_population_history(const orc_state *s, double *p1, double *p2, double *p3)
@@ -1445,6 +1553,11 @@ void orc_get_emissivity_opacity(const orc_state *s, double *emis, double *opac)
 void orc_get_convergence(const orc_state *s, double *fnc, double *rcm, double *rcmax)
 {
     *fnc = s->fnc; *rcm = s->rcm; *rcmax = s->rcmax;
+}
+int64_t orc_nlspecs(const orc_state *s) { return s->S; }
+void orc_get_convergence_species(const orc_state *s, int64_t l, double *fnc, double *rcm, double *rcmax)
+{
+    *fnc = s->sp[l].fnc; *rcm = s->sp[l].rcm; *rcmax = s->sp[l].rcmax;
 }
 void orc_get_antipod(const orc_state *s, int64_t *antipod) { memcpy(antipod, s->antipod, s->R * sizeof(int64_t)); }
 void orc_get_lines(const orc_state *s, double *line, int64_t *line_index)

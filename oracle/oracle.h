@@ -94,7 +94,10 @@ void orc_update_using_Ng_acceleration(orc_state *s);
 int64_t orc_compute_level_populations(orc_state *s, int use_Ng, int64_t max_niterations,
                                       int criterion, int nthreads);
 
-/* accessors: copy state arrays out / in */
+/* accessors: copy state arrays out / in.  Populations (and their history)
+ * hold the species one after the other, each as index(p, i) = i + p*nlev;
+ * per-line arrays (Ls, Jlin, Jeff, Jdif, emissivity, opacity) are
+ * [ncells][nlines] with the species' lines one after the other. */
 void orc_get_frequencies(const orc_state *s, double *nu, int64_t *nr_line);
 void orc_get_boundary_intensities(const orc_state *s, double *I_bdy /* [nboundary*F] for R=0 */);
 void orc_get_J(const orc_state *s, double *J);
@@ -109,7 +112,9 @@ void orc_set_populations(orc_state *s, const double *pop);
 void orc_get_population_history(const orc_state *s, double *p1, double *p2, double *p3);
 void orc_set_population_history(orc_state *s, const double *p1, const double *p2, const double *p3);
 void orc_get_emissivity_opacity(const orc_state *s, double *emis, double *opac);
-void orc_get_convergence(const orc_state *s, double *fnc, double *rcm, double *rcmax);
+void orc_get_convergence(const orc_state *s, double *fnc, double *rcm, double *rcmax);  /* largest over species */
+int64_t orc_nlspecs(const orc_state *s);
+void orc_get_convergence_species(const orc_state *s, int64_t l, double *fnc, double *rcm, double *rcmax);
 void orc_get_antipod(const orc_state *s, int64_t *antipod);
 void orc_get_lines(const orc_state *s, double *line, int64_t *line_index);
 /* appears_in_line_integral, corresponding_{l,k,z} (length nfreqs) */

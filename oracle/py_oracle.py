@@ -97,6 +97,9 @@ def lib():
         L.orc_set_population_history.argtypes = [vp, vp, vp, vp]
         L.orc_get_emissivity_opacity.argtypes = [vp, vp, vp]
         L.orc_get_convergence.argtypes = [vp, vp, vp, vp]
+        L.orc_nlspecs.argtypes = [vp]
+        L.orc_nlspecs.restype = C.c_int64
+        L.orc_get_convergence_species.argtypes = [vp, C.c_int64, vp, vp, vp]
         L.orc_get_lines.argtypes = [vp, vp, vp]
         L.orc_get_line_maps.argtypes = [vp, vp, vp, vp, vp]
         L.orc_planck.argtypes = [C.c_double, C.c_double]
@@ -123,7 +126,19 @@ class Oracle:
         self._view, self._keep = model.view()
         self._s = lib().orc_create(C.byref(self._view))
         self.N, self.R, self.F = model.ncells, model.nrays, model.nfreqs
-        self.L, self.V, self.Q, self.NB = model.nrad, model.nlev, model.nquads, model.nboundary
+        self.L, self.V, self.Q, self.NB = model.nlines, model.nlev, model.nquads, model.nboundary
+        # levels of each line-producing species; populations hold them species after species
+        self.Vs = [ld.nlev for ld in model.species_linedata]
+
+    def _split(self, a):
+        """A population array as (N, V) for one species, else a list per species."""
+        if len(self.Vs) == 1:
+            return a.reshape(self.N, self.V)
+        out, o = [], 0
+        for V in self.Vs:
+            out.append(a[o:o + self.N * V].reshape(self.N, V))
+            o += self.N * V
+        return out
 
     def __del__(self):
         if getattr(self, "_s", None):
@@ -283,20 +298,33 @@ class Oracle:
         return a.reshape(self.N, self.L), b.reshape(self.N, self.L), c.reshape(self.N, self.L)
 
     def populations(self):
-        a = np.zeros(self.N * self.V)
+        a = np.zeros(self.N * sum(self.Vs))
         lib().orc_get_populations(self._s, _p(a))
-        return a.reshape(self.N, self.V)
+        return self._split(a)
 
     def set_populations(self, pops):
+        if isinstance(pops, (list, tuple)):
+            pops = np.concatenate([np.ravel(x) for x in pops])
         a = np.ascontiguousarray(pops, dtype=np.float64).ravel()
         lib().orc_set_populations(self._s, _p(a))
 
     def population_history(self):
-        a, b, c = (np.zeros(self.N * self.V) for _ in range(3))
+        a, b, c = (np.zeros(self.N * sum(self.Vs)) for _ in range(3))
         lib().orc_get_population_history(self._s, _p(a), _p(b), _p(c))
-        return tuple(x.reshape(self.N, self.V) for x in (a, b, c))
+        return tuple(self._split(x) for x in (a, b, c))
+
+    def convergence_species(self):
+        """(fraction not converged, mean change, max change) of each species."""
+        out = []
+        for l in range(lib().orc_nlspecs(self._s)):
+            a, b, c = C.c_double(), C.c_double(), C.c_double()
+            lib().orc_get_convergence_species(self._s, l, C.byref(a), C.byref(b), C.byref(c))
+            out.append((a.value, b.value, c.value))
+        return out
 
     def set_population_history(self, p1, p2, p3):
+        p1, p2, p3 = (np.concatenate([np.ravel(y) for y in x]) if isinstance(x, (list, tuple)) else x
+                      for x in (p1, p2, p3))
         arrs = [np.ascontiguousarray(x, dtype=np.float64).ravel() for x in (p1, p2, p3)]
         lib().orc_set_population_history(self._s, *[_p(x) for x in arrs])
 
