@@ -70,14 +70,25 @@ struct alignas(32) NbRec {
 };
 
 // Small per-model line tables, read by every kernel.
+constexpr int MAX_SPECIES = 8;    // line-producing species per context
 struct LineTables {
-    int    L;              // nrad (one species)
-    int    V;              // nlev
+    int    L;              // lines of every species (Parameters::nlines)
+    int    V;              // levels of every species
     int    Q;              // nquads
     int    F;              // nfreqs = L*Q
-    double inverse_mass;
+    int    S;              // line-producing species
+    double inverse_mass;   // species 0
     double line[64];       // sorted line frequencies (Lines::line)
-    int    line_index[64]; // transition of sorted line l (Lines::line_index)
+    int    line_index[64]; // line g (species after species) of sorted line l (Lines::line_index)
+    // per species: levels, lines, first population (in units of ncells), first
+    // line g, inverse mass
+    int    sV[MAX_SPECIES], sL[MAX_SPECIES], voff[MAX_SPECIES], loff[MAX_SPECIES];
+    double inv_mass[MAX_SPECIES];
+    int    line_spec[64];  // species of line g
+    // species whose mass sets the width of sorted line l in get_eta_and_chi:
+    // Simulation::get_line_width (simulation.tpp:363-373) reads the frequency
+    // back-map corresponding_l_for_spec at index line_index[l], a bin index
+    int    width_spec[64];
 };
 constexpr int MAX_LINES = 64;
 
@@ -146,6 +157,11 @@ struct mgr_ctx {
     std::vector<int64_t> m_iterations; // per model, of the last level-population run
     std::vector<mgr_conv> m_last_conv;
     int64_t ncolpar = 0, ncol_tot = 0, ntmp_tot = 0, ctab_tot = 0;
+    // line-producing species: levels, lines, first population block (in
+    // units of N), first line, abundance index, first collision partner and
+    // partner count of each; populations are held species after species
+    int64_t S = 1;
+    std::vector<int64_t> sp_V, sp_L, sp_voff, sp_loff, sp_num, sp_cp0, sp_ncp;
     double  pop_prec = 1e-6, max_width_fraction = 0.5;
     int64_t num = 0;
 
@@ -172,7 +188,7 @@ struct mgr_ctx {
     double  *A = nullptr, *Bs = nullptr, *Ba = nullptr, *frequency = nullptr; // [L]
     int32_t *irad = nullptr, *jrad = nullptr;      // [L]
     double  *energy = nullptr, *lweight = nullptr; // [V]
-    double  *roots = nullptr, *qweights = nullptr; // [Q]
+    double  *roots = nullptr, *qweights = nullptr; // [S*Q] species after species
     int32_t *cp_num = nullptr, *cp_ncol = nullptr, *cp_ntmp = nullptr;  // [ncolpar]
     int32_t *cp_koff = nullptr, *cp_toff = nullptr, *cp_coff = nullptr; // [ncolpar]
     int32_t *cp_op = nullptr;                      // [ncolpar] 0 n, 1 o, 2 p
@@ -180,7 +196,8 @@ struct mgr_ctx {
     double  *tmp = nullptr, *Cd = nullptr, *Ce = nullptr;
 
     // device: derived per-cell tables
-    double  *pw = nullptr;          // [N] profile width / line frequency
+    double  *pw = nullptr;          // [S*N] profile width / line frequency, per species
+    double  *pw_cmax = nullptr;     // [N] the largest over species
     double  *dshift_max = nullptr;  // [N]
     double  *W11 = nullptr;         // [N*L] 11 * width of sorted line l
     double  *IW64 = nullptr;        // [N*L] 1 / width
@@ -203,7 +220,7 @@ struct mgr_ctx {
 
     // device: populations and line quantities
     double  *pop = nullptr, *prev1 = nullptr, *prev2 = nullptr, *prev3 = nullptr; // [N*V]
-    double  *pop_tot = nullptr;     // [N]
+    double  *pop_tot = nullptr;     // [S*N]
     double  *emis = nullptr, *opac = nullptr;   // [N*L]
     bool     pops_set = false;
 
@@ -309,7 +326,7 @@ struct mgr_ctx {
 
     // reductions
     double  *partials = nullptr;    // [nchunks * 8]
-    double  *reduced = nullptr;     // [8]
+    double  *reduced = nullptr;     // [4 (MAX_SPECIES + 1)]: the result, then per species
     double  *h_reduced = nullptr;   // pinned [8]
 
     // graphs of one ALI iteration

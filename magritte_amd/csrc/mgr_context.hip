@@ -5,6 +5,7 @@
 // Compiled with -ffp-contract=off: these kernels restate FP64 expressions of
 // the reference and must round exactly as its scalar code does.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -66,23 +67,31 @@ void dfree(T *&p)
 
 // ---- kernels ---------------------------------------------------------------
 
-// Thermodynamics::profile_width(inverse_mass, p) (thermodynamics.tpp:62-69),
-// Simulation::get_dshift_max (simulation.tpp:15-37), and the per-(cell, sorted
-// line) window 11*width, 1/width used by get_eta_and_chi (simulation.tpp:403-446).
+// Thermodynamics::profile_width(inverse_mass, p) (thermodynamics.tpp:62-69) of
+// every species, Simulation::get_dshift_max (simulation.tpp:15-37, the smallest
+// over species), and the per-(cell, sorted line) window 11*width, 1/width used
+// by get_eta_and_chi (simulation.tpp:403-446) with get_line_width's species.
 __global__ void k_cell_tables(int64_t N, const LineTables *__restrict__ tb,
                               const double *__restrict__ T, const double *__restrict__ vturb2,
                               double max_width_fraction, double *__restrict__ pw,
-                              double *__restrict__ dshift_max, double *__restrict__ W11,
-                              double *__restrict__ IW64, float *__restrict__ IW32)
+                              double *__restrict__ pw_cmax, double *__restrict__ dshift_max,
+                              double *__restrict__ W11, double *__restrict__ IW64, float *__restrict__ IW32)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N) return;
     const int L = tb->L;
-    const double w = sqrt(TWO_KB_OVER_AMU_C_SQUARED * tb->inverse_mass * T[p] + vturb2[p]);
-    pw[p] = w;
-    dshift_max[p] = max_width_fraction * w;
+    double dsm = DBL_MAX, wmax = 0.0;
+    for (int l = 0; l < tb->S; l++) {
+        const double w = sqrt(TWO_KB_OVER_AMU_C_SQUARED * tb->inv_mass[l] * T[p] + vturb2[p]);
+        pw[l * N + p] = w;
+        wmax = fmax(wmax, w);
+        const double nd = max_width_fraction * w;
+        if (dsm > nd) dsm = nd;
+    }
+    pw_cmax[p] = wmax;
+    dshift_max[p] = dsm;
     for (int l = 0; l < L; l++) {
-        const double width = tb->line[l] * w;
+        const double width = tb->line[l] * pw[tb->width_spec[l] * N + p];
         W11[p * L + l] = LINE_WINDOW * width;
         const double iw = 1.0 / width;
         IW64[p * L + l] = iw;
@@ -91,11 +100,25 @@ __global__ void k_cell_tables(int64_t N, const LineTables *__restrict__ tb,
 }
 
 // Simulation::compute_spectral_discretisation (sim_spectrum.tpp:13-196):
-// nu(p, i) = line[k] + line[k]*pw(p)*root_z for i = k*Q + z, sorted per cell.
-// With pw null, compute_spectral_discretisation_image (sim_spectrum.tpp:206-311):
+// nu(p, i) = line[g] + line[g]*pw(s, p)*root(s, z) for i = g*Q + z, where g
+// runs over the lines species after species, line[g] is the sorted list at
+// that running index and s is the species of line g; sorted per cell.  With
+// pw null, compute_spectral_discretisation_image (sim_spectrum.tpp:206-311):
 // the same grid with the one width `image_width` in every cell.
 // One thread per (p, i) ranks its value among the cell's F values (ties by
 // original index), which gives heapsort's order for distinct values.
+__device__ __forceinline__ double bin_value(const LineTables *__restrict__ tb, const double *__restrict__ pw,
+                                            int64_t N, int64_t p, double image_width,
+                                            const double *__restrict__ roots, int i)
+{
+    const int Q = tb->Q;
+    const int g = i / Q, z = i % Q;
+    const int s = tb->line_spec[g];
+    const double w = pw ? pw[s * N + p] : image_width;
+    const double lg = tb->line[g];
+    return lg + (lg * w) * roots[s * Q + z];
+}
+
 __global__ void k_spectral(int64_t N, const LineTables *__restrict__ tb,
                            const double *__restrict__ pw, double image_width,
                            const double *__restrict__ roots, double *__restrict__ nu,
@@ -106,15 +129,11 @@ __global__ void k_spectral(int64_t N, const LineTables *__restrict__ tb,
     if (t >= N * F) return;
     const int64_t p = t / F;
     const int i = (int)(t % F);
-    const double w = pw ? pw[p] : image_width;
     const int ki = i / Q, zi = i % Q;
-    const double li = tb->line[ki];
-    const double vi = li + (li * w) * roots[zi];
+    const double vi = bin_value(tb, pw, N, p, image_width, roots, i);
     int rank = 0;
     for (int j = 0; j < F; j++) {
-        const int kj = j / Q, zj = j % Q;
-        const double lj = tb->line[kj];
-        const double vj = lj + (lj * w) * roots[zj];
+        const double vj = bin_value(tb, pw, N, p, image_width, roots, j);
         rank += (vj < vi) || (vj == vi && j < i);
     }
     nu[p * F + rank] = vi;
@@ -177,17 +196,21 @@ __global__ void k_pop_tot(int64_t N, int nspecs, int num, const double *__restri
 __global__ void k_emissivity_opacity(int64_t N, const LineTables *__restrict__ tb,
                                      const double *__restrict__ A, const double *__restrict__ Bs,
                                      const double *__restrict__ Ba, const int32_t *__restrict__ irad,
-                                     const int32_t *__restrict__ jrad, const double *__restrict__ pop,
+                                     const int32_t *__restrict__ jrad, const double *__restrict__ pop_all,
                                      const double *__restrict__ IW64, const double *__restrict__ W11,
                                      double *__restrict__ emis, double *__restrict__ opac,
                                      float4 *__restrict__ row32, double4 *__restrict__ row64)
 {
-    const int L = tb->L, V = tb->V;
+    const int L = tb->L;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N * L) return;
     const int64_t p = t / L;
     const int l = (int)(t % L);
     const int k = tb->line_index[l];
+    // the line's species: its populations, levels i, j of its own numbering
+    const int sp = tb->line_spec[k];
+    const int V = tb->sV[sp];
+    const double *pop = pop_all + (int64_t)tb->voff[sp] * N;
     const int i = irad[k], j = jrad[k];
     const double e = HH_OVER_FOUR_PI * A[k] * pop[p * V + i];
     const double o = HH_OVER_FOUR_PI * (pop[p * V + j] * Ba[k] - pop[p * V + i] * Bs[k]);
@@ -210,7 +233,7 @@ int launch_cell_tables(mgr_ctx *ctx)
 {
     hipLaunchKernelGGL(k_cell_tables, dim3(grid_for(ctx->N, 256)), dim3(256), 0, ctx->stream,
                        ctx->N, ctx->tables, ctx->temperature, ctx->vturb2, ctx->max_width_fraction,
-                       ctx->pw, ctx->dshift_max, ctx->W11, ctx->IW64, ctx->IW32);
+                       ctx->pw, ctx->pw_cmax, ctx->dshift_max, ctx->W11, ctx->IW64, ctx->IW32);
     MGR_LAUNCH_CHECK();
     return MGR_OK;
 }
@@ -247,7 +270,33 @@ static int check_view(mgr_ctx *ctx, const mgr_model_view *m, int64_t idx)
     for (int64_t b = 0; b < m->nboundary; b++)
         if (m->boundary2cell_nr[b] < 0 || m->boundary2cell_nr[b] >= m->ncells)
             return ctx->fail(MGR_ERR_ARGUMENT, who + "boundary2cell_nr out of range");
+    if (m->nlspecs > 1) {
+        if (!m->more_species) return ctx->fail(MGR_ERR_ARGUMENT, who + "nlspecs > 1 with no more_species");
+        for (int64_t l = 1; l < m->nlspecs; l++) {
+            const mgr_species_view *v = &m->more_species[l - 1];
+            if (v->nrad <= 0 || v->nlev < 2)
+                return ctx->fail(MGR_ERR_ARGUMENT, who + "species " + std::to_string(l) + ": sizes out of range");
+            if (!v->energy || !v->weight || !v->irad || !v->jrad || !v->A || !v->Bs || !v->Ba || !v->frequency ||
+                !v->roots || !v->qweights)
+                return ctx->fail(MGR_ERR_ARGUMENT, who + "species " + std::to_string(l) + ": null array");
+        }
+    }
     return MGR_OK;
+}
+
+// The species of a model view: species 0 from its own fields, then more_species.
+static std::vector<mgr_species_view> species_of(const mgr_model_view *m)
+{
+    std::vector<mgr_species_view> v(1);
+    mgr_species_view &s = v[0];
+    s.num = m->num; s.inverse_mass = m->inverse_mass; s.nlev = m->nlev; s.nrad = m->nrad;
+    s.energy = m->energy; s.weight = m->weight; s.irad = m->irad; s.jrad = m->jrad;
+    s.A = m->A; s.Bs = m->Bs; s.Ba = m->Ba; s.frequency = m->frequency;
+    s.ncolpar = m->ncolpar; s.colpar_num = m->colpar_num; s.colpar_orth_or_para = m->colpar_orth_or_para;
+    s.colpar_ncol = m->colpar_ncol; s.colpar_ntmp = m->colpar_ntmp; s.icol = m->icol; s.jcol = m->jcol;
+    s.tmp = m->tmp; s.Cd = m->Cd; s.Ce = m->Ce; s.roots = m->roots; s.qweights = m->qweights;
+    for (int64_t l = 1; l < m->nlspecs; l++) v.push_back(m->more_species[l - 1]);
+    return v;
 }
 
 template <typename T>
@@ -399,10 +448,29 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
     }
     ctx->nchunks = (int64_t)chunk_cell.size() - 1;
 
+    // the line-producing species (Lines::setup, lines.cpp:43-87): lines and
+    // levels species after species
+    const std::vector<mgr_species_view> spv = species_of(m);
+    const int64_t S = (int64_t)spv.size();
+    if (S > MAX_SPECIES) return ctx->fail(MGR_ERR_UNSUPPORTED, "more than " + std::to_string(MAX_SPECIES) + " line-producing species");
+    if (S > 1 && Mm > 1) return ctx->fail(MGR_ERR_UNSUPPORTED, "several line-producing species in a batch of models are not supported");
+    if (S > 1 && m->n_off_diag >= 2)
+        return ctx->fail(MGR_ERR_UNSUPPORTED, "n_off_diag >= 2 (banded ALO) with several line-producing species is not supported");
+    ctx->S = S;
+    ctx->sp_V.assign(S, 0); ctx->sp_L.assign(S, 0); ctx->sp_voff.assign(S, 0); ctx->sp_loff.assign(S, 0);
+    ctx->sp_num.assign(S, 0); ctx->sp_cp0.assign(S, 0); ctx->sp_ncp.assign(S, 0);
+    int64_t Vt = 0, Lt = 0, Ct = 0;
+    for (int64_t l = 0; l < S; l++) {
+        ctx->sp_V[l] = spv[l].nlev; ctx->sp_L[l] = spv[l].nrad; ctx->sp_voff[l] = Vt; ctx->sp_loff[l] = Lt;
+        ctx->sp_num[l] = spv[l].num; ctx->sp_cp0[l] = Ct; ctx->sp_ncp[l] = spv[l].ncolpar;
+        Vt += spv[l].nlev; Lt += spv[l].nrad; Ct += spv[l].ncolpar;
+        if (spv[l].num < 0 || spv[l].num >= m->nspecs) return ctx->fail(MGR_ERR_ARGUMENT, "line species index out of range");
+    }
+
     ctx->N = N; ctx->R = Rmax; ctx->NB = nbd_tot; ctx->nspecs = m->nspecs;
-    ctx->L = m->nrad; ctx->V = m->nlev; ctx->Q = m->nquads; ctx->F = m->nrad * m->nquads;
+    ctx->L = Lt; ctx->V = Vt; ctx->Q = m->nquads; ctx->F = Lt * m->nquads;
     ctx->pop_prec = m->pop_prec; ctx->max_width_fraction = m->max_width_fraction;
-    ctx->num = m->num; ctx->ncolpar = m->ncolpar;
+    ctx->num = m->num; ctx->ncolpar = Ct;
     const int64_t R = Rtot, NB = ctx->NB, L = ctx->L, V = ctx->V, Q = ctx->Q;
 
     if (L > MAX_LINES) return ctx->fail(MGR_ERR_UNSUPPORTED, "more than 64 line transitions");
@@ -417,22 +485,47 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
         return ctx->fail(MGR_ERR_UNSUPPORTED, "n_off_diag >= 2 (banded ALO) in a batch of models is not supported");
     ctx->nband = m->n_off_diag >= 2 ? (int)m->n_off_diag - 1 : 0;
     ctx->band_S = (ctx->R / 2) * 2 * ctx->nband;
-    if (m->num < 0 || m->num >= m->nspecs) return ctx->fail(MGR_ERR_ARGUMENT, "line species index out of range");
-
     // Lines::setup (src/Model/Lines/lines.cpp:43-87): sorted line list.
     LineTables &tb = ctx->h_tables;
     std::memset(&tb, 0, sizeof(tb));
-    tb.L = (int)L; tb.V = (int)V; tb.Q = (int)Q; tb.F = (int)ctx->F; tb.inverse_mass = m->inverse_mass;
+    tb.L = (int)L; tb.V = (int)V; tb.Q = (int)Q; tb.F = (int)ctx->F; tb.S = (int)S;
+    tb.inverse_mass = m->inverse_mass;
+    std::vector<double> freq_all(L), A_all(L), Bs_all(L), Ba_all(L), energy_all(V), weight_all(V);
+    std::vector<double> roots_all(S * Q), qw_all(S * Q);
+    std::vector<int64_t> irad_all(L), jrad_all(L);
+    for (int64_t l = 0; l < S; l++) {
+        const mgr_species_view &v = spv[l];
+        tb.sV[l] = (int)v.nlev; tb.sL[l] = (int)v.nrad;
+        tb.voff[l] = (int)ctx->sp_voff[l]; tb.loff[l] = (int)ctx->sp_loff[l];
+        tb.inv_mass[l] = v.inverse_mass;
+        for (int64_t k = 0; k < v.nrad; k++) {
+            const int64_t g = ctx->sp_loff[l] + k;
+            if (v.irad[k] < 0 || v.irad[k] >= v.nlev || v.jrad[k] < 0 || v.jrad[k] >= v.nlev)
+                return ctx->fail(MGR_ERR_ARGUMENT, "radiative transition level out of range");
+            tb.line_spec[g] = (int)l;
+            freq_all[g] = v.frequency[k]; A_all[g] = v.A[k]; Bs_all[g] = v.Bs[k]; Ba_all[g] = v.Ba[k];
+            irad_all[g] = v.irad[k]; jrad_all[g] = v.jrad[k];
+        }
+        for (int64_t i = 0; i < v.nlev; i++) {
+            energy_all[ctx->sp_voff[l] + i] = v.energy[i];
+            weight_all[ctx->sp_voff[l] + i] = v.weight[i];
+        }
+        for (int64_t z = 0; z < Q; z++) { roots_all[l * Q + z] = v.roots[z]; qw_all[l * Q + z] = v.qweights[z]; }
+    }
     {
         std::vector<int> order(L);
         std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int a, int b) { return m->frequency[a] < m->frequency[b]; });
-        for (int l = 0; l < L; l++) { tb.line[l] = m->frequency[order[l]]; tb.line_index[l] = order[l]; }
-    }
-    for (int64_t k = 0; k < L; k++) {
-        if (m->irad[k] < 0 || m->irad[k] >= V || m->jrad[k] < 0 || m->jrad[k] >= V)
-            return ctx->fail(MGR_ERR_ARGUMENT, "radiative transition level out of range");
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return freq_all[a] < freq_all[b]; });
+        for (int l = 0; l < L; l++) { tb.line[l] = freq_all[order[l]]; tb.line_index[l] = order[l]; }
+        // get_line_width: corresponding_l_for_spec[line_index[l]], with the
+        // back-map laid out over bins (species, transition, root) in order
+        for (int l = 0; l < L; l++) {
+            const int64_t f = tb.line_index[l];
+            int sw = 0;
+            for (int64_t q = 0; q < S; q++)
+                if (f >= ctx->sp_loff[q] * Q) sw = (int)q;
+            tb.width_spec[l] = sw;
+        }
     }
 
     // neighbours (CSR); every index was checked against its own model above,
@@ -467,24 +560,39 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
             if (ctx->antipod[r] < 0) return ctx->fail(MGR_ERR_ARGUMENT, "a ray has no antipode");
     }
 
-    // collision partners: offsets into the concatenated tables
+    // collision partners of every species, species after species: offsets into
+    // the concatenated tables
     std::vector<int32_t> cpn(ctx->ncolpar + 1), cpc(ctx->ncolpar + 1), cpt(ctx->ncolpar + 1),
         cpk(ctx->ncolpar + 1), cptoff(ctx->ncolpar + 1), cpco(ctx->ncolpar + 1), cpop(ctx->ncolpar + 1);
+    std::vector<int64_t> icol_all, jcol_all;
+    std::vector<double> tmp_all, Cd_all, Ce_all;
     int64_t koff = 0, toff = 0, coff = 0;
-    for (int64_t c = 0; c < ctx->ncolpar; c++) {
-        if (m->colpar_num[c] < 0 || m->colpar_num[c] >= m->nspecs)
-            return ctx->fail(MGR_ERR_ARGUMENT, "collision partner species out of range");
-        if (m->colpar_ntmp[c] < 1) return ctx->fail(MGR_ERR_ARGUMENT, "collision partner without temperatures");
-        cpn[c] = (int32_t)m->colpar_num[c]; cpc[c] = (int32_t)m->colpar_ncol[c];
-        cpt[c] = (int32_t)m->colpar_ntmp[c];
-        cpk[c] = (int32_t)koff; cptoff[c] = (int32_t)toff; cpco[c] = (int32_t)coff;
-        const char op = m->colpar_orth_or_para[c];
-        cpop[c] = (op == 'o') ? 1 : (op == 'p') ? 2 : 0;
-        for (int64_t k = 0; k < m->colpar_ncol[c]; k++)
-            if (m->icol[koff + k] < 0 || m->icol[koff + k] >= V || m->jcol[koff + k] < 0 || m->jcol[koff + k] >= V)
-                return ctx->fail(MGR_ERR_ARGUMENT, "collisional transition level out of range");
-        koff += m->colpar_ncol[c]; toff += m->colpar_ntmp[c];
-        coff += m->colpar_ncol[c] * m->colpar_ntmp[c];
+    for (int64_t l = 0; l < S; l++) {
+        const mgr_species_view &v = spv[l];
+        int64_t kl = 0, tl = 0, cl = 0;
+        for (int64_t cc = 0; cc < v.ncolpar; cc++) {
+            const int64_t c = ctx->sp_cp0[l] + cc;
+            if (v.colpar_num[cc] < 0 || v.colpar_num[cc] >= m->nspecs)
+                return ctx->fail(MGR_ERR_ARGUMENT, "collision partner species out of range");
+            if (v.colpar_ntmp[cc] < 1) return ctx->fail(MGR_ERR_ARGUMENT, "collision partner without temperatures");
+            cpn[c] = (int32_t)v.colpar_num[cc]; cpc[c] = (int32_t)v.colpar_ncol[cc];
+            cpt[c] = (int32_t)v.colpar_ntmp[cc];
+            cpk[c] = (int32_t)koff; cptoff[c] = (int32_t)toff; cpco[c] = (int32_t)coff;
+            const char op = v.colpar_orth_or_para[cc];
+            cpop[c] = (op == 'o') ? 1 : (op == 'p') ? 2 : 0;
+            for (int64_t k = 0; k < v.colpar_ncol[cc]; k++) {
+                if (v.icol[kl + k] < 0 || v.icol[kl + k] >= v.nlev || v.jcol[kl + k] < 0 || v.jcol[kl + k] >= v.nlev)
+                    return ctx->fail(MGR_ERR_ARGUMENT, "collisional transition level out of range");
+                icol_all.push_back(v.icol[kl + k]);
+                jcol_all.push_back(v.jcol[kl + k]);
+            }
+            tmp_all.insert(tmp_all.end(), v.tmp + tl, v.tmp + tl + v.colpar_ntmp[cc]);
+            Cd_all.insert(Cd_all.end(), v.Cd + cl, v.Cd + cl + v.colpar_ncol[cc] * v.colpar_ntmp[cc]);
+            Ce_all.insert(Ce_all.end(), v.Ce + cl, v.Ce + cl + v.colpar_ncol[cc] * v.colpar_ntmp[cc]);
+            kl += v.colpar_ncol[cc]; tl += v.colpar_ntmp[cc]; cl += v.colpar_ncol[cc] * v.colpar_ntmp[cc];
+            koff += v.colpar_ncol[cc]; toff += v.colpar_ntmp[cc];
+            coff += v.colpar_ncol[cc] * v.colpar_ntmp[cc];
+        }
     }
     ctx->ncol_tot = koff; ctx->ntmp_tot = toff; ctx->ctab_tot = coff;
 
@@ -533,35 +641,36 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
     if ((rc = upload(ctx, &ctx->vturb2, m->vturb2, N))) return rc;
     if ((rc = upload(ctx, &ctx->abundance, m->abundance, N * m->nspecs))) return rc;
     if ((rc = upload(ctx, &ctx->tables, &tb, 1))) return rc;
-    if ((rc = upload(ctx, &ctx->A, m->A, L))) return rc;
-    if ((rc = upload(ctx, &ctx->Bs, m->Bs, L))) return rc;
-    if ((rc = upload(ctx, &ctx->Ba, m->Ba, L))) return rc;
-    if ((rc = upload(ctx, &ctx->frequency, m->frequency, L))) return rc;
-    if ((rc = upload_cast<int32_t>(ctx, &ctx->irad, m->irad, L))) return rc;
-    if ((rc = upload_cast<int32_t>(ctx, &ctx->jrad, m->jrad, L))) return rc;
-    if ((rc = upload(ctx, &ctx->energy, m->energy, V))) return rc;
-    if ((rc = upload(ctx, &ctx->lweight, m->weight, V))) return rc;
-    if ((rc = upload(ctx, &ctx->roots, m->roots, Q))) return rc;
+    if ((rc = upload(ctx, &ctx->A, A_all.data(), L))) return rc;
+    if ((rc = upload(ctx, &ctx->Bs, Bs_all.data(), L))) return rc;
+    if ((rc = upload(ctx, &ctx->Ba, Ba_all.data(), L))) return rc;
+    if ((rc = upload(ctx, &ctx->frequency, freq_all.data(), L))) return rc;
+    if ((rc = upload_cast<int32_t>(ctx, &ctx->irad, irad_all.data(), L))) return rc;
+    if ((rc = upload_cast<int32_t>(ctx, &ctx->jrad, jrad_all.data(), L))) return rc;
+    if ((rc = upload(ctx, &ctx->energy, energy_all.data(), V))) return rc;
+    if ((rc = upload(ctx, &ctx->lweight, weight_all.data(), V))) return rc;
+    if ((rc = upload(ctx, &ctx->roots, roots_all.data(), S * Q))) return rc;
     ctx->root_max = 0.0;
-    for (int64_t z = 0; z < Q; z++) ctx->root_max = std::max(ctx->root_max, std::fabs(m->roots[z]));
-    if ((rc = upload(ctx, &ctx->qweights, m->qweights, Q))) return rc;
+    for (int64_t z = 0; z < S * Q; z++) ctx->root_max = std::max(ctx->root_max, std::fabs(roots_all[z]));
+    if ((rc = upload(ctx, &ctx->qweights, qw_all.data(), S * Q))) return rc;
     if ((rc = upload(ctx, &ctx->cp_num, cpn.data(), ctx->ncolpar))) return rc;
-    ctx->h_cp_num.assign(m->colpar_num, m->colpar_num + ctx->ncolpar);
+    ctx->h_cp_num.assign(cpn.begin(), cpn.begin() + ctx->ncolpar);
     if ((rc = upload(ctx, &ctx->cp_ncol, cpc.data(), ctx->ncolpar))) return rc;
     if ((rc = upload(ctx, &ctx->cp_ntmp, cpt.data(), ctx->ncolpar))) return rc;
     if ((rc = upload(ctx, &ctx->cp_koff, cpk.data(), ctx->ncolpar))) return rc;
     if ((rc = upload(ctx, &ctx->cp_toff, cptoff.data(), ctx->ncolpar))) return rc;
     if ((rc = upload(ctx, &ctx->cp_coff, cpco.data(), ctx->ncolpar))) return rc;
     if ((rc = upload(ctx, &ctx->cp_op, cpop.data(), ctx->ncolpar))) return rc;
-    if ((rc = upload_cast<int32_t>(ctx, &ctx->icol, m->icol, koff))) return rc;
-    if ((rc = upload_cast<int32_t>(ctx, &ctx->jcol, m->jcol, koff))) return rc;
-    if ((rc = upload(ctx, &ctx->tmp, m->tmp, toff))) return rc;
-    if ((rc = upload(ctx, &ctx->Cd, m->Cd, coff))) return rc;
-    if ((rc = upload(ctx, &ctx->Ce, m->Ce, coff))) return rc;
+    if ((rc = upload_cast<int32_t>(ctx, &ctx->icol, icol_all.data(), koff))) return rc;
+    if ((rc = upload_cast<int32_t>(ctx, &ctx->jcol, jcol_all.data(), koff))) return rc;
+    if ((rc = upload(ctx, &ctx->tmp, tmp_all.data(), toff))) return rc;
+    if ((rc = upload(ctx, &ctx->Cd, Cd_all.data(), coff))) return rc;
+    if ((rc = upload(ctx, &ctx->Ce, Ce_all.data(), coff))) return rc;
 
     // derived and state arrays
     const int64_t F = ctx->F;
-    if ((rc = dalloc(ctx, &ctx->pw, N))) return rc;
+    if ((rc = dalloc(ctx, &ctx->pw, S * N))) return rc;
+    if ((rc = dalloc(ctx, &ctx->pw_cmax, N))) return rc;
     if ((rc = dalloc(ctx, &ctx->dshift_max, N))) return rc;
     if ((rc = dalloc(ctx, &ctx->W11, N * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->IW64, N * L))) return rc;
@@ -575,7 +684,7 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
     if ((rc = dalloc(ctx, &ctx->prev1, N * V))) return rc;
     if ((rc = dalloc(ctx, &ctx->prev2, N * V))) return rc;
     if ((rc = dalloc(ctx, &ctx->prev3, N * V))) return rc;
-    if ((rc = dalloc(ctx, &ctx->pop_tot, N))) return rc;
+    if ((rc = dalloc(ctx, &ctx->pop_tot, S * N))) return rc;
     if ((rc = dalloc(ctx, &ctx->emis, N * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->opac, N * L))) return rc;
     if ((rc = dalloc(ctx, &ctx->J, N * F))) return rc;
@@ -599,7 +708,7 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
     }
     if ((rc = dalloc(ctx, &ctx->err_flag, 1))) return rc;
     if ((rc = dalloc(ctx, &ctx->partials, (size_t)ctx->nchunks * 8))) return rc;
-    if ((rc = dalloc(ctx, &ctx->reduced, 8))) return rc;
+    if ((rc = dalloc(ctx, &ctx->reduced, 4 * (MAX_SPECIES + 1)))) return rc;
     if ((rc = dalloc(ctx, &ctx->shift_bits, 2))) return rc;
     MGR_HIP(hipHostMalloc((void **)&ctx->h_reduced, 8 * sizeof(double), hipHostMallocDefault));
     MGR_HIP(hipMemset(ctx->err_flag, 0, sizeof(int32_t)));
@@ -609,14 +718,16 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
     MGR_HIP(hipMemset(ctx->Ls, 0, N * L * sizeof(double)));
     MGR_HIP(hipMemset(ctx->J, 0, N * F * sizeof(double)));
 
-    hipLaunchKernelGGL(k_pop_tot, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N,
-                       (int)ctx->nspecs, (int)ctx->num, ctx->abundance, ctx->pop_tot);
-    MGR_LAUNCH_CHECK();
+    for (int64_t l = 0; l < S; l++) {
+        hipLaunchKernelGGL(k_pop_tot, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N,
+                           (int)ctx->nspecs, (int)ctx->sp_num[l], ctx->abundance, ctx->pop_tot + l * N);
+        MGR_LAUNCH_CHECK();
+    }
     if ((rc = launch_cell_tables(ctx))) return rc;
     MGR_HIP(hipStreamSynchronize(ctx->stream));
     {
-        std::vector<double> pwh(N);
-        MGR_HIP(hipMemcpy(pwh.data(), ctx->pw, N * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<double> pwh(S * N);
+        MGR_HIP(hipMemcpy(pwh.data(), ctx->pw, S * N * sizeof(double), hipMemcpyDeviceToHost));
         ctx->pw_max = *std::max_element(pwh.begin(), pwh.end());
     }
     return MGR_OK;
@@ -633,7 +744,7 @@ int free_model(mgr_ctx *ctx)
     dfree(ctx->qweights); dfree(ctx->cp_num); dfree(ctx->cp_ncol); dfree(ctx->cp_ntmp);
     dfree(ctx->cp_koff); dfree(ctx->cp_toff); dfree(ctx->cp_coff); dfree(ctx->cp_op);
     dfree(ctx->icol); dfree(ctx->jcol); dfree(ctx->tmp); dfree(ctx->Cd); dfree(ctx->Ce);
-    dfree(ctx->pw); dfree(ctx->dshift_max); dfree(ctx->W11); dfree(ctx->IW64); dfree(ctx->IW32);
+    dfree(ctx->pw); dfree(ctx->pw_cmax); dfree(ctx->dshift_max); dfree(ctx->W11); dfree(ctx->IW64); dfree(ctx->IW32);
     dfree(ctx->row32); dfree(ctx->row64); dfree(ctx->nu);
     dfree(ctx->nr_line); dfree(ctx->f_appears); dfree(ctx->f_k); dfree(ctx->f_z);
     dfree(ctx->d_lam_f_off); dfree(ctx->d_lam_f); dfree(ctx->I_bdy); dfree(ctx->pop);
@@ -759,12 +870,18 @@ static int spectral_discretisation(mgr_ctx *ctx, bool image, double width)
     hipLaunchKernelGGL(k_spectral, dim3(grid_for(N * F, 256)), dim3(256), 0, ctx->stream, N,
                        ctx->tables, image ? nullptr : ctx->pw, width, ctx->roots, ctx->nu, ctx->nr_line);
     MGR_LAUNCH_CHECK();
-    // appears_in_line_integral / corresponding_{k,z}, indexed by the unsorted
-    // position (sim_spectrum.tpp:153-184), and per transition the sorted
-    // frequency indices f whose back-map names it (used for Lambda*).
+    // appears_in_line_integral / corresponding_{l,k,z}, indexed by the unsorted
+    // position (sim_spectrum.tpp:153-184), and per line the sorted frequency
+    // indices f whose back-map names it (used for Lambda*).  Here f_k holds
+    // the line g = loff(l) + k and f_z the index l Q + z of the root's weight
+    // in the species' quadratures held one after the other.
     std::vector<int32_t> appears(F, 0), fk(F, (int32_t)F), fz(F, (int32_t)F);
     for (int64_t k = 0; k < L; k++)
-        for (int64_t z = 0; z < Q; z++) { appears[k * Q + z] = 1; fk[k * Q + z] = (int32_t)k; fz[k * Q + z] = (int32_t)z; }
+        for (int64_t z = 0; z < Q; z++) {
+            appears[k * Q + z] = 1;
+            fk[k * Q + z] = (int32_t)k;
+            fz[k * Q + z] = (int32_t)(ctx->h_tables.line_spec[k] * Q + z);
+        }
     ctx->lam_f_off.assign(L + 1, 0);
     ctx->lam_f.clear();
     for (int64_t k = 0; k < L; k++) {
@@ -829,10 +946,13 @@ int mgr_compute_LTE_level_populations(mgr_ctx *ctx)
 {
     if (!ctx) return MGR_ERR_ARGUMENT;
     MGR_HIP(hipSetDevice(ctx->device));
-    hipLaunchKernelGGL(k_lte, dim3(grid_for(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N,
-                       (int)ctx->V, (int)ctx->nspecs, (int)ctx->num, ctx->abundance, ctx->temperature,
-                       ctx->lweight, ctx->energy, ctx->pop, ctx->pop_tot);
-    MGR_LAUNCH_CHECK();
+    for (int64_t l = 0; l < ctx->S; l++) {   // every species (sim_lines.tpp:10-16)
+        hipLaunchKernelGGL(k_lte, dim3(grid_for(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N,
+                           (int)ctx->sp_V[l], (int)ctx->nspecs, (int)ctx->sp_num[l], ctx->abundance,
+                           ctx->temperature, ctx->lweight + ctx->sp_voff[l], ctx->energy + ctx->sp_voff[l],
+                           ctx->pop + ctx->N * ctx->sp_voff[l], ctx->pop_tot + ctx->N * l);
+        MGR_LAUNCH_CHECK();
+    }
     int rc = launch_emissivity_opacity(ctx);
     if (rc) return rc;
     MGR_HIP(hipStreamSynchronize(ctx->stream));

@@ -45,7 +45,9 @@ __global__ void k_lambda_sum(int64_t N, int L, int nchunk, const double *__restr
 // Jeff = Jlin - h/(4 pi) Ls n_i, Jdif = h/(4 pi) Ls n_i.
 // With band elements (band_cell non-null) every element of (p, k) enters:
 // diff = Ls(p,k) n_i(p) + sum over the slots of p of Ls(slot, k) n_i(cell).
-__global__ void k_jeff(int64_t N, int L, int Q, int F, int V, const double *__restrict__ J,
+// One species: its L lines are lines loff .. loff+L-1 of the per-line arrays
+// [N][Lst]; pop, qw and irad are the species' own.
+__global__ void k_jeff(int64_t N, int L, int Lst, int loff, int Q, int F, int V, const double *__restrict__ J,
                        const int32_t *__restrict__ nr_line, const double *__restrict__ qw,
                        const int32_t *__restrict__ irad, const double *__restrict__ pop,
                        const double *__restrict__ Ls, double *__restrict__ Jlin,
@@ -56,17 +58,18 @@ __global__ void k_jeff(int64_t N, int L, int Q, int F, int V, const double *__re
     if (t >= N * L) return;
     const int64_t p = t / L;
     const int k = (int)(t % L);
+    const int64_t pk = p * Lst + loff + k;
     double jl = 0.0;
-    for (int z = 0; z < Q; z++) jl += qw[z] * J[p * F + nr_line[(p * L + k) * Q + z]];
-    double diff = Ls[p * L + k] * pop[p * V + irad[k]];
+    for (int z = 0; z < Q; z++) jl += qw[z] * J[p * F + nr_line[pk * Q + z]];
+    double diff = Ls[pk] * pop[p * V + irad[k]];
     if (band_cell)
         for (int64_t sl = p * S; sl < (p + 1) * S; sl++) {
             const int32_t c = band_cell[sl];
-            if (c >= 0) diff += band_Ls[sl * L + k] * pop[(int64_t)c * V + irad[k]];
+            if (c >= 0) diff += band_Ls[sl * Lst + loff + k] * pop[(int64_t)c * V + irad[k]];
         }
-    Jlin[p * L + k] = jl;
-    Jeff[p * L + k] = jl - HH_OVER_FOUR_PI * diff;
-    Jdif[p * L + k] = HH_OVER_FOUR_PI * diff;
+    Jlin[pk] = jl;
+    Jeff[pk] = jl - HH_OVER_FOUR_PI * diff;
+    Jdif[pk] = HH_OVER_FOUR_PI * diff;
 }
 
 // search (src/Functions/interpolation.tpp:14-54)
@@ -84,9 +87,12 @@ __device__ __forceinline__ int search_table(const double *x, int n, double value
     return stop;
 }
 
+// One species' rate equations: its V levels, L lines (line k of its own at
+// p*Lst + k of Jeff and Ls) and ncolpar collision partners; the arrays point
+// at the species' own parts.
 struct StateqArgs {
     int64_t N;
-    int V, L, nspecs, ncolpar;
+    int V, L, Lst, nspecs, ncolpar;
     const double *Jeff, *Ls, *A, *Bs, *Ba, *abundance, *T, *pop_tot;
     const int32_t *irad, *jrad;
     const int32_t *cp_num, *cp_ncol, *cp_ntmp, *cp_koff, *cp_toff, *cp_coff, *cp_op, *icol, *jcol;
@@ -115,7 +121,7 @@ __device__ __forceinline__ void stateq_assemble(const StateqArgs &a, int64_t p, 
     const double *pop = a.prev1 + p * Vn;
     // radiative transitions (transpose convention of the reference)
     for (int k = 0; k < a.L; k++) {
-        const double Je = a.Jeff[p * a.L + k];
+        const double Je = a.Jeff[p * a.Lst + k];
         const double v_IJ = a.A[k] + a.Bs[k] * Je;
         const double v_JI = a.Ba[k] * Je;
         const int I = a.irad[k], Jl = a.jrad[k];
@@ -126,7 +132,7 @@ __device__ __forceinline__ void stateq_assemble(const StateqArgs &a, int64_t p, 
     for (int k = 0; k < a.L; k++) {
         const int I = a.irad[k], Jl = a.jrad[k];
         const double opacity = HH_OVER_FOUR_PI * (pop[Jl] * a.Ba[k] - pop[I] * a.Bs[k]);
-        const double v_IJ = -a.Ls[p * a.L + k] * opacity;
+        const double v_IJ = -a.Ls[p * a.Lst + k] * opacity;
         if (Jl != Vn - 1) M[Jl * Vn + I] += v_IJ;
         if (I != Vn - 1) M[I * Vn + I] += -v_IJ;
     }
@@ -570,6 +576,22 @@ __global__ void k_ng_apply(int64_t N, int V, const int32_t *__restrict__ cell_mo
 
 __global__ void k_zero_flag(int32_t *flag) { *flag = 0; }
 
+// The driver's convergence words from the S species' words at 4 (l+1): the
+// largest fraction, mean and maximum change; the error word is shared.
+__global__ void k_conv_species(int S, double *reduced)
+{
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int l = 0; l < S; l++) {
+        a = fmax(a, reduced[4 * (l + 1) + 0]);
+        b = fmax(b, reduced[4 * (l + 1) + 1]);
+        c = fmax(c, reduced[4 * (l + 1) + 2]);
+    }
+    reduced[0] = a;
+    reduced[1] = b;
+    reduced[2] = c;
+    reduced[3] = reduced[4 * S + 3];
+}
+
 // ---- Ng acceleration without allocation ---------------------------------------
 // mgr_ng_acceleration_noalloc runs the kernels above on caller-owned arrays of
 // one model.  Its scratch is one file-scope device array per GPU: the chunk
@@ -619,13 +641,37 @@ int launch_lambda(mgr_ctx *ctx)
 
 int launch_jeff(mgr_ctx *ctx)
 {
-    hipLaunchKernelGGL(k_jeff, dim3(grid_for(ctx->N * ctx->L, 256)), dim3(256), 0, ctx->stream,
-                       ctx->N, (int)ctx->L, (int)ctx->Q, (int)ctx->F, (int)ctx->V, ctx->J,
-                       ctx->nr_line, ctx->qweights, ctx->irad, ctx->pop, ctx->Ls, ctx->Jlin,
-                       ctx->Jeff, ctx->Jdif, ctx->band_S, ctx->nband > 0 ? ctx->band_cell : nullptr,
-                       ctx->band_Ls);
-    MGR_LAUNCH_CHECK();
+    for (int64_t l = 0; l < ctx->S; l++) {   // Simulation::compute_Jeff loops the species
+        const int64_t Ll = ctx->sp_L[l], lo = ctx->sp_loff[l];
+        hipLaunchKernelGGL(k_jeff, dim3(grid_for(ctx->N * Ll, 256)), dim3(256), 0, ctx->stream,
+                           ctx->N, (int)Ll, (int)ctx->L, (int)lo, (int)ctx->Q, (int)ctx->F, (int)ctx->sp_V[l],
+                           ctx->J, ctx->nr_line, ctx->qweights + l * ctx->Q, ctx->irad + lo,
+                           ctx->pop + ctx->N * ctx->sp_voff[l], ctx->Ls, ctx->Jlin, ctx->Jeff, ctx->Jdif,
+                           ctx->band_S, ctx->nband > 0 ? ctx->band_cell : nullptr, ctx->band_Ls);
+        MGR_LAUNCH_CHECK();
+    }
     return MGR_OK;
+}
+
+// The rate-equation arguments of species l.
+static StateqArgs stateq_args(mgr_ctx *ctx, int64_t l)
+{
+    StateqArgs a;
+    const int64_t lo = ctx->sp_loff[l], vo = ctx->N * ctx->sp_voff[l], c0 = ctx->sp_cp0[l];
+    a.N = ctx->N; a.V = (int)ctx->sp_V[l]; a.L = (int)ctx->sp_L[l]; a.Lst = (int)ctx->L;
+    a.nspecs = (int)ctx->nspecs; a.ncolpar = (int)ctx->sp_ncp[l];
+    a.Jeff = ctx->Jeff + lo; a.Ls = ctx->Ls + lo; a.A = ctx->A + lo; a.Bs = ctx->Bs + lo; a.Ba = ctx->Ba + lo;
+    a.abundance = ctx->abundance; a.T = ctx->temperature; a.pop_tot = ctx->pop_tot + ctx->N * l;
+    a.irad = ctx->irad + lo; a.jrad = ctx->jrad + lo;
+    a.cp_num = ctx->cp_num + c0; a.cp_ncol = ctx->cp_ncol + c0; a.cp_ntmp = ctx->cp_ntmp + c0;
+    a.cp_koff = ctx->cp_koff + c0; a.cp_toff = ctx->cp_toff + c0; a.cp_coff = ctx->cp_coff + c0;
+    a.cp_op = ctx->cp_op + c0;
+    a.icol = ctx->icol; a.jcol = ctx->jcol; a.tmp = ctx->tmp; a.Cd = ctx->Cd; a.Ce = ctx->Ce;
+    a.pop = ctx->pop + vo; a.prev1 = ctx->prev1 + vo; a.prev2 = ctx->prev2 + vo; a.prev3 = ctx->prev3 + vo;
+    a.err_flag = ctx->err_flag;
+    a.cell_model = ctx->d_cell_model;
+    a.m_active = ctx->d_m_active;
+    return a;
 }
 
 // The rate equations with band elements (one model): factorise the block
@@ -674,24 +720,22 @@ static int launch_stateq_band(mgr_ctx *ctx, const StateqArgs &a)
     return MGR_OK;
 }
 
+static int launch_stateq_species(mgr_ctx *ctx, const StateqArgs &a);
+
 int launch_stateq(mgr_ctx *ctx)
 {
-    StateqArgs a;
-    a.N = ctx->N; a.V = (int)ctx->V; a.L = (int)ctx->L; a.nspecs = (int)ctx->nspecs;
-    a.ncolpar = (int)ctx->ncolpar;
-    a.Jeff = ctx->Jeff; a.Ls = ctx->Ls; a.A = ctx->A; a.Bs = ctx->Bs; a.Ba = ctx->Ba;
-    a.abundance = ctx->abundance; a.T = ctx->temperature; a.pop_tot = ctx->pop_tot;
-    a.irad = ctx->irad; a.jrad = ctx->jrad;
-    a.cp_num = ctx->cp_num; a.cp_ncol = ctx->cp_ncol; a.cp_ntmp = ctx->cp_ntmp;
-    a.cp_koff = ctx->cp_koff; a.cp_toff = ctx->cp_toff; a.cp_coff = ctx->cp_coff; a.cp_op = ctx->cp_op;
-    a.icol = ctx->icol; a.jcol = ctx->jcol; a.tmp = ctx->tmp; a.Cd = ctx->Cd; a.Ce = ctx->Ce;
-    a.pop = ctx->pop; a.prev1 = ctx->prev1; a.prev2 = ctx->prev2; a.prev3 = ctx->prev3;
-    a.err_flag = ctx->err_flag;
-    a.cell_model = ctx->d_cell_model;
-    a.m_active = ctx->d_m_active;
+    if (ctx->nband > 0) return launch_stateq_band(ctx, stateq_args(ctx, 0));
+    for (int64_t l = 0; l < ctx->S; l++) {   // Lines::iteration_using_statistical_equilibrium
+        const int rc = launch_stateq_species(ctx, stateq_args(ctx, l));
+        if (rc) return rc;
+    }
+    return MGR_OK;
+}
+
+static int launch_stateq_species(mgr_ctx *ctx, const StateqArgs &a)
+{
     const dim3 g(grid_for(ctx->N, 128)), b(128);
-    if (ctx->nband > 0) return launch_stateq_band(ctx, a);
-    switch (ctx->V) {
+    switch (a.V) {
         case 2: hipLaunchKernelGGL(k_stateq<2>, g, b, 0, ctx->stream, a); break;
         case 3: hipLaunchKernelGGL(k_stateq<3>, g, b, 0, ctx->stream, a); break;
         case 4: hipLaunchKernelGGL(k_stateq<4>, g, b, 0, ctx->stream, a); break;
@@ -700,7 +744,7 @@ int launch_stateq(mgr_ctx *ctx)
         case 7: hipLaunchKernelGGL(k_stateq<7>, g, b, 0, ctx->stream, a); break;
         case 8: hipLaunchKernelGGL(k_stateq<8>, g, b, 0, ctx->stream, a); break;
         default:
-            if (ctx->V > VMAX_GENERIC)
+            if (a.V > VMAX_GENERIC)
                 return ctx->fail(MGR_ERR_UNSUPPORTED, "more than 32 levels");
             hipLaunchKernelGGL(k_stateq_generic, g, b, 0, ctx->stream, a);
     }
@@ -708,30 +752,47 @@ int launch_stateq(mgr_ctx *ctx)
     return MGR_OK;
 }
 
+// check_for_convergence of every species; with several species each species'
+// words go to reduced[4 (l+1) ..] and the driver's words are the largest over
+// the species (the ALI goes on while any species has not converged,
+// sim_lines.tpp:130-150).
 int launch_convergence(mgr_ctx *ctx)
 {
-    hipLaunchKernelGGL(k_conv_partial, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, (int)ctx->V,
-                       ctx->pop_prec, ctx->d_chunk_cell, ctx->d_cell_model, ctx->d_m_cell_off,
-                       ctx->pop, ctx->prev1, ctx->pop_tot, ctx->partials);
-    MGR_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_conv_final, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->M, ctx->d_m_chunk_off,
-                       ctx->partials, ctx->d_m_active, ctx->d_mconv, ctx->reduced, ctx->err_flag);
-    MGR_LAUNCH_CHECK();
+    for (int64_t l = 0; l < ctx->S; l++) {
+        const int64_t vo = ctx->N * ctx->sp_voff[l];
+        hipLaunchKernelGGL(k_conv_partial, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream,
+                           (int)ctx->sp_V[l], ctx->pop_prec, ctx->d_chunk_cell, ctx->d_cell_model, ctx->d_m_cell_off,
+                           ctx->pop + vo, ctx->prev1 + vo, ctx->pop_tot + ctx->N * l, ctx->partials);
+        MGR_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_conv_final, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->M, ctx->d_m_chunk_off,
+                           ctx->partials, ctx->d_m_active, ctx->d_mconv,
+                           ctx->S > 1 ? ctx->reduced + 4 * (l + 1) : ctx->reduced, ctx->err_flag);
+        MGR_LAUNCH_CHECK();
+    }
+    if (ctx->S > 1) {
+        hipLaunchKernelGGL(k_conv_species, dim3(1), dim3(1), 0, ctx->stream, (int)ctx->S, ctx->reduced);
+        MGR_LAUNCH_CHECK();
+    }
     return MGR_OK;
 }
 
+// Ng acceleration of every species in turn (Lines::iteration_using_Ng_acceleration).
 int launch_ng(mgr_ctx *ctx)
 {
-    hipLaunchKernelGGL(k_ng_partial, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, (int)ctx->V,
-                       ctx->d_chunk_cell, ctx->pop, ctx->prev1, ctx->prev2, ctx->prev3, ctx->partials);
-    MGR_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_ng_final, dim3(grid_for(ctx->M, 64)), dim3(64), 0, ctx->stream, ctx->M,
-                       ctx->d_m_chunk_off, ctx->partials, ctx->d_mng);
-    MGR_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_ng_apply, dim3(grid_for(ctx->N * ctx->V, 256)), dim3(256), 0, ctx->stream, ctx->N,
-                       (int)ctx->V, ctx->d_cell_model, ctx->d_m_active, ctx->d_mng, ctx->pop, ctx->prev1,
-                       ctx->prev2, ctx->prev3);
-    MGR_LAUNCH_CHECK();
+    for (int64_t l = 0; l < ctx->S; l++) {
+        const int64_t vo = ctx->N * ctx->sp_voff[l], V = ctx->sp_V[l];
+        hipLaunchKernelGGL(k_ng_partial, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, (int)V,
+                           ctx->d_chunk_cell, ctx->pop + vo, ctx->prev1 + vo, ctx->prev2 + vo, ctx->prev3 + vo,
+                           ctx->partials);
+        MGR_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_ng_final, dim3(grid_for(ctx->M, 64)), dim3(64), 0, ctx->stream, ctx->M,
+                           ctx->d_m_chunk_off, ctx->partials, ctx->d_mng);
+        MGR_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_ng_apply, dim3(grid_for(ctx->N * V, 256)), dim3(256), 0, ctx->stream, ctx->N,
+                           (int)V, ctx->d_cell_model, ctx->d_m_active, ctx->d_mng, ctx->pop + vo, ctx->prev1 + vo,
+                           ctx->prev2 + vo, ctx->prev3 + vo);
+        MGR_LAUNCH_CHECK();
+    }
     return MGR_OK;
 }
 

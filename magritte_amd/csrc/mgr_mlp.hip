@@ -867,6 +867,7 @@ int mgr::launch_features(mgr_ctx *ctx, float *dX)
 extern "C" int mgr_features(mgr_ctx *ctx, float *X)
 {
     if (!ctx || !X) return MGR_ERR_ARGUMENT;
+    if (ctx->S > 1) return ctx->fail(MGR_ERR_UNSUPPORTED, "the warm-start features are defined for one line-producing species");
     if (!ctx->J_set) return ctx->fail(MGR_ERR_STATE, "features need a radiation field and J_eff first");
     MGR_HIP(hipSetDevice(ctx->device));
     float *dX = nullptr;
@@ -882,7 +883,8 @@ extern "C" int mgr_features(mgr_ctx *ctx, float *X)
 extern "C" int mgr_warm_start(mgr_ctx *ctx, mgr_mlp *mlp)
 {
     if (!ctx || !mlp) return MGR_ERR_ARGUMENT;
-    if (ctx->V != OUT) return ctx->fail(MGR_ERR_UNSUPPORTED, "the warm-start network predicts 7 levels");
+    if (ctx->V != OUT || ctx->S > 1)
+        return ctx->fail(MGR_ERR_UNSUPPORTED, "the warm-start network predicts the 7 levels of one species");
     if (!ctx->J_set) return ctx->fail(MGR_ERR_STATE, "warm start needs a radiation field and J_eff first");
     if (ctx->device != mlp->device) return ctx->fail(MGR_ERR_ARGUMENT, "context and network on different devices");
     MGR_HIP(hipSetDevice(ctx->device));

@@ -766,7 +766,7 @@ __device__ __forceinline__ void lambda_partials(const RFArgs<real> &a, int32_t o
     double term = 0.0;
     if (k < L) {
         const double freq_line = a.freq_k[k];
-        const double iw = 1.0 / (freq_line * a.pw[o]);
+        const double iw = 1.0 / (freq_line * a.pw[(int64_t)a.tb->line_spec[k] * a.N + o]);   // the line's species
         const double x = iw * (nu_f - freq_line);
         const double prof = iw * INVERSE_SQRT_PI * (1.0 / exp(x * x));
         term = a.A_k[k] * a.qw[a.f_z[f]] * (nu_f * prof * lam_f);

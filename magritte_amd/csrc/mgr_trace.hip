@@ -593,7 +593,7 @@ int launch_step_masks(mgr_ctx *ctx)
                        (int)ctx->F, nchunk, rm, ctx->tables, ctx->nu, ctx->shift_min, ctx->shift_max, ctx->pw_max,
                        win, cnt);
     hipLaunchKernelGGL(k_step_masks, dim3((unsigned)((N * ctx->R + 127) / 128)), dim3(128), 0, ctx->stream, N,
-                       (int)ctx->R, nchunk, ctx->d_cell_model, ctx->d_cell_qbase, ctx->d_m_R, ctx->pw,
+                       (int)ctx->R, nchunk, ctx->d_cell_model, ctx->d_cell_qbase, ctx->d_m_R, ctx->pw_cmax,
                        ctx->dshift_max, ctx->seg_off,
                        ctx->nseg, ctx->seg_cell, ctx->seg_shift, win, cnt, ctx->step_mask);
     // the largest dz' bounds the row closings of out-of-window sub-points

@@ -138,7 +138,19 @@ class Context:
             raise HipError(f"mgr_create failed ({rc}): {self.lib.mgr_last_error(None).decode()}")
         self.h = h
         self.N, self.R, self.F = model.ncells, model.nrays, model.nfreqs
-        self.L, self.V, self.Q, self.NB = model.nrad, model.nlev, model.nquads, model.nboundary
+        self.L, self.V, self.Q, self.NB = model.nlines, model.nlev, model.nquads, model.nboundary
+        # levels of each line-producing species; populations hold them species after species
+        self.Vs = [ld.nlev for ld in model.species_linedata]
+
+    def _split(self, a):
+        """A population array as (N, V) for one species, else a list per species."""
+        if len(self.Vs) == 1:
+            return a.reshape(self.N, self.V)
+        out, o = [], 0
+        for V in self.Vs:
+            out.append(a[o:o + self.N * V].reshape(self.N, V))
+            o += self.N * V
+        return out
 
     def close(self):
         if getattr(self, "h", None):
@@ -266,20 +278,24 @@ class Context:
         return a.reshape(self.N, self.L), b.reshape(self.N, self.L), c.reshape(self.N, self.L)
 
     def populations(self):
-        a = np.zeros(self.N * self.V)
+        a = np.zeros(self.N * sum(self.Vs))
         self._check(self.lib.mgr_get_populations(self.h, _p(a)), "get_populations")
-        return a.reshape(self.N, self.V)
+        return self._split(a)
 
     def set_populations(self, pops):
+        if isinstance(pops, (list, tuple)):
+            pops = np.concatenate([np.ravel(x) for x in pops])
         a = np.ascontiguousarray(pops, dtype=np.float64).ravel()
         self._check(self.lib.mgr_set_populations(self.h, _p(a)), "set_populations")
 
     def population_history(self):
-        a, b, c = (np.zeros(self.N * self.V) for _ in range(3))
+        a, b, c = (np.zeros(self.N * sum(self.Vs)) for _ in range(3))
         self._check(self.lib.mgr_get_population_history(self.h, _p(a), _p(b), _p(c)), "get_population_history")
-        return tuple(x.reshape(self.N, self.V) for x in (a, b, c))
+        return tuple(self._split(x) for x in (a, b, c))
 
     def set_population_history(self, p1, p2, p3):
+        p1, p2, p3 = (np.concatenate([np.ravel(y) for y in x]) if isinstance(x, (list, tuple)) else x
+                      for x in (p1, p2, p3))
         arrs = [np.ascontiguousarray(x, dtype=np.float64).ravel() for x in (p1, p2, p3)]
         self._check(self.lib.mgr_set_population_history(self.h, *[_p(x) for x in arrs]),
                     "set_population_history")
@@ -425,7 +441,8 @@ class BatchContext(Context):
         self.model = m0
         self.N = sum(m.ncells for m in self.models)
         self.R = max(m.nrays for m in self.models)
-        self.F, self.L, self.V, self.Q = m0.nfreqs, m0.nrad, m0.nlev, m0.nquads
+        self.F, self.L, self.V, self.Q = m0.nfreqs, m0.nlines, m0.nlev, m0.nquads
+        self.Vs = [m0.nlev]
         self.NB = sum(m.nboundary for m in self.models)
         self.M = len(self.models)
         off = np.zeros(self.M + 1, np.int64)

@@ -1,0 +1,129 @@
+"""GPU parity with several line-producing species (Parameters::nlspecs > 1)
+against the oracle, through the C-ABI.
+
+Models: a 4-level rotor (lines at 115-346 GHz) with a two-level species at
+700 GHz, and the same with the second species at 200 GHz (its line between
+the rotor's) and a lighter mass, where the reference's species back-maps
+(the spectral grid on the merged sorted line list, get_line_width's species
+from a bin index; sim_spectrum.tpp:23-45, simulation.tpp:363-373) matter.
+Bars as for one species: grids bit-exact, J and Lambda* 1e-5 (FP32) / 1e-10
+(FP64), populations 1e-5 at the 1e-6 criterion with the same iteration count.
+"""
+import numpy as np
+import pytest
+
+from magritte_amd.setup import synthetic
+from oracle.py_oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 1e-5, "fp64": 1e-10}
+
+MODELS = {
+    "apart": lambda: synthetic.two_species_model(n=6, seed=5, nside=1),
+    "interleaved": lambda: synthetic.two_species_model(n=6, seed=6, nside=1, second_nu=200.0e9,
+                                                       second_mass=5.0),
+}
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+@pytest.fixture(scope="module", params=list(MODELS))
+def pair(request):
+    from magritte_amd.hip import Context
+    print(f"\n[species] {request.param}", flush=True)
+    m = MODELS[request.param]()
+    return request.param, m, Oracle(m).setup(), Context(m).setup()
+
+
+def test_grids_and_lte(pair):
+    _, m, orc, ctx = pair
+    nu_o, nr_o = orc.frequencies()
+    nu_g, nr_g = ctx.frequencies()
+    assert np.array_equal(nu_g, nu_o)
+    assert np.array_equal(nr_g.ravel(), nr_o.ravel())
+    assert rel(ctx.boundary_intensities(), orc.boundary_intensities()) < 1e-14
+    for pg, po in zip(ctx.populations(), orc.populations()):
+        assert rel(pg, po) < 1e-13
+    eg, og = ctx.emissivity_opacity()
+    eo, oo = orc.emissivity_opacity()
+    assert rel(eg, eo) < 1e-13 and rel(og, oo) < 1e-13
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_radiation_field_and_lambda(pair, precision):
+    name, m, orc, ctx = pair
+    orc.compute_radiation_field()
+    ctx.compute_radiation_field(precision)
+    eJ = rel(ctx.J(), orc.J())
+    Lo = orc.lambda_diag()
+    mask = Lo > 0
+    eL = rel(ctx.lambda_diag()[mask], Lo[mask])
+    print(f"[species] {name} {precision}: J {eJ:.3e}, Lambda* {eL:.3e}", flush=True)
+    assert eJ < TOL[precision] and eL < TOL[precision]
+    orc.compute_Jeff()
+    ctx.compute_Jeff()
+    Jl_g, Je_g, Jd_g = ctx.Jeff()
+    Jl_o, Je_o, Jd_o = orc.Jeff()
+    assert rel(Jl_g, Jl_o) < TOL[precision]
+    scale = np.maximum(np.abs(Jl_o), np.abs(Jd_o))
+    assert float(np.max(np.abs(Je_g - Je_o) / scale)) < TOL[precision]
+
+
+def test_one_rate_equation_solve(pair):
+    """Both species' rate equations from identical J_eff and Lambda* (FP64 field)."""
+    name, m, orc, ctx = pair
+    orc.compute_radiation_field()
+    orc.compute_Jeff()
+    ctx.compute_radiation_field("fp64")
+    ctx.compute_Jeff()
+    orc.update_using_statistical_equilibrium()
+    conv_o = orc.check_for_convergence()
+    conv_g = ctx.compute_level_populations_from_stateq()
+    for pg, po in zip(ctx.populations(), orc.populations()):
+        assert rel(pg, po) < 1e-9
+    # the driver's figures are the largest over the species
+    fo = orc.convergence()
+    assert conv_g[0] == pytest.approx(fo[0], rel=1e-9, abs=1e-12)
+    assert conv_g[2] == pytest.approx(fo[2], rel=1e-6)
+
+
+@pytest.mark.parametrize("case,precision", [("apart", "fp32"), ("apart", "fp64"), ("subthermal", "fp32"),
+                                            ("subthermal", "fp64")])
+def test_ali_matches_oracle(case, precision):
+    """subthermal (n0 = 1e12 m^-3) needs enough iterations for Ng steps, which
+    run species by species."""
+    from magritte_amd.hip import Context
+    if case == "apart":
+        m = MODELS["apart"]()
+    else:
+        m = synthetic.two_species_model(n=6, seed=3, nside=1, n0=1.0e12, T0=1000.0, abundance=1.0e-3)
+    orc = Oracle(m).setup()
+    ctx = Context(m).setup()
+    it_o = orc.compute_level_populations(True, 60, 1)
+    it_g = ctx.compute_level_populations(True, 60, "max", precision)
+    errs = [rel(pg, po) for pg, po in zip(ctx.populations(), orc.populations())]
+    print(f"[species] ALI {case} {precision}: oracle {it_o}, GPU {it_g} iterations; populations {errs}",
+          flush=True)
+    assert abs(it_g - it_o) <= 1
+    if case == "subthermal":
+        assert it_o >= 6
+    assert max(errs) < 1e-5
+
+
+def test_refusals():
+    from magritte_amd.hip import BatchContext, Context, HipError
+    m = synthetic.two_species_model(n=4, nside=1)
+    m.n_off_diag = 2
+    with pytest.raises(HipError, match="n_off_diag"):
+        Context(m)
+    with pytest.raises(HipError, match="batch"):
+        BatchContext([synthetic.two_species_model(n=4, nside=1, seed=s) for s in (1, 2)])
+    ctx = Context(synthetic.two_species_model(n=4, nside=1)).setup()
+    ctx.compute_radiation_field("fp32")
+    ctx.compute_Jeff()
+    with pytest.raises(HipError, match="species"):
+        ctx.features()
