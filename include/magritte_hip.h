@@ -187,6 +187,14 @@ int  mgr_batch_info(mgr_ctx *ctx, int64_t *nmodels, int64_t *cell_offsets /* [nm
 /* Per model, after mgr_compute_level_populations: the iteration at which it
  * converged (or the last one) and its last convergence statistics.          */
 int  mgr_batch_convergence(mgr_ctx *ctx, int64_t *iterations, mgr_conv *conv);
+/* The line-producing species of the context (Parameters::nlspecs) and each
+ * one's nlev and nrad (arrays of nlspecs, may be NULL).                      */
+int  mgr_species_info(mgr_ctx *ctx, int64_t *nlspecs, int64_t *nlev, int64_t *nrad);
+/* Species l's check_for_convergence figures of the last level-population
+ * update (LineProducingSpecies::fraction_not_converged, relative_change_mean,
+ * relative_change_max; lineProducingSpecies.tpp:120-171); the mgr_conv of the
+ * driver entry points holds the largest over the species.                   */
+int  mgr_species_convergence(mgr_ctx *ctx, int64_t l, mgr_conv *conv);
 void mgr_destroy(mgr_ctx *ctx);
 const char *mgr_last_error(const mgr_ctx *ctx);   /* ctx may be NULL (create errors) */
 int  mgr_device_count(int *count);

@@ -24,10 +24,8 @@ void Simulation::check(int rc, const char *what)
 void Simulation::ensure_context()
 {
     if (ctx_) return;
-    if (lines.lineProducingSpecies.size() != 1)
-        throw std::runtime_error("the GPU path supports one line-producing species");
-    const LineProducingSpecies &ls = lines.lineProducingSpecies[0];
-    const Linedata &ld = ls.linedata;
+    const size_t S = lines.lineProducingSpecies.size();
+    if (S < 1) throw std::runtime_error("the model has no line-producing species");
     const Cells &cells = geometry.cells;
     const size_t N = cells.position.size();
     const size_t R = geometry.rays.rays.size();
@@ -40,20 +38,27 @@ void Simulation::ensure_context()
     need(thermodynamics.temperature.gas.size() == N, "thermodynamics.temperature.gas");
     need(thermodynamics.turbulence.vturb2.size() == N, "thermodynamics.turbulence.vturb2");
     need(chemistry.species.abundance.size() == N, "chemistry.species.abundance");
-    for (const auto &row : chemistry.species.abundance)
-        need(row.size() == chemistry.species.abundance[0].size() && (long)row.size() > ld.num,
-             "chemistry.species.abundance rows");
-    need(ld.irad.size() == (size_t)ld.nrad && ld.jrad.size() == (size_t)ld.nrad && ld.A.size() == (size_t)ld.nrad &&
-             ld.Bs.size() == (size_t)ld.nrad && ld.Ba.size() == (size_t)ld.nrad &&
-             ld.frequency.size() == (size_t)ld.nrad,
-         "linedata radiative arrays (nrad)");
-    need(ld.energy.size() == (size_t)ld.nlev && ld.weight.size() == (size_t)ld.nlev, "linedata level arrays (nlev)");
-    need(ls.quadrature.weights.size() == ls.quadrature.roots.size(), "quadrature");
-    for (const auto &cp : ld.colpar) {
-        need(cp.jcol.size() == cp.icol.size(), "collision partner jcol");
-        need(cp.Cd.size() == cp.tmp.size() && cp.Ce.size() == cp.tmp.size(), "collision partner Cd/Ce rows");
-        for (size_t t = 0; t < cp.tmp.size(); t++)
-            need(cp.Cd[t].size() == cp.icol.size() && cp.Ce[t].size() == cp.icol.size(), "collision partner Cd/Ce");
+    const size_t nq = lines.lineProducingSpecies[0].quadrature.roots.size();
+    for (const LineProducingSpecies &ls : lines.lineProducingSpecies) {
+        const Linedata &ld = ls.linedata;
+        for (const auto &row : chemistry.species.abundance)
+            need(row.size() == chemistry.species.abundance[0].size() && (long)row.size() > ld.num,
+                 "chemistry.species.abundance rows");
+        need(ld.irad.size() == (size_t)ld.nrad && ld.jrad.size() == (size_t)ld.nrad &&
+                 ld.A.size() == (size_t)ld.nrad && ld.Bs.size() == (size_t)ld.nrad &&
+                 ld.Ba.size() == (size_t)ld.nrad && ld.frequency.size() == (size_t)ld.nrad,
+             "linedata radiative arrays (nrad)");
+        need(ld.energy.size() == (size_t)ld.nlev && ld.weight.size() == (size_t)ld.nlev,
+             "linedata level arrays (nlev)");
+        need(ls.quadrature.weights.size() == ls.quadrature.roots.size() && ls.quadrature.roots.size() == nq,
+             "quadrature (nquads, the same for every species)");
+        for (const auto &cp : ld.colpar) {
+            need(cp.jcol.size() == cp.icol.size(), "collision partner jcol");
+            need(cp.Cd.size() == cp.tmp.size() && cp.Ce.size() == cp.tmp.size(), "collision partner Cd/Ce rows");
+            for (size_t t = 0; t < cp.tmp.size(); t++)
+                need(cp.Cd[t].size() == cp.icol.size() && cp.Ce[t].size() == cp.icol.size(),
+                     "collision partner Cd/Ce");
+        }
     }
 
     Double1 pos(3 * N), vel(3 * N), rays(3 * R), abundance;
@@ -75,20 +80,57 @@ void Simulation::ensure_context()
 
     std::vector<int64_t> n_nb64(n_nb.begin(), n_nb.end()), nbs64(nbs.begin(), nbs.end());
     std::vector<int64_t> b2c(geometry.boundary.boundary2cell_nr.begin(), geometry.boundary.boundary2cell_nr.end());
-    std::vector<int64_t> irad(ld.irad.begin(), ld.irad.end()), jrad(ld.jrad.begin(), ld.jrad.end());
-    std::vector<int64_t> cp_num, cp_ncol, cp_ntmp, icol, jcol;
-    std::string cp_op;
-    Double1 tmp, Cd, Ce;
-    for (const auto &cp : ld.colpar) {
-        cp_num.push_back(cp.num_col_partner);
-        cp_ncol.push_back((int64_t)cp.icol.size());
-        cp_ntmp.push_back((int64_t)cp.tmp.size());
-        cp_op.push_back(cp.orth_or_para_H2.empty() ? 'n' : cp.orth_or_para_H2[0]);
-        icol.insert(icol.end(), cp.icol.begin(), cp.icol.end());
-        jcol.insert(jcol.end(), cp.jcol.begin(), cp.jcol.end());
-        tmp.insert(tmp.end(), cp.tmp.begin(), cp.tmp.end());
-        for (const auto &row : cp.Cd) Cd.insert(Cd.end(), row.begin(), row.end());
-        for (const auto &row : cp.Ce) Ce.insert(Ce.end(), row.begin(), row.end());
+
+    // every species' line data, flattened as the C-ABI takes it
+    struct SpeciesArrays {
+        std::vector<int64_t> irad, jrad, cp_num, cp_ncol, cp_ntmp, icol, jcol;
+        std::string cp_op;
+        Double1 tmp, Cd, Ce;
+    };
+    std::vector<SpeciesArrays> sa(S);
+    std::vector<mgr_species_view> sv(S);
+    for (size_t l = 0; l < S; l++) {
+        const LineProducingSpecies &ls = lines.lineProducingSpecies[l];
+        const Linedata &ld = ls.linedata;
+        SpeciesArrays &a = sa[l];
+        a.irad.assign(ld.irad.begin(), ld.irad.end());
+        a.jrad.assign(ld.jrad.begin(), ld.jrad.end());
+        for (const auto &cp : ld.colpar) {
+            a.cp_num.push_back(cp.num_col_partner);
+            a.cp_ncol.push_back((int64_t)cp.icol.size());
+            a.cp_ntmp.push_back((int64_t)cp.tmp.size());
+            a.cp_op.push_back(cp.orth_or_para_H2.empty() ? 'n' : cp.orth_or_para_H2[0]);
+            a.icol.insert(a.icol.end(), cp.icol.begin(), cp.icol.end());
+            a.jcol.insert(a.jcol.end(), cp.jcol.begin(), cp.jcol.end());
+            a.tmp.insert(a.tmp.end(), cp.tmp.begin(), cp.tmp.end());
+            for (const auto &row : cp.Cd) a.Cd.insert(a.Cd.end(), row.begin(), row.end());
+            for (const auto &row : cp.Ce) a.Ce.insert(a.Ce.end(), row.begin(), row.end());
+        }
+        mgr_species_view &w = sv[l];
+        w.num = ld.num;
+        w.inverse_mass = ld.inverse_mass;
+        w.nlev = ld.nlev;
+        w.nrad = ld.nrad;
+        w.energy = ld.energy.data();
+        w.weight = ld.weight.data();
+        w.irad = a.irad.data();
+        w.jrad = a.jrad.data();
+        w.A = ld.A.data();
+        w.Bs = ld.Bs.data();
+        w.Ba = ld.Ba.data();
+        w.frequency = ld.frequency.data();
+        w.ncolpar = (int64_t)ld.colpar.size();
+        w.colpar_num = a.cp_num.data();
+        w.colpar_orth_or_para = a.cp_op.c_str();
+        w.colpar_ncol = a.cp_ncol.data();
+        w.colpar_ntmp = a.cp_ntmp.data();
+        w.icol = a.icol.data();
+        w.jcol = a.jcol.data();
+        w.tmp = a.tmp.data();
+        w.Cd = a.Cd.data();
+        w.Ce = a.Ce.data();
+        w.roots = ls.quadrature.roots.data();
+        w.qweights = ls.quadrature.weights.data();
     }
 
     mgr_model_view v{};
@@ -96,7 +138,7 @@ void Simulation::ensure_context()
     v.nrays = (int64_t)R;
     v.nboundary = (int64_t)b2c.size();
     v.nspecs = (int64_t)nspecs;
-    v.nquads = (int64_t)ls.quadrature.roots.size();
+    v.nquads = (int64_t)nq;
     v.pop_prec = pop_prec();
     v.max_width_fraction = parameters.max_width_fraction;
     v.n_off_diag = parameters.n_off_diag;
@@ -110,30 +152,33 @@ void Simulation::ensure_context()
     v.temperature = thermodynamics.temperature.gas.data();
     v.vturb2 = thermodynamics.turbulence.vturb2.data();
     v.abundance = abundance.data();
-    v.num = ld.num;
-    v.inverse_mass = ld.inverse_mass;
-    v.nlev = ld.nlev;
-    v.nrad = ld.nrad;
-    v.energy = ld.energy.data();
-    v.weight = ld.weight.data();
-    v.irad = irad.data();
-    v.jrad = jrad.data();
-    v.A = ld.A.data();
-    v.Bs = ld.Bs.data();
-    v.Ba = ld.Ba.data();
-    v.frequency = ld.frequency.data();
-    v.ncolpar = (int64_t)ld.colpar.size();
-    v.colpar_num = cp_num.data();
-    v.colpar_orth_or_para = cp_op.c_str();
-    v.colpar_ncol = cp_ncol.data();
-    v.colpar_ntmp = cp_ntmp.data();
-    v.icol = icol.data();
-    v.jcol = jcol.data();
-    v.tmp = tmp.data();
-    v.Cd = Cd.data();
-    v.Ce = Ce.data();
-    v.roots = ls.quadrature.roots.data();
-    v.qweights = ls.quadrature.weights.data();
+    const mgr_species_view &w0 = sv[0];
+    v.num = w0.num;
+    v.inverse_mass = w0.inverse_mass;
+    v.nlev = w0.nlev;
+    v.nrad = w0.nrad;
+    v.energy = w0.energy;
+    v.weight = w0.weight;
+    v.irad = w0.irad;
+    v.jrad = w0.jrad;
+    v.A = w0.A;
+    v.Bs = w0.Bs;
+    v.Ba = w0.Ba;
+    v.frequency = w0.frequency;
+    v.ncolpar = w0.ncolpar;
+    v.colpar_num = w0.colpar_num;
+    v.colpar_orth_or_para = w0.colpar_orth_or_para;
+    v.colpar_ncol = w0.colpar_ncol;
+    v.colpar_ntmp = w0.colpar_ntmp;
+    v.icol = w0.icol;
+    v.jcol = w0.jcol;
+    v.tmp = w0.tmp;
+    v.Cd = w0.Cd;
+    v.Ce = w0.Ce;
+    v.roots = w0.roots;
+    v.qweights = w0.qweights;
+    v.nlspecs = (int64_t)S;
+    v.more_species = S > 1 ? sv.data() + 1 : nullptr;
     const int rc = mgr_create(&v, device, &ctx_);
     if (rc != MGR_OK) {
         ctx_ = nullptr;
@@ -169,11 +214,25 @@ int Simulation::compute_spectral_discretisation_image(const double width)
 
 // Frequencies::nu, nr_line and the unsorted-index back-maps of either grid
 // (sim_spectrum.tpp:153-184, 270-302).
+// Lines of every species, species after species: total and first line of l.
+size_t Simulation::nlines_total() const
+{
+    size_t K = 0;
+    for (const LineProducingSpecies &ls : lines.lineProducingSpecies) K += ls.linedata.nrad;
+    return K;
+}
+
+size_t Simulation::first_line(size_t l) const
+{
+    size_t K = 0;
+    for (size_t q = 0; q < l; q++) K += lines.lineProducingSpecies[q].linedata.nrad;
+    return K;
+}
+
 void Simulation::pull_spectral()
 {
-    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
     const size_t N = geometry.cells.position.size();
-    const size_t K = ls.linedata.nrad, Q = ls.quadrature.roots.size(), F = K * Q;
+    const size_t K = nlines_total(), Q = lines.lineProducingSpecies[0].quadrature.roots.size(), F = K * Q;
     Double1 nu(N * F);
     std::vector<int64_t> nr(N * K * Q);
     check(mgr_get_frequencies(ctx_, nu.data(), nr.data()), "get_frequencies");
@@ -181,21 +240,27 @@ void Simulation::pull_spectral()
     fr.nu.assign(N, Double1(F));
     for (size_t p = 0; p < N; p++)
         for (size_t f = 0; f < F; f++) fr.nu[p][f] = nu[p * F + f];
-    ls.nr_line.assign(N, Long2(K, Long1(Q)));
-    for (size_t p = 0; p < N; p++)
-        for (size_t k = 0; k < K; k++)
-            for (size_t z = 0; z < Q; z++) ls.nr_line[p][k][z] = (long)nr[(p * K + k) * Q + z];
     fr.appears_in_line_integral.assign(F, false);
     fr.corresponding_l_for_spec.assign(F, (long)F);
     fr.corresponding_k_for_tran.assign(F, (long)F);
     fr.corresponding_z_for_line.assign(F, (long)F);
-    for (size_t k = 0; k < K; k++)
-        for (size_t z = 0; z < Q; z++) {
-            fr.appears_in_line_integral[k * Q + z] = true;
-            fr.corresponding_l_for_spec[k * Q + z] = 0;
-            fr.corresponding_k_for_tran[k * Q + z] = (long)k;
-            fr.corresponding_z_for_line[k * Q + z] = (long)z;
-        }
+    size_t index2 = 0;
+    for (size_t l = 0; l < lines.lineProducingSpecies.size(); l++) {
+        LineProducingSpecies &ls = lines.lineProducingSpecies[l];
+        const size_t Kl = ls.linedata.nrad, g0 = first_line(l);
+        ls.nr_line.assign(N, Long2(Kl, Long1(Q)));
+        for (size_t p = 0; p < N; p++)
+            for (size_t k = 0; k < Kl; k++)
+                for (size_t z = 0; z < Q; z++) ls.nr_line[p][k][z] = (long)nr[(p * K + g0 + k) * Q + z];
+        for (size_t k = 0; k < Kl; k++)
+            for (size_t z = 0; z < Q; z++) {
+                fr.appears_in_line_integral[index2] = true;
+                fr.corresponding_l_for_spec[index2] = (long)l;
+                fr.corresponding_k_for_tran[index2] = (long)k;
+                fr.corresponding_z_for_line[index2] = (long)z;
+                index2++;
+            }
+    }
 }
 
 // Copies the device's boundary intensities into radiation.I_bdy; the
@@ -237,31 +302,42 @@ int Simulation::compute_boundary_intensities(const Double1 &temperatures)
     return 0;
 }
 
+// The device holds the species' populations one after the other, each in
+// the species' own layout i + p*nlev.
 void Simulation::pull_populations()
 {
-    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
     const size_t N = geometry.cells.position.size();
-    ls.population.resize(N * ls.linedata.nlev);
-    ls.population_prev1.resize(ls.population.size());
-    ls.population_prev2.resize(ls.population.size());
-    ls.population_prev3.resize(ls.population.size());
-    check(mgr_get_populations(ctx_, ls.population.data()), "get_populations");
-    check(mgr_get_population_history(ctx_, ls.population_prev1.data(), ls.population_prev2.data(),
-                                     ls.population_prev3.data()),
-          "get_population_history");
-    lines.emissivity.resize(N * ls.linedata.nrad);
-    lines.opacity.resize(N * ls.linedata.nrad);
+    size_t tot = 0;
+    for (const LineProducingSpecies &ls : lines.lineProducingSpecies) tot += N * ls.linedata.nlev;
+    Double1 a(tot), b(tot), c(tot), d(tot);
+    check(mgr_get_populations(ctx_, a.data()), "get_populations");
+    check(mgr_get_population_history(ctx_, b.data(), c.data(), d.data()), "get_population_history");
+    size_t o = 0;
+    for (LineProducingSpecies &ls : lines.lineProducingSpecies) {
+        const size_t n = N * ls.linedata.nlev;
+        ls.population.assign(a.begin() + o, a.begin() + o + n);
+        ls.population_prev1.assign(b.begin() + o, b.begin() + o + n);
+        ls.population_prev2.assign(c.begin() + o, c.begin() + o + n);
+        ls.population_prev3.assign(d.begin() + o, d.begin() + o + n);
+        o += n;
+    }
+    lines.emissivity.resize(N * nlines_total());
+    lines.opacity.resize(N * nlines_total());
     check(mgr_get_emissivity_opacity(ctx_, lines.emissivity.data(), lines.opacity.data()),
           "get_emissivity_opacity");
 }
 
 void Simulation::push_populations()
 {
-    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
-    check(mgr_set_populations(ctx_, ls.population.data()), "set_populations");
-    check(mgr_set_population_history(ctx_, ls.population_prev1.data(), ls.population_prev2.data(),
-                                     ls.population_prev3.data()),
-          "set_population_history");
+    Double1 a, b, c, d;
+    for (const LineProducingSpecies &ls : lines.lineProducingSpecies) {
+        a.insert(a.end(), ls.population.begin(), ls.population.end());
+        b.insert(b.end(), ls.population_prev1.begin(), ls.population_prev1.end());
+        c.insert(c.end(), ls.population_prev2.begin(), ls.population_prev2.end());
+        d.insert(d.end(), ls.population_prev3.begin(), ls.population_prev3.end());
+    }
+    check(mgr_set_populations(ctx_, a.data()), "set_populations");
+    check(mgr_set_population_history(ctx_, b.data(), c.data(), d.data()), "set_population_history");
 }
 
 // Simulation::compute_LTE_level_populations (sim_lines.tpp:10-16)
@@ -269,34 +345,42 @@ int Simulation::compute_LTE_level_populations()
 {
     ensure_context();
     check(mgr_compute_LTE_level_populations(ctx_), "compute_LTE_level_populations");
-    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
     const size_t N = geometry.cells.position.size();
-    ls.population_tot.resize(N);
-    for (size_t p = 0; p < N; p++) ls.population_tot[p] = chemistry.species.abundance[p][ls.linedata.num];
+    for (LineProducingSpecies &ls : lines.lineProducingSpecies) {
+        ls.population_tot.resize(N);
+        for (size_t p = 0; p < N; p++) ls.population_tot[p] = chemistry.species.abundance[p][ls.linedata.num];
+    }
     pull_populations();
     return 0;
 }
 
 void Simulation::pull_radiation()
 {
-    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
     const size_t N = geometry.cells.position.size();
-    const size_t K = ls.linedata.nrad;
-    const size_t F = K * ls.quadrature.roots.size();
+    const size_t Ktot = nlines_total();
+    const size_t F = Ktot * lines.lineProducingSpecies[0].quadrature.roots.size();
     radiation.J.resize(N * F);
     check(mgr_get_J(ctx_, radiation.J.data()), "get_J");
-    Double1 Ls(N * K);
-    check(mgr_get_lambda(ctx_, Ls.data()), "get_lambda");
-    // the element nr = p of every (p, k), when any was added
-    ls.lambda.Ls.assign(N, Double2(K));
-    ls.lambda.nr.assign(N, Long2(K));
-    ls.lambda.size.assign(N * K, 0);
-    for (size_t p = 0; p < N; p++)
-        for (size_t k = 0; k < K; k++)
-            if (Ls[p * K + k] != 0.0) {
-                ls.lambda.Ls[p][k] = {Ls[p * K + k]};
-                ls.lambda.nr[p][k] = {(long)p};
-            }
+    Double1 Lall(N * Ktot);
+    check(mgr_get_lambda(ctx_, Lall.data()), "get_lambda");
+    // the element nr = p of every (p, k) of every species, when any was added
+    for (size_t l = 0; l < lines.lineProducingSpecies.size(); l++) {
+        LineProducingSpecies &ls = lines.lineProducingSpecies[l];
+        const size_t K = ls.linedata.nrad, g0 = first_line(l);
+        ls.lambda.Ls.assign(N, Double2(K));
+        ls.lambda.nr.assign(N, Long2(K));
+        ls.lambda.size.assign(N * K, 0);
+        for (size_t p = 0; p < N; p++)
+            for (size_t k = 0; k < K; k++)
+                if (Lall[p * Ktot + g0 + k] != 0.0) {
+                    ls.lambda.Ls[p][k] = {Lall[p * Ktot + g0 + k]};
+                    ls.lambda.nr[p][k] = {(long)p};
+                }
+        for (size_t p = 0; p < N; p++)
+            for (size_t k = 0; k < K; k++) ls.lambda.size[k + K * p] = ls.lambda.nr[p][k].size();
+    }
+    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
+    const size_t K = ls.linedata.nrad;
     // n_off_diag >= 2: the band elements, merged per emitting cell in the
     // order RayPair::update_Lambda first adds them (ray pairs, then bands m,
     // lower before upper; raypair_lambda.tpp:129-146, lambda.tpp:130-143)
@@ -345,20 +429,36 @@ void Simulation::pull_radiation()
 
 void Simulation::pull_jeff()
 {
-    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
     const size_t N = geometry.cells.position.size();
-    const size_t K = ls.linedata.nrad;
-    Double1 a(N * K), b(N * K), c(N * K);
+    const size_t Ktot = nlines_total();
+    Double1 a(N * Ktot), b(N * Ktot), c(N * Ktot);
     check(mgr_get_Jeff(ctx_, a.data(), b.data(), c.data()), "get_Jeff");
-    ls.Jlin.assign(N, Double1(K));
-    ls.Jeff.assign(N, Double1(K));
-    ls.Jdif.assign(N, Double1(K));
-    for (size_t p = 0; p < N; p++)
-        for (size_t k = 0; k < K; k++) {
-            ls.Jlin[p][k] = a[p * K + k];
-            ls.Jeff[p][k] = b[p * K + k];
-            ls.Jdif[p][k] = c[p * K + k];
-        }
+    for (size_t l = 0; l < lines.lineProducingSpecies.size(); l++) {
+        LineProducingSpecies &ls = lines.lineProducingSpecies[l];
+        const size_t K = ls.linedata.nrad, g0 = first_line(l);
+        ls.Jlin.assign(N, Double1(K));
+        ls.Jeff.assign(N, Double1(K));
+        ls.Jdif.assign(N, Double1(K));
+        for (size_t p = 0; p < N; p++)
+            for (size_t k = 0; k < K; k++) {
+                ls.Jlin[p][k] = a[p * Ktot + g0 + k];
+                ls.Jeff[p][k] = b[p * Ktot + g0 + k];
+                ls.Jdif[p][k] = c[p * Ktot + g0 + k];
+            }
+    }
+}
+
+// LineProducingSpecies::check_for_convergence's figures of every species
+void Simulation::pull_convergence()
+{
+    for (size_t l = 0; l < lines.lineProducingSpecies.size(); l++) {
+        mgr_conv c{};
+        check(mgr_species_convergence(ctx_, (int64_t)l, &c), "species_convergence");
+        LineProducingSpecies &ls = lines.lineProducingSpecies[l];
+        ls.fraction_not_converged = c.fraction_not_converged;
+        ls.relative_change_mean = c.relative_change_mean;
+        ls.relative_change_max = c.relative_change_max;
+    }
 }
 
 int Simulation::run_radiation_field(int prec)
@@ -410,10 +510,7 @@ void Simulation::compute_level_populations_from_stateq()
     ensure_context();
     mgr_conv conv{};
     check(mgr_compute_level_populations_from_stateq(ctx_, &conv), "compute_level_populations_from_stateq");
-    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
-    ls.fraction_not_converged = conv.fraction_not_converged;
-    ls.relative_change_mean = conv.relative_change_mean;
-    ls.relative_change_max = conv.relative_change_max;
+    pull_convergence();
     pull_populations();
 }
 
@@ -432,8 +529,8 @@ long Simulation::compute_level_populations(const Io &io, bool use_Ng_acceleratio
         return -1;
     }
     ensure_context();
-    LineProducingSpecies &ls = lines.lineProducingSpecies[0];
-    ls.write_populations(io, 0, "_rank_0_iteration_0");
+    const size_t S = lines.lineProducingSpecies.size();
+    for (size_t l = 0; l < S; l++) lines.lineProducingSpecies[l].write_populations(io, (long)l, "_rank_0_iteration_0");
     push_populations();
     long iteration = 0, iteration_normal = 0;
     error_mean.clear();
@@ -446,19 +543,26 @@ long Simulation::compute_level_populations(const Io &io, bool use_Ng_acceleratio
         mgr_conv conv{};
         check(mgr_ali_iteration(ctx_, ng ? 1 : 0, precision, &conv), "ALI iteration");
         iteration_normal = ng ? 0 : iteration_normal + 1;
-        ls.fraction_not_converged = conv.fraction_not_converged;
-        ls.relative_change_mean = conv.relative_change_mean;
-        ls.relative_change_max = conv.relative_change_max;
-        error_mean.push_back(conv.relative_change_mean);
-        error_max.push_back(conv.relative_change_max);
-        if (convergence_criterion == MGR_CRIT_MAX) {
-            if (conv.relative_change_max >= pop_prec()) some_not_converged = true;
-        } else if (conv.fraction_not_converged > 0.005) {
-            some_not_converged = true;
+        if (S > 1) pull_convergence();
+        for (size_t l = 0; l < S; l++) {   // sim_lines.tpp:130-150, per species
+            LineProducingSpecies &ls = lines.lineProducingSpecies[l];
+            if (S == 1) {
+                ls.fraction_not_converged = conv.fraction_not_converged;
+                ls.relative_change_mean = conv.relative_change_mean;
+                ls.relative_change_max = conv.relative_change_max;
+            }
+            error_mean.push_back(ls.relative_change_mean);
+            error_max.push_back(ls.relative_change_max);
+            if (convergence_criterion == MGR_CRIT_MAX) {
+                if (ls.relative_change_max >= pop_prec()) some_not_converged = true;
+            } else if (ls.fraction_not_converged > 0.005) {
+                some_not_converged = true;
+            }
         }
         pull_populations();
         if (!ng) pull_jeff();
-        ls.write_populations(io, 0, "_rank_0_iteration_" + std::to_string(iteration));
+        for (size_t l = 0; l < S; l++)
+            lines.lineProducingSpecies[l].write_populations(io, (long)l, "_rank_0_iteration_" + std::to_string(iteration));
     }
     pull_radiation();
     return iteration;

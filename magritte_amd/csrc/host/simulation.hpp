@@ -71,6 +71,9 @@ struct Simulation : public Model {
     void pull_jeff();
     void pull_boundary_intensities();
     void pull_spectral();
+    void pull_convergence();
+    size_t nlines_total() const;
+    size_t first_line(size_t l) const;
     bool use_scattering() const;
     double pop_prec() const { return parameters.pop_prec_.is_set ? parameters.pop_prec() : 1.0e-6; }
     int run_radiation_field(int prec);

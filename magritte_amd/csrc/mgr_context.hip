@@ -847,6 +847,32 @@ int mgr_batch_convergence(mgr_ctx *ctx, int64_t *iterations, mgr_conv *conv)
     return MGR_OK;
 }
 
+static int copy_out(mgr_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+int mgr_species_info(mgr_ctx *ctx, int64_t *nlspecs, int64_t *nlev, int64_t *nrad)
+{
+    if (!ctx || !nlspecs) return MGR_ERR_ARGUMENT;
+    *nlspecs = ctx->S;
+    for (int64_t l = 0; l < ctx->S; l++) {
+        if (nlev) nlev[l] = ctx->sp_V[l];
+        if (nrad) nrad[l] = ctx->sp_L[l];
+    }
+    return MGR_OK;
+}
+
+int mgr_species_convergence(mgr_ctx *ctx, int64_t l, mgr_conv *conv)
+{
+    if (!ctx || !conv) return MGR_ERR_ARGUMENT;
+    if (l < 0 || l >= ctx->S) return ctx->fail(MGR_ERR_ARGUMENT, "species index out of range");
+    double w[3];
+    const int rc = copy_out(ctx, w, ctx->reduced + (ctx->S > 1 ? 4 * (l + 1) : 0), sizeof(w));
+    if (rc) return rc;
+    conv->fraction_not_converged = w[0];
+    conv->relative_change_mean = w[1];
+    conv->relative_change_max = w[2];
+    return MGR_OK;
+}
+
 void mgr_destroy(mgr_ctx *ctx)
 {
     if (!ctx) return;

@@ -49,6 +49,8 @@ SIGNATURES = {
     "mgr_lambda_band_slots": ([_vp, _vp], _i),
     "mgr_get_lambda_band": ([_vp, _vp, _vp], _i),
     "mgr_band_sweeps": ([_vp, _vp, _vp], _i),
+    "mgr_species_info": ([_vp, _vp, _vp, _vp], _i),
+    "mgr_species_convergence": ([_vp, _i64, _vp], _i),
     "mgr_get_Jeff": ([_vp, _vp, _vp, _vp], _i),
     "mgr_get_frequencies": ([_vp, _vp, _vp], _i),
     "mgr_get_boundary_intensities": ([_vp, _vp], _i),
@@ -246,6 +248,16 @@ class Context:
         Ls = np.zeros(self.N * S * self.L)
         self._check(self.lib.mgr_get_lambda_band(self.h, _p(cells), _p(Ls)), "get_lambda_band")
         return cells.reshape(self.N, S), Ls.reshape(self.N, S, self.L)
+
+    def species_convergence(self):
+        """(fraction not converged, mean change, max change) of every species
+        after the last level-population update."""
+        out = []
+        for l in range(len(self.Vs)):
+            c = ConvStats()
+            self._check(self.lib.mgr_species_convergence(self.h, l, C.byref(c)), "species_convergence")
+            out.append((c.fraction_not_converged, c.relative_change_mean, c.relative_change_max))
+        return out
 
     def band_sweeps(self):
         """Block-Jacobi sweeps of the last rate-equation solve with bands and

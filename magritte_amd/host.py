@@ -23,18 +23,17 @@ def to_simulation(m: Model, use_scattering: bool = False):
     c = core()
     sim = c.Simulation()
     N, R, NB = m.ncells, m.nrays, m.nboundary
-    ld = m.linedata
     p = sim.parameters
     p.set_ncells(N)
     p.set_nrays(R)
     p.set_nrays_red(R // 2)
     p.set_nboundary(NB)
     p.set_nspecs(len(m.species))
-    p.set_nlspecs(1)
-    p.set_nlines(ld.nrad)
+    p.set_nlspecs(m.nlspecs)
+    p.set_nlines(m.nlines)
     p.set_nquads(m.nquads)
-    p.set_nfreqs(ld.nrad * m.nquads)
-    p.set_nfreqs_red(ld.nrad * m.nquads)
+    p.set_nfreqs(m.nfreqs)
+    p.set_nfreqs_red(m.nfreqs)
     p.set_pop_prec(float(m.pop_prec))
     p.max_width_fraction = float(m.max_width_fraction)
     p.n_off_diag = int(m.n_off_diag)
@@ -73,6 +72,21 @@ def to_simulation(m: Model, use_scattering: bool = False):
     sim.chemistry.species.abundance = np.asarray(m.abundance, dtype=np.float64).tolist()
     sim.chemistry.species.sym = list(m.species)
 
+    species = [(m.linedata, m.roots, m.qweights)] + [(x.linedata, x.roots, x.qweights) for x in m.more_species]
+    for ld, roots, qweights in species:
+        sim.lines.lineProducingSpecies.append(_line_species(c, m, ld, roots, qweights))
+    freqs = np.concatenate([ld.frequency for ld, _, _ in species])
+    sim.lines.line = sorted(freqs.tolist())
+    sim.lines.line_index = [int(i) for i in np.argsort(freqs, kind="stable")]
+    if use_scattering:   # Radiation::read sizes u (radiation.cpp:117-126)
+        sim.radiation.u = [[0.0] * (N * m.nfreqs) for _ in range(R // 2)]
+    return sim
+
+
+def _line_species(c, m, ld, roots, qweights):
+    """One Lines/LineProducingSpecies_<l> of the host Model, sized as
+    LineProducingSpecies::read leaves it (lineProducingSpecies.cpp:21-70)."""
+    N = m.ncells
     lsp = c.LineProducingSpecies()
     cl = c.Linedata()
     cl.num = int(ld.num)
@@ -105,11 +119,10 @@ def to_simulation(m: Model, use_scattering: bool = False):
     cl.colpar = cps
     lsp.linedata = cl
     qd = c.Quadrature()
-    qd.roots = np.asarray(m.roots, dtype=np.float64).tolist()
-    qd.weights = np.asarray(m.qweights, dtype=np.float64).tolist()
+    qd.roots = np.asarray(roots, dtype=np.float64).tolist()
+    qd.weights = np.asarray(qweights, dtype=np.float64).tolist()
     lsp.quadrature = qd
     lsp.ncells = N
-    # sized as LineProducingSpecies::read leaves them (lineProducingSpecies.cpp:21-70)
     zeros_nv = [0.0] * (N * ld.nlev)
     lsp.population = zeros_nv
     lsp.population_prev1 = zeros_nv
@@ -119,9 +132,4 @@ def to_simulation(m: Model, use_scattering: bool = False):
     lsp.Jlin = [[0.0] * ld.nrad for _ in range(N)]
     lsp.Jeff = [[0.0] * ld.nrad for _ in range(N)]
     lsp.Jdif = [[0.0] * ld.nrad for _ in range(N)]
-    sim.lines.lineProducingSpecies.append(lsp)
-    sim.lines.line = sorted(ld.frequency.tolist())
-    sim.lines.line_index = [int(i) for i in np.argsort(ld.frequency, kind="stable")]
-    if use_scattering:   # Radiation::read sizes u (radiation.cpp:117-126)
-        sim.radiation.u = [[0.0] * (N * ld.nrad * m.nquads) for _ in range(R // 2)]
-    return sim
+    return lsp

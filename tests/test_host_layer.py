@@ -92,6 +92,60 @@ def test_io_round_trip(tmp_path, kind):
 
 
 @pytest.mark.parametrize("kind", ["text", "hdf5"])
+def test_io_round_trip_of_two_species(tmp_path, kind):
+    """A model with Lines/LineProducingSpecies_0 and _1 reads back both
+    species (Lines::read, lines.cpp:19-40) with their line data and quadratures,
+    and the merged sorted line list."""
+    m = synthetic.two_species_model(n=4, nside=1, second_nu=200.0e9)
+    sim = to_simulation(m)
+    io = _io(kind, tmp_path)
+    sim.write(io)
+    back = core.Model()
+    back.read(io)
+    assert back.parameters.nlspecs() == 2
+    assert back.parameters.nlines() == m.nlines
+    for ls, ld in zip(back.lines.lineProducingSpecies, m.species_linedata):
+        assert ls.linedata.nlev == ld.nlev and ls.linedata.nrad == ld.nrad and ls.linedata.num == ld.num
+        np.testing.assert_array_equal(np.array(ls.linedata.frequency), ld.frequency)
+        np.testing.assert_array_equal(np.array(ls.linedata.energy), ld.energy)
+    freqs = np.concatenate([ld.frequency for ld in m.species_linedata])
+    np.testing.assert_array_equal(np.array(back.lines.line), np.sort(freqs))
+
+
+@pytest.mark.gpu
+def test_two_species_simulation_matches_oracle(tmp_path):
+    """The drop-in Simulation with two line-producing species: J and the
+    converged populations of each species against the oracle; error_mean and
+    error_max get one entry per species and iteration, and every species
+    writes its populations (sim_lines.tpp:78-83, 130-151)."""
+    from oracle.py_oracle import Oracle
+    m = synthetic.two_species_model(n=5, nside=1, seed=4)
+    print("\n[host] two species: Simulation vs oracle", flush=True)
+    orc = Oracle(m).setup()
+    sim = to_simulation(m)
+    assert sim.compute_spectral_discretisation() == 0
+    assert sim.compute_boundary_intensities() == 0
+    assert sim.compute_LTE_level_populations() == 0
+    assert sim.compute_radiation_field() == 0
+    orc.compute_radiation_field()
+    J_o = orc.J().ravel()
+    assert float(np.max(np.abs(np.array(sim.radiation.J) - J_o) / np.abs(J_o))) < 1e-5
+    for l, ls in enumerate(sim.lines.lineProducingSpecies):
+        assert np.array(ls.nr_line).shape == (m.ncells, m.species_linedata[l].nrad, m.nquads)
+    sim.convergence_criterion = core.CRIT_MAX
+    it = sim.compute_level_populations(core.IoText(str(tmp_path) + "/"), True, 60)
+    it_o = orc.compute_level_populations(True, 60, 1)
+    assert 0 < it < 60 and abs(it - it_o) <= 1
+    for ls, po in zip(sim.lines.lineProducingSpecies, orc.populations()):
+        pg = np.array(ls.population)
+        assert float(np.max(np.abs(pg - po.ravel()) / np.abs(po.ravel()))) < 1e-5
+    assert len(sim.error_max) == 2 * it
+    for l in (0, 1):
+        written = os.listdir(str(tmp_path) + f"/Lines/LineProducingSpecies_{l}")
+        assert f"population_rank_0_iteration_{it}.txt" in written
+
+
+@pytest.mark.parametrize("kind", ["text", "hdf5"])
 def test_io_round_trip_of_the_iteration_state(tmp_path, kind):
     """What a resumed ALI needs (LineProducingSpecies::read/write,
     lineProducingSpecies.cpp:21-134, 179-206): population, population_prev1..3,
