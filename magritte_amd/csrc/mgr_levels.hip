@@ -577,8 +577,9 @@ __global__ void k_ng_apply(int64_t N, int V, const int32_t *__restrict__ cell_mo
 __global__ void k_zero_flag(int32_t *flag) { *flag = 0; }
 
 // The driver's convergence words from the S species' words at 4 (l+1): the
-// largest fraction, mean and maximum change; the error word is shared.
-__global__ void k_conv_species(int S, double *reduced)
+// largest fraction, mean and maximum change; the error word is shared.  The
+// context then holds one model, whose per-model words get the same figures.
+__global__ void k_conv_species(int S, double *reduced, double *mconv)
 {
     double a = 0.0, b = 0.0, c = 0.0;
     for (int l = 0; l < S; l++) {
@@ -590,6 +591,9 @@ __global__ void k_conv_species(int S, double *reduced)
     reduced[1] = b;
     reduced[2] = c;
     reduced[3] = reduced[4 * S + 3];
+    mconv[0] = a;
+    mconv[1] = b;
+    mconv[2] = c;
 }
 
 // ---- Ng acceleration without allocation ---------------------------------------
@@ -770,7 +774,7 @@ int launch_convergence(mgr_ctx *ctx)
         MGR_LAUNCH_CHECK();
     }
     if (ctx->S > 1) {
-        hipLaunchKernelGGL(k_conv_species, dim3(1), dim3(1), 0, ctx->stream, (int)ctx->S, ctx->reduced);
+        hipLaunchKernelGGL(k_conv_species, dim3(1), dim3(1), 0, ctx->stream, (int)ctx->S, ctx->reduced, ctx->d_mconv);
         MGR_LAUNCH_CHECK();
     }
     return MGR_OK;
