@@ -176,6 +176,9 @@ struct mgr_ctx {
     int64_t *nb_off = nullptr;                     // [N+1]
     int32_t *nb = nullptr;                         // [sum n_neighbors]
     mgr::NbRec *nb_rec = nullptr;                  // [sum n_neighbors] neighbour index and position
+    int64_t  nb_kmax = 0;                          // longest neighbour list
+    double   fwd_tau = 0.0;                        // candidate-mask margin (1e-9 of the extent)
+    uint64_t *fwd_mask = nullptr;                  // [N*R] candidate neighbours per (cell, ray), or null
     uint8_t *is_boundary = nullptr;                // [N]
     int32_t *cell2bdy = nullptr;                   // [N], -1 for interior cells
     int32_t *bdy2cell = nullptr;                   // [NB]
@@ -353,6 +356,7 @@ int free_model(mgr_ctx *ctx);
 int launch_cell_tables(mgr_ctx *ctx);
 int launch_emissivity_opacity(mgr_ctx *ctx);
 int launch_trace(mgr_ctx *ctx);
+int launch_forward_masks(mgr_ctx *ctx);   // candidate neighbours of every (cell, ray), once
 int launch_radiation_field(mgr_ctx *ctx, int precision);   // kernel + Lambda*
 int launch_far_cache(mgr_ctx *ctx, int precision);   // outside graph capture
 int launch_rf_kernel(mgr_ctx *ctx, int precision);

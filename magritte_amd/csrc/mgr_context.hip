@@ -612,6 +612,16 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
         }
         MGR_HIP(hipMalloc((void **)&ctx->nb_rec, rec.size() * sizeof(NbRec)));
         MGR_HIP(hipMemcpy(ctx->nb_rec, rec.data(), rec.size() * sizeof(NbRec), hipMemcpyHostToDevice));
+        // the tracer's candidate masks: list lengths, and a margin of 1e-9 of
+        // the largest |p - o|_1 |n|_inf over cells and rays
+        ctx->nb_kmax = 0;
+        for (int64_t c = 0; c < N; c++) ctx->nb_kmax = std::max<int64_t>(ctx->nb_kmax, off[c + 1] - off[c]);
+        double ext = 0.0, nmax = 0.0;
+        for (int64_t c = 0; c < N; c++)
+            ext = std::max(ext, std::fabs(m->position[3 * c]) + std::fabs(m->position[3 * c + 1]) +
+                                    std::fabs(m->position[3 * c + 2]));
+        for (int64_t i = 0; i < 3 * R; i++) nmax = std::max(nmax, std::fabs(m->rays[i]));
+        ctx->fwd_tau = 1e-9 * (2.0 * ext) * nmax;
     }
     if ((rc = upload(ctx, &ctx->is_boundary, isb.data(), N))) return rc;
     if ((rc = upload(ctx, &ctx->cell2bdy, c2b.data(), N))) return rc;
@@ -736,7 +746,7 @@ int upload_models(mgr_ctx *ctx, const mgr_model_view *models, int64_t nmodels)
 int free_model(mgr_ctx *ctx)
 {
     dfree(ctx->pos); dfree(ctx->vel); dfree(ctx->nb_off); dfree(ctx->nb); dfree(ctx->is_boundary);
-    dfree(ctx->nb_rec);
+    dfree(ctx->nb_rec); dfree(ctx->fwd_mask);
     dfree(ctx->cell2bdy); dfree(ctx->bdy2cell); dfree(ctx->rays); dfree(ctx->ray_w);
     dfree(ctx->d_antipod); dfree(ctx->temperature); dfree(ctx->vturb2); dfree(ctx->abundance);
     dfree(ctx->tables); dfree(ctx->A); dfree(ctx->Bs); dfree(ctx->Ba); dfree(ctx->frequency);

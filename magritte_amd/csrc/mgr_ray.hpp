@@ -19,6 +19,26 @@ __device__ __forceinline__ double dot3(double ax, double ay, double az, double b
     return (ax * bx + ay * by) + az * bz;
 }
 
+// One candidate of get_next, the neighbour record r.
+__device__ __forceinline__ void next_candidate(const NbRec r, double ox, double oy, double oz, double nx,
+                                               double ny, double nz, double Z, double &dmin, int32_t &next,
+                                               double &dZ)
+{
+#pragma clang fp contract(off)
+    const double Rx = r.x - ox;
+    const double Ry = r.y - oy;
+    const double Rz = r.z - oz;
+    const double Z_new = dot3(Rx, Ry, Rz, nx, ny, nz);
+    if (Z_new > Z) {
+        const double distance_from_ray2 = dot3(Rx, Ry, Rz, Rx, Ry, Rz) - Z_new * Z_new;
+        if (distance_from_ray2 < dmin) {
+            dmin = distance_from_ray2;
+            next = (int32_t)r.c;
+            dZ = Z_new - Z;
+        }
+    }
+}
+
 // Geometry::get_next_general (src/Model/Geometry/geometry.tpp:328-421):
 // the neighbour with Z_new > Z closest to the ray (strict <, first wins).
 // (ox, oy, oz) is the origin's position; nbr holds each neighbour's index and
@@ -31,21 +51,40 @@ __device__ __forceinline__ int32_t get_next(const NbRec *__restrict__ nbr,
 #pragma clang fp contract(off)
     double dmin = DBL_MAX;
     int32_t next = -1;
-    const int64_t q1 = nb_off[current + 1];
-    for (int64_t q = nb_off[current]; q < q1; q++) {
-        const NbRec r = nbr[q];
-        const double Rx = r.x - ox;
-        const double Ry = r.y - oy;
-        const double Rz = r.z - oz;
-        const double Z_new = dot3(Rx, Ry, Rz, nx, ny, nz);
-        if (Z_new > Z) {
-            const double distance_from_ray2 = dot3(Rx, Ry, Rz, Rx, Ry, Rz) - Z_new * Z_new;
-            if (distance_from_ray2 < dmin) {
-                dmin = distance_from_ray2;
-                next = (int32_t)r.c;
-                dZ = Z_new - Z;
-            }
+    const int64_t b = nb_off[current];
+    const int deg = (int)(nb_off[current + 1] - b);
+    for (int j = 0; j < deg; j++) next_candidate(nbr[b + j], ox, oy, oz, nx, ny, nz, Z, dmin, next, dZ);
+    Z = Z + dZ;
+    return next;
+}
+
+// get_next over the entries of the list at `list` whose bit is set in
+// `cand`, in list order: the cleared entries lie behind the current cell
+// along the ray by far more than the rounding of Z_new and Z, so that
+// Z_new > Z fails for them.  Candidates are taken NEXT_GROUP at a time,
+// their records loaded together and then tested in list order.
+constexpr int NEXT_GROUP = 4;
+__device__ __forceinline__ int32_t get_next_masked(const NbRec *__restrict__ list, uint64_t cand, double ox,
+                                                   double oy, double oz, double nx, double ny, double nz,
+                                                   double &Z, double &dZ)
+{
+#pragma clang fp contract(off)
+    double dmin = DBL_MAX;
+    int32_t next = -1;
+    while (cand) {
+        int j[NEXT_GROUP];
+#pragma unroll
+        for (int u = 0; u < NEXT_GROUP; u++) {
+            j[u] = cand ? __builtin_ctzll(cand) : -1;
+            cand &= cand - 1;
         }
+        NbRec r[NEXT_GROUP];
+#pragma unroll
+        for (int u = 0; u < NEXT_GROUP; u++)
+            if (j[u] >= 0) r[u] = list[j[u]];
+#pragma unroll
+        for (int u = 0; u < NEXT_GROUP; u++)
+            if (j[u] >= 0) next_candidate(r[u], ox, oy, oz, nx, ny, nz, Z, dmin, next, dZ);
     }
     Z = Z + dZ;
     return next;
@@ -60,6 +99,14 @@ __device__ __forceinline__ double doppler_shift(const double *__restrict__ vel, 
     const double dy = vel[3 * (int64_t)current + 1] - vel[3 * (int64_t)origin + 1];
     const double dz = vel[3 * (int64_t)current + 2] - vel[3 * (int64_t)origin + 2];
     return 1.0 - dot3(dx, dy, dz, nx, ny, nz);
+}
+
+// doppler_shift from the two cells' velocities already loaded
+__device__ __forceinline__ double doppler_shift_v(double vcx, double vcy, double vcz, double vox, double voy,
+                                                  double voz, double nx, double ny, double nz)
+{
+#pragma clang fp contract(off)
+    return 1.0 - dot3(vcx - vox, vcy - voy, vcz - voz, nx, ny, nz);
 }
 
 // Geometry::get_doppler_shift<Rest> (geometry.tpp:448-463): 1 - v(current).n

@@ -1,13 +1,16 @@
 // mgr_trace.hip -- ray tracer (Geometry::trace_ray<CoMoving>,
 // src/Model/Geometry/geometry.tpp:21-95) over the point-cloud neighbour lists.
 //
-// One thread per (origin, ray).  A counting pass sizes every ray, the host
-// scans the counts, and a filling pass writes the traced steps (next cell,
-// shift, dZ) into one device-resident SoA cache that every later radiation
-// field reuses: geometry and velocities are static, so the reference's
-// re-tracing on every ALI iteration is not repeated.  Interpolation points
-// (set_data) are not stored; the solver expands them from the stored steps
-// with the same FP64 arithmetic (mgr_ray.hpp).
+// One thread per (origin, ray).  A counting pass walks every ray, scanning
+// only the neighbours that lie ahead along the ray (candidate masks per
+// cell and ray, k_forward_masks), and records the cells it visits; the host
+// scans the counts, and a filling pass replays the recorded cells (no
+// neighbour scans) into one device-resident SoA cache, in blocks of four
+// steps, that every later radiation field reuses: geometry and velocities are
+// static, so the reference's re-tracing on every ALI iteration is not
+// repeated.  Interpolation points (set_data) are not stored; the solver
+// expands them from the stored steps with the same FP64 arithmetic
+// (mgr_ray.hpp).
 //
 // Compiled with -ffp-contract=off.
 #include <algorithm>
@@ -25,6 +28,9 @@ constexpr int32_t ERR_NO_NEIGHBOUR = 1;
 constexpr int32_t ERR_TOO_LONG     = 8;
 constexpr int32_t ERR_TOO_MANY     = 16;   // a step needs more than 10000 points
 
+constexpr int64_t TRACE_RECORD_STEPS = 256;        // recorded cells per walk, at most
+constexpr int64_t TRACE_RECORD_BYTES = 64LL << 30;  // record buffer, at most
+
 // positive doubles order like their bit patterns
 __device__ __forceinline__ void atomic_minmax_pos_double(uint64_t *addr, double lo, double hi)
 {
@@ -32,24 +38,59 @@ __device__ __forceinline__ void atomic_minmax_pos_double(uint64_t *addr, double 
     if (hi > 0.0) atomicMax((unsigned long long *)addr + 1, (unsigned long long)__double_as_longlong(hi));
 }
 
+// Cells recorded by the counting pass: the next cell of every step, four to
+// an int4, up to `cap` steps per (origin, ray).  The filling pass replays
+// them (replay_walk) instead of scanning the neighbours again.  The step
+// index j runs in step on every active lane of a wave, so the selects below
+// branch on a uniform value.
+__device__ __forceinline__ void put_cell(int4 &acc, int j, int32_t c)
+{
+    switch (j & 3) {
+    case 0: acc.x = c; break;
+    case 1: acc.y = c; break;
+    case 2: acc.z = c; break;
+    default: acc.w = c; break;
+    }
+}
+
+__device__ __forceinline__ int32_t get_cell(const int4 &acc, int j)
+{
+    switch (j & 3) {
+    case 0: return acc.x;
+    case 1: return acc.y;
+    case 2: return acc.z;
+    default: return acc.w;
+    }
+}
+
 // The walk of one (origin, ray): Geometry::trace_ray (geometry.tpp:21-95)
 // with set_data's interpolation split of every step (geometry.tpp:100-207).
 // The ray advances along (nx, ny, nz); shifts are co-moving,
 // 1 - (v_c - v_o).n, or for REST the rest-frame 1 - v_c.s with s the
-// direction of get_doppler_shift<Rest>.  FILL writes the steps from w on.
+// direction of get_doppler_shift<Rest>.  FILL writes the steps from w on;
+// otherwise, with rec non-null, the walk records its first `cap` cells.
 struct Walk {
     int32_t steps, points, last_cell;   // last_cell: cell of the last emitted point, -1: none
     double  smin, smax, last_shift;     // shift range of the walk (origin included)
 };
 
-template <bool FILL, bool REST>
+// Candidate masks of the walk's ray r: entry c Rmax + r holds the bits of
+// cell c's candidate neighbours and, in bit 63, whether c is a boundary cell
+// (read when the walk is MASKED).
+constexpr uint64_t FWD_BOUNDARY = 1ull << 63;
+struct Cands {
+    const uint64_t *fwd;
+    int Rmax, r;
+    __device__ __forceinline__ uint64_t of(int32_t c) const { return fwd[(int64_t)c * Rmax + r]; }
+};
+
+template <bool FILL, bool REST, bool MASKED>
 __device__ __forceinline__ Walk trace_walk(int64_t N, const double *__restrict__ pos,
-                                           const double *__restrict__ vel,
-                                           const int64_t *__restrict__ nb_off,
-                                           const NbRec *__restrict__ nb,
+                                           const double *__restrict__ vel, const int64_t *__restrict__ nb_off,
+                                           const NbRec *__restrict__ nb, Cands cd,
                                            const uint8_t *__restrict__ is_boundary, int32_t o,
                                            double nx, double ny, double nz, double sx, double sy,
-                                           double sz, double dsm, int64_t w,
+                                           double sz, double dsm, int64_t w, int4 *__restrict__ rec, int cap,
                                            int32_t *__restrict__ seg_cell,
                                            double *__restrict__ seg_shift, double *__restrict__ seg_dZ,
                                            int4 *__restrict__ seg_f32, float *__restrict__ seg_hdz32,
@@ -60,8 +101,15 @@ __device__ __forceinline__ Walk trace_walk(int64_t N, const double *__restrict__
     k.smin = 1.0; k.smax = 1.0; k.last_shift = 1.0;
     double Z = 0.0, dZ = 0.0;
     const double ox = pos[3 * (int64_t)o + 0], oy = pos[3 * (int64_t)o + 1], oz = pos[3 * (int64_t)o + 2];
-    int32_t nxt = get_next(nb, nb_off, ox, oy, oz, nx, ny, nz, o, Z, dZ);
+    int4 acc = make_int4(0, 0, 0, 0);
+    int32_t nxt = MASKED ? get_next_masked(nb + nb_off[o], cd.of(o) & ~FWD_BOUNDARY, ox, oy, oz, nx, ny, nz, Z, dZ)
+                         : get_next(nb, nb_off, ox, oy, oz, nx, ny, nz, o, Z, dZ);
     if (nxt == -1) return k;
+    if (!FILL && rec) put_cell(acc, 0, nxt);
+    // what the next step needs of nxt, loaded together: its candidates (and
+    // boundary bit) and its neighbour list
+    uint64_t fn = MASKED ? cd.of(nxt) : 0;
+    const NbRec *ln = MASKED ? nb + nb_off[nxt] : nullptr;
     double shift_crt = REST ? rest_shift(vel, o, sx, sy, sz) : doppler_shift(vel, o, o, nx, ny, nz);
     double shift_nxt = REST ? rest_shift(vel, nxt, sx, sy, sz) : doppler_shift(vel, o, nxt, nx, ny, nz);
     k.smin = fmin(k.smin, shift_crt);
@@ -91,26 +139,125 @@ __device__ __forceinline__ Walk trace_walk(int64_t N, const double *__restrict__
             w++;
         }
         k.steps++;
-        if (is_boundary[nxt]) break;
+        if (MASKED ? (fn & FWD_BOUNDARY) != 0 : is_boundary[nxt] != 0) break;
         if (k.steps >= N) { atomicOr(err_flag, ERR_TOO_LONG); break; }
         shift_crt = shift_nxt;
-        nxt = get_next(nb, nb_off, ox, oy, oz, nx, ny, nz, nxt, Z, dZ);
+        nxt = MASKED ? get_next_masked(ln, fn & ~FWD_BOUNDARY, ox, oy, oz, nx, ny, nz, Z, dZ)
+                     : get_next(nb, nb_off, ox, oy, oz, nx, ny, nz, nxt, Z, dZ);
         if (nxt < 0) { atomicOr(err_flag, ERR_NO_NEIGHBOUR); break; }
+        if (MASKED) {
+            fn = cd.of(nxt);
+            ln = nb + nb_off[nxt];
+        }
+        const int j = k.steps;   // index of this next step
+        if (!FILL && rec && j < cap) {
+            if ((j & 3) == 0) rec[(j >> 2) - 1] = acc;
+            put_cell(acc, j, nxt);
+        }
         shift_nxt = REST ? rest_shift(vel, nxt, sx, sy, sz) : doppler_shift(vel, o, nxt, nx, ny, nz);
+    }
+    // the last, partly filled block of the record
+    if (!FILL && rec && k.steps <= cap) rec[(k.steps - 1) >> 2] = acc;
+    return k;
+}
+
+// The filling pass of a walk whose `ns` cells the counting pass recorded:
+// the same steps as trace_walk<true, false> without the neighbour scans.
+// Z_new of each step is recomputed from the cell's position (the neighbour
+// records hold copies of the same positions) with get_next's operations, so
+// dZ, Z and the shifts are bit-identical.  The steps go four at a time, one
+// record block each: the four cells' positions and velocities are loaded
+// together, and the block is written with 16-byte stores from w (a multiple
+// of 4: every walk's steps start on a block), the slots past the walk's
+// last step zeroed.
+__device__ __forceinline__ Walk replay_walk(const double *__restrict__ pos, const double *__restrict__ vel,
+                                            int32_t o, double nx, double ny, double nz, double dsm, int64_t w,
+                                            const int4 *__restrict__ rec, int ns,
+                                            int32_t *__restrict__ seg_cell, double *__restrict__ seg_shift,
+                                            double *__restrict__ seg_dZ, int4 *__restrict__ seg_f32,
+                                            float *__restrict__ seg_hdz32)
+{
+#pragma clang fp contract(off)
+    Walk k;
+    k.steps = ns; k.points = 0; k.last_cell = -1;
+    k.smin = 1.0; k.smax = 1.0; k.last_shift = 1.0;
+    const double ox = pos[3 * (int64_t)o + 0], oy = pos[3 * (int64_t)o + 1], oz = pos[3 * (int64_t)o + 2];
+    const double vox = vel[3 * (int64_t)o + 0], voy = vel[3 * (int64_t)o + 1], voz = vel[3 * (int64_t)o + 2];
+    double Z = 0.0;
+    double shift_crt = doppler_shift_v(vox, voy, voz, vox, voy, voz, nx, ny, nz);
+    for (int j0 = 0; j0 < ns; j0 += 4) {
+        const int4 blk = rec[j0 >> 2];
+        const int32_t cb[4] = {blk.x, blk.y, blk.z, blk.w};
+        double p[4][3], v[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (j0 + u < ns)
+                for (int d = 0; d < 3; d++) {
+                    p[u][d] = pos[3 * (int64_t)cb[u] + d];
+                    v[u][d] = vel[3 * (int64_t)cb[u] + d];
+                }
+        int32_t c4[4];
+        double sh4[4], dz4[4];
+        float hz4[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (j0 + u < ns) {
+                const double Rx = p[u][0] - ox;
+                const double Ry = p[u][1] - oy;
+                const double Rz = p[u][2] - oz;
+                const double Z_new = dot3(Rx, Ry, Rz, nx, ny, nz);
+                const double dZ = Z_new - Z;
+                Z = Z + dZ;
+                const double shift_nxt = doppler_shift_v(v[u][0], v[u][1], v[u][2], vox, voy, voz, nx, ny, nz);
+                const StepSplit sp = split_step(shift_crt, shift_nxt, dZ, dsm);
+                k.points += sp.n;
+                const int half = sp.interp ? sp.half : 0;
+                const float s0 = (float)((sp.interp ? shift_crt : shift_nxt) - 1.0);
+                const float ds = (float)sp.dshift;
+                seg_f32[w + u] = make_int4(cb[u], sp.n | (half << STEP_HALF_SHIFT), __float_as_int(s0),
+                                           __float_as_int(ds));
+                c4[u] = cb[u];
+                sh4[u] = shift_nxt;
+                dz4[u] = dZ;
+                hz4[u] = (float)(sp.dZ * HDZ_SCALE);
+                if (sp.n > 0) {
+                    k.last_cell = cb[u];
+                    k.last_shift = sp.interp ? sub_shift(shift_crt, sp.n, sp.dshift) : shift_nxt;
+                }
+                shift_crt = shift_nxt;
+            } else {   // past the last step: an empty slot (no points)
+                seg_f32[w + u] = make_int4(0, 0, 0, 0);
+                c4[u] = 0;
+                sh4[u] = 0.0;
+                dz4[u] = 0.0;
+                hz4[u] = 0.0f;
+            }
+        }
+        *reinterpret_cast<int4 *>(seg_cell + w) = make_int4(c4[0], c4[1], c4[2], c4[3]);
+        reinterpret_cast<double2 *>(seg_shift + w)[0] = make_double2(sh4[0], sh4[1]);
+        reinterpret_cast<double2 *>(seg_shift + w)[1] = make_double2(sh4[2], sh4[3]);
+        reinterpret_cast<double2 *>(seg_dZ + w)[0] = make_double2(dz4[0], dz4[1]);
+        reinterpret_cast<double2 *>(seg_dZ + w)[1] = make_double2(dz4[2], dz4[3]);
+        *reinterpret_cast<float4 *>(seg_hdz32 + w) = make_float4(hz4[0], hz4[1], hz4[2], hz4[3]);
+        w += 4;
     }
     return k;
 }
 
 // One thread per (origin o, ray r < Rmax); r beyond the origin's own ray
 // count exits.  Entry q = cell_qbase[o] + r of the (origin, ray) arrays; the
-// ray direction is the global ray m_rbase[model] + r.
-template <bool FILL>
+// ray direction is the global ray m_rbase[model] + r.  With `cells` set the
+// counting pass records the first `cap` cells of every walk, and the filling
+// pass leaves the walks that fit to k_replay.
+template <bool FILL, bool MASKED>
 __global__ void k_trace(int64_t N, int Rmax, const int32_t *__restrict__ cell_model,
                         const int64_t *__restrict__ cell_qbase, const int32_t *__restrict__ m_R,
                         const int32_t *__restrict__ m_rbase, const double *__restrict__ pos,
                         const double *__restrict__ vel, const int64_t *__restrict__ nb_off,
-                        const NbRec *__restrict__ nb, const uint8_t *__restrict__ is_boundary,
+                        const NbRec *__restrict__ nb, const uint64_t *__restrict__ fwd,
+                        const uint8_t *__restrict__ is_boundary,
                         const double *__restrict__ rays, const double *__restrict__ dshift_max,
+                        int4 *__restrict__ cells, int cap,
                         int32_t *__restrict__ nseg, int32_t *__restrict__ npts,
                         const int64_t *__restrict__ seg_off, int32_t *__restrict__ seg_cell,
                         double *__restrict__ seg_shift, double *__restrict__ seg_dZ,
@@ -127,9 +274,11 @@ __global__ void k_trace(int64_t N, int Rmax, const int32_t *__restrict__ cell_mo
     const int64_t q = cell_qbase[o] + r;
     const int64_t rg = (int64_t)m_rbase[mdl] + r;
     const double nx = rays[3 * rg + 0], ny = rays[3 * rg + 1], nz = rays[3 * rg + 2];
-    const Walk k = trace_walk<FILL, false>(N, pos, vel, nb_off, nb, is_boundary, o, nx, ny, nz, nx, ny, nz,
-                                           dshift_max[o], FILL ? seg_off[q] : 0, seg_cell, seg_shift,
-                                           seg_dZ, seg_f32, seg_hdz32, err_flag);
+    if (FILL && cells && nseg[q] <= cap) return;   // replayed by k_replay
+    const Walk k = trace_walk<FILL, false, MASKED>(N, pos, vel, nb_off, nb, Cands{fwd, Rmax, r}, is_boundary,
+                                                   o, nx, ny, nz, nx, ny, nz, dshift_max[o], FILL ? seg_off[q] : 0,
+                                                   cells ? cells + q * (cap >> 2) : nullptr, cap, seg_cell,
+                                                   seg_shift, seg_dZ, seg_f32, seg_hdz32, err_flag);
     if (!FILL) {
         nseg[q] = k.steps;
         npts[q] = k.points;
@@ -137,7 +286,44 @@ __global__ void k_trace(int64_t N, int Rmax, const int32_t *__restrict__ cell_mo
     } else {
         far_cell[q] = k.last_cell;
         far_shift[q] = k.last_shift;
+        // empty slots up to the next block of four
+        for (int64_t e = seg_off[q] + k.steps; e & 3; e++) {
+            seg_cell[e] = 0;
+            seg_shift[e] = 0.0;
+            seg_dZ[e] = 0.0;
+            seg_f32[e] = make_int4(0, 0, 0, 0);
+            seg_hdz32[e] = 0.0f;
+        }
     }
+}
+
+// The filling pass of the walks the counting pass recorded (nseg <= cap),
+// one thread per (origin, ray) as k_trace.
+__global__ void k_replay(int64_t N, int Rmax, const int32_t *__restrict__ cell_model,
+                         const int64_t *__restrict__ cell_qbase, const int32_t *__restrict__ m_R,
+                         const int32_t *__restrict__ m_rbase, const double *__restrict__ pos,
+                         const double *__restrict__ vel, const double *__restrict__ rays,
+                         const double *__restrict__ dshift_max, const int4 *__restrict__ cells, int cap,
+                         const int32_t *__restrict__ nseg, const int64_t *__restrict__ seg_off,
+                         int32_t *__restrict__ seg_cell, double *__restrict__ seg_shift, double *__restrict__ seg_dZ,
+                         int4 *__restrict__ seg_f32, float *__restrict__ seg_hdz32, int32_t *__restrict__ far_cell,
+                         double *__restrict__ far_shift)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * Rmax) return;
+    const int32_t o = (int32_t)(t / Rmax);
+    const int r = (int)(t % Rmax);
+    const int32_t mdl = cell_model[o];
+    if (r >= m_R[mdl]) return;
+    const int64_t q = cell_qbase[o] + r;
+    const int ns = nseg[q];
+    if (ns > cap) return;   // scanned again by k_trace<true>
+    const int64_t rg = (int64_t)m_rbase[mdl] + r;
+    const double nx = rays[3 * rg + 0], ny = rays[3 * rg + 1], nz = rays[3 * rg + 2];
+    const Walk k = replay_walk(pos, vel, o, nx, ny, nz, dshift_max[o], seg_off[q], cells + q * (cap >> 2), ns,
+                               seg_cell, seg_shift, seg_dZ, seg_f32, seg_hdz32);
+    far_cell[q] = k.last_cell;
+    far_shift[q] = k.last_shift;
 }
 
 // Rest-frame walks of one model (Geometry::trace_ray<Rest>).  With r_img >= 0
@@ -145,10 +331,11 @@ __global__ void k_trace(int64_t N, int Rmax, const int32_t *__restrict__ cell_mo
 // itself; with r_img < 0 every ray, entry o R + r (for npoints_on_rays<Rest>).
 // The rest-frame direction of ray r is r for r < R/2 and its antipode
 // otherwise (geometry.tpp:452-460).
-template <bool FILL>
+template <bool FILL, bool MASKED>
 __global__ void k_trace_rest(int64_t N, int R, int r_img, const double *__restrict__ pos,
                              const double *__restrict__ vel, const int64_t *__restrict__ nb_off,
-                             const NbRec *__restrict__ nb, const uint8_t *__restrict__ is_boundary,
+                             const NbRec *__restrict__ nb, const uint64_t *__restrict__ fwd,
+                             const uint8_t *__restrict__ is_boundary,
                              const double *__restrict__ rays, const int32_t *__restrict__ antipod,
                              const double *__restrict__ dshift_max, int32_t *__restrict__ nseg,
                              int32_t *__restrict__ npts, const int64_t *__restrict__ seg_off,
@@ -167,9 +354,9 @@ __global__ void k_trace_rest(int64_t N, int R, int r_img, const double *__restri
     const int rs = ray >= R / 2 ? antipod[ray] : ray;
     const double nx = rays[3 * ray + 0], ny = rays[3 * ray + 1], nz = rays[3 * ray + 2];
     const double sx = rays[3 * rs + 0], sy = rays[3 * rs + 1], sz = rays[3 * rs + 2];
-    const Walk k = trace_walk<FILL, true>(N, pos, vel, nb_off, nb, is_boundary, o, nx, ny, nz, sx, sy, sz,
-                                          dshift_max[o], FILL ? seg_off[t] : 0, seg_cell, seg_shift,
-                                          seg_dZ, seg_f32, seg_hdz32, err_flag);
+    const Walk k = trace_walk<FILL, true, MASKED>(N, pos, vel, nb_off, nb, Cands{fwd, R, ray}, is_boundary, o,
+                                          nx, ny, nz, sx, sy, sz, dshift_max[o], FILL ? seg_off[t] : 0,
+                                          nullptr, 0, seg_cell, seg_shift, seg_dZ, seg_f32, seg_hdz32, err_flag);
     if (!FILL) {
         nseg[t] = k.steps;
         npts[t] = k.points;
@@ -179,6 +366,39 @@ __global__ void k_trace_rest(int64_t N, int R, int r_img, const double *__restri
         far_shift[t] = k.last_shift;
         origin_shift[t] = rest_shift(vel, o, sx, sy, sz);
     }
+}
+
+// Candidate masks of every (cell c, ray r < its model's ray count), entry
+// c Rmax + r: bit j is set unless entry j of c's neighbour list lies behind c
+// along the ray, (p_j - p_c).n < -tau; bit 63 is set for a boundary cell.
+// The walk's test Z_new > Z compares (p_j - o).n with the running Z, which
+// equals (p_c - o).n up to a few roundings of numbers no larger than
+// |p| + |o|; tau = 1e-9 of the model's extent dwarfs that (some 1e-15 of
+// it), so a cleared neighbour never passes the test and the walk's choice is
+// unchanged.
+__global__ void k_forward_masks(int64_t N, int Rmax, const int32_t *__restrict__ cell_model,
+                                const int32_t *__restrict__ m_R, const int32_t *__restrict__ m_rbase,
+                                const double *__restrict__ pos, const int64_t *__restrict__ nb_off,
+                                const NbRec *__restrict__ nb, const uint8_t *__restrict__ is_boundary,
+                                const double *__restrict__ rays, double tau, uint64_t *__restrict__ fwd)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * Rmax) return;
+    const int32_t c = (int32_t)(t / Rmax);
+    const int r = (int)(t % Rmax);
+    const int32_t mdl = cell_model[c];
+    if (r >= m_R[mdl]) return;
+    const int64_t rg = (int64_t)m_rbase[mdl] + r;
+    const double nx = rays[3 * rg + 0], ny = rays[3 * rg + 1], nz = rays[3 * rg + 2];
+    const double cx = pos[3 * (int64_t)c + 0], cy = pos[3 * (int64_t)c + 1], cz = pos[3 * (int64_t)c + 2];
+    const int64_t b = nb_off[c];
+    const int deg = (int)(nb_off[c + 1] - b);
+    uint64_t m = is_boundary[c] ? FWD_BOUNDARY : 0;
+    for (int j = 0; j < deg; j++) {
+        const NbRec e = nb[b + j];
+        if (!((e.x - cx) * nx + (e.y - cy) * ny + (e.z - cz) * nz < -tau)) m |= 1ull << j;   // NaN: kept
+    }
+    fwd[t] = m;
 }
 
 // Expands the cached steps of one ray into its projected points (what
@@ -465,11 +685,32 @@ int launch_band_cells(mgr_ctx *ctx)
     return MGR_OK;
 }
 
+// Device memory held for the length of a host function.
+struct DeviceBuffer {
+    void *p = nullptr;
+    void release() { if (p) (void)hipFree(p); p = nullptr; }
+    ~DeviceBuffer() { release(); }
+};
+
 static int trace_error(mgr_ctx *ctx, int32_t errv)
 {
     if (errv & ERR_TOO_MANY) return ctx->fail(MGR_ERR_TRACE, "Too many (> 10 000) interpolations needed!");
     return ctx->fail(MGR_ERR_TRACE, errv & ERR_NO_NEIGHBOUR ? "a ray found no forward neighbour inside the mesh"
                                                             : "a ray visited more steps than there are cells");
+}
+
+// Candidate masks of every (cell, ray), once per context (the geometry is
+// static); none when a neighbour list is longer than 63.
+int launch_forward_masks(mgr_ctx *ctx)
+{
+    const int64_t n = ctx->N * ctx->R;
+    if (ctx->fwd_mask || ctx->nb_kmax > 63 || n == 0) return MGR_OK;
+    MGR_HIP(hipMalloc((void **)&ctx->fwd_mask, n * sizeof(uint64_t)));
+    hipLaunchKernelGGL(k_forward_masks, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, ctx->stream, ctx->N,
+                       (int)ctx->R, ctx->d_cell_model, ctx->d_m_R, ctx->d_m_rbase, ctx->pos, ctx->nb_off, ctx->nb_rec,
+                       ctx->is_boundary, ctx->rays, ctx->fwd_tau, ctx->fwd_mask);
+    MGR_LAUNCH_CHECK();
+    return MGR_OK;
 }
 
 int launch_trace(mgr_ctx *ctx)
@@ -484,11 +725,31 @@ int launch_trace(mgr_ctx *ctx)
     static const uint64_t one_bits[2] = {0x3FF0000000000000ull, 0x3FF0000000000000ull};   // 1.0, 1.0
     MGR_HIP(hipMemcpyAsync(ctx->shift_bits, one_bits, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
     MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
-    hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->d_cell_model,
-                       ctx->d_cell_qbase, ctx->d_m_R, ctx->d_m_rbase, ctx->pos,
-                       ctx->vel, ctx->nb_off, ctx->nb_rec, ctx->is_boundary, ctx->rays, ctx->dshift_max,
-                       ctx->nseg, ctx->npts, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, ctx->shift_bits, ctx->err_flag);
+    // The counting pass records the cells of every walk of up to `cap` steps
+    // (a multiple of 4) in a buffer of at most a quarter of the free memory;
+    // the filling pass replays them.  Longer walks are scanned again.
+    DeviceBuffer cells;
+    int cap = 0;
+    if (NR > 0) {
+        size_t freeb = 0, totb = 0;
+        if (hipMemGetInfo(&freeb, &totb) == hipSuccess) {
+            const int64_t budget = std::min<int64_t>((int64_t)(freeb / 4), TRACE_RECORD_BYTES);
+            cap = (int)std::min<int64_t>(TRACE_RECORD_STEPS, budget / NR / (int64_t)sizeof(int32_t) / 4 * 4);
+            if (cap < 4 || hipMalloc(&cells.p, (size_t)NR * cap * sizeof(int32_t)) != hipSuccess) {
+                (void)hipGetLastError();
+                cells.p = nullptr;
+                cap = 0;
+            }
+        }
+    }
+    int rc = launch_forward_masks(ctx);
+    if (rc) return rc;
+    auto count = ctx->fwd_mask ? k_trace<false, true> : k_trace<false, false>;
+    hipLaunchKernelGGL(count, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->d_cell_model,
+                       ctx->d_cell_qbase, ctx->d_m_R, ctx->d_m_rbase, ctx->pos, ctx->vel, ctx->nb_off, ctx->nb_rec,
+                       ctx->fwd_mask, ctx->is_boundary, ctx->rays, ctx->dshift_max, (int4 *)cells.p, cap, ctx->nseg,
+                       ctx->npts, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       ctx->shift_bits, ctx->err_flag);
     MGR_LAUNCH_CHECK();
     std::vector<int32_t> ns(NR), np(NR);
     int32_t errv = 0;
@@ -500,7 +761,11 @@ int launch_trace(mgr_ctx *ctx)
     MGR_HIP(hipMemcpy(np.data(), ctx->npts, NR * sizeof(int32_t), hipMemcpyDeviceToHost));
     MGR_HIP(hipMemcpy(sbits, ctx->shift_bits, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost));
     std::vector<int64_t> off(NR + 1, 0);
-    for (int64_t q = 0; q < NR; q++) off[q + 1] = off[q] + ns[q];
+    int64_t max_steps = 0;
+    for (int64_t q = 0; q < NR; q++) {   // every walk starts on a block of four steps
+        off[q + 1] = off[q] + (ns[q] + 3) / 4 * 4;
+        max_steps = std::max<int64_t>(max_steps, ns[q]);
+    }
     ctx->total_segments = off[NR];
     double smin, smax;
     std::memcpy(&smin, &sbits[0], sizeof(double));
@@ -546,13 +811,22 @@ int launch_trace(mgr_ctx *ctx)
         MGR_HIP(hipMalloc((void **)&ctx->far_shift, std::max<int64_t>(NR, 1) * sizeof(double)));
     }
     MGR_HIP(hipMemcpy(ctx->seg_off, off.data(), (NR + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->d_cell_model,
-                       ctx->d_cell_qbase, ctx->d_m_R, ctx->d_m_rbase, ctx->pos,
-                       ctx->vel, ctx->nb_off, ctx->nb_rec, ctx->is_boundary, ctx->rays, ctx->dshift_max,
-                       nullptr, nullptr, ctx->seg_off, ctx->seg_cell, ctx->seg_shift, ctx->seg_dZ,
-                       ctx->seg_f32, ctx->seg_hdz32, ctx->far_cell, ctx->far_shift, nullptr, ctx->err_flag);
+    if (cells.p)
+        hipLaunchKernelGGL(k_replay, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->d_cell_model,
+                           ctx->d_cell_qbase, ctx->d_m_R, ctx->d_m_rbase, ctx->pos, ctx->vel, ctx->rays,
+                           ctx->dshift_max, (const int4 *)cells.p, cap, ctx->nseg, ctx->seg_off, ctx->seg_cell,
+                           ctx->seg_shift, ctx->seg_dZ, ctx->seg_f32, ctx->seg_hdz32, ctx->far_cell, ctx->far_shift);
+    if (!cells.p || max_steps > cap) {   // the walks not recorded
+        auto fill = ctx->fwd_mask ? k_trace<true, true> : k_trace<true, false>;
+        hipLaunchKernelGGL(fill, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, ctx->d_cell_model,
+                           ctx->d_cell_qbase, ctx->d_m_R, ctx->d_m_rbase, ctx->pos, ctx->vel, ctx->nb_off, ctx->nb_rec,
+                           ctx->fwd_mask, ctx->is_boundary, ctx->rays, ctx->dshift_max, (int4 *)cells.p, cap,
+                           ctx->nseg, nullptr, ctx->seg_off, ctx->seg_cell, ctx->seg_shift, ctx->seg_dZ,
+                           ctx->seg_f32, ctx->seg_hdz32, ctx->far_cell, ctx->far_shift, nullptr, ctx->err_flag);
+    }
     MGR_LAUNCH_CHECK();
     MGR_HIP(hipStreamSynchronize(ctx->stream));
+    cells.release();
     if (ctx->nband > 0) {
         const int rc = launch_band_cells(ctx);
         if (rc) return rc;
@@ -669,6 +943,8 @@ int launch_trace_image(mgr_ctx *ctx, int64_t ray)
     const int64_t N = ctx->N, R = ctx->R, NQ = 2 * N;
     mgr_ctx::ImageTrace &g = ctx->img;
     g.ray = -1;
+    const int mrc = launch_forward_masks(ctx);
+    if (mrc) return mrc;
     if (!g.nseg) {
         MGR_HIP(hipMalloc((void **)&g.nseg, NQ * sizeof(int32_t)));
         MGR_HIP(hipMalloc((void **)&g.npts, NQ * sizeof(int32_t)));
@@ -681,10 +957,11 @@ int launch_trace_image(mgr_ctx *ctx, int64_t ray)
     static const uint64_t one_bits[2] = {0x3FF0000000000000ull, 0x3FF0000000000000ull};   // 1.0, 1.0
     MGR_HIP(hipMemcpyAsync(ctx->shift_bits, one_bits, 2 * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
     MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
-    hipLaunchKernelGGL(k_trace_rest<false>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, (int)ray, ctx->pos,
-                       ctx->vel, ctx->nb_off, ctx->nb_rec, ctx->is_boundary, ctx->rays, ctx->d_antipod,
-                       ctx->dshift_max, g.nseg, g.npts, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, ctx->shift_bits, ctx->err_flag);
+    auto count = ctx->fwd_mask ? k_trace_rest<false, true> : k_trace_rest<false, false>;
+    hipLaunchKernelGGL(count, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, (int)ray, ctx->pos, ctx->vel,
+                       ctx->nb_off, ctx->nb_rec, ctx->fwd_mask, ctx->is_boundary, ctx->rays,
+                       ctx->d_antipod, ctx->dshift_max, g.nseg, g.npts, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, ctx->shift_bits, ctx->err_flag);
     MGR_LAUNCH_CHECK();
     MGR_HIP(hipStreamSynchronize(ctx->stream));
     int32_t errv = 0;
@@ -715,10 +992,11 @@ int launch_trace_image(mgr_ctx *ctx, int64_t ray)
         g.total_segments = T;
     }
     MGR_HIP(hipMemcpy(g.seg_off, off.data(), (NQ + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_trace_rest<true>, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, (int)ray, ctx->pos,
-                       ctx->vel, ctx->nb_off, ctx->nb_rec, ctx->is_boundary, ctx->rays, ctx->d_antipod,
-                       ctx->dshift_max, nullptr, nullptr, g.seg_off, g.seg_cell, g.seg_shift, g.seg_dZ,
-                       g.seg_f32, g.seg_hdz32, g.far_cell, g.far_shift, g.origin_shift, nullptr,
+    auto fill = ctx->fwd_mask ? k_trace_rest<true, true> : k_trace_rest<true, false>;
+    hipLaunchKernelGGL(fill, dim3(grid), dim3(128), 0, ctx->stream, N, (int)R, (int)ray, ctx->pos, ctx->vel,
+                       ctx->nb_off, ctx->nb_rec, ctx->fwd_mask, ctx->is_boundary, ctx->rays,
+                       ctx->d_antipod, ctx->dshift_max, nullptr, nullptr, g.seg_off, g.seg_cell, g.seg_shift,
+                       g.seg_dZ, g.seg_f32, g.seg_hdz32, g.far_cell, g.far_shift, g.origin_shift, nullptr,
                        ctx->err_flag);
     MGR_LAUNCH_CHECK();
     MGR_HIP(hipStreamSynchronize(ctx->stream));
@@ -782,6 +1060,8 @@ int mgr_get_npoints_on_rays_rest(mgr_ctx *ctx, int64_t *npoints)
     if (!ctx || !npoints) return MGR_ERR_ARGUMENT;
     if (ctx->M != 1) return ctx->fail(MGR_ERR_UNSUPPORTED, "rest-frame traces are per model (batch of one)");
     MGR_HIP(hipSetDevice(ctx->device));
+    const int mrc = mgr::launch_forward_masks(ctx);
+    if (mrc) return mrc;
     const int64_t N = ctx->N, R = ctx->R, NQ = N * R;
     int32_t *d_ns = nullptr, *d_np = nullptr;
     MGR_HIP(hipMalloc((void **)&d_ns, std::max<int64_t>(NQ, 1) * sizeof(int32_t)));
@@ -794,10 +1074,11 @@ int mgr_get_npoints_on_rays_rest(mgr_ctx *ctx, int64_t *npoints)
     int32_t errv = 0;
     if (hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream) != hipSuccess) rc = MGR_ERR_DEVICE;
     if (!rc) {
-        hipLaunchKernelGGL(k_trace_rest<false>, dim3((unsigned)((NQ + 127) / 128)), dim3(128), 0, ctx->stream, N,
-                           (int)R, -1, ctx->pos, ctx->vel, ctx->nb_off, ctx->nb_rec, ctx->is_boundary, ctx->rays,
-                           ctx->d_antipod, ctx->dshift_max, d_ns, d_np, nullptr, nullptr, nullptr, nullptr,
-                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->err_flag);
+        auto count = ctx->fwd_mask ? k_trace_rest<false, true> : k_trace_rest<false, false>;
+        hipLaunchKernelGGL(count, dim3((unsigned)((NQ + 127) / 128)), dim3(128), 0, ctx->stream, N, (int)R, -1,
+                           ctx->pos, ctx->vel, ctx->nb_off, ctx->nb_rec, ctx->fwd_mask,
+                           ctx->is_boundary, ctx->rays, ctx->d_antipod, ctx->dshift_max, d_ns, d_np, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->err_flag);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess ||
             hipMemcpy(&errv, ctx->err_flag, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(np.data(), d_np, NQ * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
