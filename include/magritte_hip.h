@@ -223,6 +223,15 @@ int mgr_compute_LTE_level_populations(mgr_ctx *ctx);
  * (src/Model/Geometry/geometry.tpp:21-207,328-465).  The reference re-traces
  * inside every compute_radiation_field; traces here are computed once.       */
 int mgr_trace(mgr_ctx *ctx);
+/* Options of the next mgr_trace (no counterpart in the reference; traces are
+ * bit-identical under every setting).  record_steps: how many cells of each
+ * walk the counting pass records for the filling pass to replay instead of
+ * scanning the neighbour lists again (-1: the default, 256; 0: none; else a
+ * multiple of 4 up to 4096; the buffer is also held to a quarter of the free
+ * memory).  candidate_masks: scan only the neighbours ahead of the current
+ * cell along the ray (on by default; lists longer than 63 entries are always
+ * scanned whole).                                                             */
+int mgr_set_trace_options(mgr_ctx *ctx, int64_t record_steps, int candidate_masks);
 
 /* ------------------------------------------------------------------------ */
 /* Hot path                                                                  */

@@ -179,6 +179,8 @@ struct mgr_ctx {
     int64_t  nb_kmax = 0;                          // longest neighbour list
     double   fwd_tau = 0.0;                        // candidate-mask margin (1e-9 of the extent)
     uint64_t *fwd_mask = nullptr;                  // [N*R] candidate neighbours per (cell, ray), or null
+    int64_t  trace_record_steps = 256;             // cells recorded per walk by the counting pass (0: none)
+    bool     trace_masks = true;                   // candidate masks on
     uint8_t *is_boundary = nullptr;                // [N]
     int32_t *cell2bdy = nullptr;                   // [N], -1 for interior cells
     int32_t *bdy2cell = nullptr;                   // [NB]

@@ -28,7 +28,7 @@ constexpr int32_t ERR_NO_NEIGHBOUR = 1;
 constexpr int32_t ERR_TOO_LONG     = 8;
 constexpr int32_t ERR_TOO_MANY     = 16;   // a step needs more than 10000 points
 
-constexpr int64_t TRACE_RECORD_STEPS = 256;        // recorded cells per walk, at most
+constexpr int64_t TRACE_RECORD_MAX   = 4096;       // recorded cells per walk, at most
 constexpr int64_t TRACE_RECORD_BYTES = 64LL << 30;  // record buffer, at most
 
 // positive doubles order like their bit patterns
@@ -704,7 +704,7 @@ static int trace_error(mgr_ctx *ctx, int32_t errv)
 int launch_forward_masks(mgr_ctx *ctx)
 {
     const int64_t n = ctx->N * ctx->R;
-    if (ctx->fwd_mask || ctx->nb_kmax > 63 || n == 0) return MGR_OK;
+    if (ctx->fwd_mask || !ctx->trace_masks || ctx->nb_kmax > 63 || n == 0) return MGR_OK;
     MGR_HIP(hipMalloc((void **)&ctx->fwd_mask, n * sizeof(uint64_t)));
     hipLaunchKernelGGL(k_forward_masks, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, ctx->stream, ctx->N,
                        (int)ctx->R, ctx->d_cell_model, ctx->d_m_R, ctx->d_m_rbase, ctx->pos, ctx->nb_off, ctx->nb_rec,
@@ -734,7 +734,7 @@ int launch_trace(mgr_ctx *ctx)
         size_t freeb = 0, totb = 0;
         if (hipMemGetInfo(&freeb, &totb) == hipSuccess) {
             const int64_t budget = std::min<int64_t>((int64_t)(freeb / 4), TRACE_RECORD_BYTES);
-            cap = (int)std::min<int64_t>(TRACE_RECORD_STEPS, budget / NR / (int64_t)sizeof(int32_t) / 4 * 4);
+            cap = (int)std::min<int64_t>(ctx->trace_record_steps, budget / NR / (int64_t)sizeof(int32_t) / 4 * 4);
             if (cap < 4 || hipMalloc(&cells.p, (size_t)NR * cap * sizeof(int32_t)) != hipSuccess) {
                 (void)hipGetLastError();
                 cells.p = nullptr;
@@ -1013,6 +1013,22 @@ int mgr_trace(mgr_ctx *ctx)
     if (!ctx) return MGR_ERR_ARGUMENT;
     MGR_HIP(hipSetDevice(ctx->device));
     return launch_trace(ctx);
+}
+
+int mgr_set_trace_options(mgr_ctx *ctx, int64_t record_steps, int candidate_masks)
+{
+    if (!ctx) return MGR_ERR_ARGUMENT;
+    if (record_steps < -1 || record_steps > TRACE_RECORD_MAX || (record_steps > 0 && record_steps % 4))
+        return ctx->fail(MGR_ERR_ARGUMENT, "record_steps: -1, 0 or a multiple of 4 up to 4096");
+    MGR_HIP(hipSetDevice(ctx->device));
+    ctx->trace_record_steps = record_steps < 0 ? 256 : record_steps;
+    ctx->trace_masks = candidate_masks != 0;
+    if (!ctx->trace_masks && ctx->fwd_mask) {
+        MGR_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(ctx->fwd_mask);
+        ctx->fwd_mask = nullptr;
+    }
+    return MGR_OK;
 }
 
 int mgr_get_ray(mgr_ctx *ctx, int64_t origin, int64_t ray, int64_t capacity, int64_t *cellNr,

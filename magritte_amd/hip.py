@@ -61,6 +61,7 @@ SIGNATURES = {
     "mgr_set_scattering": ([_vp, _i], _i),
     "mgr_set_window_skip": ([_vp, _i], _i),
     "mgr_set_far_cache": ([_vp, _i], _i),
+    "mgr_set_trace_options": ([_vp, _i64, _i], _i),
     "mgr_window_skip_stats": ([_vp, _vp, _vp], _i),
     "mgr_get_u": ([_vp, _vp], _i),
     "mgr_compute_spectral_discretisation_image": ([_vp, _d], _i),
@@ -350,6 +351,13 @@ class Context:
         """FP32 kernel: eliminate steps out of every line window without
         evaluating their profiles (bit-identical; on by default)."""
         self._check(self.lib.mgr_set_window_skip(self.h, 1 if enable else 0), "set_window_skip")
+
+    def set_trace_options(self, record_steps=-1, candidate_masks=True):
+        """Options of the next trace: cells recorded per walk for the filling
+        pass (-1 default, 0 none) and the candidate masks (bit-identical
+        traces either way)."""
+        self._check(self.lib.mgr_set_trace_options(self.h, int(record_steps), 1 if candidate_masks else 0),
+                    "set_trace_options")
 
     def set_far_cache(self, enable=True):
         """FP32 kernel: keep the far-end intensities of every ray and bin
