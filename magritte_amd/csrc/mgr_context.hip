@@ -119,6 +119,30 @@ __device__ __forceinline__ double bin_value(const LineTables *__restrict__ tb, c
     return lg + (lg * w) * roots[s * Q + z];
 }
 
+// The same with one workgroup per cell: the cell's F values are formed once
+// into LDS and each ranked against them there (F doubles of LDS).
+__global__ void k_spectral_cell(int64_t N, const LineTables *__restrict__ tb,
+                                const double *__restrict__ pw, double image_width,
+                                const double *__restrict__ roots, double *__restrict__ nu,
+                                int32_t *__restrict__ nr_line)
+{
+    extern __shared__ double vals[];
+    const int F = tb->F, Q = tb->Q, L = tb->L;
+    const int64_t p = blockIdx.x;
+    for (int i = threadIdx.x; i < F; i += blockDim.x) vals[i] = bin_value(tb, pw, N, p, image_width, roots, i);
+    __syncthreads();
+    for (int i = threadIdx.x; i < F; i += blockDim.x) {
+        const double vi = vals[i];
+        int rank = 0;
+        for (int j = 0; j < F; j++) {
+            const double vj = vals[j];
+            rank += (vj < vi) || (vj == vi && j < i);
+        }
+        nu[p * F + rank] = vi;
+        nr_line[(p * L + i / Q) * Q + i % Q] = rank;
+    }
+}
+
 __global__ void k_spectral(int64_t N, const LineTables *__restrict__ tb,
                            const double *__restrict__ pw, double image_width,
                            const double *__restrict__ roots, double *__restrict__ nu,
@@ -903,8 +927,12 @@ static int spectral_discretisation(mgr_ctx *ctx, bool image, double width)
 {
     MGR_HIP(hipSetDevice(ctx->device));
     const int64_t N = ctx->N, F = ctx->F, L = ctx->L, Q = ctx->Q;
-    hipLaunchKernelGGL(k_spectral, dim3(grid_for(N * F, 256)), dim3(256), 0, ctx->stream, N,
-                       ctx->tables, image ? nullptr : ctx->pw, width, ctx->roots, ctx->nu, ctx->nr_line);
+    if (F * (int64_t)sizeof(double) <= 48 * 1024)
+        hipLaunchKernelGGL(k_spectral_cell, dim3((unsigned)N), dim3(256), F * sizeof(double), ctx->stream, N,
+                           ctx->tables, image ? nullptr : ctx->pw, width, ctx->roots, ctx->nu, ctx->nr_line);
+    else
+        hipLaunchKernelGGL(k_spectral, dim3(grid_for(N * F, 256)), dim3(256), 0, ctx->stream, N,
+                           ctx->tables, image ? nullptr : ctx->pw, width, ctx->roots, ctx->nu, ctx->nr_line);
     MGR_LAUNCH_CHECK();
     // appears_in_line_integral / corresponding_{l,k,z}, indexed by the unsorted
     // position (sim_spectrum.tpp:153-184), and per line the sorted frequency

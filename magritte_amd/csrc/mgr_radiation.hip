@@ -290,23 +290,34 @@ __device__ __forceinline__ void eta_chi32x(const RFArgs<float> &a, const float4 
     chi = c_acc;
 }
 
-// Radiation::rescale_I_bdy, scalar branch (radiation.tpp:341-373): first bin
-// >= freq of the boundary cell's own grid (search_with_notch from 0 is a lower
-// bound), edge bins as they are, otherwise linear interpolation.
-__device__ __forceinline__ double rescale_I_bdy(const double *__restrict__ nup,
-                                                const double *__restrict__ Ib, int F, double fs)
+// Radiation::rescale_I_bdy, scalar branch (radiation.tpp:341-373), at the
+// count lo of the boundary cell's bins below the frequency (the grid is
+// ascending, so that is the first bin >= freq, where search_with_notch from
+// 0 stops): edge bins as they are, otherwise linear interpolation.
+__device__ __forceinline__ double rescale_at(const double *__restrict__ nup, const double *__restrict__ Ib,
+                                             int F, double fs, int lo)
 {
 #pragma clang fp contract(off)
-    int lo = 0, hi = F;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (nup[mid] < fs) lo = mid + 1; else hi = mid;
-    }
     const int notch = lo < F ? lo : F - 1;
     if (notch == 0 || notch == F - 1) return Ib[notch];
     const double nu1 = nup[notch], nu2 = nup[notch - 1];
     const double I1 = Ib[notch], I2 = Ib[notch - 1];
     return (I2 - I1) / (nu2 - nu1) * (fs - nu1) + I1;
+}
+
+// The largest power of two <= F: the first step of the searches below.
+__device__ __forceinline__ int search_top(int F) { return 1 << (31 - __builtin_clz((unsigned)F)); }
+
+// Radiation::rescale_I_bdy of the boundary cell whose bins are nup (its
+// intensities Ib) at frequency fs.  The count of bins below fs is found by
+// halving steps from search_top(F): the same probes for every lane.
+__device__ __forceinline__ double rescale_I_bdy(const double *__restrict__ nup,
+                                                const double *__restrict__ Ib, int F, double fs)
+{
+    int lo = 0;
+    for (int step = search_top(F); step > 0; step >>= 1)
+        if (lo + step <= F && nup[lo + step - 1] < fs) lo += step;
+    return rescale_at(nup, Ib, F, fs, lo);
 }
 
 // ---- Feautrier elimination state of one side ----------------------------------
@@ -521,24 +532,41 @@ __device__ __forceinline__ real far_in(const RFArgs<real> &a, int64_t q, double 
     return (real)(far_intensity(a, q, nu_f, F, err) * I_SCALE);
 }
 
-// The far-end intensity cache: one thread per (origin, ray, bin), the value
-// far_in computes in the kernel of precision T.
+// The far-end intensity cache: one wave per (origin, ray) entry, its lanes
+// over the bins, each the value far_in computes in the kernel of precision T
+// (far_intensity with the entry's far cell, boundary and shift read once).
 template <typename T>
-__global__ void k_far_cache(RFArgs<float> a, int Rmax, T *__restrict__ out, int32_t *__restrict__ err_flag)
+__global__ __launch_bounds__(256) void k_far_cache(RFArgs<float> a, int Rmax, T *__restrict__ out,
+                                                   int32_t *__restrict__ err_flag)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int F = a.F;
-    const int64_t per = (int64_t)Rmax * F;
-    if (t >= a.N * per) return;
-    const int32_t o = (int32_t)(t / per);
-    const int64_t rem = t - (int64_t)o * per;
-    const int r = (int)(rem / F);
-    const int f = (int)(rem - (int64_t)r * F);
+    const int64_t e = (int64_t)blockIdx.x * (blockDim.x / WAVE) + rfl_int((int)(threadIdx.x / WAVE));
+    if (e >= a.N * Rmax) return;
+    const int32_t o = (int32_t)(e / Rmax);
+    const int r = (int)(e - (int64_t)o * Rmax);
     if (r >= a.m_R[a.cell_model[o]]) return;
     const int64_t q = a.cell_qbase[o] + r;
-    int32_t err = 0;
-    out[q * F + f] = (T)(far_intensity(a, q, a.nu[(int64_t)o * F + f], F, err) * I_SCALE);
-    if (err) atomicOr(err_flag, err);
+    const int F = a.F;
+    const int lane = (int)(threadIdx.x & (WAVE - 1));
+    T *dst = out + q * F;
+    const int32_t cf = a.far_cell[q];
+    const int32_t b = cf < 0 ? -1 : a.cell2bdy[cf];
+    if (b < 0) {   // no points (0), or a far cell off the boundary (0 and an error)
+        if (cf >= 0 && lane == 0) atomicOr(err_flag, ERR_NOT_BOUNDARY);
+        for (int f = lane; f < F; f += WAVE) dst[f] = (T)(0.0 * I_SCALE);
+        return;
+    }
+    const double fsh = a.far_shift[q];
+    const double *nup = a.nu + (int64_t)cf * F;
+    const double *Ib = a.I_bdy + (int64_t)b * F;
+    const double *nuo = a.nu + (int64_t)o * F;
+    for (int f = lane; f < F; f += WAVE) {
+        double fs;
+        {
+#pragma clang fp contract(off)
+            fs = fsh * nuo[f];
+        }
+        dst[f] = (T)(rescale_I_bdy(nup, Ib, F, fs) * I_SCALE);
+    }
 }
 
 // One ray of a pair, eliminated from its boundary end (incoming intensity
@@ -1804,8 +1832,8 @@ int launch_far_cache(mgr_ctx *ctx, int precision)
     a.I_bdy = ctx->I_bdy;
     a.far_cell = ctx->far_cell;
     a.far_shift = ctx->far_shift;
-    const int64_t threads = ctx->N * ctx->R * ctx->F;
-    const dim3 grid((unsigned)((threads + 255) / 256));
+    const int64_t entries = ctx->N * ctx->R;   // one wave each, four to a workgroup
+    const dim3 grid((unsigned)((entries + 3) / 4));
     MGR_HIP(hipMemsetAsync(ctx->err_flag, 0, sizeof(int32_t), ctx->stream));
     if (pi)
         hipLaunchKernelGGL(k_far_cache<double>, grid, dim3(256), 0, ctx->stream, a, (int)ctx->R, ctx->far_I64,

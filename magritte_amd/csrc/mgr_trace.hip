@@ -449,42 +449,42 @@ __global__ void k_expand_ray(int32_t o, int64_t q, const int64_t *__restrict__ s
 // line, the hull (a, b) = (min c, max c) over the chunk's bins whose c can
 // reach the trace's shift range [smin, smax] at the widest width pw_max covers
 // all of them; it is widened by the step's own width when tested.  Hulls that
-// overlap (the same roots of different lines) merge.  One thread per (origin,
-// chunk); cnt = -1 when a chunk needs more than MAX_WIN intervals (never
-// skipped).
+// overlap (the same roots of different lines) merge.  cnt = -1 when a chunk
+// needs more than MAX_WIN intervals (never skipped).
 constexpr int MAX_WIN = 4;
 
 __global__ void k_chunk_windows(int64_t N, int F, int nchunk, bool root_major, const LineTables *__restrict__ tb,
                                 const double *__restrict__ nu, double smin, double smax, double pw_max,
                                 double *__restrict__ win, int8_t *__restrict__ cnt)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // one wave per (origin, chunk), lane i on the chunk's bin 64 k + i; the
+    // hull of each line is a wave minimum and maximum (exact, so in any order)
+    const int64_t t = (int64_t)blockIdx.x * (blockDim.x / WAVE) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     if (t >= N * nchunk) return;
     const int64_t o = t / nchunk;
     const int k = (int)(t % nchunk);
-    const int f0 = k * WAVE, f1 = min(F, f0 + WAVE);
+    const int i = k * WAVE + (int)(threadIdx.x & (WAVE - 1));
+    const bool valid = i < F;
+    const double nuv = valid ? nu[o * F + lane_bin(i, tb->L, F, root_major)] : 1.0;
     const double reach = MASK_WINDOW * pw_max * 1.01;
     double wa[MAX_WIN], wb[MAX_WIN];
     int n = 0;
     for (int l = 0; l < tb->L && n >= 0; l++) {
         const double nl = tb->line[l];
-        double a = 0.0, b = 0.0;
-        bool any = false;
-        for (int i = f0; i < f1; i++) {
-            const double c = nl / nu[o * F + lane_bin(i, tb->L, F, root_major)];
-            if (c * (1.0 - reach) <= smax * (1.0 + 1e-9) && c * (1.0 + reach) >= smin * (1.0 - 1e-9)) {
-                if (!any) { a = c; b = c; any = true; }
-                a = fmin(a, c);
-                b = fmax(b, c);
-            }
+        const double c = nl / nuv;
+        const bool in = valid && c * (1.0 - reach) <= smax * (1.0 + 1e-9) && c * (1.0 + reach) >= smin * (1.0 - 1e-9);
+        if (!__any(in)) continue;
+        double a = in ? c : DBL_MAX, b = in ? c : -DBL_MAX;
+        for (int off = WAVE / 2; off >= 1; off >>= 1) {
+            a = fmin(a, __shfl_xor(a, off));
+            b = fmax(b, __shfl_xor(b, off));
         }
-        if (!any) continue;
         // overlapping intervals (the same roots of different lines) merge
         bool merged = false;
-        for (int i = 0; i < n && !merged; i++) {
-            if (a <= wb[i] && b >= wa[i]) {
-                wa[i] = fmin(wa[i], a);
-                wb[i] = fmax(wb[i], b);
+        for (int j = 0; j < n && !merged; j++) {
+            if (a <= wb[j] && b >= wa[j]) {
+                wa[j] = fmin(wa[j], a);
+                wb[j] = fmax(wb[j], b);
                 merged = true;
             }
         }
@@ -494,9 +494,10 @@ __global__ void k_chunk_windows(int64_t N, int F, int nchunk, bool root_major, c
         wb[n] = b;
         n++;
     }
-    for (int i = 0; i < n; i++) {
-        win[((o * STEP_MASK_CHUNKS + k) * MAX_WIN + i) * 2 + 0] = wa[i];
-        win[((o * STEP_MASK_CHUNKS + k) * MAX_WIN + i) * 2 + 1] = wb[i];
+    if ((threadIdx.x & (WAVE - 1)) != 0) return;
+    for (int j = 0; j < n; j++) {
+        win[((o * STEP_MASK_CHUNKS + k) * MAX_WIN + j) * 2 + 0] = wa[j];
+        win[((o * STEP_MASK_CHUNKS + k) * MAX_WIN + j) * 2 + 1] = wb[j];
     }
     cnt[o * STEP_MASK_CHUNKS + k] = (int8_t)n;
 }
@@ -505,7 +506,10 @@ __global__ void k_chunk_windows(int64_t N, int F, int nchunk, bool root_major, c
 // its own shift in its own cell; the sub-points of an interpolated step lie
 // between the shifts of its two ends and in its two cells.  Bit k is set when
 // that shift range [lo, hi] misses every interval of chunk k widened by the
-// (wider) cell width.  One thread per (origin, ray).
+// (wider) cell width.  One thread per (origin, ray); every ray's steps start
+// on a block of four, so a block's cells and shifts are read with 16-byte
+// loads and its four mask bytes written as one word (zero past the last
+// step).
 __global__ void k_step_masks(int64_t N, int Rmax, int nchunk, const int32_t *__restrict__ cell_model,
                              const int64_t *__restrict__ cell_qbase, const int32_t *__restrict__ m_R,
                              const double *__restrict__ pw, const double *__restrict__ dshift_max,
@@ -523,29 +527,43 @@ __global__ void k_step_masks(int64_t N, int Rmax, int nchunk, const int32_t *__r
     const int64_t s0 = seg_off[q];
     const int ns = nseg[q];
     const double dsm = dshift_max[o];
+    int cn[STEP_MASK_CHUNKS];
+    for (int k = 0; k < nchunk; k++) cn[k] = cnt[(int64_t)o * STEP_MASK_CHUNKS + k];
+    const double *wo = win + (int64_t)o * STEP_MASK_CHUNKS * MAX_WIN * 2;
     int32_t c_prev = o;
     double s_prev = 1.0;   // co-moving shift of the origin
-    for (int j = 0; j < ns; j++) {
-        const int32_t c = seg_cell[s0 + j];
-        const double s = seg_shift[s0 + j];
-        const bool interp = fabs(s - s_prev) > dsm;   // as split_step
-        const double lo = interp ? fmin(s_prev, s) : s, hi = interp ? fmax(s_prev, s) : s;
-        const double w = MASK_WINDOW * (interp ? fmax(pw[c_prev], pw[c]) : pw[c]);
-        unsigned bits = 0;
-        for (int k = 0; k < nchunk; k++) {
-            const int n = cnt[(int64_t)o * STEP_MASK_CHUNKS + k];
-            if (n < 0) continue;
-            bool out = true;
-            for (int i = 0; i < n; i++) {
-                const double a = win[(((int64_t)o * STEP_MASK_CHUNKS + k) * MAX_WIN + i) * 2 + 0];
-                const double b = win[(((int64_t)o * STEP_MASK_CHUNKS + k) * MAX_WIN + i) * 2 + 1];
-                if (hi >= (1.0 - w) * a && lo <= (1.0 + w) * b) out = false;
+    for (int j0 = 0; j0 < ns; j0 += 4) {
+        const int4 cb = *reinterpret_cast<const int4 *>(seg_cell + s0 + j0);
+        const double2 sa = reinterpret_cast<const double2 *>(seg_shift + s0 + j0)[0];
+        const double2 sb = reinterpret_cast<const double2 *>(seg_shift + s0 + j0)[1];
+        const int32_t cs[4] = {cb.x, cb.y, cb.z, cb.w};
+        const double ss[4] = {sa.x, sa.y, sb.x, sb.y};
+        unsigned word = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (j0 + u >= ns) break;
+            const int32_t c = cs[u];
+            const double s = ss[u];
+            const bool interp = fabs(s - s_prev) > dsm;   // as split_step
+            const double lo = interp ? fmin(s_prev, s) : s, hi = interp ? fmax(s_prev, s) : s;
+            const double w = MASK_WINDOW * (interp ? fmax(pw[c_prev], pw[c]) : pw[c]);
+            unsigned bits = 0;
+            for (int k = 0; k < nchunk; k++) {
+                const int n = cn[k];
+                if (n < 0) continue;
+                bool out = true;
+                for (int i = 0; i < n; i++) {
+                    const double a = wo[(k * MAX_WIN + i) * 2 + 0];
+                    const double b = wo[(k * MAX_WIN + i) * 2 + 1];
+                    if (hi >= (1.0 - w) * a && lo <= (1.0 + w) * b) out = false;
+                }
+                if (out) bits |= 1u << k;
             }
-            if (out) bits |= 1u << k;
+            word |= bits << (8 * u);
+            c_prev = c;
+            s_prev = s;
         }
-        mask[s0 + j] = (uint8_t)bits;
-        c_prev = c;
-        s_prev = s;
+        *reinterpret_cast<uint32_t *>(mask + s0 + j0) = word;
     }
 }
 
@@ -863,7 +881,7 @@ int launch_step_masks(mgr_ctx *ctx)
         (void)hipFree(win);
         return ctx->fail(MGR_ERR_DEVICE, "out of device memory");
     }
-    hipLaunchKernelGGL(k_chunk_windows, dim3((unsigned)((N * nchunk + 127) / 128)), dim3(128), 0, ctx->stream, N,
+    hipLaunchKernelGGL(k_chunk_windows, dim3((unsigned)((N * nchunk + 3) / 4)), dim3(4 * WAVE), 0, ctx->stream, N,
                        (int)ctx->F, nchunk, rm, ctx->tables, ctx->nu, ctx->shift_min, ctx->shift_max, ctx->pw_max,
                        win, cnt);
     hipLaunchKernelGGL(k_step_masks, dim3((unsigned)((N * ctx->R + 127) / 128)), dim3(128), 0, ctx->stream, N,
