@@ -380,11 +380,12 @@ int mgr_radiation_field_shape(mgr_ctx *ctx, int precision, int *waves_per_group,
  * the shape.                                                                  */
 int mgr_set_radiation_field_shape(mgr_ctx *ctx, int precision, int waves_per_group, int steps_per_chunk);
 
-/* Out-of-window skip of the FP32 radiation-field kernel (no counterpart in
- * the reference; on by default): steps at which no bin of a 64-bin wave comes
- * within 10.5 line widths of any line are eliminated with eta = 0 and chi at
- * its floor without evaluating the profiles, which are exactly 0 there in
- * FP32; results are bit-identical either way.  Off evaluates every point.    */
+/* Out-of-window skip of the radiation-field kernels (no counterpart in the
+ * reference; on by default): steps at which no bin of a 64-bin wave comes
+ * within 10.5 line widths of any line (FP32; the profiles are exactly 0 there)
+ * or 11 (1 + 1e-6) widths (FP64; get_eta_and_chi's own window holds no line)
+ * are eliminated with eta = 0 and chi at its floor without evaluating the
+ * profiles; results are bit-identical either way.  Off evaluates every point. */
 int mgr_set_window_skip(mgr_ctx *ctx, int enable);
 /* Far-end intensity cache (no counterpart in the reference; on by default):
  * the rescaled boundary intensity entering every (origin, ray) and bin,

@@ -60,6 +60,11 @@ constexpr int WAVE = 64;                 // CDNA wavefront width
 constexpr int STEP_N_MASK = 0x3FFF, STEP_HALF_SHIFT = 14, STEP_HALF_MASK = 0x1FFF, STEP_OUT_SHIFT = 27;
 constexpr int STEP_MASK_CHUNKS = 5;
 constexpr double MASK_WINDOW = 10.5;
+// The FP64 kernel's bits: more than 11 (1 + 1e-6) widths from every line, so
+// that get_eta_and_chi's own +-11-width window (LINE_WINDOW) holds no line at
+// any sub-point whatever the rounding of the shift (~1e-16): eta = 0 and chi
+// at the floor exactly, the values the out-of-window branch uses.
+constexpr double MASK_WINDOW64 = 11.0 * (1.0 + 1e-6);
 constexpr int MAX_INTERPOLATION = 10000; // geometry.tpp:138
 
 // One neighbour entry of the tracer: the neighbour's index and position, so
@@ -292,6 +297,8 @@ struct mgr_ctx {
     // any of the step's sub-points; valid for the current trace and frequencies
     uint8_t *step_mask = nullptr;   // [total segments]
     bool     masks_valid = false;
+    uint8_t *step_mask64 = nullptr; // [total segments] the FP64 kernel's bits (MASK_WINDOW64)
+    bool     masks64_valid = false;
     float    hdz32_max = 0.0f;      // largest dz' of the cached steps (set with the masks)
     bool     rf_window_skip = true;
     // lane order of the radiation-field kernel (lane_bin), chosen with the
@@ -373,6 +380,6 @@ int launch_ng(mgr_ctx *ctx);
 int read_convergence(mgr_ctx *ctx, mgr_conv *conv);
 int set_active(mgr_ctx *ctx, bool all);   // all: every model of the batch iterates
 int launch_trace_image(mgr_ctx *ctx, int64_t ray);   // rest-frame trace of ray and its antipode
-int launch_step_masks(mgr_ctx *ctx);   // out-of-window bits of the cached steps (when stale)
+int launch_step_masks(mgr_ctx *ctx, bool fp64);   // out-of-window bits of the cached steps (when stale)
 void free_image(mgr_ctx *ctx);
 }  // namespace mgr

@@ -787,7 +787,7 @@ int free_model(mgr_ctx *ctx)
     dfree(ctx->Jeff); dfree(ctx->Jdif); dfree(ctx->err_flag); dfree(ctx->seg_off); dfree(ctx->nseg);
     dfree(ctx->npts); dfree(ctx->seg_cell); dfree(ctx->seg_shift); dfree(ctx->seg_dZ);
     dfree(ctx->seg_f32); dfree(ctx->seg_hdz32); dfree(ctx->far_cell); dfree(ctx->far_shift);
-    dfree(ctx->step_mask);
+    dfree(ctx->step_mask); dfree(ctx->step_mask64);
     dfree(ctx->far_I32);
     dfree(ctx->far_I64);
     dfree(ctx->shift_bits); dfree(ctx->partials); dfree(ctx->reduced);
@@ -967,6 +967,7 @@ static int spectral_discretisation(mgr_ctx *ctx, bool image, double width)
     ctx->spectral_set = true;
     ctx->spectral_image = image;
     ctx->masks_valid = false;   // the out-of-window bits depend on the frequencies
+    ctx->masks64_valid = false;
     ctx->far_invalidate();
     return MGR_OK;
 }

@@ -998,7 +998,7 @@ static int ensure_graphs(mgr_ctx *ctx, int precision)
     if (!ctx->traced && (rc = launch_trace(ctx))) return rc;
     if ((rc = set_active(ctx, true))) return rc;
     // recomputed in place when the frequencies changed; the graphs read them at replay
-    if (precision == MGR_FP32 && (rc = launch_step_masks(ctx))) return rc;   // FP32 kernel only
+    if ((rc = launch_step_masks(ctx, precision == MGR_FP64))) return rc;
     if ((rc = launch_far_cache(ctx, precision))) return rc;
     if ((rc = tune_radiation_field(ctx, precision))) return rc;
     if (ctx->graph_precision != precision) {

@@ -84,7 +84,17 @@ template <> struct Prec<float> {
 };
 template <> struct Prec<double> {
     using row_t = double4;  // (11 width, 1/width, E, K)
-    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    // the hardware reciprocal and two Newton steps (within an ulp or so of
+    // 1/x; the elimination's arguments are normal numbers, chi scaled and
+    // 1 + c x >= 1)
+    static __device__ __forceinline__ double rcp(double x)
+    {
+        double r = __builtin_amdgcn_rcp(x);
+        double e = __builtin_fma(-x, r, 1.0);
+        r = __builtin_fma(r, e, r);
+        e = __builtin_fma(-x, r, 1.0);
+        return __builtin_fma(r, e, r);
+    }
 };
 
 // One traced step in LDS.  Sub-point m = 1..n of the step lies at
@@ -165,6 +175,7 @@ struct RFArgs {
     int            nband;                     // Lambda* bands (n_off_diag - 1) of k_lambda_band
     int64_t        band_S;                    // band slots per origin
     double        *band_part;                 // [N][nchunk][band_S][L]
+    size_t         line_offset;               // FP64: byte offset of the line-frequency copy in LDS
 };
 
 // ---- line profiles ----------------------------------------------------------
@@ -172,17 +183,18 @@ struct RFArgs {
 // FP64: get_eta_and_chi (simulation.tpp:385-481) for one lane with the exact
 // window: skip while freq_diff > 11 width, include while freq_diff >= -11 width.
 __device__ __forceinline__ void eta_chi64(const RFArgs<double> &a, const double4 *__restrict__ row,
-                                          double fs, int lstart, double &eta, double &chi)
+                                          const double *__restrict__ line, double fs, int lstart, double &eta,
+                                          double &chi)
 {
 #pragma clang fp contract(off)
     const int L = a.L;
     int l = lstart;
-    double fd = fs - a.tb->line[l];
+    double fd = fs - line[l];
     double4 v = row[l];
     while (fd > v.x) {
         if (l >= L - 1) break;
         l++;
-        fd = fs - a.tb->line[l];
+        fd = fs - line[l];
         v = row[l];
     }
     double e_acc = 0.0, c_acc = CHI_FLOOR * CHI_SCALE;
@@ -193,7 +205,7 @@ __device__ __forceinline__ void eta_chi64(const RFArgs<double> &a, const double4
         c_acc += v.w * g;
         if (l >= L - 1) break;
         l++;
-        fd = fs - a.tb->line[l];
+        fd = fs - line[l];
         v = row[l];
     }
     eta = e_acc;
@@ -412,7 +424,8 @@ __device__ __forceinline__ void stage_chunk(const RFArgs<real> &a, int32_t o, in
             const double sc = (j > 0) ? a.seg_shift[s0 + j - 1] : (a.origin_shift ? a.origin_shift[q] : 1.0);
             const StepSplit sp = split_step(sc, sn, z, dsm);
             s.cell = c;
-            s.nh = sp.n | ((sp.interp ? sp.half : 0) << STEP_HALF_SHIFT);
+            s.nh = sp.n | ((sp.interp ? sp.half : 0) << STEP_HALF_SHIFT) |
+                   (a.step_mask ? (int)a.step_mask[s0 + j] << STEP_OUT_SHIFT : 0);
             s.s0 = sp.interp ? sc : sn;               // absolute FP64 shift
             s.ds = sp.dshift;
             s.dz = sp.dZ * HDZ_SCALE;                 // dz'
@@ -440,6 +453,7 @@ template <>
 struct LaneCtx<double> {
     double nu_f;
     int    lstart;
+    const double *line;   // line frequencies (a copy in LDS in the field kernel)
 };
 
 template <typename real>
@@ -465,7 +479,7 @@ __device__ __forceinline__ void eval_point<double>(const RFArgs<double> &a, cons
 #pragma clang fp contract(off)
         fs = s * lc.nu_f;
     }
-    eta_chi64(a, row, fs, lc.lstart, eta, chi);
+    eta_chi64(a, row, lc.line, fs, lc.lstart, eta, chi);
 }
 
 // The point evaluation of a kernel: eval_point, or the FP32 image variant
@@ -851,6 +865,13 @@ void k_radiation_field(RFArgs<real> a)
     const row_t *orow = a.rows + (int64_t)o * a.L;
 
     LaneCtx<real> lc;
+    if constexpr (sizeof(real) == 8) {
+        // FP64: the line frequencies the profile windows read at every point, in LDS
+        double *lt = reinterpret_cast<double *>(smem + a.line_offset);
+        for (int i = (int)threadIdx.x; i < a.L; i += (int)blockDim.x) lt[i] = a.tb->line[i];
+        __syncthreads();
+        lc.line = lt;
+    }
     real S_o, chi_o;
     origin_point<real, false, ONE>(a, nu_f, orow, lc, S_o, chi_o);
 
@@ -946,6 +967,7 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_image(RFArgs<real> a, doub
     const row_t *orow = a.rows + (int64_t)o * a.L;
 
     LaneCtx<real> lc;
+    if constexpr (sizeof(real) == 8) lc.line = a.tb->line;
     real S_o, chi_o;
     origin_point<real, X64>(a, nu_f, orow, lc, S_o, chi_o);
 
@@ -1110,6 +1132,7 @@ __global__ __launch_bounds__(WAVE) void k_lambda_band(RFArgs<double> a)
     const double nu_f = a.nu[(int64_t)o * F + fa];
     const double dsm = a.dshift_max[o];
     LaneCtx<double> lc;
+    lc.line = a.tb->line;
     double S_o, chi_o;
     origin_point<double>(a, nu_f, a.rows + (int64_t)o * L, lc, S_o, chi_o);
     const double freq_line = k < L ? a.freq_k[k] : 1.0;
@@ -1328,13 +1351,14 @@ int make_args(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     if (ch < 2 || ch > max_chunk<real, X64>(ctx)) return ctx->fail(MGR_ERR_UNSUPPORTED, "line rows do not fit the LDS budget");
     a.ch = ch;
     a.row_offset = ((sizeof(step_t<real, X64>) * (ch + 1) + 63) / 64) * 64;
+    a.line_offset = a.row_offset + sizeof(typename Prec<real>::row_t) * (ch + 1) * ctx->L;
     return MGR_OK;
 }
 
 template <typename real>
 size_t lds_bytes(const mgr_ctx *ctx, const RFArgs<real> &a)
 {
-    return a.row_offset + sizeof(typename Prec<real>::row_t) * (a.ch + 1) * ctx->L;
+    return a.line_offset + (sizeof(real) == 8 ? sizeof(double) * ctx->L : 0);
 }
 
 template <typename real>
@@ -1365,9 +1389,12 @@ int launch_rf(mgr_ctx *ctx, const typename Prec<real>::row_t *rows, int nwg, int
     a.origin_shift = nullptr;
     a.pair_stride = ctx->rf_pair_stride;
     a.u_out = ctx->u_store;
-    // the out-of-window bits are exact for the FP32 profile only
-    a.step_mask = (sizeof(real) == 4 && ctx->masks_valid && ctx->rf_window_skip) ? ctx->step_mask : nullptr;
-    if (a.step_mask && ctx->hdz32_max > 0.0f) {
+    // the out-of-window bits of the kernel's precision (each exact for its own profile)
+    if constexpr (sizeof(real) == 4)
+        a.step_mask = (ctx->masks_valid && ctx->rf_window_skip) ? ctx->step_mask : nullptr;
+    else
+        a.step_mask = (ctx->masks64_valid && ctx->rf_window_skip) ? ctx->step_mask64 : nullptr;
+    if (sizeof(real) == 4 && a.step_mask && ctx->hdz32_max > 0.0f) {
         // An out-of-window sub-point closes its row with c = chi_floor dz'_a + chi_floor dz'_b (each
         // product rounded), at most cb = 2 chi_floor dz'_max (1 + 2^-20).  The no-op needs
         // cb < y 2^-25 (then c is below half an ulp of y) and cb y < 2^-25 (then 1 + c y rounds to 1).
@@ -1412,6 +1439,7 @@ int launch_image(mgr_ctx *ctx, const typename Prec<real>::row_t *rows)
     int rc = make_args<real, X64>(ctx, rows, std::min(nchunk, MAX_WAVES), max_chunk<real, X64>(ctx), 0,
                                   ctx->N, g.shift_min, g.shift_max, a);
     if (rc) return rc;
+    a.step_mask = nullptr;   // the masks index the co-moving cache, not the image's trace
     a.seg_off = g.seg_off;
     a.nseg = g.nseg;
     a.npts = g.npts;
@@ -1626,7 +1654,7 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
         if (rc) return rc;
     }
     int rc0 = set_active(ctx, true);
-    if (!rc0 && precision == MGR_FP32) rc0 = launch_step_masks(ctx);   // both serve the FP32 kernel only
+    if (!rc0) rc0 = launch_step_masks(ctx, precision == MGR_FP64);
     if (!rc0) rc0 = launch_far_cache(ctx, precision);
     if (!rc0) rc0 = tune_radiation_field(ctx, precision);
     if (rc0) return rc0;
@@ -1673,7 +1701,7 @@ extern "C" int mgr_first_pass(mgr_ctx *ctx, int precision, int pair_stride)
         if (rc) return rc;
     }
     int rc = set_active(ctx, true);
-    if (!rc && precision == MGR_FP32) rc = launch_step_masks(ctx);
+    if (!rc) rc = launch_step_masks(ctx, precision == MGR_FP64);
     if (!rc) rc = launch_far_cache(ctx, precision);
     if (!rc) rc = tune_radiation_field(ctx, precision);
     if (rc) return rc;

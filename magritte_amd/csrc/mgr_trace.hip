@@ -444,8 +444,9 @@ __global__ void k_expand_ray(int32_t o, int64_t q, const int64_t *__restrict__ s
 
 // Shift intervals in which some bin of a 64-bin chunk (the bins of lane slots
 // 64k .. 64k + 63, lane_bin) can meet a line window.  Bin f meets line l at
-// shift s when |s nu_f - nu_l| <= MASK_WINDOW nu_l pw, i.e. s in
-// [(1 - MASK_WINDOW pw) c, (1 + MASK_WINDOW pw) c] with c = nu_l / nu_f.  Per
+// shift s when |s nu_f - nu_l| <= window nu_l pw, i.e. s in
+// [(1 - window pw) c, (1 + window pw) c] with c = nu_l / nu_f (window:
+// MASK_WINDOW for the FP32 kernel, MASK_WINDOW64 for the FP64 one).  Per
 // line, the hull (a, b) = (min c, max c) over the chunk's bins whose c can
 // reach the trace's shift range [smin, smax] at the widest width pw_max covers
 // all of them; it is widened by the step's own width when tested.  Hulls that
@@ -455,7 +456,7 @@ constexpr int MAX_WIN = 4;
 
 __global__ void k_chunk_windows(int64_t N, int F, int nchunk, bool root_major, const LineTables *__restrict__ tb,
                                 const double *__restrict__ nu, double smin, double smax, double pw_max,
-                                double *__restrict__ win, int8_t *__restrict__ cnt)
+                                double window, double *__restrict__ win, int8_t *__restrict__ cnt)
 {
     // one wave per (origin, chunk), lane i on the chunk's bin 64 k + i; the
     // hull of each line is a wave minimum and maximum (exact, so in any order)
@@ -466,7 +467,7 @@ __global__ void k_chunk_windows(int64_t N, int F, int nchunk, bool root_major, c
     const int i = k * WAVE + (int)(threadIdx.x & (WAVE - 1));
     const bool valid = i < F;
     const double nuv = valid ? nu[o * F + lane_bin(i, tb->L, F, root_major)] : 1.0;
-    const double reach = MASK_WINDOW * pw_max * 1.01;
+    const double reach = window * pw_max * 1.01;
     double wa[MAX_WIN], wb[MAX_WIN];
     int n = 0;
     for (int l = 0; l < tb->L && n >= 0; l++) {
@@ -515,7 +516,7 @@ __global__ void k_step_masks(int64_t N, int Rmax, int nchunk, const int32_t *__r
                              const double *__restrict__ pw, const double *__restrict__ dshift_max,
                              const int64_t *__restrict__ seg_off, const int32_t *__restrict__ nseg,
                              const int32_t *__restrict__ seg_cell, const double *__restrict__ seg_shift,
-                             const double *__restrict__ win,
+                             double window, const double *__restrict__ win,
                              const int8_t *__restrict__ cnt, uint8_t *__restrict__ mask)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -546,7 +547,7 @@ __global__ void k_step_masks(int64_t N, int Rmax, int nchunk, const int32_t *__r
             const double s = ss[u];
             const bool interp = fabs(s - s_prev) > dsm;   // as split_step
             const double lo = interp ? fmin(s_prev, s) : s, hi = interp ? fmax(s_prev, s) : s;
-            const double w = MASK_WINDOW * (interp ? fmax(pw[c_prev], pw[c]) : pw[c]);
+            const double w = window * (interp ? fmax(pw[c_prev], pw[c]) : pw[c]);
             unsigned bits = 0;
             for (int k = 0; k < nchunk; k++) {
                 const int n = cn[k];
@@ -810,10 +811,13 @@ int launch_trace(mgr_ctx *ctx)
     // the ALI graphs captured pointers into the old cache
     ctx->graph_precision = -1;
     ctx->masks_valid = false;
+    ctx->masks64_valid = false;
     ctx->far_invalidate();
     if (ctx->seg_cell) {
         (void)hipFree(ctx->seg_cell); (void)hipFree(ctx->seg_shift); (void)hipFree(ctx->seg_dZ);
         (void)hipFree(ctx->seg_f32); (void)hipFree(ctx->seg_hdz32); (void)hipFree(ctx->step_mask);
+        if (ctx->step_mask64) (void)hipFree(ctx->step_mask64);
+        ctx->step_mask64 = nullptr;
         ctx->seg_cell = nullptr; ctx->seg_shift = nullptr; ctx->seg_dZ = nullptr;
         ctx->seg_f32 = nullptr; ctx->seg_hdz32 = nullptr; ctx->step_mask = nullptr;
     }
@@ -854,11 +858,16 @@ int launch_trace(mgr_ctx *ctx)
 }
 
 // Computes the out-of-window bits of the cached steps for the current
-// frequencies (once per trace and spectral discretisation).  Models with more
-// than STEP_MASK_CHUNKS chunks of 64 bins get bits for the first ones only.
-int launch_step_masks(mgr_ctx *ctx)
+// frequencies (once per trace and spectral discretisation), for the FP32
+// kernel or (fp64) the FP64 one.  Models with more than STEP_MASK_CHUNKS
+// chunks of 64 bins get bits for the first ones only.
+int launch_step_masks(mgr_ctx *ctx, bool fp64)
 {
-    if (ctx->masks_valid || !ctx->traced || !ctx->spectral_set) return MGR_OK;
+    bool &valid = fp64 ? ctx->masks64_valid : ctx->masks_valid;
+    if (valid || !ctx->traced || !ctx->spectral_set) return MGR_OK;
+    uint8_t *&bits_out = fp64 ? ctx->step_mask64 : ctx->step_mask;
+    if (!bits_out) MGR_HIP(hipMalloc((void **)&bits_out, std::max<int64_t>(ctx->total_segments, 1)));
+    const double window = fp64 ? MASK_WINDOW64 : MASK_WINDOW;
     const int64_t N = ctx->N;
     const int nchunk = (int)std::min<int64_t>((ctx->F + WAVE - 1) / WAVE, STEP_MASK_CHUNKS);
     // Root-major lanes when the lines are further apart than the trace's
@@ -883,16 +892,16 @@ int launch_step_masks(mgr_ctx *ctx)
     }
     hipLaunchKernelGGL(k_chunk_windows, dim3((unsigned)((N * nchunk + 3) / 4)), dim3(4 * WAVE), 0, ctx->stream, N,
                        (int)ctx->F, nchunk, rm, ctx->tables, ctx->nu, ctx->shift_min, ctx->shift_max, ctx->pw_max,
-                       win, cnt);
+                       window, win, cnt);
     hipLaunchKernelGGL(k_step_masks, dim3((unsigned)((N * ctx->R + 127) / 128)), dim3(128), 0, ctx->stream, N,
                        (int)ctx->R, nchunk, ctx->d_cell_model, ctx->d_cell_qbase, ctx->d_m_R, ctx->pw_cmax,
                        ctx->dshift_max, ctx->seg_off,
-                       ctx->nseg, ctx->seg_cell, ctx->seg_shift, win, cnt, ctx->step_mask);
-    // the largest dz' bounds the row closings of out-of-window sub-points
+                       ctx->nseg, ctx->seg_cell, ctx->seg_shift, window, win, cnt, bits_out);
+    // the largest dz' bounds the row closings of out-of-window sub-points (FP32)
     unsigned *dmax = reinterpret_cast<unsigned *>(cnt);   // N * STEP_MASK_CHUNKS >= 4 bytes, no longer read
     hipError_t le = hipGetLastError();
     if (le == hipSuccess) le = hipMemsetAsync(dmax, 0, sizeof(unsigned), ctx->stream);
-    if (le == hipSuccess && ctx->total_segments > 0) {
+    if (le == hipSuccess && ctx->total_segments > 0 && !fp64) {
         hipLaunchKernelGGL(k_max_hdz, dim3(1024), dim3(256), 0, ctx->stream, ctx->total_segments, ctx->seg_hdz32,
                            dmax);
         le = hipGetLastError();
@@ -903,8 +912,8 @@ int launch_step_masks(mgr_ctx *ctx)
     (void)hipFree(win);
     (void)hipFree(cnt);
     if (le != hipSuccess || se != hipSuccess) return ctx->fail(MGR_ERR_DEVICE, "step-mask kernels failed");
-    std::memcpy(&ctx->hdz32_max, &bits, sizeof(float));
-    ctx->masks_valid = true;
+    if (!fp64) std::memcpy(&ctx->hdz32_max, &bits, sizeof(float));
+    valid = true;
     return MGR_OK;
 }
 
@@ -918,7 +927,7 @@ extern "C" int mgr_window_skip_stats(mgr_ctx *ctx, int64_t *point_chunks, int64_
     if (!ctx->traced) return ctx->fail(MGR_ERR_STATE, "mgr_trace first");
     if (!ctx->spectral_set) return ctx->fail(MGR_ERR_STATE, "compute_spectral_discretisation first");
     MGR_HIP(hipSetDevice(ctx->device));
-    int rc = mgr::launch_step_masks(ctx);
+    int rc = mgr::launch_step_masks(ctx, false);
     if (rc) return rc;
     const int64_t T = ctx->total_segments;
     const int nchunk = (int)std::min<int64_t>((ctx->F + mgr::WAVE - 1) / mgr::WAVE, mgr::STEP_MASK_CHUNKS);

@@ -348,7 +348,7 @@ class Context:
         self._check(self.lib.mgr_set_scattering(self.h, 1 if use_scattering else 0), "set_scattering")
 
     def set_window_skip(self, enable=True):
-        """FP32 kernel: eliminate steps out of every line window without
+        """Both kernels: eliminate steps out of every line window without
         evaluating their profiles (bit-identical; on by default)."""
         self._check(self.lib.mgr_set_window_skip(self.h, 1 if enable else 0), "set_window_skip")
 

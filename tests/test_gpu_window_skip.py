@@ -1,12 +1,15 @@
-"""The out-of-window skip and the far-end intensity cache of the FP32
-radiation-field kernel against the full evaluation, through the C-ABI.
+"""The out-of-window skip and the far-end intensity cache of the
+radiation-field kernels against the full evaluation, through the C-ABI.
 
 A step whose sub-points keep every bin of a 64-bin wave more than 10.5 line
 widths from every line has FP32 profiles of exactly 0 (exp2 of less than
 -149), so eta = 0 and chi sits at its floor; the skip eliminates such a step
-without evaluating the profiles.  Bar: J, the Lambda diagonal, Radiation::u
-and the populations after ALI iterations are bit-identical with the skip on
-and off, and a model with velocity fields does skip a share of its points.
+without evaluating the profiles.  The FP64 kernel has bits of its own, set
+beyond 11 (1 + 1e-6) widths, where get_eta_and_chi's +-11-width window
+(simulation.tpp:385-481) holds no line: eta = 0 and chi at the floor there
+too.  Bar: J, the Lambda diagonal, Radiation::u and the populations after ALI
+iterations are bit-identical with the skip on and off, in both precisions,
+and a model with velocity fields does skip a share of its points.
 The masks follow the frequencies: after an image discretisation and back the
 line-grid result is unchanged.  The far-end intensity cache (the rescaled
 boundary intensity of every ray and bin, kept in FP32) gives the same bits as
@@ -37,34 +40,36 @@ MODELS = {
 }
 
 
-def _field(ctx):
-    ctx.compute_radiation_field("fp32")
+def _field(ctx, precision="fp32"):
+    ctx.compute_radiation_field(precision)
     return ctx.J(), ctx.lambda_diag(), ctx.u()
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
 @pytest.mark.parametrize("name", list(MODELS))
-def test_skip_is_bit_identical(name):
+def test_skip_is_bit_identical(name, precision):
     from magritte_amd.hip import Context
     make, must_skip = MODELS[name]
-    print(f"\n[skip] model {name}", flush=True)
+    print(f"\n[skip] model {name} {precision}", flush=True)
     ctx = Context(make()).setup()
     ctx.set_scattering(True)
-    J1, L1, u1 = _field(ctx)
+    J1, L1, u1 = _field(ctx, precision)
     total, skipped = ctx.window_skip_stats()
     print(f"[skip] {name}: {skipped} of {total} point-chunks skipped ({skipped / max(total, 1):.1%})", flush=True)
     assert 0 <= skipped <= total
     if must_skip:
         assert skipped > 0
     ctx.set_window_skip(False)
-    J0, L0, u0 = _field(ctx)
+    J0, L0, u0 = _field(ctx, precision)
     assert np.array_equal(J1, J0)
     assert np.array_equal(L1, L0)
     assert np.array_equal(u1, u0)
 
 
-def test_skip_in_ali_iterations():
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_skip_in_ali_iterations(precision):
     from magritte_amd.hip import Context
-    print("\n[skip] envelope10 ALI", flush=True)
+    print(f"\n[skip] envelope10 ALI {precision}", flush=True)
     ctx = Context(synthetic.envelope_model(n=10, seed=9)).setup()
     pops = {}
     for on in (True, False):
@@ -72,7 +77,7 @@ def test_skip_in_ali_iterations():
         ctx.set_far_cache(on)
         ctx.compute_LTE_level_populations()
         for _ in range(3):
-            ctx.ali_iteration(ng=False, precision="fp32")
+            ctx.ali_iteration(ng=False, precision=precision)
         pops[on] = ctx.populations()
     assert np.array_equal(pops[True], pops[False])
 
