@@ -999,6 +999,7 @@ static int ensure_graphs(mgr_ctx *ctx, int precision)
     if ((rc = set_active(ctx, true))) return rc;
     // recomputed in place when the frequencies changed; the graphs read them at replay
     if ((rc = launch_step_masks(ctx, precision == MGR_FP64))) return rc;
+    if (ctx->nband > 0 && (rc = launch_step_masks(ctx, true))) return rc;   // the band kernel's
     if ((rc = launch_far_cache(ctx, precision))) return rc;
     if ((rc = tune_radiation_field(ctx, precision))) return rc;
     if (ctx->graph_precision != precision) {

@@ -1029,56 +1029,94 @@ __global__ __launch_bounds__(WAVE * MAX_WAVES) void k_image(RFArgs<real> a, doub
 
 // Sub-points of traced step j of the entry at base: m = 1 .. n at shift
 // s0 + m ds with dZ dz, in cell c_prev for m < half and c_next otherwise
-// (Geometry::set_data, geometry.tpp:100-207).
+// (Geometry::set_data, geometry.tpp:100-207).  step_raw loads what the split
+// needs, split_points forms it, so that the loads of a step can be issued
+// before the previous step's points are visited.
 struct StepPoints {
     int     n, half;
     double  s0, ds, dz;
     int32_t c_prev, c_next;
 };
 
+struct StepRaw {
+    double  sc, sn, z;
+    int32_t c_prev, c_next;
+};
+
+__device__ __forceinline__ StepRaw step_raw(const RFArgs<double> &a, int64_t base, int j, int32_t o)
+{
+    StepRaw r;
+    r.sn = a.seg_shift[base + j];
+    r.sc = j > 0 ? a.seg_shift[base + j - 1] : 1.0;   // co-moving: the origin's shift is 1
+    r.z = a.seg_dZ[base + j];
+    r.c_prev = j > 0 ? a.seg_cell[base + j - 1] : o;
+    r.c_next = a.seg_cell[base + j];
+    return r;
+}
+
+__device__ __forceinline__ StepPoints split_points(const StepRaw &r, double dsm)
+{
+    StepPoints p;
+    const StepSplit sp = split_step(r.sc, r.sn, r.z, dsm);
+    p.n = sp.n;
+    p.half = sp.interp ? sp.half : 0;
+    p.s0 = sp.interp ? r.sc : r.sn;
+    p.ds = sp.dshift;
+    p.dz = sp.dZ;
+    p.c_prev = r.c_prev;
+    p.c_next = r.c_next;
+    return p;
+}
+
 __device__ __forceinline__ StepPoints step_points(const RFArgs<double> &a, int64_t base, int j, int32_t o,
                                                   double dsm)
 {
-    StepPoints p;
-    const double sn = a.seg_shift[base + j];
-    const double sc = j > 0 ? a.seg_shift[base + j - 1] : 1.0;   // co-moving: the origin's shift is 1
-    const StepSplit sp = split_step(sc, sn, a.seg_dZ[base + j], dsm);
-    p.n = sp.n;
-    p.half = sp.interp ? sp.half : 0;
-    p.s0 = sp.interp ? sc : sn;
-    p.ds = sp.dshift;
-    p.dz = sp.dZ;
-    p.c_prev = j > 0 ? a.seg_cell[base + j - 1] : o;
-    p.c_next = a.seg_cell[base + j];
-    return p;
+    return split_points(step_raw(a, base, j, o), dsm);
 }
 
 // The points of a ray pair in order: entry qx from its far end inwards, the
 // origin, then entry qy outwards, at most ny points of it.  visit(t, cell,
-// shift, dz) gets the visit count t and the dZ of the interval between the
-// point and the one visited before it, which is the dZ of the interval's
-// outer point (Simulation::setup, simulation.tpp:233-240, 292-299).
+// shift, dz, out) gets the visit count t and the dZ of the interval between
+// the point and the one visited before it, which is the dZ of the interval's
+// outer point (Simulation::setup, simulation.tpp:233-240, 292-299); out is
+// set when the step's out-of-window bit `cbit` (of a.step_mask) is set: no
+// line is then in the window of any bin of this wave (eta = 0, chi at the
+// floor).
 template <typename Visit>
 __device__ __forceinline__ void walk_pair(const RFArgs<double> &a, int32_t o, int64_t qx, int64_t qy, int ny,
-                                          double dsm, Visit &&visit)
+                                          double dsm, unsigned cbit, Visit &&visit)
 {
     int t = 0;
     double dz_prev = 0.0;
     const int64_t bx = a.seg_off[qx];
-    for (int j = a.nseg[qx] - 1; j >= 0; --j) {
-        const StepPoints sp = step_points(a, bx, j, o, dsm);
+    // each step's loads are issued before the previous step's points are visited
+    int j = a.nseg[qx] - 1;
+    StepRaw nx;
+    uint8_t nmask = 0;
+    if (j >= 0) {
+        nx = step_raw(a, bx, j, o);
+        nmask = cbit ? a.step_mask[bx + j] : 0;
+    }
+    for (; j >= 0; --j) {
+        const StepPoints sp = split_points(nx, dsm);
+        const bool out = (nmask & cbit) != 0;
+        if (j > 0) {
+            nx = step_raw(a, bx, j - 1, o);
+            nmask = cbit ? a.step_mask[bx + j - 1] : 0;
+        }
         for (int m = sp.n; m >= 1; --m) {
-            visit(t++, m < sp.half ? sp.c_prev : sp.c_next, sub_point_shift(sp.s0, m, sp.ds), dz_prev);
+            visit(t++, m < sp.half ? sp.c_prev : sp.c_next, sub_point_shift(sp.s0, m, sp.ds), dz_prev, out);
             dz_prev = sp.dz;
         }
     }
-    visit(t++, o, 1.0, dz_prev);
+    visit(t++, o, 1.0, dz_prev, false);
     const int64_t by = a.seg_off[qy];
     const int nsy = a.nseg[qy];
-    for (int j = 0; j < nsy && ny > 0; ++j) {
-        const StepPoints sp = step_points(a, by, j, o, dsm);
+    for (int jy = 0; jy < nsy && ny > 0; ++jy) {
+        const StepPoints sp = step_points(a, by, jy, o, dsm);
+        const bool out = cbit && (a.step_mask[by + jy] & cbit);
         for (int m = 1; m <= sp.n && ny > 0; ++m, --ny)
-            visit(t++, m < sp.half ? sp.c_prev : sp.c_next, sub_point_shift(sp.s0, m, sp.ds), sp.dz);
+            visit(t++, m < sp.half ? sp.c_prev : sp.c_next, sub_point_shift(sp.s0, m, sp.ds), sp.dz, out);
     }
 }
 
@@ -1125,14 +1163,21 @@ __global__ __launch_bounds__(WAVE) void k_lambda_band(RFArgs<double> a)
     const int lane = (int)threadIdx.x;
     const int F = a.F, L = a.L, nb = a.nband;
     const int W = 3 * (nb + 1);
-    const int f = chunk * WAVE + lane;
-    const bool active = f < F;
-    const int fa = active ? f : F - 1;
+    // the lane order of the field kernels (lane_bin), which the out-of-window bits follow
+    const int slot = chunk * WAVE + lane;
+    const bool active = slot < F;
+    const int fa = lane_bin(active ? slot : F - 1, L, F, a.root_major);
     const int k = active ? a.f_k[fa] : L;
     const double nu_f = a.nu[(int64_t)o * F + fa];
     const double dsm = a.dshift_max[o];
     LaneCtx<double> lc;
-    lc.line = a.tb->line;
+    {
+        // the line frequencies the profile windows read at every point, in LDS after the window arrays
+        double *lt = win + (size_t)BW_ARRAYS * 3 * (nb + 1) * WAVE;
+        for (int i = lane; i < L; i += WAVE) lt[i] = a.tb->line[i];
+        __syncthreads();
+        lc.line = lt;
+    }
     double S_o, chi_o;
     origin_point<double>(a, nu_f, a.rows + (int64_t)o * L, lc, S_o, chi_o);
     const double freq_line = k < L ? a.freq_k[k] : 1.0;
@@ -1142,6 +1187,14 @@ __global__ __launch_bounds__(WAVE) void k_lambda_band(RFArgs<double> a)
     const int64_t qb = a.cell_qbase[o];
     const int Rh = a.m_R[0] / 2;
     double *part = a.band_part + ((int64_t)o * a.nchunk + chunk) * a.band_S * L;
+    // this wave's out-of-window bit (the FP64 kernel's masks), none past the masked chunks
+    const unsigned cbit = (a.step_mask && chunk < STEP_MASK_CHUNKS) ? 1u << chunk : 0u;
+    // chi of a point: only the opacity enters dtau; at an out-of-window point the floor
+    auto opacity = [&](int32_t c, double s, bool out) -> double {
+        double eta, chi = CHI_FLOOR * CHI_SCALE;
+        if (!out) eval_point<double>(a, lc, a.rows + (int64_t)c * L, s, eta, chi);
+        return chi * (1.0 / CHI_SCALE);
+    };
 
     for (int r = 0; r < Rh; r++) {
         const int64_t qr = qb + r, qa = qb + a.antipod[r];
@@ -1156,10 +1209,8 @@ __global__ __launch_bounds__(WAVE) void k_lambda_band(RFArgs<double> a)
             // pass 1: from index 0 (the far end of the longer ray) upwards
             double chi_p = 0.0, dt_p = 0.0, F_p = 0.0, i1F_p = 0.0;
             const int top = nA + 2 * nb + 2 < last ? nA + 2 * nb + 2 : last;
-            walk_pair(a, o, qA, qB, top - nA, dsm, [&](int n, int32_t c, double s, double dz) {
-                double eta, chi;
-                eval_point<double>(a, lc, a.rows + (int64_t)c * L, s, eta, chi);
-                chi *= 1.0 / CHI_SCALE;
+            walk_pair(a, o, qA, qB, top - nA, dsm, cbit, [&](int n, int32_t c, double s, double dz, bool out) {
+                const double chi = opacity(c, s, out);
                 const int w = n - w0;
                 if (w >= 0 && w < W) {
                     // nu phi(nu) at the point's own cell and frequency, and chi
@@ -1178,8 +1229,8 @@ __global__ __launch_bounds__(WAVE) void k_lambda_band(RFArgs<double> a)
                     } else {
                         const double iA = 0.5 * (dt_p + dt) * dt_p;   // inverse_A[n-1]
                         const double iC = 0.5 * (dt_p + dt) * dt;     // inverse_C[n-1]
-                        F_p = (1.0 + (1.0 / iA) * F_p * i1F_p) * iC;  // F[n-1]
-                        i1F_p = 1.0 / (1.0 + F_p);
+                        F_p = (1.0 + Prec<double>::rcp(iA) * F_p * i1F_p) * iC;  // F[n-1]
+                        i1F_p = Prec<double>::rcp(1.0 + F_p);
                         if (w - 1 >= 0 && w - 1 < W) {
                             wref(BW_F, w - 1) = F_p;
                             wref(BW_IC, w - 1) = iC;
@@ -1194,11 +1245,9 @@ __global__ __launch_bounds__(WAVE) void k_lambda_band(RFArgs<double> a)
             chi_p = 0.0;
             dt_p = 0.0;
             const int bottom = nA - nb > 0 ? nA - nb : 0;
-            walk_pair(a, o, qB, qA, nA - bottom, dsm, [&](int t, int32_t c, double s, double dz) {
+            walk_pair(a, o, qB, qA, nA - bottom, dsm, cbit, [&](int t, int32_t c, double s, double dz, bool out) {
                 const int n = last - t;
-                double eta, chi;
-                eval_point<double>(a, lc, a.rows + (int64_t)c * L, s, eta, chi);
-                chi *= 1.0 / CHI_SCALE;
+                const double chi = opacity(c, s, out);
                 if (t >= 1) {
                     const double dt = 0.5 * (chi + chi_p) * dz;   // dtau[n]
                     if (n == last - 1) {
@@ -1208,9 +1257,9 @@ __global__ __launch_bounds__(WAVE) void k_lambda_band(RFArgs<double> a)
                     } else {
                         const double iA = 0.5 * (dt + dt_p) * dt;     // inverse_A[n+1]
                         const double iC = 0.5 * (dt + dt_p) * dt_p;   // inverse_C[n+1]
-                        G_p = (1.0 + (1.0 / iC) * G_p * i1G_p) * iA;  // G[n+1]
+                        G_p = (1.0 + Prec<double>::rcp(iC) * G_p * i1G_p) * iA;  // G[n+1]
                     }
-                    i1G_p = 1.0 / (1.0 + G_p);
+                    i1G_p = Prec<double>::rcp(1.0 + G_p);
                     const int w = n + 1 - w0;
                     if (w >= 0 && w < W) wref(BW_G, w) = G_p;
                     dt_p = dt;
@@ -1478,7 +1527,8 @@ int launch_band(mgr_ctx *ctx)
     a.nband = ctx->nband;
     a.band_S = ctx->band_S;
     a.band_part = ctx->band_part;
-    const size_t lds = (size_t)BW_ARRAYS * 3 * (ctx->nband + 1) * WAVE * sizeof(double);
+    a.step_mask = (ctx->masks64_valid && ctx->rf_window_skip) ? ctx->step_mask64 : nullptr;
+    const size_t lds = ((size_t)BW_ARRAYS * 3 * (ctx->nband + 1) * WAVE + ctx->L) * sizeof(double);
     hipLaunchKernelGGL(k_lambda_band, dim3(grid_of(a)), dim3(WAVE), lds, ctx->stream, a);
     MGR_LAUNCH_CHECK();
     const int64_t SL = ctx->band_S * ctx->L;
@@ -1655,6 +1705,7 @@ extern "C" int mgr_compute_radiation_field(mgr_ctx *ctx, int precision)
     }
     int rc0 = set_active(ctx, true);
     if (!rc0) rc0 = launch_step_masks(ctx, precision == MGR_FP64);
+    if (!rc0 && ctx->nband > 0) rc0 = launch_step_masks(ctx, true);   // the band kernel's
     if (!rc0) rc0 = launch_far_cache(ctx, precision);
     if (!rc0) rc0 = tune_radiation_field(ctx, precision);
     if (rc0) return rc0;
@@ -1702,6 +1753,7 @@ extern "C" int mgr_first_pass(mgr_ctx *ctx, int precision, int pair_stride)
     }
     int rc = set_active(ctx, true);
     if (!rc) rc = launch_step_masks(ctx, precision == MGR_FP64);
+    if (!rc && ctx->nband > 0) rc = launch_step_masks(ctx, true);   // the band kernel's
     if (!rc) rc = launch_far_cache(ctx, precision);
     if (!rc) rc = tune_radiation_field(ctx, precision);
     if (rc) return rc;
