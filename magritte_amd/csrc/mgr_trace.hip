@@ -41,8 +41,8 @@ __device__ __forceinline__ void atomic_minmax_pos_double(uint64_t *addr, double 
 // Cells recorded by the counting pass: the next cell of every step, four to
 // an int4, up to `cap` steps per (origin, ray).  The filling pass replays
 // them (replay_walk) instead of scanning the neighbours again.  The step
-// index j runs in step on every active lane of a wave, so the selects below
-// branch on a uniform value.
+// index j runs in step on every active lane of a wave, so the select below
+// branches on a uniform value.
 __device__ __forceinline__ void put_cell(int4 &acc, int j, int32_t c)
 {
     switch (j & 3) {
@@ -50,16 +50,6 @@ __device__ __forceinline__ void put_cell(int4 &acc, int j, int32_t c)
     case 1: acc.y = c; break;
     case 2: acc.z = c; break;
     default: acc.w = c; break;
-    }
-}
-
-__device__ __forceinline__ int32_t get_cell(const int4 &acc, int j)
-{
-    switch (j & 3) {
-    case 0: return acc.x;
-    case 1: return acc.y;
-    case 2: return acc.z;
-    default: return acc.w;
     }
 }
 
