@@ -451,8 +451,15 @@ int mgr_mlp_train(mgr_mlp *mlp, const float *X, const float *T, int64_t n, int64
  * transitions k = 0..3.                                                       */
 int mgr_features(mgr_ctx *ctx, float *X);
 /* Features -> network -> populations (pop_tot 10^y_i / sum 10^y) in place of
- * the current populations; needs 7 levels.                                    */
+ * the current populations; needs 7 levels.  A cell whose prediction is not
+ * finite and positive, or gives a line zero or negative opacity
+ * (n_j B_ji <= n_i B_ij: an inversion, which a network far outside its
+ * training data can predict and on which the iteration diverges), keeps the
+ * populations it held before the call (the LTE start, after mgr_first_pass
+ * from LTE).                                                                  */
 int mgr_warm_start(mgr_ctx *ctx, mgr_mlp *mlp);
+/* Cells that kept their previous populations in the last mgr_warm_start.     */
+int mgr_warm_start_fallbacks(const mgr_ctx *ctx, int64_t *cells);
 /* The warm start's first pass from the current populations: a radiation
  * field over every pair_stride-th ray pair of each origin (ray weights scaled
  * by pair_stride, an approximation of J at 1/pair_stride of the cost; 1 is

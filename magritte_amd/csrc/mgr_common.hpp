@@ -273,6 +273,7 @@ struct mgr_ctx {
     int      rf_nwg[2] = {0, 0}, rf_ch[2] = {0, 0};
     bool     rf_autotune = true;
     int      rf_pair_stride = 1;    // every k-th ray pair (weights times k): the warm start's first pass only
+    int64_t  warm_fallbacks = 0;    // cells that kept their populations in the last warm start
     struct TuneEntry { int nwg, ch; float ms; };
     std::vector<TuneEntry> tune_log;
 

@@ -23,6 +23,7 @@ using namespace mgr;
 namespace {
 
 constexpr int32_t ERR_SINGULAR = 16;
+constexpr int32_t ERR_NONFINITE = 64;   // a level population is NaN or infinite
 
 // The Lambda* diagonal of RayPair::update_Lambda (raypair_lambda.tpp:94-149),
 // n_off_diag = 0: Ls(p,k) = sum over bins f whose back-map names transition k
@@ -452,17 +453,23 @@ __device__ __forceinline__ double block_max(double v, double *sh)
 __global__ void k_conv_partial(int V, double pop_prec, const int64_t *__restrict__ chunk_cell,
                                const int32_t *__restrict__ cell_model, const int64_t *__restrict__ m_cell_off,
                                const double *__restrict__ pop, const double *__restrict__ prev1,
-                               const double *__restrict__ pop_tot, double *__restrict__ partials)
+                               const double *__restrict__ pop_tot, double *__restrict__ partials,
+                               int32_t *__restrict__ err_flag)
 {
     __shared__ double sh[RED_BLOCK];
     const int64_t c0 = chunk_cell[blockIdx.x], c1 = chunk_cell[blockIdx.x + 1];
     const int32_t mdl = cell_model ? cell_model[c0] : 0;   // null: one model
     const double weight = 1.0 / (double)((m_cell_off[mdl + 1] - m_cell_off[mdl]) * V);
     double fnc = 0.0, rcm = 0.0, rmax = 0.0;
+    bool finite = true;
     for (int64_t p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
         const double min_pop = 1.0E-10 * pop_tot[p];
         for (int i = 0; i < V; i++) {
             const int64_t ind = p * V + i;
+            // the criterion passes over NaN populations (the comparison below
+            // is false for them), so a diverged iteration would look converged:
+            // it is reported instead (err_flag null: not checked)
+            finite = finite && __builtin_isfinite(pop[ind]);
             if (pop[ind] > min_pop) {
                 double rc = 2.0;
                 rc *= fabs(pop[ind] - prev1[ind]);
@@ -473,6 +480,7 @@ __global__ void k_conv_partial(int V, double pop_prec, const int64_t *__restrict
             }
         }
     }
+    if (!finite && err_flag) atomicOr(err_flag, ERR_NONFINITE);
     fnc = block_sum(fnc, sh);
     rcm = block_sum(rcm, sh);
     rmax = block_max(rmax, sh);
@@ -766,7 +774,7 @@ int launch_convergence(mgr_ctx *ctx)
         const int64_t vo = ctx->N * ctx->sp_voff[l];
         hipLaunchKernelGGL(k_conv_partial, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream,
                            (int)ctx->sp_V[l], ctx->pop_prec, ctx->d_chunk_cell, ctx->d_cell_model, ctx->d_m_cell_off,
-                           ctx->pop + vo, ctx->prev1 + vo, ctx->pop_tot + ctx->N * l, ctx->partials);
+                           ctx->pop + vo, ctx->prev1 + vo, ctx->pop_tot + ctx->N * l, ctx->partials, ctx->err_flag);
         MGR_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_conv_final, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->M, ctx->d_m_chunk_off,
                            ctx->partials, ctx->d_m_active, ctx->d_mconv,
@@ -835,6 +843,8 @@ int read_convergence(mgr_ctx *ctx, mgr_conv *conv)
     if (errv & ERR_SINGULAR) return ctx->fail(MGR_ERR_NUMERIC, "singular rate matrix in a cell");
     if (errv & ERR_BAND)
         return ctx->fail(MGR_ERR_NUMERIC, "rate equations with Lambda* bands: block-Jacobi sweeps did not converge");
+    if (errv & ERR_NONFINITE)
+        return ctx->fail(MGR_ERR_NUMERIC, "non-finite level populations: the iteration diverged");
     if (errv) return ctx->fail(MGR_ERR_TRACE, "a ray pair ended off the boundary or an interior origin has an empty ray");
     return MGR_OK;
 }
@@ -975,7 +985,7 @@ int mgr_ng_acceleration_noalloc(int64_t ncells, int64_t nlev, double pop_prec, d
     hipLaunchKernelGGL(k_ng_apply, dim3(grid_for(ncells * nlev, 256)), dim3(256), 0, st, ncells, V, nullptr,
                        m_active, coef, population, prev1, prev2, prev3);
     hipLaunchKernelGGL(k_conv_partial, dim3((unsigned)nch), dim3(RED_BLOCK), 0, st, V, pop_prec, chunk_cell,
-                       nullptr, m_cell_off, population, prev1, population_tot, partials);
+                       nullptr, m_cell_off, population, prev1, population_tot, partials, nullptr);
     hipLaunchKernelGGL(k_conv_final, dim3(1), dim3(RED_BLOCK), 0, st, (int64_t)1, m_chunk_off, partials, m_active,
                        mconv, reduced, err_flag);
     hipError_t e = hipGetLastError();

@@ -522,6 +522,27 @@ __global__ __launch_bounds__(THREADS) void k_mlp_forward(const float *__restrict
     }
 }
 
+// The warm start's guard: a cell whose predicted populations are not all
+// finite and positive, or give one of its lines an opacity <= 0 (the sign of
+// n_j B_a - n_i B_s, as the emissivity/opacity kernel forms it), takes back
+// the populations it held before the prediction.
+__global__ void k_warm_guard(int64_t N, int V, int L, const int32_t *__restrict__ irad,
+                             const int32_t *__restrict__ jrad, const double *__restrict__ Ba,
+                             const double *__restrict__ Bs, const double *__restrict__ prev,
+                             double *__restrict__ pop, unsigned long long *__restrict__ count)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    double *x = pop + p * V;
+    bool ok = true;
+    for (int i = 0; i < V; i++) ok = ok && x[i] > 0.0 && __builtin_isfinite(x[i]);
+    for (int k = 0; k < L; k++) ok = ok && (x[jrad[k]] * Ba[k] - x[irad[k]] * Bs[k] > 0.0);
+    if (!ok) {
+        for (int i = 0; i < V; i++) x[i] = prev[p * V + i];
+        atomicAdd(count, 1ull);
+    }
+}
+
 // Per-cell features of the warm start (after a first radiation field and
 // J_eff from the starting populations):
 //   0 log10 n(collision partner 0)   1 log10 T   2 log10 n(species)
@@ -889,17 +910,47 @@ extern "C" int mgr_warm_start(mgr_ctx *ctx, mgr_mlp *mlp)
     if (ctx->device != mlp->device) return ctx->fail(MGR_ERR_ARGUMENT, "context and network on different devices");
     MGR_HIP(hipSetDevice(ctx->device));
     MGR_HIP(hipStreamSynchronize(mlp->stream));
+    ctx->warm_fallbacks = 0;
     float *dX = nullptr;
-    MGR_HIP(hipMalloc(&dX, ctx->N * IN * sizeof(float)));
+    double *dPrev = nullptr;
+    unsigned long long *dCount = nullptr;
+    const size_t npop = (size_t)ctx->N * OUT;
+    if (hipMalloc(&dX, ctx->N * IN * sizeof(float)) != hipSuccess ||
+        hipMalloc(&dPrev, npop * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dCount, sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipFree(dX);
+        (void)hipFree(dPrev);
+        return ctx->fail(MGR_ERR_NOMEM, "warm start: device allocation failed");
+    }
     int rc = mgr::launch_features(ctx, dX);
+    if (!rc && (hipMemcpyAsync(dPrev, ctx->pop, npop * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
+                hipMemsetAsync(dCount, 0, sizeof(unsigned long long), ctx->stream) != hipSuccess))
+        rc = ctx->fail(MGR_ERR_DEVICE, "warm start: copy failed");
     if (!rc) {
         hipLaunchKernelGGL(k_mlp_forward, dim3(grid_for_rows(ctx->N)), dim3(THREADS), LDS_TRAIN, ctx->stream, dX,
                            ctx->N, mlp->wbf, mlp->P, mlp->norm, (float *)nullptr, ctx->pop_tot, ctx->pop);
+        hipLaunchKernelGGL(k_warm_guard, dim3((unsigned)((ctx->N + 255) / 256)), dim3(256), 0, ctx->stream, ctx->N,
+                           (int)OUT, (int)ctx->L, ctx->irad, ctx->jrad, ctx->Ba, ctx->Bs, dPrev, ctx->pop, dCount);
         if (hipGetLastError() != hipSuccess) rc = ctx->fail(MGR_ERR_DEVICE, "warm-start launch failed");
     }
     if (!rc) rc = launch_emissivity_opacity(ctx);
-    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ctx->fail(MGR_ERR_DEVICE, "warm start failed");
+    unsigned long long nfb = 0;
+    if (!rc && (hipMemcpyAsync(&nfb, dCount, sizeof(nfb), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess))
+        rc = ctx->fail(MGR_ERR_DEVICE, "warm start failed");
     (void)hipFree(dX);
-    if (!rc) ctx->pops_set = true;
+    (void)hipFree(dPrev);
+    (void)hipFree(dCount);
+    if (!rc) {
+        ctx->warm_fallbacks = (int64_t)nfb;
+        ctx->pops_set = true;
+    }
     return rc;
+}
+
+extern "C" int mgr_warm_start_fallbacks(const mgr_ctx *ctx, int64_t *cells)
+{
+    if (!ctx || !cells) return MGR_ERR_ARGUMENT;
+    *cells = ctx->warm_fallbacks;
+    return MGR_OK;
 }

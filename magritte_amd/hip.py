@@ -86,6 +86,7 @@ SIGNATURES = {
     "mgr_mlp_train": ([_vp, _vp, _vp, _i64, _i64, _i64, C.c_float, C.c_uint64, _vp], _i),
     "mgr_features": ([_vp, _vp], _i),
     "mgr_warm_start": ([_vp, _vp], _i),
+    "mgr_warm_start_fallbacks": ([_vp, _vp], _i),
 }
 
 _LIB = None
@@ -436,8 +437,14 @@ class Context:
         self._check(self.lib.mgr_first_pass(self.h, _prec(precision), int(pair_stride)), "first_pass")
 
     def warm_start(self, mlp):
-        """Replace the populations by the network's prediction."""
+        """Replace the populations by the network's prediction; returns the
+        number of cells whose prediction was rejected (not finite and
+        positive, or a line with opacity <= 0) and that kept their previous
+        populations."""
         self._check(self.lib.mgr_warm_start(self.h, mlp.h), "warm_start")
+        n = C.c_int64(0)
+        self._check(self.lib.mgr_warm_start_fallbacks(self.h, C.byref(n)), "warm_start_fallbacks")
+        return n.value
 
 
 class BatchContext(Context):
