@@ -31,6 +31,14 @@ per second over the timed steps.  The JSON line also carries:
                    and from the learned warm start (trained weights in
                    magritte_amd/data), with the converged populations' match.
 
+--workload regular measures the same on the 16^3 regular grid (4096 cells,
+6-neighbour lists, 12 rays, one 2-level species, 39 bins at 20 K), where the
+CPU leg also runs the oracle's whole ALI: its time to converged populations
+and their difference from the GPU's.
+
+--workload stress measures the same on the optically thick 64^3 envelope
+(n0 = 1e17 m^-3, 20 % of cells with +-30 km/s radial velocity jumps).
+
 --workload batch measures the same on 512 small envelopes of uneven sizes
 and ray counts solved together in one context (one launch per kernel).
 
@@ -75,8 +83,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_workload(n):
+def build_workload(n, kind="envelope"):
     from magritte_amd.setup import synthetic
+    if kind == "regular":
+        return synthetic.regular_grid_model(n=n)
+    if kind == "stress":
+        return synthetic.stress_model(n=n)
     return synthetic.envelope_model(n=n)
 
 
@@ -148,6 +160,25 @@ def cpu_baseline(model, target_s, J_gpu=None, seed=0):
         J_o = orc.J()[sample]
         Jg = J_gpu[sample]
         out["J_max_rel_err_gpu_vs_oracle"] = float(np.max(np.abs(Jg - J_o) / np.abs(J_o)))
+    return out
+
+
+def cpu_time_to_converged(model, P_gpu=None):
+    """The oracle's whole ALI from LTE (max relative change < 1e-6, Ng), on
+    all host cores, and the largest relative difference of the GPU's
+    converged populations from the oracle's (levels above 1e-10 of the
+    cell's total)."""
+    from oracle import py_oracle
+    orc = py_oracle.Oracle(model).setup()
+    t0 = time.perf_counter()
+    it = orc.compute_level_populations(True, 400, 1)
+    dt = time.perf_counter() - t0
+    out = {"seconds": round(dt, 3), "iterations": int(it), "cores": int(py_oracle.lib().orc_nthreads_max()),
+           "kind": "port"}
+    if P_gpu is not None:
+        P_o = orc.populations()   # (N, V): one species
+        mask = P_o > 1e-10 * P_o.sum(axis=1, keepdims=True)
+        out["pops_max_rel_diff_gpu_vs_oracle"] = float(np.max(np.abs(P_gpu - P_o)[mask] / P_o[mask]))
     return out
 
 
@@ -262,6 +293,7 @@ def time_to_converged(make_ctx, precision, weights):
     it = ctx.compute_level_populations(True, 400, "max", precision)
     out["lte"] = {"seconds": round(time.perf_counter() - t0, 3), "iterations": int(it)}
     P_lte = ctx.populations()
+    out["_populations"] = P_lte
     ctx.close()
     log(f"[bench] converged from LTE in {it} iterations, {out['lte']['seconds']:.2f} s")
     if weights and os.path.exists(weights):
@@ -291,12 +323,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="envelope", choices=["envelope", "batch", "mlp"],
-                    help="envelope: one n^3 model; batch: many small models in one context; "
+    ap.add_argument("--workload", default="envelope", choices=["envelope", "regular", "stress", "batch", "mlp"],
+                    help="envelope: one n^3 model; regular: the 16^3 regular grid; stress: the optically "
+                         "thick envelope; batch: many small models in one context; "
                          "mlp: training of the warm-start network (a step is one epoch)")
     ap.add_argument("--mlp-rows", type=int, default=2_000_000)
     ap.add_argument("--mlp-batch", type=int, default=16384)
-    ap.add_argument("--n", type=int, default=64, help="cells per side of the envelope (64 = full workload)")
+    ap.add_argument("--n", type=int, default=None,
+                    help="cells per side (default 64 for the envelopes = full workload, 16 for the regular grid)")
     ap.add_argument("--models", type=int, default=512, help="models of the batch workload")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -319,13 +353,21 @@ def main():
 
     from magritte_amd.hip import BatchContext, Context
     t0 = time.perf_counter()
-    if args.workload == "envelope":
-        model = build_workload(args.n)
+    single = args.workload in ("envelope", "regular", "stress")
+    if args.n is None:
+        args.n = 16 if args.workload == "regular" else 64
+    if single:
+        model = build_workload(args.n, args.workload)
         models = [model]
         make_ctx = lambda: Context(model).setup()   # noqa: E731
-        workload = (f"envelope point cloud {args.n}^3 jittered, 26-neighbour lists, {model.nrays} rays, "
-                    f"7-level rotor, 6 lines x 31 bins")
-        tag = f"envelope{args.n}"
+        if args.workload == "regular":
+            workload = (f"regular grid {args.n}^3, 6-neighbour lists, {model.nrays} rays, 2-level species, "
+                        f"{model.nfreqs} bins, n_H2 1e12, 20 K")
+        else:
+            workload = (f"{'stress ' if args.workload == 'stress' else ''}envelope point cloud {args.n}^3 "
+                        f"jittered, 26-neighbour lists, {model.nrays} rays, 7-level rotor, 6 lines x 31 bins"
+                        + (", n0 1e17, 20% +-30 km/s jumps" if args.workload == "stress" else ""))
+        tag = f"{args.workload}{args.n}"
     else:
         from magritte_amd.warmstart import training_model
         rng = np.random.default_rng(2024)
@@ -349,7 +391,7 @@ def main():
     log(f"[bench] {kernel}: launch shape (waves per workgroup, steps per chunk) {shape}; tried {tried}")
     ms_per_step = 1e3 * wall / args.steps
     value = solves_per_step * args.steps / wall
-    J_gpu = ctx.J() if args.workload == "envelope" and not args.no_cpu else None
+    J_gpu = ctx.J() if single and not args.no_cpu else None
     other = None
     if args.precision == "fp32" and args.fp64_steps > 0:
         # the FP64 kernel on the same workload, for comparison
@@ -374,14 +416,22 @@ def main():
         out["fp64_kernel"] = other
     ctx.close()
 
+    P_gpu = None
     if not args.no_converge:
-        out["time_to_converged"] = time_to_converged(make_ctx_traced(make_ctx), args.precision, args.weights)
+        # the network predicts the 7 levels of the rotor: no warm start for the 2-level grid
+        weights = None if args.workload == "regular" else args.weights
+        out["time_to_converged"] = time_to_converged(make_ctx_traced(make_ctx), args.precision, weights)
+        P_gpu = out["time_to_converged"].pop("_populations")
 
-    if not args.no_cpu and args.workload == "envelope":
+    if not args.no_cpu and single:
         log("[bench] CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(model, args.cpu_seconds, J_gpu)
         log(f"[bench] GPU J against the oracle on the sample: max rel err "
             f"{out['cpu_baseline']['J_max_rel_err_gpu_vs_oracle']:.3e}")
+        if args.workload == "regular":
+            # small enough for the oracle's whole ALI
+            out["cpu_baseline"]["time_to_converged"] = cpu_time_to_converged(model, P_gpu)
+            log(f"[bench] oracle ALI: {out['cpu_baseline']['time_to_converged']}")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
