@@ -130,34 +130,36 @@ def converge(model, mlp=None, precision="fp32", max_iter=MAX_ITER):
     or from the network's populations.  The clock starts after the one-off
     per-model set-up (trace, masks, far-end cache, launch tuning and the ALI
     graphs: mgr_prepare) and includes, for the warm start, the first pass,
-    the features and the inference."""
+    the features and the inference.  Also returns the cells whose prediction
+    the warm start rejected (they kept LTE)."""
     ctx = Context(model).setup()
     ctx.trace()
     ctx.prepare(precision)
     t0 = time.perf_counter()
+    fallbacks = 0
     if mlp is not None:
         ctx.first_pass(precision, getattr(mlp, "first_pass_stride", 1))
-        ctx.warm_start(mlp)
+        fallbacks = ctx.warm_start(mlp)
     it = ctx.compute_level_populations(True, max_iter, CRITERION, precision)
     dt = time.perf_counter() - t0
     pops = ctx.populations()
     ctx.close()
-    return int(it), dt, pops
+    return int(it), dt, pops, fallbacks
 
 
 def evaluate_model(model, mlp, precision="fp32", max_iter=MAX_ITER):
     """LTE start against the warm start on one model.  `converged` is False
     when the LTE-started iteration (the reference for the comparison) does not
     meet the criterion within max_iter iterations."""
-    it_l, t_l, p_l = converge(model, None, precision, max_iter)
-    it_n, t_n, p_n = converge(model, mlp, precision, max_iter)
+    it_l, t_l, p_l, _ = converge(model, None, precision, max_iter)
+    it_n, t_n, p_n, fb = converge(model, mlp, precision, max_iter)
     ntot = model.abundance[:, model.linedata.num]
     mask = p_l > 1e-10 * ntot[:, None]
     rel = np.abs(p_n - p_l)[mask] / np.abs(p_l[mask])
     return {"converged": bool(it_l < max_iter and it_n < max_iter),
             "lte_iterations": it_l, "lte_seconds": round(t_l, 4), "nn_iterations": it_n,
             "nn_seconds": round(t_n, 4), "speedup": round(t_l / t_n, 3),
-            "pops_max_rel_diff": float(rel.max())}
+            "pops_max_rel_diff": float(rel.max()), "cells_kept_lte": int(fb)}
 
 
 def main(argv=None):
@@ -225,7 +227,9 @@ def main(argv=None):
                "mean_nn_iterations": float(np.mean([r["nn_iterations"] for r in ok])),
                "models_fewer_iterations": sum(r["nn_iterations"] < r["lte_iterations"] for r in rows if r["converged"]),
                "models_more_iterations": sum(r["nn_iterations"] > r["lte_iterations"] for r in rows if r["converged"]),
-               "max_pops_rel_diff": float(np.max([r["pops_max_rel_diff"] for r in ok])), "rows": rows}
+               "max_pops_rel_diff": float(np.max([r["pops_max_rel_diff"] for r in ok])),
+               "models_with_cells_kept_lte": sum(r["cells_kept_lte"] > 0 for r in rows),
+               "cells_kept_lte": sum(r["cells_kept_lte"] for r in rows), "rows": rows}
     if a.out:
         with open(a.out, "w") as fh:
             json.dump(summary, fh, indent=1)
