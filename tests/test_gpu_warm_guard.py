@@ -4,7 +4,8 @@ starts, through the C-ABI.
 - mgr_warm_start keeps a cell's previous populations when the network's
   prediction is not finite and positive or inverts a line (opacity <= 0):
   a network whose output inverts every line leaves the LTE start unchanged,
-  bit for bit; one whose output inverts none is taken as predicted.
+  bit for bit; one whose output inverts none is taken as predicted.  The
+  same holds per cell over the concatenated models of a batch.
 - The trained network far outside its training data (the optically thick
   stress envelope, n0 = 1e17 m^-3, where its predictions invert lines) still
   converges to the LTE-started populations.
@@ -105,3 +106,23 @@ def test_nonfinite_populations_stop_the_iteration():
     with pytest.raises(HipError, match="non-finite"):
         ctx.compute_level_populations(True, 10, "max", "fp32")
     ctx.close()
+
+
+def test_guard_in_a_batch():
+    """The guard per cell over the concatenated models of a batch."""
+    from magritte_amd.hip import BatchContext
+    ms = [synthetic.envelope_model(n=n, nside=1, nquads=9, seed=s) for n, s in ((5, 1), (6, 2))]
+    print("\n[guard] batch of 5^3 and 6^3 envelopes", flush=True)
+    b = BatchContext(ms).setup()
+    b.trace()
+    b.first_pass("fp32", 1)
+    P_lte = b.populations().copy()
+    bad = constant_network(0.5 * np.arange(MLP_OUT))
+    assert b.warm_start(bad) == b.N
+    assert np.array_equal(b.populations(), P_lte)
+    good = constant_network(-np.arange(MLP_OUT, dtype=np.float32))
+    assert b.warm_start(good) == 0
+    assert not np.array_equal(b.populations(), P_lte)
+    b.close()
+    bad.close()
+    good.close()
