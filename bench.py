@@ -329,7 +329,8 @@ def time_to_converged(make_ctx, precision, weights, max_iter=400):
         ctx = make_ctx()
         ctx.prepare(precision)
         t0 = time.perf_counter()
-        ctx.first_pass(precision, net.first_pass_stride)
+        if net.first_pass_stride > 0:
+            ctx.first_pass(precision, net.first_pass_stride)
         fallbacks = ctx.warm_start(net)
         it = ctx.compute_level_populations(True, max_iter, "max", precision)
         dt = time.perf_counter() - t0

@@ -445,10 +445,10 @@ int mgr_mlp_loss_grad(mgr_mlp *mlp, const float *X, const float *T, int64_t n, f
  * loss_per_epoch [epochs] may be NULL.                                        */
 int mgr_mlp_train(mgr_mlp *mlp, const float *X, const float *T, int64_t n, int64_t epochs,
                   int64_t batch, float lr, uint64_t seed, float *loss_per_epoch);
-/* Per-cell features [ncells][8] after a radiation field and J_eff:
- * log10 n(collision partner), log10 T, log10 n(species), log10 of the
- * largest neighbour velocity gradient (s^-1), log10 Jlin_k / B_nu_k(T) for
- * transitions k = 0..3.                                                       */
+/* Per-cell features [ncells][8]: log10 n(collision partner), log10 T,
+ * log10 n(species), log10 of the largest neighbour velocity gradient (s^-1),
+ * log10 Jlin_k / B_nu_k(T) for transitions k = 0..3 after a radiation field
+ * and J_eff (0 before any).                                                   */
 int mgr_features(mgr_ctx *ctx, float *X);
 /* Features -> network -> populations (pop_tot 10^y_i / sum 10^y) in place of
  * the current populations; needs 7 levels.  A cell whose prediction is not
@@ -460,6 +460,10 @@ int mgr_features(mgr_ctx *ctx, float *X);
 int mgr_warm_start(mgr_ctx *ctx, mgr_mlp *mlp);
 /* Cells that kept their previous populations in the last mgr_warm_start.     */
 int mgr_warm_start_fallbacks(const mgr_ctx *ctx, int64_t *cells);
+/* Whether the network reads the Jlin inputs (4..7; the default).  A network
+ * trained with those inputs at 0 (use = 0) needs no first pass: mgr_warm_start
+ * then sets them to 0 and needs no radiation field before it.                 */
+int mgr_mlp_set_jlin_inputs(mgr_mlp *mlp, int use);
 /* The warm start's first pass from the current populations: a radiation
  * field over every pair_stride-th ray pair of each origin (ray weights scaled
  * by pair_stride, an approximation of J at 1/pair_stride of the cost; 1 is

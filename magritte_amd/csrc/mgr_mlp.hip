@@ -577,7 +577,7 @@ __global__ void k_features(int64_t N, int L, int nspecs, int num, int partner, c
     x[3] = (float)log10(fmax(gmax * CC, floor_));
     for (int k = 0; k < 4; k++) {
         float f = 0.0f;
-        if (k < L) {
+        if (k < L && Jlin) {   // Jlin null: no first pass, the inputs are 0
             const double nu = frequency[k];
             const double B = TWO_HH_OVER_CC_SQUARED * nu * nu * nu / expm1(HH_OVER_KB * nu / T[p]);
             f = (float)log10(fmax(Jlin[p * L + k] / B, floor_));
@@ -597,6 +597,7 @@ struct mgr_mlp {
     Norm *norm = nullptr;
     Norm h_norm;
     int64_t step = 0;
+    bool jlin_inputs = true;   // reads inputs 4..7 (Jlin / B_nu after a first pass)
     int fail(int code, const std::string &msg)
     {
         err = msg;
@@ -872,15 +873,15 @@ extern "C" int mgr_mlp_train(mgr_mlp *mlp, const float *X, const float *T, int64
 // ---- context side: features and warm start -----------------------------------
 
 namespace mgr {
-int launch_features(mgr_ctx *ctx, float *dX);
+int launch_features(mgr_ctx *ctx, float *dX, bool jlin);
 }
 
-int mgr::launch_features(mgr_ctx *ctx, float *dX)
+int mgr::launch_features(mgr_ctx *ctx, float *dX, bool jlin)
 {
     const int partner = ctx->h_cp_num.empty() ? -1 : (int)ctx->h_cp_num[0];
     hipLaunchKernelGGL(k_features, dim3((unsigned)((ctx->N + 255) / 256)), dim3(256), 0, ctx->stream, ctx->N,
                        (int)ctx->L, (int)ctx->nspecs, (int)ctx->num, partner, ctx->abundance, ctx->temperature,
-                       ctx->pos, ctx->vel, ctx->nb_off, ctx->nb, ctx->frequency, ctx->Jlin, dX);
+                       ctx->pos, ctx->vel, ctx->nb_off, ctx->nb, ctx->frequency, jlin ? ctx->Jlin : nullptr, dX);
     MGR_LAUNCH_CHECK();
     return MGR_OK;
 }
@@ -889,11 +890,10 @@ extern "C" int mgr_features(mgr_ctx *ctx, float *X)
 {
     if (!ctx || !X) return MGR_ERR_ARGUMENT;
     if (ctx->S > 1) return ctx->fail(MGR_ERR_UNSUPPORTED, "the warm-start features are defined for one line-producing species");
-    if (!ctx->J_set) return ctx->fail(MGR_ERR_STATE, "features need a radiation field and J_eff first");
     MGR_HIP(hipSetDevice(ctx->device));
     float *dX = nullptr;
     MGR_HIP(hipMalloc(&dX, ctx->N * IN * sizeof(float)));
-    int rc = mgr::launch_features(ctx, dX);
+    int rc = mgr::launch_features(ctx, dX, ctx->J_set);   // before any radiation field the J inputs are 0
     if (!rc && (hipStreamSynchronize(ctx->stream) != hipSuccess ||
                 hipMemcpy(X, dX, ctx->N * IN * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess))
         rc = ctx->fail(MGR_ERR_DEVICE, "features copy failed");
@@ -906,7 +906,8 @@ extern "C" int mgr_warm_start(mgr_ctx *ctx, mgr_mlp *mlp)
     if (!ctx || !mlp) return MGR_ERR_ARGUMENT;
     if (ctx->V != OUT || ctx->S > 1)
         return ctx->fail(MGR_ERR_UNSUPPORTED, "the warm-start network predicts the 7 levels of one species");
-    if (!ctx->J_set) return ctx->fail(MGR_ERR_STATE, "warm start needs a radiation field and J_eff first");
+    if (mlp->jlin_inputs && !ctx->J_set)
+        return ctx->fail(MGR_ERR_STATE, "warm start needs a radiation field and J_eff first (the network reads J)");
     if (ctx->device != mlp->device) return ctx->fail(MGR_ERR_ARGUMENT, "context and network on different devices");
     MGR_HIP(hipSetDevice(ctx->device));
     MGR_HIP(hipStreamSynchronize(mlp->stream));
@@ -922,7 +923,7 @@ extern "C" int mgr_warm_start(mgr_ctx *ctx, mgr_mlp *mlp)
         (void)hipFree(dPrev);
         return ctx->fail(MGR_ERR_NOMEM, "warm start: device allocation failed");
     }
-    int rc = mgr::launch_features(ctx, dX);
+    int rc = mgr::launch_features(ctx, dX, mlp->jlin_inputs);
     if (!rc && (hipMemcpyAsync(dPrev, ctx->pop, npop * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
                 hipMemsetAsync(dCount, 0, sizeof(unsigned long long), ctx->stream) != hipSuccess))
         rc = ctx->fail(MGR_ERR_DEVICE, "warm start: copy failed");
@@ -946,6 +947,13 @@ extern "C" int mgr_warm_start(mgr_ctx *ctx, mgr_mlp *mlp)
         ctx->pops_set = true;
     }
     return rc;
+}
+
+extern "C" int mgr_mlp_set_jlin_inputs(mgr_mlp *mlp, int use)
+{
+    if (!mlp) return MGR_ERR_ARGUMENT;
+    mlp->jlin_inputs = use != 0;
+    return MGR_OK;
 }
 
 extern "C" int mgr_warm_start_fallbacks(const mgr_ctx *ctx, int64_t *cells)

@@ -87,6 +87,7 @@ SIGNATURES = {
     "mgr_features": ([_vp, _vp], _i),
     "mgr_warm_start": ([_vp, _vp], _i),
     "mgr_warm_start_fallbacks": ([_vp, _vp], _i),
+    "mgr_mlp_set_jlin_inputs": ([_vp, _i], _i),
 }
 
 _LIB = None
@@ -541,6 +542,10 @@ class Mlp:
         a = np.ascontiguousarray(_pack_params(p), dtype=np.float32)
         assert a.size == self.nparams
         self._check(self.lib.mgr_mlp_set_params(self.h, _p(a)), "set_params")
+
+    def set_jlin_inputs(self, use):
+        """Whether the network reads the first-pass Jlin inputs (4..7)."""
+        self._check(self.lib.mgr_mlp_set_jlin_inputs(self.h, 1 if use else 0), "set_jlin_inputs")
 
     def set_normalisation(self, x_mean, x_std, y_mean, y_std):
         arrs = [np.ascontiguousarray(v, dtype=np.float32) for v in (x_mean, x_std, y_mean, y_std)]

@@ -47,10 +47,12 @@ def training_model(rng):
 
 def first_pass(model, stride=FIRST_PASS_STRIDE):
     """Context after set-up, trace and the warm start's first pass from LTE
-    (a radiation field over every stride-th ray pair, and J_eff)."""
+    (a radiation field over every stride-th ray pair, and J_eff); stride 0:
+    no first pass (the Jlin features are 0)."""
     ctx = Context(model).setup()
     ctx.trace()
-    ctx.first_pass("fp32", stride)
+    if stride > 0:
+        ctx.first_pass("fp32", stride)
     return ctx
 
 
@@ -122,6 +124,8 @@ def load(path, device=0):
     mlp.set_normalisation(d["x_mean"], d["x_std"], d["y_mean"], d["y_std"])
     meta = json.loads(str(d["meta"])) if "meta" in d.files else {}
     mlp.first_pass_stride = int(meta.get("first_pass_stride", 1))
+    if mlp.first_pass_stride == 0:
+        mlp.set_jlin_inputs(False)   # trained without the first pass
     return mlp
 
 
@@ -138,7 +142,9 @@ def converge(model, mlp=None, precision="fp32", max_iter=MAX_ITER):
     t0 = time.perf_counter()
     fallbacks = 0
     if mlp is not None:
-        ctx.first_pass(precision, getattr(mlp, "first_pass_stride", 1))
+        stride = getattr(mlp, "first_pass_stride", 1)
+        if stride > 0:
+            ctx.first_pass(precision, stride)
         fallbacks = ctx.warm_start(mlp)
     it = ctx.compute_level_populations(True, max_iter, CRITERION, precision)
     dt = time.perf_counter() - t0
@@ -170,7 +176,8 @@ def main(argv=None):
     tr.add_argument("--epochs", type=int, default=30)
     tr.add_argument("--batch", type=int, default=16384)
     tr.add_argument("--lr", type=float, default=4e-3)
-    tr.add_argument("--stride", type=int, default=FIRST_PASS_STRIDE, help="ray-pair stride of the first pass")
+    tr.add_argument("--stride", type=int, default=FIRST_PASS_STRIDE,
+                    help="ray-pair stride of the first pass; 0: no first pass (the Jlin inputs are 0)")
     tr.add_argument("--seed", type=int, default=0)
     tr.add_argument("--out", required=True)
     ev = sub.add_parser("evaluate")
