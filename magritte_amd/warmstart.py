@@ -2,15 +2,17 @@
 
 The network (include/magritte_hip.h, mgr_mlp_*) maps per-cell features --
 log10 collision-partner density, temperature, species density, neighbour
-velocity gradient and log10 Jlin/B_nu(T) of the first four transitions after
-a first pass from LTE -- to log10 level-population fractions.  The first pass
-is a radiation field over every FIRST_PASS_STRIDE-th ray pair (mgr_first_pass),
-a quarter of a full one, so that the warm start costs less than the ALI
-iteration it saves.  Its training data are populations the plain LTE-started
-ALI iteration converged on many small envelope models; training and inference
-run in the project's own HIP kernels.
+velocity gradient and log10 Jlin/B_nu(T) of the first four transitions -- to
+log10 level-population fractions.  Jlin comes from a first pass from LTE, a
+radiation field over every stride-th ray pair (mgr_first_pass; --stride,
+default FIRST_PASS_STRIDE), or, with stride 0, the Jlin inputs are 0 and no
+radiation field is paid before the ALI: the shipped weights are trained that
+way (--stride 0), which converged as fast as first-pass networks on held-out
+models and costs less.  Its training data are populations the plain
+LTE-started ALI iteration converged on many small envelope models; training
+and inference run in the project's own HIP kernels.
 
-    python -m magritte_amd.warmstart train --models 1000 --out weights.npz
+    python -m magritte_amd.warmstart train --models 2000 --epochs 80 --stride 0 --out weights.npz
     python -m magritte_amd.warmstart evaluate --weights weights.npz --models 8
 
 A warm start replaces only the starting populations: the ALI loop after it is
